@@ -1,0 +1,211 @@
+/*
+ * hdnnp.h -- C ABI of the MI355X HDNNP energy-and-force engine.
+ *
+ * This is the drop-in boundary under n2p2's libnnp Mode API.  Every entry
+ * point replaces one reference interface, cited as file:line relative to the
+ * n2p2 tree (src/libnnp unless stated).  The header-compatible C++ layer in
+ * n2p2_amd/csrc/nnp (nnp::Mode, nnp::Structure, nnp::Prediction) calls
+ * these functions; ctypes / cgo / JNI bindings can call them directly.
+ *
+ * Conventions
+ *   - plain C types only; arrays are caller-owned host memory unless the
+ *     name ends in _device (then they are device pointers on the context's
+ *     GPU);
+ *   - every function returning int returns HDNNP_OK (0) on success or a
+ *     negative HDNNP_E_* code; hdnnp_last_error() then holds a message that
+ *     starts with "ERROR: " like the reference's exceptions;
+ *   - a context is single-threaded (like a reference Mode object): calls on
+ *     one context must not overlap; distinct contexts are independent;
+ *   - all compute calls are synchronous at return unless stated.
+ *   - units are the model's internal units (normalised when the settings
+ *     file has mean_energy / conv_energy / conv_length, as in Mode).
+ */
+#ifndef HDNNP_H
+#define HDNNP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDNNP_OK            0
+#define HDNNP_E_INVALID    -1   /* bad argument / inconsistent call order */
+#define HDNNP_E_FILE       -2   /* file missing or malformed */
+#define HDNNP_E_SETTINGS   -3   /* settings keyword error */
+#define HDNNP_E_DEVICE     -4   /* HIP runtime error */
+#define HDNNP_E_CAPACITY   -5   /* a size exceeds a documented limit */
+#define HDNNP_E_UNSUPPORTED -6  /* feature outside this engine's scope */
+
+/* Force-gather modes for hdnnp_forces / hdnnp_evaluate. */
+#define HDNNP_FORCES_REFERENCE 0  /* Mode::calculateForces: an atom's own
+                                     periodic self-image slots are skipped
+                                     (Mode.cpp:1885-1887) */
+#define HDNNP_FORCES_CONSISTENT 1 /* adds those slots: forces = -dE/dr also
+                                     when the cell is smaller than the cutoff */
+
+/* Arithmetic of the descriptor and network kernels. */
+#define HDNNP_PRECISION_FP64  0   /* plain path: FP64 throughout */
+#define HDNNP_PRECISION_MIXED 1   /* FP32 descriptors with compensated sums,
+                                     16-bit MFMA network layers with FP32
+                                     accumulation; FP64 energy total, force
+                                     accumulation and SF scaling */
+
+typedef struct hdnnp_model hdnnp_model;
+typedef struct hdnnp_ctx   hdnnp_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hdnnp_last_error(void);
+
+/* ------------------------------------------------------------ model -- */
+
+/* Reads the settings file, the scaling file and one weight file per element
+ * and sets the model up exactly as the reference sequence
+ *   Mode::initialize; loadSettingsFile; setupGeneric;
+ *   setupSymmetryFunctionScaling (or ...ScalingNone if scaling_file is NULL
+ *   or ""); setupNeuralNetworkWeights
+ * (Mode.cpp:55-236, :692-840, :1158-1478, :2270-2289; Prediction.cpp:34-46).
+ * weights_fmt is a printf format with one size_t conversion for the atomic
+ * number, e.g. "weights.%03zu.data".  Symmetry-function types 2, 3 and 9
+ * and all cutoff, scaling and activation types of the reference are
+ * supported; other SF types return HDNNP_E_UNSUPPORTED. */
+int  hdnnp_model_load(const char* settings_file, const char* scaling_file,
+                      const char* weights_fmt, hdnnp_model** out);
+void hdnnp_model_free(hdnnp_model* m);
+
+/* Element count / atomic number of element e (elements sorted by Z,
+ * ElementMap.cpp:36-54) / SF count of element e (Element::
+ * numSymmetryFunctions) / Mode::getMaxCutoffRadius (Mode.h:468). */
+int    hdnnp_model_num_elements(const hdnnp_model* m);
+int    hdnnp_model_atomic_number(const hdnnp_model* m, int e);
+int    hdnnp_model_num_sf(const hdnnp_model* m, int e);
+double hdnnp_model_max_cutoff(const hdnnp_model* m);
+/* Element index of a chemical symbol (ElementMap::operator[]), or -1. */
+int    hdnnp_model_element_index(const hdnnp_model* m, const char* symbol);
+/* Normalisation constants (Mode.cpp:238-299); returns 1 if used, else 0. */
+int    hdnnp_model_normalization(const hdnnp_model* m, double* mean_energy,
+                                 double* conv_energy, double* conv_length);
+/* Element::getAtomicEnergyOffset (atom_energy keyword, Mode.cpp:340-350). */
+double hdnnp_model_energy_offset(const hdnnp_model* m, int e);
+/* Sorted SF table of element e, 12 doubles per SF in NN-input order:
+ * type, e1, e2 (-1 for radial), eta, rs, lambda, zeta, rc, cutoff_type,
+ * cutoff_alpha, scaling_factor, settings line number
+ * (Element::sortSymmetryFunctions, Element.cpp:154-166). */
+int    hdnnp_model_sf_table(const hdnnp_model* m, int e, double* out);
+/* Replaces element e's scaling statistics (4 doubles per SF: Gmin, Gmax,
+ * Gmean, Gsigma) and recomputes the scaling factors, as
+ * SymFnc::setScalingType + SymGrp*::setScalingFactors do when a scaling.data
+ * line is read (SymFnc.cpp:52-98, SymGrpBaseExpAng.cpp:45-57).  scale_type
+ * is 0 none, 1 scale, 2 center, 3 scale+center, 4 scale-sigma. */
+int    hdnnp_model_set_scaling(hdnnp_model* m, int e, int scale_type,
+                               double smin, double smax,
+                               const double* stats);
+
+/* ---------------------------------------------------------- context -- */
+
+/* One engine context per Mode object: device copies of the model tables,
+ * structure and work buffers, one HIP stream.  device = HIP ordinal. */
+int  hdnnp_ctx_create(const hdnnp_model* m, int device, hdnnp_ctx** out);
+void hdnnp_ctx_destroy(hdnnp_ctx* c);
+/* HDNNP_PRECISION_FP64 (default) or HDNNP_PRECISION_MIXED. */
+int  hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision);
+/* Propagates a model change made after hdnnp_ctx_create (e.g.
+ * hdnnp_model_set_scaling) to the context's device tables. */
+int  hdnnp_ctx_update_model(hdnnp_ctx* c);
+
+/* --------------------------------------------------------- structure -- */
+
+/* Loads a structure: n atoms, positions r (n x 3, row-major), element
+ * indices elem (n), box = three lattice vectors as rows (9 doubles) or NULL
+ * for a non-periodic structure.  Positions must already lie in the cell as
+ * after Structure::remap (Structure.cpp:1223-1247), which is what
+ * Structure::readFromLines leaves (Structure.cpp:268).  Replaces the
+ * Structure fields atoms[i].r, atoms[i].element, box, isPeriodic that Mode
+ * reads (Structure.h:67-118). */
+int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
+                        const int32_t* elem, const double* box);
+/* New positions for the same atoms (host array, n x 3). */
+int hdnnp_set_positions(hdnnp_ctx* c, const double* r);
+/* New positions already on the device (n x 3 doubles); no copy to host. */
+int hdnnp_set_positions_device(hdnnp_ctx* c, const double* r_device);
+
+/* ----------------------------------------------------------- compute -- */
+
+/* Structure::calculateNeighborList(cutoff, false) (Structure.cpp:287-399):
+ * per atom all (j, image) with |r_i - r_j + image| <= cutoff, ordered by j
+ * then image components ascending; bit-identical index, d and dr.
+ * Built with a cell list on the device. */
+int hdnnp_neighbor_list(hdnnp_ctx* c, double cutoff);
+/* Mode::calculateSymmetryFunctionGroups(structure, derivatives)
+ * (Mode.cpp:1561-1640).  Computes the scaled SF values G.  With
+ * derivatives = 1 the per-neighbour dG/dr of the reference are not stored:
+ * the force kernel recomputes them (see hdnnp_forces). */
+int hdnnp_symmetry_functions(hdnnp_ctx* c, int derivatives);
+/* Mode::calculateAtomicNeuralNetworks(structure, derivatives)
+ * (Mode.cpp:1642-1663): atomic energies and, with derivatives, dE_i/dG_i. */
+int hdnnp_atomic_networks(hdnnp_ctx* c, int derivatives);
+/* Mode::calculateEnergy (Mode.cpp:1803-1828): energy = sum of atomic
+ * energies, summed in FP64 with a fixed order. */
+int hdnnp_energy(hdnnp_ctx* c, double* energy);
+/* Mode::calculateForces (Mode.cpp:1849-1913) with Atom::
+ * calculateSelfForceShort / calculatePairForceShort (Atom.cpp:371-401):
+ * F = -dE/dr by contracting dE/dG with dG/dr recomputed on the fly and a
+ * deterministic per-atom gather (no atomics).  mode = HDNNP_FORCES_*. */
+int hdnnp_forces(hdnnp_ctx* c, int mode);
+/* Mode::evaluateNNP (Mode.cpp:1967-2016): neighbour list with the model's
+ * maximum cutoff, SFs, networks, energy and (if force_mode >= 0) forces. */
+int hdnnp_evaluate(hdnnp_ctx* c, int force_mode, double* energy);
+
+/* ----------------------------------------------------------- results -- */
+
+int64_t hdnnp_num_atoms(const hdnnp_ctx* c);
+/* Total neighbour-list entries after hdnnp_neighbor_list. */
+int64_t hdnnp_num_neighbor_entries(const hdnnp_ctx* c);
+/* Structure::pbc (Structure.cpp:486-517) of the last neighbour list. */
+int hdnnp_get_pbc(hdnnp_ctx* c, int32_t* pbc);
+/* Neighbour list as CSR: offsets (n+1), index (M), image (3M, the integer
+ * image vector bc of Structure.cpp:309-322), d (M), dr (3M).  Any output
+ * pointer may be NULL.  Replaces Atom::neighbors / Atom::Neighbor
+ * (Atom.h:37-79). */
+int hdnnp_get_neighbors(hdnnp_ctx* c, int64_t* offsets, int32_t* index,
+                        int32_t* image, double* d, double* dr);
+/* Atom::G (n x ld, ld >= max SF count, zero padded). */
+int hdnnp_get_G(hdnnp_ctx* c, double* G, int ld);
+/* Atom::dEdG (n x ld). */
+int hdnnp_get_dEdG(hdnnp_ctx* c, double* dEdG, int ld);
+/* Atom::energy (n). */
+int hdnnp_get_atomic_energies(hdnnp_ctx* c, double* e);
+/* Atom::f (n x 3). */
+int hdnnp_get_forces(hdnnp_ctx* c, double* f);
+/* Device pointer of the n x 3 force array (valid until the next
+ * hdnnp_set_structure); for callers that keep MD state on the GPU. */
+const double* hdnnp_forces_device(hdnnp_ctx* c);
+/* Number of extrapolation warnings of the last SF evaluation: G outside
+ * [Gmin, Gmax] +- 1000 eps before scaling (Element.cpp:462-541).
+ * per_atom (n, may be NULL) receives the count per atom. */
+int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom);
+
+/* Symmetry-function statistics over the loaded structure as nnp-scaling
+ * writes them to scaling.data (Dataset.cpp:974-1100): SFs evaluated without
+ * scaling, per SF of element e: min, max, mean, sigma (4 doubles each).
+ * Needs hdnnp_neighbor_list first.  Returns the number of atoms of e. */
+int64_t hdnnp_sf_statistics(hdnnp_ctx* c, int e, double* out);
+
+/* ---------------------------------------------------------- profiling -- */
+
+/* Average device time in milliseconds of each kernel class over the calls
+ * since the last reset, measured with HIP events on the context's stream.
+ * names: "neighbor", "symfunc", "network", "force", "gather".  Returns -1
+ * for an unknown name.  enable != 0 turns event recording on. */
+int    hdnnp_profile_enable(hdnnp_ctx* c, int enable);
+double hdnnp_profile_avg_ms(hdnnp_ctx* c, const char* name, int64_t* calls);
+int    hdnnp_profile_reset(hdnnp_ctx* c);
+/* Algorithmic work of the last evaluation: counts of neighbour pairs,
+ * element-matched angular pairs, triplets inside the cutoff, and SF member
+ * evaluations (radial, angular), for roofline accounting. */
+int    hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts /* 6 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDNNP_H */

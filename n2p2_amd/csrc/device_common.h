@@ -1,0 +1,197 @@
+// Device helpers shared by the HIP kernels: cutoff functions, integer
+// powers, activations and 64-lane wave reductions.  All FP64.
+#ifndef HDNNP_DEVICE_COMMON_H
+#define HDNNP_DEVICE_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tables.h"
+
+namespace hdnnp {
+
+constexpr int kWave = 64;
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// CutoffFunction::fdf (CutoffFunction.h:210-220, CutoffFunction.cpp:128-213,
+// CoreFunction.h:214-318): value and derivative, zero at and beyond rc.
+__device__ __forceinline__ void cutoff_fdf(CutCfg const& c, double r,
+                                           double& fc, double& dfc)
+{
+    if (r >= c.rc) { fc = 0.0; dfc = 0.0; return; }
+    int const t = c.type;
+    if (t == CT_TANHU || t == CT_TANH)
+    {
+        double const th = tanh(1.0 - r * c.rcinv);
+        double const th2 = th * th;
+        // CT_TANH multiplies by tanh(1)^-3 so that f(0) = 1 (host-computed).
+        double const pre = (t == CT_TANH) ? c.tanhpre : 1.0;
+        fc = pre * th * th2;
+        dfc = 3.0 * pre * th2 * (th2 - 1.0) * c.rcinv;
+        return;
+    }
+    if (t == CT_HARD) { fc = 1.0; dfc = 0.0; return; }
+    if (r < c.rci) { fc = 1.0; dfc = 0.0; return; }
+    double const x = (r - c.rci) * c.iw;
+    if (t == CT_COS)
+    {
+        double const cs = cos(kPi * x);
+        fc = 0.5 * (cs + 1.0);
+        dfc = -0.5 * c.iw * kPi * sqrt(1.0 - cs * cs);
+        return;
+    }
+    double f, df;
+    if (t == CT_POLY1)
+    {
+        f = (2.0 * x - 3.0) * x * x + 1.0;
+        df = x * (6.0 * x - 6.0);
+    }
+    else if (t == CT_POLY2)
+    {
+        double const x2 = x * x;
+        f = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
+        df = x2 * ((60.0 - 30.0 * x) * x - 30.0);
+    }
+    else if (t == CT_POLY3)
+    {
+        double const x2 = x * x;
+        f = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
+        df = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
+    }
+    else if (t == CT_POLY4)
+    {
+        double x4 = x * x;
+        x4 *= x4;
+        f = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0)
+          * x * x4 + 1.0;
+        df = x4 * (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0)
+           - 630.0);
+    }
+    else  // CT_EXP: f(x) = e^(1 - 1/(1-x^2))
+    {
+        double const e = 2.718281828459045090795598298427648842334747314453125;
+        double const tmp = 1.0 / (x * x - 1.0);
+        double const ex = exp(tmp);
+        f = e * ex;
+        df = -2.0 * e * x * tmp * tmp * ex;
+    }
+    fc = f;
+    dfc = df * c.iw;
+}
+
+// Exponentiation by squaring (utility.cpp:292-317 semantics).
+__device__ __forceinline__ double pow_int(double x, int n)
+{
+    unsigned m;
+    if (n < 0) { x = 1.0 / x; m = (unsigned)(-n); }
+    else m = (unsigned)n;
+    double r = 1.0;
+    do
+    {
+        if (m & 1u) r *= x;
+        m >>= 1;
+        x *= x;
+    } while (m);
+    return r;
+}
+
+// NeuralNetwork::propagateLayer activations (NeuralNetwork.cpp:761-896):
+// value and first derivative.  Logistic and softplus clamp at |x| > 35.
+__device__ __forceinline__ void activate(int a, double x, double& v, double& d)
+{
+    switch (a)
+    {
+    case AF_IDENTITY: v = x; d = 1.0; break;
+    case AF_TANH:
+    {
+        double const t = tanh(x);
+        v = t; d = 1.0 - t * t;
+        break;
+    }
+    case AF_LOGISTIC:
+        if (x > 35.0) { v = 1.0; d = 0.0; }
+        else if (x < -35.0) { v = 0.0; d = 0.0; }
+        else
+        {
+            double const s = 1.0 / (1.0 + exp(-x));
+            v = s; d = s * (1.0 - s);
+        }
+        break;
+    case AF_SOFTPLUS:
+        if (x > 35.0) { v = x; d = 1.0; }
+        else if (x < -35.0) { v = 0.0; d = 0.0; }
+        else
+        {
+            double const ex = exp(x);
+            v = log(1.0 + ex);
+            d = 1.0 / (1.0 + 1.0 / ex);
+        }
+        break;
+    case AF_RELU:
+        if (x > 0.0) { v = x; d = 1.0; } else { v = 0.0; d = 0.0; }
+        break;
+    case AF_GAUSSIAN:
+    {
+        double const g = exp(-0.5 * x * x);
+        v = g; d = -x * g;
+        break;
+    }
+    case AF_COS: v = cos(x); d = -sin(x); break;
+    case AF_REVLOGISTIC:
+    {
+        double const s = 1.0 / (1.0 + exp(-x));
+        v = 1.0 - s; d = s * (s - 1.0);
+        break;
+    }
+    case AF_EXP:
+    {
+        double const e = exp(-x);
+        v = e; d = -e;
+        break;
+    }
+    case AF_HARMONIC: v = x * x; d = 2.0 * x; break;
+    default: v = 0.0; d = 0.0; break;
+    }
+}
+
+// SymFnc::scale (SymFnc.cpp:169-195).
+__device__ __forceinline__ double scale_sf(ScaleDesc const& s, double v)
+{
+    switch (s.type)
+    {
+    case ST_NONE: return v;
+    case ST_SCALE: return s.smin + s.sfac * (v - s.gmin);
+    case ST_CENTER: return v - s.gmean;
+    case ST_SCALECENTER:
+    case ST_SCALESIGMA: return s.smin + s.sfac * (v - s.gmean);
+    }
+    return 0.0;
+}
+
+// Butterfly sum over the 64 lanes of a wave; every lane gets the total.
+// The pairing pattern is fixed, so the result is deterministic.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+// Number of set bits of a 64-lane ballot below this lane.
+__device__ __forceinline__ int lanes_below(uint64_t mask)
+{
+    return __popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+}  // namespace hdnnp
+
+#endif

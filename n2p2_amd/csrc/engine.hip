@@ -1,0 +1,1063 @@
+// Engine context and the C ABI declared in include/hdnnp.h.
+//
+// One context per Mode object: device copies of the model tables, the
+// structure, neighbour list, per-atom results and a HIP stream.  Device
+// buffers grow on demand and are reused between calls (a structure of the
+// same size re-uses everything; nothing is allocated per step).
+#include "hdnnp.h"
+
+#include "kernels.h"
+#include "model.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace hdnnp;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, std::string const& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw Error(HDNNP_E_DEVICE, std::string("ERROR: HIP error in ") + what
+                    + ": " + hipGetErrorString(e) + "\n");
+}
+
+template <class F> int guarded(F&& f)
+{
+    try
+    {
+        g_err.clear();
+        f();
+        return HDNNP_OK;
+    }
+    catch (Error const& e)
+    {
+        g_err = e.what();
+        return e.code;
+    }
+    catch (std::exception const& e)
+    {
+        g_err = e.what();
+        std::string const w = e.what();
+        if (w.find("open file") != std::string::npos) return HDNNP_E_FILE;
+        if (w.find("outside this engine") != std::string::npos)
+            return HDNNP_E_UNSUPPORTED;
+        return HDNNP_E_SETTINGS;
+    }
+}
+
+// Grow-only device buffer.
+template <class T> struct DBuf
+{
+    T* p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n)
+    {
+        if (n <= cap) return;
+        if (p) hipCheck(hipFree(p), "hipFree");
+        p = nullptr;
+        size_t const want = std::max<size_t>(n, 1);
+        hipCheck(hipMalloc((void**)&p, want * sizeof(T)), "hipMalloc");
+        cap = want;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+enum KClass { K_NEIGHBOR = 0, K_SYMFUNC, K_NETWORK, K_FORCE, K_GATHER,
+              K_NUM };
+const char* const kClassNames[K_NUM] = {"neighbor", "symfunc", "network",
+                                        "force", "gather"};
+
+}  // namespace
+
+struct hdnnp_model
+{
+    Model m;
+};
+
+struct hdnnp_ctx
+{
+    const hdnnp_model* model = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Model::Flat flat;
+    ModelTables T{};
+    DBuf<ElemDesc> dElem;
+    DBuf<CutCfg> dCfg;
+    DBuf<GroupDesc> dGroup;
+    DBuf<MemberDesc> dMember;
+    DBuf<ScaleDesc> dScale;
+    DBuf<int> dPairGroups, dPairCount;
+    DBuf<double> dWeights;
+    int maxMembers = 0, maxCfg = 1, ld = 1;
+
+    // structure
+    int64_t n = 0;
+    bool periodic = false;
+    double box[3][3] = {{0}};
+    double invbox[3][3] = {{0}};
+    std::vector<int32_t> hElem;
+    std::vector<double> hR;          // host copy of positions (if set from host)
+    bool hostPosValid = false;
+    std::vector<std::vector<int>> hAtomsOfElem;
+    std::vector<DBuf<int>> dAtomsOfElem;
+    DBuf<double> dR;
+    DBuf<int> dElemAtom;
+
+    // neighbour list
+    NLParams nl{};
+    int pbc[3] = {0, 0, 0};
+    int ncell = 0;
+    int64_t M = 0;
+    int maxNeigh = 0;
+    bool haveNL = false;
+    DBuf<int> dCellOfAtom, dRank, dCellCount, dCellStart, dCellAtoms, dCount,
+        dOff, dStats, dBlockSums, dTotal;
+    DBuf<uint64_t> dKeys;
+    DBuf<int> dNbrJ, dPartner;
+    DBuf<double4> dGeo;
+
+    // results
+    DBuf<double> dG, dE, dEdG, dScratch, dCslot, dCself, dF, dEnergy, dStatOut;
+    DBuf<int> dEw;
+    bool haveSF = false, haveNN = false, haveNNDeriv = false;
+    double energy = 0.0;
+    int precision = HDNNP_PRECISION_FP64;
+
+    // profiling
+    bool prof = false;
+    struct Pending { int cls; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> eventPool;
+    double profMs[K_NUM] = {0};
+    int64_t profCalls[K_NUM] = {0};
+
+    hipEvent_t getEvent()
+    {
+        if (!eventPool.empty())
+        {
+            hipEvent_t e = eventPool.back();
+            eventPool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipCheck(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    void begin(int cls, hipEvent_t& a)
+    {
+        (void)cls;
+        if (!prof) return;
+        a = getEvent();
+        hipCheck(hipEventRecord(a, stream), "hipEventRecord");
+    }
+    void end(int cls, hipEvent_t a)
+    {
+        if (!prof) return;
+        hipEvent_t b = getEvent();
+        hipCheck(hipEventRecord(b, stream), "hipEventRecord");
+        pending.push_back(Pending{cls, a, b});
+    }
+    void collect()
+    {
+        for (auto& p : pending)
+        {
+            float ms = 0.0f;
+            hipCheck(hipEventSynchronize(p.b), "hipEventSynchronize");
+            hipCheck(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+            profMs[p.cls] += ms;
+            profCalls[p.cls] += 1;
+            eventPool.push_back(p.a);
+            eventPool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void sync()
+    {
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        collect();
+    }
+
+    void setDevice() { hipCheck(hipSetDevice(device), "hipSetDevice"); }
+
+    void uploadModel()
+    {
+        flat = model->m.flatten();
+        auto up = [&](auto& buf, auto const& v) {
+            buf.ensure(v.size());
+            if (!v.empty())
+                hipCheck(hipMemcpy(buf.p, v.data(),
+                                   v.size() * sizeof(v[0]),
+                                   hipMemcpyHostToDevice), "upload model");
+        };
+        up(dElem, flat.elem);
+        up(dCfg, flat.cfg);
+        up(dGroup, flat.group);
+        up(dMember, flat.member);
+        up(dScale, flat.scale);
+        up(dPairGroups, flat.pairGroups);
+        up(dPairCount, flat.pairCount);
+        up(dWeights, flat.weights);
+        T.ne = (int)flat.elem.size();
+        T.maxnsf = model->m.maxNumSF();
+        T.maxCutoff = model->m.maxCutoffRadius;
+        T.elem = dElem.p;
+        T.cfg = dCfg.p;
+        T.group = dGroup.p;
+        T.member = dMember.p;
+        T.scale = dScale.p;
+        T.pairGroups = dPairGroups.p;
+        T.pairCount = dPairCount.p;
+        T.weights = dWeights.p;
+        ld = std::max(1, T.maxnsf);
+        maxMembers = 0;
+        maxCfg = 1;
+        for (auto const& e : flat.elem)
+        {
+            int nm = 0;
+            for (int g = e.g0; g < e.g0 + e.ng; ++g) nm += flat.group[g].nmem;
+            maxMembers = std::max(maxMembers, nm);
+            maxCfg = std::max(maxCfg, e.ncfg);
+        }
+    }
+};
+
+namespace {
+
+void requireStructure(hdnnp_ctx* c)
+{
+    if (c->n <= 0)
+        throw Error(HDNNP_E_INVALID, "ERROR: No structure loaded.\n");
+}
+
+void requireNL(hdnnp_ctx* c)
+{
+    requireStructure(c);
+    if (!c->haveNL)
+        throw Error(HDNNP_E_INVALID, "ERROR: Neighbor list not calculated.\n");
+}
+
+// Structure::calculateInverseBox (Structure.cpp:519-545).
+void inverseBox(double const b[3][3], double ib[3][3])
+{
+    double invdet = b[0][0] * b[1][1] * b[2][2] + b[1][0] * b[2][1] * b[0][2]
+                  + b[2][0] * b[0][1] * b[1][2] - b[2][0] * b[1][1] * b[0][2]
+                  - b[0][0] * b[2][1] * b[1][2] - b[1][0] * b[0][1] * b[2][2];
+    invdet = 1.0 / invdet;
+    ib[0][0] = b[1][1] * b[2][2] - b[2][1] * b[1][2];
+    ib[0][1] = b[2][1] * b[0][2] - b[0][1] * b[2][2];
+    ib[0][2] = b[0][1] * b[1][2] - b[1][1] * b[0][2];
+    ib[1][0] = b[2][0] * b[1][2] - b[1][0] * b[2][2];
+    ib[1][1] = b[0][0] * b[2][2] - b[2][0] * b[0][2];
+    ib[1][2] = b[1][0] * b[0][2] - b[0][0] * b[1][2];
+    ib[2][0] = b[1][0] * b[2][1] - b[2][0] * b[1][1];
+    ib[2][1] = b[2][0] * b[0][1] - b[0][0] * b[2][1];
+    ib[2][2] = b[0][0] * b[1][1] - b[1][0] * b[0][1];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ib[i][j] *= invdet;
+}
+
+// Perpendicular widths of the cell (|a . (b x c)| / |b x c| etc.) and the
+// image counts of Structure::calculatePbcCopies (Structure.cpp:486-517).
+void cellWidths(double const b[3][3], double rc, double proj[3], int pbc[3])
+{
+    auto cross = [](double const* x, double const* y, double* o) {
+        o[0] = x[1] * y[2] - x[2] * y[1];
+        o[1] = x[2] * y[0] - x[0] * y[2];
+        o[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    auto normalize = [](double* v) {
+        double const n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] /= n; v[1] /= n; v[2] /= n;
+    };
+    auto dot = [](double const* x, double const* y) {
+        return x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+    };
+    double axb[3], axc[3], bxc[3];
+    cross(b[0], b[1], axb); normalize(axb);
+    cross(b[0], b[2], axc); normalize(axc);
+    cross(b[1], b[2], bxc); normalize(bxc);
+    proj[0] = std::fabs(dot(b[0], bxc));
+    proj[1] = std::fabs(dot(b[1], axc));
+    proj[2] = std::fabs(dot(b[2], axb));
+    for (int k = 0; k < 3; ++k)
+    {
+        pbc[k] = 0;
+        while (pbc[k] * proj[k] <= rc) pbc[k]++;
+    }
+}
+
+void buildNeighborList(hdnnp_ctx* c, double rc)
+{
+    requireStructure(c);
+    c->setDevice();
+    NLParams& p = c->nl;
+    p = NLParams{};
+    p.n = (int)c->n;
+    p.periodic = c->periodic ? 1 : 0;
+    p.rc2 = rc * rc;
+    double const margin = 1.0 + 1e-6;
+    if (c->periodic)
+    {
+        double proj[3];
+        cellWidths(c->box, rc, proj, c->pbc);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                p.box[i][j] = c->box[i][j];
+                p.invbox[i][j] = c->invbox[i][j];
+            }
+        for (int k = 0; k < 3; ++k)
+        {
+            int nc = (int)std::floor(proj[k] / (rc * margin));
+            nc = std::max(1, std::min(nc, 1024));
+            p.nc[k] = nc;
+            p.m[k] = std::max(1, (int)std::ceil(rc * nc / proj[k] * margin));
+            if (p.m[k] > 60)
+                throw Error(HDNNP_E_CAPACITY,
+                            "ERROR: Cell much smaller than the cutoff "
+                            "(more than 60 periodic images per direction).\n");
+        }
+    }
+    else
+    {
+        if (!c->hostPosValid)
+        {
+            c->hR.resize(3 * c->n);
+            hipCheck(hipMemcpy(c->hR.data(), c->dR.p, 3 * c->n * sizeof(double),
+                               hipMemcpyDeviceToHost), "positions to host");
+            c->hostPosValid = true;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t i = 0; i < c->n; ++i)
+            for (int k = 0; k < 3; ++k)
+            {
+                lo[k] = std::min(lo[k], c->hR[3 * i + k]);
+                hi[k] = std::max(hi[k], c->hR[3 * i + k]);
+            }
+        c->pbc[0] = c->pbc[1] = c->pbc[2] = 0;
+        for (int k = 0; k < 3; ++k)
+        {
+            double const w = std::max(hi[k] - lo[k], 0.0) + 1e-9 * (1 + std::fabs(hi[k]));
+            int nc = (int)std::floor(w / (rc * margin));
+            nc = std::max(1, std::min(nc, 1024));
+            p.nc[k] = nc;
+            p.origin[k] = lo[k];
+            p.invbox[k][k] = 1.0 / w;
+            p.m[k] = 1;
+        }
+    }
+    int64_t const ncell64 = (int64_t)p.nc[0] * p.nc[1] * p.nc[2];
+    if (ncell64 > (int64_t)1 << 27)
+        throw Error(HDNNP_E_CAPACITY, "ERROR: Too many cells.\n");
+    int const ncell = (int)ncell64;
+    c->ncell = ncell;
+    int const n = (int)c->n;
+    hipStream_t s = c->stream;
+    hipEvent_t ev0 = nullptr;
+    c->begin(K_NEIGHBOR, ev0);
+    c->dCellOfAtom.ensure(n);
+    c->dRank.ensure(n);
+    c->dCellCount.ensure(ncell);
+    c->dCellStart.ensure(ncell + 1);
+    c->dCellAtoms.ensure(n);
+    c->dCount.ensure(n);
+    c->dOff.ensure(n + 1);
+    c->dStats.ensure(2);
+    c->dBlockSums.ensure((std::max(n, ncell) + 1023) / 1024 + 1);
+    c->dTotal.ensure(1);
+    hipCheck(hipMemsetAsync(c->dCellCount.p, 0, ncell * sizeof(int), s), "memset");
+    hipCheck(hipMemsetAsync(c->dStats.p, 0, 2 * sizeof(int), s), "memset");
+    hipCheck(launch_nl_bin(p, c->dR.p, c->dCellOfAtom.p, c->dRank.p,
+                           c->dCellCount.p, s), "nl bin");
+    hipCheck(scan_exclusive(c->dCellCount.p, c->dCellStart.p, ncell,
+                            c->dBlockSums.p, c->dTotal.p, s), "cell scan");
+    hipCheck(launch_nl_scatter(p, c->dCellOfAtom.p, c->dRank.p, c->dCellStart.p,
+                               c->dCellAtoms.p, s), "nl scatter");
+    hipCheck(launch_nl_count(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
+                             c->dCellStart.p, c->dCellCount.p, c->dCellAtoms.p,
+                             c->dCount.p, c->dStats.p, s), "nl count");
+    hipCheck(scan_exclusive(c->dCount.p, c->dOff.p, n, c->dBlockSums.p,
+                            c->dTotal.p, s), "count scan");
+    int hstat[3];
+    hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, s), "stats");
+    hipCheck(hipMemcpyAsync(hstat + 2, c->dTotal.p, sizeof(int),
+                            hipMemcpyDeviceToHost, s), "total");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    int const maxc = hstat[0];
+    int64_t const M = hstat[2];
+    if (M < 0 || M > (int64_t)2000000000)
+        throw Error(HDNNP_E_CAPACITY, "ERROR: Neighbour list too large.\n");
+    int keyCap = 64;
+    while (keyCap < maxc) keyCap <<= 1;
+    if (keyCap > 8192)
+        throw Error(HDNNP_E_CAPACITY,
+                    "ERROR: More than 8192 neighbours for one atom.\n");
+    c->dKeys.ensure(M);
+    c->dNbrJ.ensure(M);
+    c->dGeo.ensure(M);
+    c->dPartner.ensure(M);
+    hipCheck(launch_nl_fill(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
+                            c->dCellStart.p, c->dCellCount.p, c->dCellAtoms.p,
+                            c->dStats.p, c->dOff.p, keyCap, c->dKeys.p,
+                            c->dNbrJ.p, c->dGeo.p, s), "nl fill");
+    hipCheck(launch_nl_partner(n, c->dOff.p, c->dKeys.p, c->dPartner.p, s),
+             "nl partner");
+    c->end(K_NEIGHBOR, ev0);
+    c->M = M;
+    c->maxNeigh = maxc;
+    c->haveNL = true;
+    c->haveSF = c->haveNN = c->haveNNDeriv = false;
+}
+
+void computeSF(hdnnp_ctx* c, bool raw)
+{
+    requireNL(c);
+    c->setDevice();
+    int const cap = std::max(c->maxNeigh, 1);
+    if (sf_forward_lds_bytes(cap) > 65536)
+        throw Error(HDNNP_E_CAPACITY,
+                    "ERROR: Too many neighbours per atom for the symmetry-"
+                    "function kernel's LDS staging.\n");
+    c->dG.ensure((size_t)c->n * c->ld);
+    SFArgs a{};
+    a.T = c->T;
+    a.n = (int)c->n;
+    a.off = c->dOff.p;
+    a.geo = c->dGeo.p;
+    a.nbrJ = c->dNbrJ.p;
+    a.elem = c->dElemAtom.p;
+    a.G = c->dG.p;
+    a.ldG = c->ld;
+    a.cap = cap;
+    a.raw = raw ? 1 : 0;
+    hipEvent_t ev = nullptr;
+    c->begin(K_SYMFUNC, ev);
+    hipCheck(launch_sf_forward(a, c->stream), "symmetry functions");
+    c->end(K_SYMFUNC, ev);
+    c->haveSF = !raw;
+    c->haveNN = c->haveNNDeriv = false;
+}
+
+void computeNN(hdnnp_ctx* c, bool deriv)
+{
+    if (!c->haveSF)
+        throw Error(HDNNP_E_INVALID,
+                    "ERROR: Symmetry functions not calculated.\n");
+    c->setDevice();
+    int const n = (int)c->n;
+    c->dE.ensure(n);
+    c->dEdG.ensure((size_t)n * c->ld);
+    size_t need = 0;
+    for (size_t e = 0; e < c->flat.elem.size(); ++e)
+    {
+        int plane = 0;
+        for (int l = 1; l < c->flat.elem[e].nlayers; ++l)
+            plane += c->flat.elem[e].width[l];
+        need = std::max(need, (size_t)plane * 3 * c->hAtomsOfElem[e].size());
+    }
+    c->dScratch.ensure(need);
+    hipEvent_t ev = nullptr;
+    c->begin(K_NETWORK, ev);
+    for (size_t e = 0; e < c->flat.elem.size(); ++e)
+    {
+        NNArgs a{};
+        a.T = c->T;
+        a.e = (int)e;
+        a.atoms = c->dAtomsOfElem[e].p;
+        a.na = (int)c->hAtomsOfElem[e].size();
+        a.G = c->dG.p;
+        a.ldG = c->ld;
+        a.E = c->dE.p;
+        a.dEdG = c->dEdG.p;
+        a.scratch = c->dScratch.p;
+        a.deriv = deriv ? 1 : 0;
+        hipCheck(launch_nn(a, c->stream), "networks");
+    }
+    c->end(K_NETWORK, ev);
+    c->haveNN = true;
+    c->haveNNDeriv = deriv;
+}
+
+double computeEnergy(hdnnp_ctx* c)
+{
+    if (!c->haveNN)
+        throw Error(HDNNP_E_INVALID, "ERROR: Networks not evaluated.\n");
+    c->setDevice();
+    c->dEnergy.ensure(1);
+    hipCheck(launch_energy_sum(c->dE.p, (int)c->n, c->dEnergy.p, c->stream),
+             "energy");
+    double e = 0.0;
+    hipCheck(hipMemcpyAsync(&e, c->dEnergy.p, sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream), "energy copy");
+    c->sync();
+    c->energy = e;
+    return e;
+}
+
+void computeForces(hdnnp_ctx* c, int mode)
+{
+    if (!c->haveNNDeriv)
+        throw Error(HDNNP_E_INVALID,
+                    "ERROR: dE/dG not available (call the networks with "
+                    "derivatives).\n");
+    if (mode != HDNNP_FORCES_REFERENCE && mode != HDNNP_FORCES_CONSISTENT)
+        throw Error(HDNNP_E_INVALID, "ERROR: Unknown force mode.\n");
+    c->setDevice();
+    int const cap = std::max(c->maxNeigh, 1);
+    size_t const shm = force_backward_lds_bytes(cap, c->maxMembers, c->maxCfg,
+                                                c->T.maxnsf);
+    if (shm > 65536)
+        throw Error(HDNNP_E_CAPACITY,
+                    "ERROR: Too many neighbours per atom for the force "
+                    "kernel's LDS staging.\n");
+    c->dCslot.ensure(3 * (size_t)std::max<int64_t>(c->M, 1));
+    c->dCself.ensure(3 * (size_t)c->n);
+    c->dF.ensure(3 * (size_t)c->n);
+    ForceArgs a{};
+    a.T = c->T;
+    a.n = (int)c->n;
+    a.off = c->dOff.p;
+    a.geo = c->dGeo.p;
+    a.nbrJ = c->dNbrJ.p;
+    a.elem = c->dElemAtom.p;
+    a.dEdG = c->dEdG.p;
+    a.ldG = c->ld;
+    a.cslot = c->dCslot.p;
+    a.cself = c->dCself.p;
+    a.cap = cap;
+    a.maxMembers = std::max(c->maxMembers, 1);
+    a.maxCfg = c->maxCfg;
+    hipEvent_t ev = nullptr, ev2 = nullptr;
+    c->begin(K_FORCE, ev);
+    hipCheck(launch_force_backward(a, c->stream), "force backward");
+    c->end(K_FORCE, ev);
+    c->begin(K_GATHER, ev2);
+    hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dNbrJ.p,
+                                 c->dPartner.p, c->dCslot.p, c->dCself.p, mode,
+                                 c->dF.p, c->stream), "force gather");
+    c->end(K_GATHER, ev2);
+    c->sync();
+}
+
+template <class T>
+void copyOut(hdnnp_ctx* c, T* dst, const T* src, size_t count)
+{
+    if (!dst || !count) return;
+    hipCheck(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost,
+                            c->stream), "copy to host");
+}
+
+}  // namespace
+
+// =========================================================== C ABI
+extern "C" {
+
+const char* hdnnp_last_error(void) { return g_err.c_str(); }
+
+int hdnnp_model_load(const char* settings_file, const char* scaling_file,
+                     const char* weights_fmt, hdnnp_model** out)
+{
+    return guarded([&]() {
+        if (!out || !settings_file || !weights_fmt)
+            throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        std::unique_ptr<hdnnp_model> m(new hdnnp_model);
+        m->m.load(settings_file, scaling_file ? scaling_file : "",
+                  weights_fmt);
+        *out = m.release();
+    });
+}
+
+void hdnnp_model_free(hdnnp_model* m) { delete m; }
+int hdnnp_model_num_elements(const hdnnp_model* m) { return m->m.numElements(); }
+int hdnnp_model_atomic_number(const hdnnp_model* m, int e)
+{
+    if (e < 0 || e >= m->m.numElements()) return -1;
+    return m->m.el[e].Z;
+}
+int hdnnp_model_num_sf(const hdnnp_model* m, int e)
+{
+    if (e < 0 || e >= m->m.numElements()) return -1;
+    return (int)m->m.el[e].sf.size();
+}
+double hdnnp_model_max_cutoff(const hdnnp_model* m)
+{
+    return m->m.maxCutoffRadius;
+}
+int hdnnp_model_element_index(const hdnnp_model* m, const char* symbol)
+{
+    return symbol ? m->m.elementIndex(symbol) : -1;
+}
+int hdnnp_model_normalization(const hdnnp_model* m, double* me, double* ce,
+                              double* cl)
+{
+    if (me) *me = m->m.meanEnergy;
+    if (ce) *ce = m->m.convEnergy;
+    if (cl) *cl = m->m.convLength;
+    return m->m.normalize ? 1 : 0;
+}
+double hdnnp_model_energy_offset(const hdnnp_model* m, int e)
+{
+    if (e < 0 || e >= m->m.numElements()) return 0.0;
+    return m->m.el[e].energyOffset;
+}
+int hdnnp_model_sf_table(const hdnnp_model* m, int e, double* out)
+{
+    return guarded([&]() {
+        if (e < 0 || e >= m->m.numElements())
+            throw Error(HDNNP_E_INVALID, "ERROR: Element out of range.\n");
+        auto const& sf = m->m.el[e].sf;
+        for (size_t i = 0; i < sf.size(); ++i)
+        {
+            double* o = out + 12 * i;
+            o[0] = sf[i].type; o[1] = sf[i].e1; o[2] = sf[i].type == 2 ? -1 : sf[i].e2;
+            o[3] = sf[i].eta; o[4] = sf[i].rs; o[5] = sf[i].lambda;
+            o[6] = sf[i].zeta; o[7] = sf[i].rc; o[8] = sf[i].cutoffType;
+            o[9] = sf[i].cutoffAlpha; o[10] = sf[i].scalingFactor;
+            o[11] = (double)sf[i].line;
+        }
+    });
+}
+int hdnnp_model_set_scaling(hdnnp_model* m, int e, int type, double smin,
+                            double smax, const double* stats)
+{
+    return guarded([&]() {
+        if (e < 0 || e >= m->m.numElements() || type < 0 || type > 4 || !stats)
+            throw Error(HDNNP_E_INVALID, "ERROR: Invalid scaling arguments.\n");
+        m->m.setScaling(e, type, smin, smax, stats);
+    });
+}
+
+int hdnnp_ctx_create(const hdnnp_model* m, int device, hdnnp_ctx** out)
+{
+    return guarded([&]() {
+        if (!m || !out) throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        int ndev = 0;
+        hipCheck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (device < 0 || device >= ndev)
+            throw Error(HDNNP_E_DEVICE, "ERROR: No such HIP device.\n");
+        std::unique_ptr<hdnnp_ctx> c(new hdnnp_ctx);
+        c->model = m;
+        c->device = device;
+        c->setDevice();
+        hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
+                 "hipStreamCreate");
+        c->uploadModel();
+        *out = c.release();
+    });
+}
+
+void hdnnp_ctx_destroy(hdnnp_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending)
+    {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->eventPool) (void)hipEventDestroy(e);
+    c->dElem.release(); c->dCfg.release(); c->dGroup.release();
+    c->dMember.release(); c->dScale.release(); c->dPairGroups.release();
+    c->dPairCount.release(); c->dWeights.release();
+    for (auto& b : c->dAtomsOfElem) b.release();
+    c->dR.release(); c->dElemAtom.release();
+    c->dCellOfAtom.release(); c->dRank.release(); c->dCellCount.release();
+    c->dCellStart.release(); c->dCellAtoms.release(); c->dCount.release();
+    c->dOff.release(); c->dStats.release(); c->dBlockSums.release();
+    c->dTotal.release(); c->dKeys.release(); c->dNbrJ.release();
+    c->dPartner.release(); c->dGeo.release();
+    c->dG.release(); c->dE.release(); c->dEdG.release(); c->dScratch.release();
+    c->dCslot.release(); c->dCself.release(); c->dF.release();
+    c->dEnergy.release(); c->dStatOut.release(); c->dEw.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision)
+{
+    return guarded([&]() {
+        if (precision == HDNNP_PRECISION_FP64) { c->precision = precision; return; }
+        if (precision == HDNNP_PRECISION_MIXED)
+            throw Error(HDNNP_E_UNSUPPORTED,
+                        "ERROR: Mixed precision is not available in this "
+                        "build.\n");
+        throw Error(HDNNP_E_INVALID, "ERROR: Unknown precision.\n");
+    });
+}
+
+int hdnnp_ctx_update_model(hdnnp_ctx* c)
+{
+    return guarded([&]() {
+        c->setDevice();
+        hipCheck(hipStreamSynchronize(c->stream), "sync");
+        c->uploadModel();
+        c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
+                        const int32_t* elem, const double* box)
+{
+    return guarded([&]() {
+        if (n <= 0 || !r || !elem)
+            throw Error(HDNNP_E_INVALID, "ERROR: Empty structure.\n");
+        if (n > 200000000)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: Too many atoms.\n");
+        c->setDevice();
+        int const ne = c->T.ne;
+        c->n = n;
+        c->hElem.assign(elem, elem + n);
+        c->hAtomsOfElem.assign(ne, {});
+        for (int64_t i = 0; i < n; ++i)
+        {
+            if (elem[i] < 0 || elem[i] >= ne)
+                throw Error(HDNNP_E_INVALID,
+                            "ERROR: Element index out of range.\n");
+            c->hAtomsOfElem[elem[i]].push_back((int)i);
+        }
+        c->periodic = box != nullptr;
+        if (box)
+        {
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) c->box[i][j] = box[3 * i + j];
+            inverseBox(c->box, c->invbox);
+        }
+        c->dR.ensure(3 * n);
+        c->dElemAtom.ensure(n);
+        hipCheck(hipMemcpy(c->dR.p, r, 3 * n * sizeof(double),
+                           hipMemcpyHostToDevice), "positions");
+        hipCheck(hipMemcpy(c->dElemAtom.p, elem, n * sizeof(int32_t),
+                           hipMemcpyHostToDevice), "elements");
+        c->hR.assign(r, r + 3 * n);
+        c->hostPosValid = true;
+        c->dAtomsOfElem.resize(ne);
+        for (int e = 0; e < ne; ++e)
+        {
+            c->dAtomsOfElem[e].ensure(c->hAtomsOfElem[e].size());
+            if (!c->hAtomsOfElem[e].empty())
+                hipCheck(hipMemcpy(c->dAtomsOfElem[e].p,
+                                   c->hAtomsOfElem[e].data(),
+                                   c->hAtomsOfElem[e].size() * sizeof(int),
+                                   hipMemcpyHostToDevice), "element lists");
+        }
+        c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_set_positions(hdnnp_ctx* c, const double* r)
+{
+    return guarded([&]() {
+        requireStructure(c);
+        c->setDevice();
+        hipCheck(hipMemcpyAsync(c->dR.p, r, 3 * c->n * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream), "positions");
+        c->hR.assign(r, r + 3 * c->n);
+        c->hostPosValid = true;
+        c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_set_positions_device(hdnnp_ctx* c, const double* r_device)
+{
+    return guarded([&]() {
+        requireStructure(c);
+        c->setDevice();
+        hipCheck(hipMemcpyAsync(c->dR.p, r_device, 3 * c->n * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream),
+                 "positions");
+        c->hostPosValid = false;
+        c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_neighbor_list(hdnnp_ctx* c, double cutoff)
+{
+    return guarded([&]() {
+        if (!(cutoff > 0.0))
+            throw Error(HDNNP_E_INVALID, "ERROR: Cutoff must be positive.\n");
+        buildNeighborList(c, cutoff);
+    });
+}
+
+int hdnnp_symmetry_functions(hdnnp_ctx* c, int derivatives)
+{
+    (void)derivatives;
+    return guarded([&]() { computeSF(c, false); });
+}
+
+int hdnnp_atomic_networks(hdnnp_ctx* c, int derivatives)
+{
+    return guarded([&]() { computeNN(c, derivatives != 0); });
+}
+
+int hdnnp_energy(hdnnp_ctx* c, double* energy)
+{
+    return guarded([&]() {
+        double const e = computeEnergy(c);
+        if (energy) *energy = e;
+    });
+}
+
+int hdnnp_forces(hdnnp_ctx* c, int mode)
+{
+    return guarded([&]() { computeForces(c, mode); });
+}
+
+int hdnnp_evaluate(hdnnp_ctx* c, int force_mode, double* energy)
+{
+    return guarded([&]() {
+        buildNeighborList(c, c->T.maxCutoff);
+        computeSF(c, false);
+        computeNN(c, force_mode >= 0);
+        double const e = computeEnergy(c);
+        if (force_mode >= 0) computeForces(c, force_mode);
+        if (energy) *energy = e;
+    });
+}
+
+int64_t hdnnp_num_atoms(const hdnnp_ctx* c) { return c->n; }
+int64_t hdnnp_num_neighbor_entries(const hdnnp_ctx* c)
+{
+    return c->haveNL ? c->M : 0;
+}
+
+int hdnnp_get_pbc(hdnnp_ctx* c, int32_t* pbc)
+{
+    return guarded([&]() {
+        requireNL(c);
+        for (int k = 0; k < 3; ++k) pbc[k] = c->pbc[k];
+    });
+}
+
+int hdnnp_get_neighbors(hdnnp_ctx* c, int64_t* offsets, int32_t* index,
+                        int32_t* image, double* d, double* dr)
+{
+    return guarded([&]() {
+        requireNL(c);
+        c->setDevice();
+        size_t const M = (size_t)c->M;
+        int const n = (int)c->n;
+        std::vector<int> off(n + 1);
+        copyOut(c, off.data(), c->dOff.p, n + 1);
+        copyOut(c, index, c->dNbrJ.p, M);
+        std::vector<uint64_t> keys;
+        if (image)
+        {
+            keys.resize(M);
+            copyOut(c, keys.data(), c->dKeys.p, M);
+        }
+        std::vector<double4> geo;
+        if (d || dr)
+        {
+            geo.resize(M);
+            copyOut(c, geo.data(), c->dGeo.p, M);
+        }
+        c->sync();
+        if (offsets)
+            for (int i = 0; i <= n; ++i) offsets[i] = off[i];
+        if (image)
+            for (size_t q = 0; q < M; ++q)
+            {
+                uint64_t const k = keys[q];
+                image[3 * q] = (int)((k >> 14) & 127) - 64;
+                image[3 * q + 1] = (int)((k >> 7) & 127) - 64;
+                image[3 * q + 2] = (int)(k & 127) - 64;
+            }
+        for (size_t q = 0; q < geo.size(); ++q)
+        {
+            if (d) d[q] = geo[q].w;
+            if (dr)
+            {
+                dr[3 * q] = geo[q].x;
+                dr[3 * q + 1] = geo[q].y;
+                dr[3 * q + 2] = geo[q].z;
+            }
+        }
+    });
+}
+
+int hdnnp_get_G(hdnnp_ctx* c, double* G, int ld)
+{
+    return guarded([&]() {
+        if (!c->haveSF)
+            throw Error(HDNNP_E_INVALID, "ERROR: No symmetry functions.\n");
+        if (ld < c->ld) throw Error(HDNNP_E_INVALID, "ERROR: ld too small.\n");
+        c->setDevice();
+        std::vector<double> h((size_t)c->n * c->ld);
+        copyOut(c, h.data(), c->dG.p, h.size());
+        c->sync();
+        for (int64_t i = 0; i < c->n; ++i)
+        {
+            int const nsf = c->flat.elem[c->hElem[i]].nsf;
+            for (int k = 0; k < ld; ++k)
+                G[(size_t)i * ld + k] = k < nsf ? h[(size_t)i * c->ld + k] : 0.0;
+        }
+    });
+}
+
+int hdnnp_get_dEdG(hdnnp_ctx* c, double* dEdG, int ld)
+{
+    return guarded([&]() {
+        if (!c->haveNNDeriv)
+            throw Error(HDNNP_E_INVALID, "ERROR: No dE/dG.\n");
+        if (ld < c->ld) throw Error(HDNNP_E_INVALID, "ERROR: ld too small.\n");
+        c->setDevice();
+        std::vector<double> h((size_t)c->n * c->ld);
+        copyOut(c, h.data(), c->dEdG.p, h.size());
+        c->sync();
+        for (int64_t i = 0; i < c->n; ++i)
+        {
+            int const nsf = c->flat.elem[c->hElem[i]].nsf;
+            for (int k = 0; k < ld; ++k)
+                dEdG[(size_t)i * ld + k] = k < nsf ? h[(size_t)i * c->ld + k] : 0.0;
+        }
+    });
+}
+
+int hdnnp_get_atomic_energies(hdnnp_ctx* c, double* e)
+{
+    return guarded([&]() {
+        if (!c->haveNN) throw Error(HDNNP_E_INVALID, "ERROR: No energies.\n");
+        c->setDevice();
+        copyOut(c, e, c->dE.p, (size_t)c->n);
+        c->sync();
+    });
+}
+
+int hdnnp_get_forces(hdnnp_ctx* c, double* f)
+{
+    return guarded([&]() {
+        if (!c->dF.p) throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
+        c->setDevice();
+        copyOut(c, f, c->dF.p, 3 * (size_t)c->n);
+        c->sync();
+    });
+}
+
+const double* hdnnp_forces_device(hdnnp_ctx* c) { return c->dF.p; }
+
+int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom)
+{
+    int64_t total = -1;
+    int rc = guarded([&]() {
+        if (!c->haveSF)
+            throw Error(HDNNP_E_INVALID, "ERROR: No symmetry functions.\n");
+        c->setDevice();
+        c->dEw.ensure(c->n);
+        hipCheck(launch_ew(c->T, c->dG.p, c->ld, c->dElemAtom.p, (int)c->n,
+                           c->dEw.p, c->stream), "extrapolation warnings");
+        std::vector<int32_t> h(c->n);
+        copyOut(c, h.data(), c->dEw.p, (size_t)c->n);
+        c->sync();
+        total = 0;
+        for (int64_t i = 0; i < c->n; ++i)
+        {
+            total += h[i];
+            if (per_atom) per_atom[i] = h[i];
+        }
+    });
+    return rc == HDNNP_OK ? total : rc;
+}
+
+int64_t hdnnp_sf_statistics(hdnnp_ctx* c, int e, double* out)
+{
+    int64_t na = -1;
+    int rc = guarded([&]() {
+        requireNL(c);
+        if (e < 0 || e >= c->T.ne)
+            throw Error(HDNNP_E_INVALID, "ERROR: Element out of range.\n");
+        bool const hadSF = c->haveSF;
+        computeSF(c, true);
+        int const nsf = c->flat.elem[e].nsf;
+        na = (int64_t)c->hAtomsOfElem[e].size();
+        c->dStatOut.ensure(4 * (size_t)nsf);
+        hipCheck(launch_sf_stats(c->dG.p, c->ld, c->dAtomsOfElem[e].p, (int)na,
+                                 nsf, c->dStatOut.p, c->stream), "sf stats");
+        std::vector<double> h(4 * nsf);
+        copyOut(c, h.data(), c->dStatOut.p, h.size());
+        c->sync();
+        // Dataset.cpp:1074-1085: mean = sum / n,
+        // sigma = sqrt((sum2 - sum^2 / n) / (n - 1)).
+        double const nd = (double)na;
+        for (int k = 0; k < nsf; ++k)
+        {
+            double const sum = h[4 * k + 2], sum2 = h[4 * k + 3];
+            out[4 * k] = h[4 * k];
+            out[4 * k + 1] = h[4 * k + 1];
+            out[4 * k + 2] = na ? sum / nd : 0.0;
+            out[4 * k + 3] = na > 1 ? std::sqrt((sum2 - sum * sum / nd)
+                                                / (nd - 1.0)) : 0.0;
+        }
+        // G now holds unscaled values.
+        c->haveSF = false;
+        (void)hadSF;
+    });
+    return rc == HDNNP_OK ? na : rc;
+}
+
+int hdnnp_profile_enable(hdnnp_ctx* c, int enable)
+{
+    c->prof = enable != 0;
+    return HDNNP_OK;
+}
+
+double hdnnp_profile_avg_ms(hdnnp_ctx* c, const char* name, int64_t* calls)
+{
+    try { c->collect(); } catch (...) {}
+    for (int k = 0; k < K_NUM; ++k)
+        if (name && std::strcmp(name, kClassNames[k]) == 0)
+        {
+            if (calls) *calls = c->profCalls[k];
+            return c->profCalls[k] ? c->profMs[k] / c->profCalls[k] : 0.0;
+        }
+    return -1.0;
+}
+
+int hdnnp_profile_reset(hdnnp_ctx* c)
+{
+    try { c->collect(); } catch (...) {}
+    for (int k = 0; k < K_NUM; ++k)
+    {
+        c->profMs[k] = 0.0;
+        c->profCalls[k] = 0;
+    }
+    return HDNNP_OK;
+}
+
+int hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts)
+{
+    return guarded([&]() {
+        for (int k = 0; k < 6; ++k) counts[k] = 0;
+        counts[0] = c->M;
+        throw Error(HDNNP_E_UNSUPPORTED,
+                    "ERROR: Work counters are not available yet.\n");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// Host-callable launchers of the HIP kernels (one translation unit each).
+#ifndef HDNNP_KERNELS_H
+#define HDNNP_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tables.h"
+
+namespace hdnnp {
+
+// Cell-list geometry for the neighbour build.
+struct NLParams
+{
+    int    n;
+    int    periodic;
+    double box[3][3];      // lattice vectors as rows
+    double invbox[3][3];   // Structure::invbox (Structure.cpp:519-545)
+    double origin[3];      // non-periodic: lower corner of the bounding box
+    int    nc[3];          // cells per direction
+    int    m[3];           // cells searched on each side
+    double rc2;
+};
+
+hipError_t scan_exclusive(const int* in, int* out, int n, int* blockSums,
+                          int* total, hipStream_t s);
+hipError_t launch_nl_bin(NLParams const& p, const double* r, int* cellOfAtom,
+                         int* rank, int* cellCount, hipStream_t s);
+hipError_t launch_nl_scatter(NLParams const& p, const int* cellOfAtom,
+                             const int* rank, const int* cellStart,
+                             int* cellAtoms, hipStream_t s);
+hipError_t launch_nl_count(NLParams const& p, const double* r, const int* elem,
+                           const int* cellOfAtom, const int* cellStart,
+                           const int* cellCount, const int* cellAtoms,
+                           int* count, int* stats, hipStream_t s);
+hipError_t launch_nl_fill(NLParams const& p, const double* r, const int* elem,
+                          const int* cellOfAtom, const int* cellStart,
+                          const int* cellCount, const int* cellAtoms,
+                          int* stats, const int* off, int keyCap,
+                          uint64_t* keys, int* nbrJ, double4* geo,
+                          hipStream_t s);
+hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
+                             int* partner, hipStream_t s);
+
+// Symmetry functions (FP64).  raw != 0 skips scaling (statistics).
+struct SFArgs
+{
+    ModelTables T;
+    int n;                       // atoms
+    const int* off;              // CSR offsets (n+1)
+    const double4* geo;          // dx, dy, dz, d per entry
+    const int* nbrJ;             // neighbour atom per entry
+    const int* elem;             // element per atom
+    double* G;                   // n x ldG
+    int ldG;
+    int cap;                     // LDS neighbour capacity per wave
+    int raw;
+};
+hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
+size_t sf_forward_lds_bytes(int cap);
+
+struct ForceArgs
+{
+    ModelTables T;
+    int n;
+    const int* off;
+    const double4* geo;
+    const int* nbrJ;
+    const int* elem;
+    const double* dEdG;          // n x ldG
+    int ldG;
+    double* cslot;               // 3 x M (x, y, z per entry)
+    double* cself;               // 3 x n
+    int cap;
+    int maxMembers;              // members of the largest element
+    int maxCfg;
+};
+hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
+size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf);
+
+hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
+                               const int* partner, const double* cslot,
+                               const double* cself, int mode, double* F,
+                               hipStream_t s);
+
+// Atomic networks (FP64) for the atoms of one element.
+struct NNArgs
+{
+    ModelTables T;
+    int e;                       // element
+    const int* atoms;            // atom indices of element e
+    int na;
+    const double* G;
+    int ldG;
+    double* E;                   // per atom energy (indexed by atom)
+    double* dEdG;                // n x ldG (indexed by atom)
+    double* scratch;             // sum(width[l]) * 3 * na doubles
+    int deriv;
+};
+hipError_t launch_nn(NNArgs const& a, hipStream_t s);
+
+hipError_t launch_energy_sum(const double* E, int n, double* out, hipStream_t s);
+
+// Unscaled-SF statistics for one element (min, max, sum, sum of squares)
+// over atoms of that element: out 4 x nsf.
+hipError_t launch_sf_stats(const double* G, int ldG, const int* atoms, int na,
+                           int nsf, double* out, hipStream_t s);
+
+// Extrapolation-warning count per atom (Element.cpp:462-541 test).
+hipError_t launch_ew(ModelTables T, const double* G, int ldG, const int* elem,
+                     int n, int* perAtom, hipStream_t s);
+
+}  // namespace hdnnp
+
+#endif

@@ -1,0 +1,409 @@
+// Neighbour-list kernels: cell binning, per-atom candidate counting, sorted
+// fill and reverse-slot (partner) lookup.
+//
+// Output matches Structure::calculateNeighborList(rc, false)
+// (Structure.cpp:287-399) bit for bit: for each atom i every (j, image bc)
+// with |dr|^2 <= rc^2, where dr = r_i - r_j + bc0*a + bc1*b + bc2*c is
+// evaluated left to right and |dr|^2 with the contraction the reference
+// build uses for Vec3D::norm2 (fma(z, z, fma(x, x, y*y))), ordered by j,
+// then bc0, bc1, bc2 ascending, d = sqrt(|dr|^2).
+//
+// Candidates come from a cell grid in fractional coordinates whose cells
+// are at least rc wide, searched over +-m cells with the periodic shift of
+// each wrapped cell; hits are sorted per atom in LDS (bitonic) by a key
+// (j << 21 | biased bc), which is exactly the reference loop order.
+#include "device_common.h"
+#include "kernels.h"
+
+namespace hdnnp {
+
+namespace {
+
+// Left-to-right dr of Structure.cpp:319-322 with integer image vector b.
+__device__ __forceinline__ void image_dr(NLParams const& p, double const* ri,
+                                         double const* rj, int b0, int b1,
+                                         int b2, double* dr)
+{
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c)
+    {
+        double t = ri[c] - rj[c];
+        t = t + (double)b0 * p.box[0][c];
+        t = t + (double)b1 * p.box[1][c];
+        t = t + (double)b2 * p.box[2][c];
+        dr[c] = t;
+    }
+}
+
+// Vec3D::norm2 as contracted by GCC -O3 -march=native.
+__device__ __forceinline__ double norm2_pinned(double const* r)
+{
+    double t = r[1] * r[1];
+    t = fma(r[0], r[0], t);
+    return fma(r[2], r[2], t);
+}
+
+__device__ __forceinline__ void cell_coords(NLParams const& p,
+                                            double const* r, int* ci)
+{
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c)
+    {
+        double f;
+        if (p.periodic)
+        {
+            f = r[0] * p.invbox[0][c];
+            f = f + r[1] * p.invbox[1][c];
+            f = f + r[2] * p.invbox[2][c];
+        }
+        else
+            f = (r[c] - p.origin[c]) * p.invbox[c][c];
+        int k = (int)floor(f * (double)p.nc[c]);
+        ci[c] = k < 0 ? 0 : (k >= p.nc[c] ? p.nc[c] - 1 : k);
+    }
+}
+
+constexpr int kBias = 64;   // image components biased into 7 bits
+
+__device__ __forceinline__ uint64_t make_key(int j, int b0, int b1, int b2)
+{
+    return ((uint64_t)(uint32_t)j << 21)
+         | ((uint64_t)(b0 + kBias) << 14)
+         | ((uint64_t)(b1 + kBias) << 7)
+         | (uint64_t)(b2 + kBias);
+}
+
+__device__ __forceinline__ void split_key(uint64_t k, int& j, int& b0, int& b1,
+                                          int& b2)
+{
+    j = (int)(k >> 21);
+    b0 = (int)((k >> 14) & 127) - kBias;
+    b1 = (int)((k >> 7) & 127) - kBias;
+    b2 = (int)(k & 127) - kBias;
+}
+
+__global__ void k_bin(NLParams p, const double* __restrict__ r,
+                      int* __restrict__ cellOfAtom, int* __restrict__ rank,
+                      int* __restrict__ cellCount)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    int ci[3];
+    cell_coords(p, r + 3 * (size_t)i, ci);
+    int id = (ci[0] * p.nc[1] + ci[1]) * p.nc[2] + ci[2];
+    cellOfAtom[i] = id;
+    rank[i] = atomicAdd(&cellCount[id], 1);
+}
+
+__global__ void k_scatter(NLParams p, const int* __restrict__ cellOfAtom,
+                          const int* __restrict__ rank,
+                          const int* __restrict__ cellStart,
+                          int* __restrict__ cellAtoms)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    cellAtoms[cellStart[cellOfAtom[i]] + rank[i]] = i;
+}
+
+// One wave per atom.  Candidate cells are processed 64 at a time: lane c
+// owns candidate cell c (its start, count and image shift); a wave prefix
+// sum over the counts flattens the candidates so every lane tests one atom
+// per step.  MODE 0 counts hits, MODE 1 collects keys into LDS, sorts them
+// and writes the atom's slice of the list.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_nl(
+    NLParams p, const double* __restrict__ r, const int* __restrict__ elem,
+    const int* __restrict__ cellOfAtom, const int* __restrict__ cellStart,
+    const int* __restrict__ cellCount, const int* __restrict__ cellAtoms,
+    int* __restrict__ count, int* __restrict__ stats,
+    const int* __restrict__ off, int keyCap, uint64_t* __restrict__ keysOut,
+    int* __restrict__ nbrJ, double4* __restrict__ geo)
+{
+    extern __shared__ uint64_t keys[];
+    __shared__ int cStart[64], cCnt[64], cPre[64], cShift[64][3];
+    int const i = blockIdx.x;
+    int const lane = threadIdx.x;
+    if (i >= p.n) return;
+    double ri[3] = {r[3 * (size_t)i], r[3 * (size_t)i + 1], r[3 * (size_t)i + 2]};
+    int const id = cellOfAtom[i];
+    int const ci[3] = {id / (p.nc[1] * p.nc[2]), (id / p.nc[2]) % p.nc[1],
+                       id % p.nc[2]};
+    int const w0 = 2 * p.m[0] + 1, w1 = 2 * p.m[1] + 1, w2 = 2 * p.m[2] + 1;
+    int const ncand = w0 * w1 * w2;
+    int hits = 0;       // uniform running count (MODE 1: keys stored)
+    for (int c0 = 0; c0 < ncand; c0 += 64)
+    {
+        int const c = c0 + lane;
+        int cnt = 0;
+        if (c < ncand)
+        {
+            int d[3] = {c / (w1 * w2) - p.m[0], (c / w2) % w1 - p.m[1],
+                        c % w2 - p.m[2]};
+            int cw[3], sh[3];
+            bool ok = true;
+            for (int k = 0; k < 3; ++k)
+            {
+                int a = ci[k] + d[k];
+                int w = ((a % p.nc[k]) + p.nc[k]) % p.nc[k];
+                sh[k] = (a - w) / p.nc[k];
+                cw[k] = w;
+                if (!p.periodic && sh[k] != 0) ok = false;
+            }
+            if (ok)
+            {
+                int cid = (cw[0] * p.nc[1] + cw[1]) * p.nc[2] + cw[2];
+                cStart[lane] = cellStart[cid];
+                cnt = cellCount[cid];
+                // The image of j at r_j + sh*box has bc = -sh.
+                cShift[lane][0] = -sh[0];
+                cShift[lane][1] = -sh[1];
+                cShift[lane][2] = -sh[2];
+            }
+        }
+        cCnt[lane] = cnt;
+        // inclusive prefix over lanes
+        int pre = cnt;
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            int v = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += v;
+        }
+        cPre[lane] = pre - cnt;   // exclusive
+        int const total = __shfl(pre, 63, 64);
+        __syncthreads();
+        for (int t0 = 0; t0 < total; t0 += 64)
+        {
+            int const t = t0 + lane;
+            bool hit = false;
+            int j = 0, b0 = 0, b1 = 0, b2 = 0;
+            if (t < total)
+            {
+                // last cell whose exclusive prefix <= t (and count > 0)
+                int lo = 0, hi = 63;
+                while (lo < hi)
+                {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (cPre[mid] <= t) lo = mid; else hi = mid - 1;
+                }
+                while (lo > 0 && (cCnt[lo] == 0 || cPre[lo] + cCnt[lo] <= t))
+                    --lo;
+                j = cellAtoms[cStart[lo] + (t - cPre[lo])];
+                b0 = cShift[lo][0];
+                b1 = cShift[lo][1];
+                b2 = cShift[lo][2];
+                if (!(j == i && b0 == 0 && b1 == 0 && b2 == 0))
+                {
+                    double rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                                    r[3 * (size_t)j + 2]};
+                    double dr[3];
+                    image_dr(p, ri, rj, b0, b1, b2, dr);
+                    hit = norm2_pinned(dr) <= p.rc2;
+                }
+            }
+            uint64_t const mask = __ballot(hit);
+            if (MODE == 1 && hit)
+            {
+                int const pos = hits + lanes_below(mask);
+                if (pos < keyCap) keys[pos] = make_key(j, b0, b1, b2);
+            }
+            hits += __popcll(mask);
+        }
+        __syncthreads();
+    }
+    if (MODE == 0)
+    {
+        if (lane == 0)
+        {
+            count[i] = hits;
+            atomicMax(&stats[0], hits);
+        }
+        return;
+    }
+    // MODE 1: bitonic sort of keys[0, hits) padded to a power of two.
+    if (hits > keyCap)
+    {
+        if (lane == 0) atomicMax(&stats[1], hits);
+        return;
+    }
+    int P = 1;
+    while (P < hits) P <<= 1;
+    for (int t = hits + lane; t < P; t += 64) keys[t] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+    {
+        for (int jj = k >> 1; jj > 0; jj >>= 1)
+        {
+            for (int t = lane; t < P; t += 64)
+            {
+                int const q = t ^ jj;
+                if (q > t)
+                {
+                    uint64_t a = keys[t], b = keys[q];
+                    bool const up = (t & k) == 0;
+                    if ((a > b) == up)
+                    {
+                        keys[t] = b;
+                        keys[q] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int const o = off[i];
+    for (int t = lane; t < hits; t += 64)
+    {
+        uint64_t const key = keys[t];
+        int j, b0, b1, b2;
+        split_key(key, j, b0, b1, b2);
+        double rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                        r[3 * (size_t)j + 2]};
+        double dr[3];
+        image_dr(p, ri, rj, b0, b1, b2, dr);
+        double const d = sqrt(norm2_pinned(dr));
+        keysOut[o + t] = key;
+        nbrJ[o + t] = j;
+        geo[o + t] = make_double4(dr[0], dr[1], dr[2], d);
+    }
+    (void)elem;
+}
+
+// Partner slot: for entry q of atom i pointing at (j, bc) the entry of j
+// pointing at (i, -bc); binary search in j's sorted keys.  One wave per atom.
+__global__ __launch_bounds__(64) void k_partner(
+    int n, const int* __restrict__ off, const uint64_t* __restrict__ keys,
+    int* __restrict__ partner)
+{
+    int const i = blockIdx.x;
+    if (i >= n) return;
+    int const s0 = off[i], s1 = off[i + 1];
+    for (int q = s0 + (int)threadIdx.x; q < s1; q += 64)
+    {
+        int j, b0, b1, b2;
+        split_key(keys[q], j, b0, b1, b2);
+        uint64_t const want = make_key(i, -b0, -b1, -b2);
+        int lo = off[j], hi = off[j + 1] - 1;
+        int found = -1;
+        while (lo <= hi)
+        {
+            int mid = (lo + hi) >> 1;
+            uint64_t const v = keys[mid];
+            if (v == want) { found = mid; break; }
+            if (v < want) lo = mid + 1; else hi = mid - 1;
+        }
+        partner[q] = found;
+    }
+}
+
+// Exclusive scan helpers (block-level, then block offsets).
+__global__ void k_scan_block(const int* __restrict__ in, int* __restrict__ out,
+                             int* __restrict__ blockSums, int n)
+{
+    __shared__ int s[1024];
+    int const t = threadIdx.x;
+    int const g = blockIdx.x * 1024 + t;
+    int v = g < n ? in[g] : 0;
+    s[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1)
+    {
+        int a = t >= o ? s[t - o] : 0;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    if (g < n) out[g] = s[t] - v;
+    if (t == 1023) blockSums[blockIdx.x] = s[t];
+}
+
+__global__ void k_scan_sums(int* __restrict__ blockSums, int nb,
+                            int* __restrict__ total)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int acc = 0;
+    for (int b = 0; b < nb; ++b)
+    {
+        int v = blockSums[b];
+        blockSums[b] = acc;
+        acc += v;
+    }
+    *total = acc;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ blockSums,
+                           int n, const int* __restrict__ total, int writeTotal)
+{
+    int const g = blockIdx.x * 1024 + threadIdx.x;
+    if (g < n) out[g] += blockSums[blockIdx.x];
+    if (writeTotal && g == 0) out[n] = *total;
+}
+
+}  // namespace
+
+hipError_t scan_exclusive(const int* in, int* out, int n, int* blockSums,
+                          int* total, hipStream_t s)
+{
+    int const nb = (n + 1023) / 1024;
+    if (nb > 0)
+    {
+        k_scan_block<<<nb, 1024, 0, s>>>(in, out, blockSums, n);
+        k_scan_sums<<<1, 64, 0, s>>>(blockSums, nb, total);
+        k_scan_add<<<nb, 1024, 0, s>>>(out, blockSums, n, total, 1);
+    }
+    else
+    {
+        k_scan_sums<<<1, 64, 0, s>>>(blockSums, 0, total);
+        k_scan_add<<<1, 1024, 0, s>>>(out, blockSums, 0, total, 1);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_bin(NLParams const& p, const double* r, int* cellOfAtom,
+                         int* rank, int* cellCount, hipStream_t s)
+{
+    k_bin<<<(p.n + 255) / 256, 256, 0, s>>>(p, r, cellOfAtom, rank, cellCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_scatter(NLParams const& p, const int* cellOfAtom,
+                             const int* rank, const int* cellStart,
+                             int* cellAtoms, hipStream_t s)
+{
+    k_scatter<<<(p.n + 255) / 256, 256, 0, s>>>(p, cellOfAtom, rank, cellStart,
+                                                cellAtoms);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_count(NLParams const& p, const double* r, const int* elem,
+                           const int* cellOfAtom, const int* cellStart,
+                           const int* cellCount, const int* cellAtoms,
+                           int* count, int* stats, hipStream_t s)
+{
+    k_nl<0><<<p.n, 64, 0, s>>>(p, r, elem, cellOfAtom, cellStart, cellCount,
+                               cellAtoms, count, stats, nullptr, 0, nullptr,
+                               nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_fill(NLParams const& p, const double* r, const int* elem,
+                          const int* cellOfAtom, const int* cellStart,
+                          const int* cellCount, const int* cellAtoms,
+                          int* stats, const int* off, int keyCap,
+                          uint64_t* keys, int* nbrJ, double4* geo,
+                          hipStream_t s)
+{
+    size_t const shm = (size_t)keyCap * sizeof(uint64_t);
+    k_nl<1><<<p.n, 64, shm, s>>>(p, r, elem, cellOfAtom, cellStart, cellCount,
+                                 cellAtoms, nullptr, stats, off, keyCap, keys,
+                                 nbrJ, geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
+                             int* partner, hipStream_t s)
+{
+    k_partner<<<n, 64, 0, s>>>(n, off, keys, partner);
+    return hipGetLastError();
+}
+
+}  // namespace hdnnp

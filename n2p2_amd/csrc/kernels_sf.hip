@@ -1,0 +1,428 @@
+// Symmetry-function forward kernel (FP64): G values of every atom.
+//
+// Restates SymGrpExpRad::calculate (SymGrpExpRad.cpp:144-239),
+// SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403) and
+// SymGrpExpAngw::calculate (SymGrpExpAngw.cpp:156-376) without derivatives
+// (the force kernel recomputes those).  One wavefront per atom:
+//   1. the atom's neighbours are staged in LDS, partitioned by element
+//      (stable ballot compaction), so every group reads exactly the
+//      neighbour blocks it needs;
+//   2. groups run one after another; a group's member parameters are
+//      wave-uniform (scalar loads) and its accumulators live in registers;
+//   3. angular groups enumerate the unordered pairs of their blocks, test
+//      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
+//      with a ballot prefix; whenever 64 are queued every lane evaluates
+//      one, so the expensive triplet work runs without divergence.
+//   4. member sums are reduced with a fixed butterfly (deterministic).
+#include "device_common.h"
+#include "kernels.h"
+
+namespace hdnnp {
+
+namespace {
+
+constexpr int kQueue = 128;
+
+struct Stage
+{
+    double* X;
+    double* Y;
+    double* Z;
+    double* D;
+    double* FC;
+};
+
+// Unordered-pair decoding.  Different elements: p -> (p / nb, p % nb).
+// Same element (n atoms): circle pairing, p -> {r, r + 1 + c mod n} for
+// p < n*h (h = (n-1)/2), and {r, r + n/2} for the n/2 remaining pairs when
+// n is even; every unordered pair appears exactly once.
+__device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
+                                             int& r)
+{
+    q = (int)((float)p * inv);
+    r = p - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+}
+
+template <int MAXM>
+__device__ __forceinline__ void triplet_forward(
+    Stage const& S, int ja, int kb, bool wide, CutCfg const& cc,
+    MemberDesc const* __restrict__ mem, int nmem, double (&acc)[MAXM])
+{
+    double const rij = S.D[ja], rik = S.D[kb];
+    double const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
+    double const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
+    double const fcij = S.FC[ja], fcik = S.FC[kb];
+    double const r2ij = rij * rij, r2ik = rik * rik;
+    double pfc, r2sum, rjk = 0.0;
+    if (!wide)
+    {
+        double const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+        double const rjk2 = dx * dx + dy * dy + dz * dz;
+        rjk = sqrt(rjk2);
+        double fcjk, dfcjk;
+        cutoff_fdf(cc, rjk, fcjk, dfcjk);
+        pfc = fcij * fcik * fcjk;
+        r2sum = r2ij + r2ik + rjk * rjk;
+    }
+    else
+    {
+        pfc = fcij * fcik;
+        r2sum = r2ij + r2ik;
+    }
+    double const rinvijik = 1.0 / rij / rik;
+    double const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
+    double vexp = 0.0;
+#pragma unroll
+    for (int m = 0; m < MAXM; ++m)
+    {
+        if (m < nmem)
+        {
+            MemberDesc const& md = mem[m];
+            if (md.calcexp)
+            {
+                if (md.rs > 0.0)
+                {
+                    double const a = rij - md.rs, b = rik - md.rs;
+                    double s = a * a + b * b;
+                    if (!wide)
+                    {
+                        double const c = rjk - md.rs;
+                        s += c * c;
+                    }
+                    vexp = exp(-md.eta * s);
+                }
+                else vexp = exp(-md.eta * r2sum);
+            }
+            double const plambda = 1.0 + md.lambda * costijk;
+            double fg = vexp;
+            if (plambda <= 0.0) fg = 0.0;
+            else if (md.useint) fg *= pow_int(plambda, md.zint - 1);
+            else fg *= pow(plambda, md.zeta - 1.0);
+            acc[m] += fg * plambda * pfc;
+        }
+    }
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
+{
+    extern __shared__ double sm[];
+    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    int const cap = a.cap;
+    Stage S;
+    S.X = sm;
+    S.Y = S.X + cap;
+    S.Z = S.Y + cap;
+    S.D = S.Z + cap;
+    S.FC = S.D + cap;
+    int* Q = (int*)(S.FC + cap);
+
+    int const i = blockIdx.x;
+    int const lane = threadIdx.x;
+    ModelTables const& T = a.T;
+    int const ne = T.ne;
+    int const ec = a.elem[i];
+    ElemDesc const& ed = T.elem[ec];
+    int const s0 = a.off[i];
+    int const n = a.off[i + 1] - s0;
+
+    // 1. element counts and stable placement.
+    int cnt[kMaxElements];
+#pragma unroll
+    for (int q = 0; q < kMaxElements; ++q) cnt[q] = 0;
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int const e = s < n ? a.elem[a.nbrJ[s0 + s]] : -1;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+            if (q < ne) cnt[q] += __popcll(__ballot(e == q));
+    }
+    int start[kMaxElements];
+    {
+        int acc = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            start[q] = acc;
+            acc += (q < ne) ? cnt[q] : 0;
+        }
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            sStart[q] = start[q];
+            sCnt[q] = (q < ne) ? cnt[q] : 0;
+        }
+    }
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int e = -1;
+        double4 g = make_double4(0, 0, 0, 0);
+        if (s < n)
+        {
+            e = a.elem[a.nbrJ[s0 + s]];
+            g = a.geo[s0 + s];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            if (q < ne)
+            {
+                uint64_t const mask = __ballot(e == q);
+                if (e == q)
+                {
+                    int const pos = start[q] + lanes_below(mask);
+                    S.X[pos] = g.x;
+                    S.Y[pos] = g.y;
+                    S.Z[pos] = g.z;
+                    S.D[pos] = g.w;
+                }
+                start[q] += __popcll(mask);
+            }
+        }
+    }
+    __syncthreads();
+
+    double rcCached = -1.0;
+    double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
+    for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        if (g.rc != rcCached)
+        {
+            for (int t = lane; t < n; t += 64)
+            {
+                double f, df;
+                cutoff_fdf(cc, S.D[t], f, df);
+                S.FC[t] = f;
+            }
+            rcCached = g.rc;
+            __syncthreads();
+        }
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        double acc[MAXM];
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) acc[m] = 0.0;
+
+        if (g.type == 2)
+        {
+            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            for (int t = lane; t < na; t += 64)
+            {
+                double const r = S.D[a0 + t];
+                if (r < g.rc)
+                {
+                    double const fc = S.FC[a0 + t];
+#pragma unroll
+                    for (int m = 0; m < MAXM; ++m)
+                        if (m < g.nmem)
+                        {
+                            double const eta = mem[m].eta, rs = mem[m].rs;
+                            acc[m] += exp(-eta * (r - rs) * (r - rs)) * fc;
+                        }
+                }
+            }
+        }
+        else
+        {
+            bool const wide = (g.type == 9);
+            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            int const b0 = sStart[g.e2], nb = sCnt[g.e2];
+            bool const same = (g.e1 == g.e2);
+            int const hh = same ? (na - 1) / 2 : 0;
+            int const npairs = same ? na * (na - 1) / 2 : na * nb;
+            int const dv = same ? hh : nb;
+            float const inv = dv > 0 ? 1.0f / (float)dv : 0.0f;
+            int qn = 0;
+            for (int base = 0; base < npairs; base += 64)
+            {
+                int const p = base + lane;
+                bool pass = false;
+                int ja = 0, kb = 0;
+                if (p < npairs)
+                {
+                    if (same)
+                    {
+                        int r, c;
+                        if (p < na * hh)
+                        {
+                            divmod_small(p, hh, inv, r, c);
+                            int v = r + 1 + c;
+                            if (v >= na) v -= na;
+                            ja = a0 + r;
+                            kb = a0 + v;
+                        }
+                        else
+                        {
+                            r = p - na * hh;
+                            ja = a0 + r;
+                            kb = a0 + r + na / 2;
+                        }
+                    }
+                    else
+                    {
+                        int r, c;
+                        divmod_small(p, nb, inv, r, c);
+                        ja = a0 + r;
+                        kb = b0 + c;
+                    }
+                    double const rij = S.D[ja], rik = S.D[kb];
+                    if (rij < g.rc && rik < g.rc)
+                    {
+                        if (wide) pass = true;
+                        else
+                        {
+                            double const dx = S.X[kb] - S.X[ja];
+                            double const dy = S.Y[kb] - S.Y[ja];
+                            double const dz = S.Z[kb] - S.Z[ja];
+                            pass = (dx * dx + dy * dy + dz * dz) < g.rc2;
+                        }
+                    }
+                }
+                uint64_t const mask = __ballot(pass);
+                if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
+                qn += __popcll(mask);
+                __syncthreads();
+                if (qn >= 64)
+                {
+                    int const e = Q[qn - 64 + lane];
+                    triplet_forward<MAXM>(S, e & 0xffff, e >> 16, wide, cc,
+                                          mem, g.nmem, acc);
+                    qn -= 64;
+                    __syncthreads();
+                }
+            }
+            if (lane < qn)
+            {
+                int const e = Q[lane];
+                triplet_forward<MAXM>(S, e & 0xffff, e >> 16, wide, cc, mem,
+                                      g.nmem, acc);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m)
+        {
+            if (m < g.nmem)
+            {
+                double const v = wave_sum(acc[m]);
+                if (lane == m)
+                {
+                    double const raw = (g.type == 2) ? v : v * mem[m].gfac;
+                    int const sf = mem[m].sf;
+                    Gi[sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + sf], raw);
+                }
+            }
+        }
+    }
+}
+
+// Per SF of one element: min, max, sum, sum of squares over its atoms.  One
+// block per SF, fixed-order tree reduction.
+__global__ __launch_bounds__(256) void k_sf_stats(const double* __restrict__ G,
+                                                  int ldG,
+                                                  const int* __restrict__ atoms,
+                                                  int na, int nsf,
+                                                  double* __restrict__ out)
+{
+    __shared__ double smin[256], smax[256], ssum[256], ssq[256];
+    int const k = blockIdx.x;
+    int const t = threadIdx.x;
+    double mn = 1.0e300, mx = -1.0e300, su = 0.0, sq = 0.0;
+    for (int q = t; q < na; q += 256)
+    {
+        double const v = G[(size_t)atoms[q] * ldG + k];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+        su += v;
+        sq += v * v;
+    }
+    smin[t] = mn; smax[t] = mx; ssum[t] = su; ssq[t] = sq;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (t < o)
+        {
+            smin[t] = fmin(smin[t], smin[t + o]);
+            smax[t] = fmax(smax[t], smax[t + o]);
+            ssum[t] += ssum[t + o];
+            ssq[t] += ssq[t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+    {
+        out[4 * k] = smin[0];
+        out[4 * k + 1] = smax[0];
+        out[4 * k + 2] = ssum[0];
+        out[4 * k + 3] = ssq[0];
+    }
+    (void)nsf;
+}
+
+// Element::updateSymmetryFunctionStatistics (Element.cpp:462-541): count
+// SFs whose unscaled value is outside [Gmin, Gmax] by more than 1000 eps.
+__global__ void k_ew(ModelTables T, const double* __restrict__ G, int ldG,
+                     const int* __restrict__ elem, int n, int* __restrict__ ew)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ElemDesc const& ed = T.elem[elem[i]];
+    double const eps = 1000.0 * 2.220446049250313e-16;
+    int c = 0;
+    for (int k = 0; k < ed.nsf; ++k)
+    {
+        ScaleDesc const& s = T.scale[ed.sf0 + k];
+        double const g = G[(size_t)i * ldG + k];
+        double v;
+        switch (s.type)   // SymFnc::unscale (SymFnc.cpp:197-223)
+        {
+        case ST_SCALE: v = (g - s.smin) / s.sfac + s.gmin; break;
+        case ST_CENTER: v = g + s.gmean; break;
+        case ST_SCALECENTER:
+        case ST_SCALESIGMA: v = (g - s.smin) / s.sfac + s.gmean; break;
+        default: v = g; break;
+        }
+        if (s.type != ST_NONE && (v + eps < s.gmin || v - eps > s.gmax)) ++c;
+    }
+    ew[i] = c;
+}
+
+}  // namespace
+
+size_t sf_forward_lds_bytes(int cap)
+{
+    return (size_t)cap * 5 * sizeof(double) + kQueue * sizeof(int);
+}
+
+hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    k_sf_forward<kMaxMembers><<<a.n, 64, sf_forward_lds_bytes(a.cap), s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sf_stats(const double* G, int ldG, const int* atoms, int na,
+                           int nsf, double* out, hipStream_t s)
+{
+    if (nsf <= 0) return hipSuccess;
+    k_sf_stats<<<nsf, 256, 0, s>>>(G, ldG, atoms, na, nsf, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ew(ModelTables T, const double* G, int ldG, const int* elem,
+                     int n, int* perAtom, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_ew<<<(n + 255) / 256, 256, 0, s>>>(T, G, ldG, elem, n, perAtom);
+    return hipGetLastError();
+}
+
+}  // namespace hdnnp

@@ -1,0 +1,111 @@
+// Device-side model tables shared by the host setup (model.cpp) and the HIP
+// kernels.  Plain structs of doubles and ints, uploaded once per context.
+//
+// Layout (one flat buffer per table):
+//   ElemDesc[ne]        per centre element: SF count, group ranges, NN
+//   CutCfg[..]          distinct (cutoff radius) configurations per element
+//   GroupDesc[..]       SF groups, split into chunks of <= kMaxMembers
+//   MemberDesc[..]      group members (one per SF of a group chunk)
+//   ScaleDesc[..]       per SF scaling (SymFnc::scale)
+//   pair lists          per (centre, e_j, e_k) the angular group ids
+//   NN weights          per element, per layer row-major W[k][j] then b[k]
+#ifndef HDNNP_TABLES_H
+#define HDNNP_TABLES_H
+
+#include <stdint.h>
+
+namespace hdnnp {
+
+constexpr int kMaxElements   = 8;
+constexpr int kMaxMembers    = 16;   // SF members per device group chunk
+constexpr int kMaxLayers     = 8;
+constexpr int kMaxCutCfg     = 4;    // distinct cutoff radii per element
+constexpr int kMaxPairGroups = 8;    // angular group chunks per element pair
+
+// CutoffFunction.h:42-72 numbering.
+enum CutType { CT_HARD = 0, CT_COS, CT_TANHU, CT_TANH, CT_EXP, CT_POLY1,
+               CT_POLY2, CT_POLY3, CT_POLY4 };
+// SymFnc.h ScalingType numbering.
+enum ScaleType { ST_NONE = 0, ST_SCALE, ST_CENTER, ST_SCALECENTER,
+                 ST_SCALESIGMA };
+// NeuralNetwork.h ActivationFunction numbering (AF_UNSET = 0).
+enum ActType { AF_UNSET = 0, AF_IDENTITY, AF_TANH, AF_LOGISTIC, AF_SOFTPLUS,
+               AF_RELU, AF_GAUSSIAN, AF_COS, AF_REVLOGISTIC, AF_EXP,
+               AF_HARMONIC };
+
+struct CutCfg
+{
+    int    type;
+    int    pad;
+    double rc, rcinv, rci, iw;
+    double tanhpre;   // ((e + 1/e) / (e - 1/e))^3 (CutoffFunction.cpp:28)
+};
+
+struct GroupDesc
+{
+    int    type;      // 2, 3 or 9
+    int    e1, e2;    // neighbour elements (e2 = -1 for radial)
+    int    nmem;      // members in this chunk (<= kMaxMembers)
+    int    mem0;      // first member in the member table
+    int    cfg;       // index into the element's CutCfg range (local)
+    double rc, rc2;
+};
+
+struct MemberDesc
+{
+    double eta, rs, lambda, zeta;
+    double zl;        // zeta * lambda
+    double fnorm;     // angular: 2^(1-zeta) * scalingFactor (factorNorm
+                      // after SymGrpBaseExpAng::setScalingFactors)
+    double fderiv;    // angular: 2 eta / zeta / lambda
+    double gfac;      // angular: fnorm / scalingFactor (applied to the sum)
+    double sfac;      // scalingFactor
+    int    zint;      // round(zeta)
+    int    useint;    // integer power path (SymFncBaseExpAng.cpp:82-90)
+    int    calcexp;   // recompute the exponential for this member
+    int    sf;        // SF index within the centre element (NN input slot)
+};
+
+struct ScaleDesc
+{
+    int    type;
+    int    pad;
+    double smin, sfac, gmin, gmax, gmean;
+};
+
+struct ElemDesc
+{
+    int nsf;
+    int g0, ng;          // group chunks [g0, g0+ng) (radial first)
+    int nrad;            // of which radial chunks: [g0, g0+nrad)
+    int cfg0, ncfg;      // cutoff configurations
+    int sf0;             // first ScaleDesc of this element
+    // network
+    int nlayers;         // including input and output layer
+    int width[kMaxLayers];
+    int act[kMaxLayers];
+    int woff[kMaxLayers];   // offset of layer l's W (l >= 1) in weights
+    int boff[kMaxLayers];   // offset of layer l's b
+    int normalize;          // normalize_nodes
+    int maxwidth;           // widest hidden/output layer
+};
+
+struct ModelTables
+{
+    int ne;
+    int maxnsf;
+    double maxCutoff;
+    const ElemDesc*   elem;
+    const CutCfg*     cfg;
+    const GroupDesc*  group;
+    const MemberDesc* member;
+    const ScaleDesc*  scale;
+    // angular group chunks per (centre c, e_j, e_k): block pb = (c*ne+ej)*ne+ek
+    const int*        pairGroups;   // [pb * kMaxPairGroups + t], global ids
+    const int*        pairCount;    // [pb]
+    const double*     weights;
+};
+
+}  // namespace hdnnp
+
+#endif

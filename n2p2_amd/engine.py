@@ -1,0 +1,321 @@
+"""ctypes binding of the HIP engine (include/hdnnp.h) with a Mode-like API.
+
+Class and method names follow the reference's libnnp call sequence
+(Mode.h:297-386, Structure.h:188-190) so that parity tests read like the
+reference's own tests:
+
+    m = Model.load("input.nn", "scaling.data", "weights.%03zu.data")
+    ctx = Context(m)
+    ctx.set_structure(r, elem, box)
+    ctx.calculate_neighbor_list(m.max_cutoff)        # Structure::calculateNeighborList
+    ctx.calculate_symmetry_function_groups(True)     # Mode::calculateSymmetryFunctionGroups
+    ctx.calculate_atomic_neural_networks(True)       # Mode::calculateAtomicNeuralNetworks
+    e = ctx.calculate_energy()                       # Mode::calculateEnergy
+    ctx.calculate_forces()                           # Mode::calculateForces
+    f = ctx.forces()
+
+Errors raise RuntimeError with the engine's "ERROR: ..." message, like the
+reference's std::runtime_error.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhdnnp.so")
+
+FORCES_REFERENCE = 0
+FORCES_CONSISTENT = 1
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+# (name, restype, argtypes) of every exported function in include/hdnnp.h.
+_SIGNATURES = [
+    ("hdnnp_last_error", ctypes.c_char_p, []),
+    ("hdnnp_model_load", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hdnnp_model_free", None, [ctypes.c_void_p]),
+    ("hdnnp_model_num_elements", ctypes.c_int, [ctypes.c_void_p]),
+    ("hdnnp_model_atomic_number", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_model_num_sf", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_model_max_cutoff", ctypes.c_double, [ctypes.c_void_p]),
+    ("hdnnp_model_element_index", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("hdnnp_model_normalization", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p]),
+    ("hdnnp_model_energy_offset", ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_model_sf_table", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("hdnnp_model_set_scaling", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_double_p]),
+    ("hdnnp_ctx_create", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hdnnp_ctx_destroy", None, [ctypes.c_void_p]),
+    ("hdnnp_ctx_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_ctx_update_model", ctypes.c_int, [ctypes.c_void_p]),
+    ("hdnnp_set_structure", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_int32_p, _c_double_p]),
+    ("hdnnp_set_positions", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_set_positions_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("hdnnp_neighbor_list", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("hdnnp_symmetry_functions", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_atomic_networks", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_energy", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_forces", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_evaluate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("hdnnp_num_atoms", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_num_neighbor_entries", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_get_pbc", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_get_neighbors", ctypes.c_int,
+     [ctypes.c_void_p, _c_int64_p, _c_int32_p, _c_int32_p, _c_double_p, _c_double_p]),
+    ("hdnnp_get_G", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
+    ("hdnnp_get_dEdG", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
+    ("hdnnp_get_atomic_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_get_forces", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_forces_device", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_profile_avg_ms", ctypes.c_double, [ctypes.c_void_p, ctypes.c_char_p, _c_int64_p]),
+    ("hdnnp_profile_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("hdnnp_work_counts", ctypes.c_int, [ctypes.c_void_p, _c_int64_p]),
+]
+
+EXPORTED = [s[0] for s in _SIGNATURES]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libhdnnp.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            "n2p2_amd: HIP engine library %s is missing; build it with "
+            "`make -C n2p2_amd` (or __graft_entry__.build())" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.hdnnp_last_error().decode(errors="replace")
+        raise RuntimeError("hdnnp error %d: %s" % (rc, msg.strip()))
+
+
+class Model:
+    """Settings + scaling + weights (Mode setup sequence)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def load(cls, settings, scaling, weights_fmt):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        sc = scaling.encode() if scaling else None
+        _check(lib.hdnnp_model_load(settings.encode(), sc, weights_fmt.encode(),
+                                    ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_dir(cls, d, scaling=True):
+        return cls.load(os.path.join(d, "input.nn"),
+                        os.path.join(d, "scaling.data") if scaling else None,
+                        os.path.join(d, "weights.%03zu.data"))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hdnnp_model_free(self._h)
+            self._h = None
+
+    @property
+    def num_elements(self):
+        return _lib.hdnnp_model_num_elements(self._h)
+
+    @property
+    def max_cutoff(self):
+        return _lib.hdnnp_model_max_cutoff(self._h)
+
+    def num_sf(self, e):
+        return _lib.hdnnp_model_num_sf(self._h, e)
+
+    def atomic_number(self, e):
+        return _lib.hdnnp_model_atomic_number(self._h, e)
+
+    def element_index(self, sym):
+        return _lib.hdnnp_model_element_index(self._h, sym.encode())
+
+    def normalization(self):
+        me, ce, cl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        used = _lib.hdnnp_model_normalization(self._h, ctypes.byref(me),
+                                              ctypes.byref(ce), ctypes.byref(cl))
+        return bool(used), me.value, ce.value, cl.value
+
+    def energy_offset(self, e):
+        return _lib.hdnnp_model_energy_offset(self._h, e)
+
+    def sf_table(self, e):
+        out = np.zeros((self.num_sf(e), 12))
+        _check(_lib.hdnnp_model_sf_table(self._h, e, _ptr(out, ctypes.c_double)))
+        return out
+
+    def set_scaling(self, e, scale_type, smin, smax, stats):
+        stats = np.ascontiguousarray(stats, dtype=np.float64)
+        _check(_lib.hdnnp_model_set_scaling(self._h, e, scale_type, smin, smax,
+                                            _ptr(stats, ctypes.c_double)))
+
+
+class Context:
+    """One engine context (device buffers + stream), like one Mode object."""
+
+    def __init__(self, model, device=0):
+        lib = load_library()
+        self.model = model
+        h = ctypes.c_void_p()
+        _check(lib.hdnnp_ctx_create(model._h, device, ctypes.byref(h)))
+        self._h = h
+        self.n = 0
+        self.maxnsf = max(model.num_sf(e) for e in range(model.num_elements))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.hdnnp_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_precision(self, p):
+        _check(_lib.hdnnp_ctx_set_precision(self._h, p))
+
+    def update_model(self):
+        _check(_lib.hdnnp_ctx_update_model(self._h))
+
+    def set_structure(self, r, elem, box=None):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        self._r, self._e = r, elem
+        bp = None
+        if box is not None:
+            self._box = np.ascontiguousarray(box, dtype=np.float64).reshape(3, 3)
+            bp = _ptr(self._box, ctypes.c_double)
+        _check(_lib.hdnnp_set_structure(self._h, r.shape[0], _ptr(r, ctypes.c_double),
+                                        _ptr(elem, ctypes.c_int32), bp))
+        self.n = r.shape[0]
+
+    def set_positions(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
+        _check(_lib.hdnnp_set_positions(self._h, _ptr(r, ctypes.c_double)))
+
+    def set_positions_device(self, ptr):
+        _check(_lib.hdnnp_set_positions_device(self._h, ctypes.c_void_p(ptr)))
+
+    # --- Mode / Structure compute sequence
+    def calculate_neighbor_list(self, cutoff):
+        _check(_lib.hdnnp_neighbor_list(self._h, cutoff))
+
+    def calculate_symmetry_function_groups(self, derivatives=True):
+        _check(_lib.hdnnp_symmetry_functions(self._h, int(derivatives)))
+
+    def calculate_atomic_neural_networks(self, derivatives=True):
+        _check(_lib.hdnnp_atomic_networks(self._h, int(derivatives)))
+
+    def calculate_energy(self):
+        e = ctypes.c_double()
+        _check(_lib.hdnnp_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def calculate_forces(self, mode=FORCES_REFERENCE):
+        _check(_lib.hdnnp_forces(self._h, mode))
+
+    def evaluate(self, force_mode=FORCES_REFERENCE):
+        """Mode::evaluateNNP; force_mode < 0 skips forces."""
+        e = ctypes.c_double()
+        _check(_lib.hdnnp_evaluate(self._h, force_mode, ctypes.byref(e)))
+        return e.value
+
+    # --- results
+    def num_neighbor_entries(self):
+        return _lib.hdnnp_num_neighbor_entries(self._h)
+
+    def pbc(self):
+        p = np.zeros(3, dtype=np.int32)
+        _check(_lib.hdnnp_get_pbc(self._h, _ptr(p, ctypes.c_int32)))
+        return p
+
+    def neighbors(self):
+        m = self.num_neighbor_entries()
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        idx = np.zeros(m, dtype=np.int32)
+        img = np.zeros((m, 3), dtype=np.int32)
+        d = np.zeros(m)
+        dr = np.zeros((m, 3))
+        _check(_lib.hdnnp_get_neighbors(self._h, _ptr(off, ctypes.c_int64),
+                                        _ptr(idx, ctypes.c_int32), _ptr(img, ctypes.c_int32),
+                                        _ptr(d, ctypes.c_double), _ptr(dr, ctypes.c_double)))
+        return off, idx, img, d, dr
+
+    def G(self):
+        out = np.zeros((self.n, self.maxnsf))
+        _check(_lib.hdnnp_get_G(self._h, _ptr(out, ctypes.c_double), self.maxnsf))
+        return out
+
+    def dEdG(self):
+        out = np.zeros((self.n, self.maxnsf))
+        _check(_lib.hdnnp_get_dEdG(self._h, _ptr(out, ctypes.c_double), self.maxnsf))
+        return out
+
+    def atomic_energies(self):
+        out = np.zeros(self.n)
+        _check(_lib.hdnnp_get_atomic_energies(self._h, _ptr(out, ctypes.c_double)))
+        return out
+
+    def forces(self):
+        out = np.zeros((self.n, 3))
+        _check(_lib.hdnnp_get_forces(self._h, _ptr(out, ctypes.c_double)))
+        return out
+
+    def forces_device_ptr(self):
+        return _lib.hdnnp_forces_device(self._h)
+
+    def extrapolation_warnings(self):
+        per = np.zeros(self.n, dtype=np.int32)
+        tot = _lib.hdnnp_extrapolation_warnings(self._h, _ptr(per, ctypes.c_int32))
+        if tot < 0:
+            _check(int(tot))
+        return tot, per
+
+    def sf_statistics(self, e):
+        out = np.zeros((self.model.num_sf(e), 4))
+        na = _lib.hdnnp_sf_statistics(self._h, e, _ptr(out, ctypes.c_double))
+        if na < 0:
+            _check(int(na))
+        return na, out
+
+    # --- profiling
+    def profile(self, enable=True):
+        _check(_lib.hdnnp_profile_enable(self._h, int(enable)))
+
+    def profile_reset(self):
+        _check(_lib.hdnnp_profile_reset(self._h))
+
+    def profile_avg_ms(self, name):
+        calls = ctypes.c_int64()
+        ms = _lib.hdnnp_profile_avg_ms(self._h, name.encode(), ctypes.byref(calls))
+        return ms, calls.value

@@ -1935,14 +1935,18 @@ int orc_sf_statistics(orc_struct* s, orc_model* m, int e, double* out)
                 sum2[k] += v * v;
             }
         }
+        // Dataset.cpp:1074-1085: mean = sum / n,
+        // sigma = sqrt((sum2 - sum^2 / n) / (n - 1)).
         for (size_t k = 0; k < nsf; ++k)
         {
-            double mean = cnt ? sum[k] / cnt : 0.0;
-            double var = cnt ? sum2[k] / cnt - mean * mean : 0.0;
+            double const n = (double)cnt;
+            double const mean = cnt ? sum[k] / n : 0.0;
+            double const sigma = cnt > 1
+                ? std::sqrt((sum2[k] - sum[k] * sum[k] / n) / (n - 1)) : 0.0;
             out[4 * k] = mn[k];
             out[4 * k + 1] = mx[k];
             out[4 * k + 2] = mean;
-            out[4 * k + 3] = std::sqrt(std::max(var, 0.0));
+            out[4 * k + 3] = sigma;
         }
         for (size_t i = 0; i < el.sfs.size(); ++i)
         {

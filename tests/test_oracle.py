@@ -1,0 +1,260 @@
+"""Pins the oracle (CPU restatement of the reference path) before it is used
+as the checker for the HIP engine.
+
+What pins what:
+  - Prediction goldens of test/cpp/ExamplePrediction.h (H2O_RPBE-D3: G2/G3,
+    CT_TANHU, scale+center, normalisation, tanh; Cu2S_PBE: G2/G3/G9,
+    CT_POLY2, sigma scaling, softplus): energy and every force component.
+  - Cutoff goldens of test/cpp/test_CutoffFunction.cpp (all 9 types, f and
+    df at r = 2.4, rc = 3, alpha = 0.2) through a two-atom structure whose
+    energy is the cutoff value itself.
+  - SF goldens of test/cpp/fixtures.h (types 2, 3, 9 incl. rs > 0, lambda
+    -1, zeta 4) on the three- and four-atom fixtures.
+  - Derivatives: forces = -dE/dr against central differences (the nnp-checkf
+    procedure) for every SF type and for a periodic water box.
+  - Neighbour list: an independent numpy enumeration (set) and the cell-list
+    variant (set, order and bits).
+  - Finding of the reference: Mode::calculateForces omits an atom's own
+    periodic self-images; forces then differ from finite differences when
+    the cell is smaller than the cutoff, and the consistent mode fixes it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from n2p2_amd import io as nio
+from n2p2_amd import workloads as W
+from tests import cases
+from tests.oracle_lib import OModel, OStruct
+
+EPS = np.finfo(float).eps
+
+
+@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+def test_example_prediction_goldens(name):
+    g = cases.load_json("example_prediction.json")["examples"][name]
+    d = os.path.join(cases.EXAMPLES, name)
+    m = OModel.from_dir(d)
+    e, f = m.predict_file(os.path.join(d, "input.data"))
+    f0 = np.array(g["forces"])
+    assert f.shape == f0.shape
+    # The reference test demands 1000 eps absolute; the energy is large
+    # (|E| ~ 3e4), so compare it relative to its magnitude.
+    assert abs(e - g["energy"]) <= 1000 * EPS * max(1.0, abs(g["energy"]))
+    assert np.abs(f - f0).max() <= 1000 * EPS
+
+
+def _two_atom_model(workdir, ct, alpha):
+    return cases.single_sf_model(workdir, "cut%d" % ct, "H",
+                                 ["H 2 H 0.0 0.0 3.0"], cutoff_type=ct,
+                                 cutoff_alpha=alpha)
+
+
+@pytest.mark.parametrize("ct", range(9))
+def test_cutoff_goldens_through_pipeline(workdir, ct):
+    g = cases.load_json("cutoff_functions.json")
+    d = _two_atom_model(workdir, ct, g["alpha"])
+    m = OModel.from_dir(d, scaling=False)
+    r = np.array([[0.0, 0.0, 0.0], [g["r"], 0.0, 0.0]])
+    s = OStruct(m, r, [0, 0])
+    e = s.evaluate(cells=False)
+    G = s.G()
+    # eta = 0: G_0 = fc(r) exactly; E = G_0 + G_1.
+    assert abs(G[0, 0] - g["f"][ct]) <= 100 * EPS
+    assert abs(e - 2 * g["f"][ct]) <= 200 * EPS
+    # F_0x = -dE/dx_0 = 2 * fc'(r) (each atom's G depends on the distance).
+    f = s.forces()
+    assert abs(f[0, 0] - 2 * g["df"][ct]) <= 200 * EPS
+    assert abs(f[1, 0] + 2 * g["df"][ct]) <= 200 * EPS
+
+
+def _fixture_models(workdir):
+    fx = cases.load_json("symfunc_fixtures.json")
+    for key, st in fx["sets"].items():
+        syms = st["elements"].split()
+        for k, sf in enumerate(st["sf"]):
+            lines = [sf["line"]]
+            centre = sf["line"].split()[0]
+            for s in syms:
+                if s != centre:
+                    lines.append("%s 2 %s 0.001 0.0 10.0" % (s, centre))
+            d = cases.single_sf_model(workdir, "fx_%s_%d" % (key, k),
+                                      st["elements"], lines, cutoff_type=2)
+            yield key, k, st, sf, d
+
+
+def test_symfunc_fixture_values(workdir):
+    n = 0
+    for key, k, st, sf, d in _fixture_models(workdir):
+        m = OModel.from_dir(d, scaling=False)
+        s = OStruct(m, st["r"], st["elem"])
+        s.neighbor_list(10.0)
+        s.symmetry_functions(True)
+        G = s.G()
+        assert abs(G[0, 0] - sf["value"]) <= 1000 * EPS, (key, k, sf)
+        n += 1
+    assert n == 11
+
+
+def test_symfunc_derivatives_vs_finite_differences(workdir):
+    for key, k, st, sf, d in _fixture_models(workdir):
+        m = OModel.from_dir(d, scaling=False)
+        s = OStruct(m, st["r"], st["elem"])
+        s.evaluate()
+        f = s.forces()
+        fn = s.numeric_forces(np.arange(s.n), delta=1e-5)
+        assert np.abs(f - fn).max() < 1e-8, (key, k, sf["line"])
+
+
+def _numpy_neighbors(r, box, rc):
+    """Independent enumeration: all (i, j, image) with |dr| <= rc."""
+    n = len(r)
+    out = set()
+    if box is None:
+        for i in range(n):
+            d = r[i] - r
+            dd = np.einsum("ij,ij->i", d, d)
+            for j in np.nonzero(dd <= rc * rc)[0]:
+                if j != i:
+                    out.add((i, int(j), 0, 0, 0))
+        return out
+    inv = np.linalg.inv(box)
+    widths = 1.0 / np.linalg.norm(inv, axis=0)
+    p = [int(np.floor(rc / w)) + 1 for w in widths]
+    imgs = np.array([(a, b, c) for a in range(-p[0], p[0] + 1)
+                     for b in range(-p[1], p[1] + 1)
+                     for c in range(-p[2], p[2] + 1)])
+    shifts = imgs @ box
+    for i in range(n):
+        d = (r[i] - r)[:, None, :] + shifts[None, :, :]
+        dd = np.einsum("ijk,ijk->ij", d, d)
+        for j, q in zip(*np.nonzero(dd <= rc * rc)):
+            b = imgs[q]
+            if j == i and not b.any():
+                continue
+            out.add((i, int(j), int(b[0]), int(b[1]), int(b[2])))
+    return out
+
+
+def _oracle_neighbor_set(s, r, box):
+    off, idx, d, dr = s.neighbors()
+    res = set()
+    for i in range(s.n):
+        for q in range(off[i], off[i + 1]):
+            j = idx[q]
+            if box is None:
+                b = (0, 0, 0)
+            else:
+                b = np.rint(np.linalg.solve(box.T, dr[q] - (r[i] - r[j]))).astype(int)
+            res.add((i, int(j), int(b[0]), int(b[1]), int(b[2])))
+    return res, off, idx, d, dr
+
+
+@pytest.mark.parametrize("kind", ["water", "small_box", "triclinic", "open"])
+def test_neighbor_list_sets_and_cells(workdir, kind):
+    d, _, _, _ = cases.water_case(workdir)
+    m = OModel.from_dir(d)
+    rng = np.random.default_rng(5)
+    if kind == "water":
+        r, e, box = W.water(64, 12.42, 1234)
+        rc = 6.35
+    elif kind == "small_box":          # rc > L: several images per pair
+        box = np.diag([4.0, 4.5, 5.0])
+        r = rng.uniform(0, 4.0, (7, 3))
+        e = rng.integers(0, 2, 7)
+        rc = 6.35
+    elif kind == "triclinic":
+        box = np.array([[9.0, 0.0, 0.0], [2.0, 8.5, 0.0], [1.0, -1.5, 10.0]])
+        r = nio.remap(rng.uniform(0, 9.0, (40, 3)), box)
+        e = rng.integers(0, 2, 40)
+        rc = 4.0
+    else:
+        box = None
+        r = rng.uniform(0, 12.0, (50, 3))
+        e = rng.integers(0, 2, 50)
+        rc = 6.35
+    s = OStruct(m, r, e, box)
+    s.neighbor_list(rc, cells=False)
+    got, off, idx, dd, dr = _oracle_neighbor_set(s, r, box)
+    assert got == _numpy_neighbors(r, box, rc)
+    # Reference order: j ascending, then image components ascending.
+    for i in range(s.n):
+        keys = []
+        for q in range(off[i], off[i + 1]):
+            j = idx[q]
+            b = (0, 0, 0) if box is None else tuple(np.rint(np.linalg.solve(
+                box.T, dr[q] - (r[i] - r[j]))).astype(int))
+            keys.append((j,) + b)
+        assert keys == sorted(keys)
+    # Cell-list variant: identical entries and bits.
+    s2 = OStruct(m, r, e, box)
+    s2.neighbor_list(rc, cells=True)
+    off2, idx2, d2, dr2 = s2.neighbors()
+    assert np.array_equal(off, off2) and np.array_equal(idx, idx2)
+    assert np.array_equal(dd, d2) and np.array_equal(dr, dr2)
+
+
+def test_water_forces_vs_finite_differences(workdir):
+    d, r, e, box = cases.water_case(workdir)
+    m = OModel.from_dir(d)
+    s = OStruct(m, r, e, box)
+    s.evaluate(cells=True)
+    f = s.forces()
+    atoms = [0, 1, 2, 97, 191]
+    fn = s.numeric_forces(atoms, delta=1e-4, cells=True)
+    # nnp-checkf warns above 10 delta^2 (nnp-checkf.cpp:288).
+    assert np.abs(f[atoms] - fn).max() < 10 * 1e-4 ** 2
+
+
+def test_self_image_forces(workdir):
+    """Cell smaller than the cutoff: the reference gather misses the atom's
+    own images (mode 0); mode 1 agrees with finite differences."""
+    d = cases.single_sf_model(
+        workdir, "selfimg", "H",
+        ["H 2 H 0.01 0.0 6.0", "H 3 H H 0.01 1.0 1.0 6.0",
+         "H 9 H H 0.01 -1.0 2.0 6.0"], cutoff_type=1, act="t l",
+        all_sf=True)
+    m = OModel.from_dir(d, scaling=False)
+    r = np.array([[0.3, 0.4, 0.2], [1.9, 1.1, 0.7], [0.8, 2.6, 1.9]])
+    box = np.diag([5.0, 5.0, 5.0])
+    s = OStruct(m, r, [0, 0, 0], box)
+    s.evaluate(force_mode=0)
+    f0 = s.forces()
+    fn = s.numeric_forces(np.arange(3), delta=1e-5)
+    s.forces_calc(1)
+    f1 = s.forces()
+    assert np.abs(f1 - fn).max() < 1e-8
+    assert np.abs(f0 - fn).max() > 1e-6      # the reference behaviour
+    assert np.all(s.pbc() == 2)
+
+
+def test_sf_statistics_definition(workdir):
+    d, r, e, box = cases.water_case(workdir)
+    m = OModel.from_dir(d, scaling=False)
+    s = OStruct(m, r, e, box)
+    s.neighbor_list(m.max_cutoff, cells=True)
+    s.symmetry_functions(False)
+    G = s.G()
+    for el in range(2):
+        n, st = s.sf_statistics(el)
+        g = G[e == el, :m.num_sf(el)]
+        assert n == len(g)
+        assert np.allclose(st[:, 0], g.min(0), rtol=0, atol=0)
+        assert np.allclose(st[:, 1], g.max(0), rtol=0, atol=0)
+        assert np.allclose(st[:, 2], g.mean(0), rtol=1e-13)
+        assert np.allclose(st[:, 3], g.std(0, ddof=1), rtol=1e-9)
+
+
+def test_sf_sort_order_interleaves_elements(workdir):
+    """Within an element the SFs are sorted by (type, rc, eta, ..., e1): the
+    H- and O-neighbour G2 of one eta sit next to each other."""
+    d, _, _, _ = cases.water_case(workdir)
+    m = OModel.from_dir(d)
+    t = m.sf_table(0)
+    assert list(t[:16, 0]) == [2] * 16 and list(t[16:, 0]) == [3] * 24
+    assert list(t[:4, 1]) == [0, 1, 0, 1]
+    assert np.all(np.diff(t[:16:2, 3]) > 0)
+    groups = m.groups(0)
+    assert [g[0] for g in groups] == [2, 2, 3, 3, 3]
+    assert all(len(g[3]) == 8 for g in groups)
