@@ -1,0 +1,210 @@
+"""Benchmark: energy + force atom-steps/s of the HDNNP engine on one MI355X.
+
+Workload (default): water-like H/O box of 196,608 atoms (65,536 molecules,
+cubic 124.2 A cell), 8 radial G2 per neighbour element + 8 angular G3 per
+element pair, rc = 6.35 A cosine cutoff, min/max scaling from this
+structure's SF statistics (computed on the GPU, the nnp-scaling
+definition), random-init nsf-25-25-1 tanh networks (Xavier-uniform, seeded).
+One step = one full Mode::evaluateNNP on the GPU: neighbour list rebuilt,
+symmetry functions, networks, energy, forces; inputs resident in HBM.
+
+Prints one JSON line (see the driver contract) with a roofline object for
+the dominant kernel and a CPU baseline (the oracle, OpenMP on the host
+cores, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from n2p2_amd import costmodel, engine, workloads as W  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X vector FP64
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_model(tmp, r, elem, box):
+    """Model files for the water workload; scaling from GPU statistics."""
+    hidden = (25, 25)
+    W.write_model(tmp, W.water_settings(), W.WATER_ELEMENTS, (1, 8), hidden,
+                  seed=7)
+    m0 = engine.Model.load_dir(tmp, scaling=False)
+    c0 = engine.Context(m0)
+    c0.set_structure(r, elem, box)
+    c0.calculate_neighbor_list(m0.max_cutoff)
+    stats = []
+    for e in range(m0.num_elements):
+        _, st = c0.sf_statistics(e)
+        st = st.copy()
+        flat = st[:, 1] - st[:, 0] <= 0
+        st[flat, 1] = st[flat, 0] + 1.0
+        stats.append(st)
+    c0.close()
+    W.write_scaling(os.path.join(tmp, "scaling.data"), stats)
+    return engine.Model.load_dir(tmp), hidden
+
+
+def cpu_baseline(tmp, seconds_target=20.0):
+    """The oracle (CPU restatement of the reference path, OpenMP over all
+    host cores) on a 24,576-atom sample of the same water workload: cell-list
+    neighbour list + SFs with derivatives + networks + energy + forces,
+    median of 3 timed steps after one warm-up."""
+    from tests.oracle_lib import OModel, OStruct, lib
+    nmol, L = 8192, 62.1
+    r, elem, box = W.water(nmol, L, 4321)
+    m = OModel.from_dir(tmp)
+    s = OStruct(m, r, elem, box)
+    s.evaluate(cells=True)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        s.evaluate(cells=True)
+        times.append(time.perf_counter() - t0)
+        if sum(times) > seconds_target:
+            break
+    t = float(np.median(times))
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": len(r) / t, "unit": "atom-steps/s",
+            "cores": int(lib().orc_num_threads()), "kind": "port",
+            "sample": "oracle on a 24,576-atom water box of the same model "
+                      "(8192 molecules, 62.1 A), cell-list neighbour list, "
+                      "%d timed steps, median %.3f s/step, %s"
+                      % (len(times), t, cpu)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nmol", type=int, default=65536)
+    ap.add_argument("--box", type=float, default=124.2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the multi-GPU path "
+              "(spatial decomposition with halo exchange) is not built, so "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    r, elem, box = W.water(args.nmol, args.box, 20261202)
+    n = len(r)
+    with tempfile.TemporaryDirectory() as tmp:
+        log("workload: water %d atoms, L=%.2f" % (n, args.box))
+        model, hidden = make_model(tmp, r, elem, box)
+        ctx = engine.Context(model)
+        ctx.set_structure(r, elem, box)
+        ctx.calculate_neighbor_list(model.max_cutoff)
+        counts = ctx.work_counts()
+        log("work counts", counts.tolist())
+
+        for _ in range(args.warmup):
+            ctx.evaluate(engine.FORCES_REFERENCE)
+        log("warmup done")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            energy = ctx.evaluate(engine.FORCES_REFERENCE)
+        t1 = time.perf_counter()
+        dt = (t1 - t0) / args.steps
+        log("timed %d steps: %.3f ms/step, E=%.10f" % (args.steps, dt * 1e3,
+                                                        energy))
+
+        # Per-kernel device time (HIP events on the engine's stream).
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(max(3, min(args.steps, 10))):
+            ctx.evaluate(engine.FORCES_REFERENCE)
+        kt = {k: ctx.profile_avg_ms(k)[0]
+              for k in ("neighbor", "symfunc", "network", "force", "gather")}
+        ctx.profile(False)
+        log("kernel ms", kt)
+
+        fwd = costmodel.forward_flops(counts)
+        bwd = costmodel.backward_flops(counts)
+        nnf = sum(np.count_nonzero(elem == e) * costmodel.network_flops(
+                      [[model.num_sf(e)] + list(hidden) + [1]])
+                  for e in range(model.num_elements))
+        total = fwd + bwd + nnf
+        dom = max(kt, key=kt.get)
+        if dom == "force":
+            flops, ms = bwd, kt["force"]
+        elif dom == "symfunc":
+            flops, ms = fwd, kt["symfunc"]
+        elif dom == "network":
+            flops, ms = nnf, kt["network"]
+        else:
+            flops, ms = None, kt[dom]
+        if flops is not None:
+            achieved = flops / (ms * 1e-3) / 1e12
+            roof = {"kernel": {"force": "k_force_backward",
+                               "symfunc": "k_sf_forward",
+                               "network": "k_nn"}[dom],
+                    "bound": "valu-fp64", "achieved": achieved,
+                    "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                    "algorithmic_flops_per_launch": flops,
+                    "avg_launch_ms": ms}
+        else:
+            nb = costmodel.neighbor_bytes(n, int(counts[0]))
+            achieved = nb / (ms * 1e-3) / 1e9
+            roof = {"kernel": "neighbor build", "bound": "hbm",
+                    "achieved": achieved, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                    "traffic": None, "avg_launch_ms": ms}
+
+        cpu = None
+        if not args.no_cpu_baseline:
+            log("cpu baseline (oracle, 24,576-atom sample)")
+            cpu = cpu_baseline(tmp)
+            log("cpu baseline", cpu)
+
+        out = {
+            "metric": "E+F atom-steps/s, mixed-precision vs plain FP64 GPU, "
+                      "at force err <=1 meV/A",
+            "value": n / dt,
+            "unit": "atom-steps/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": dt * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "water %d atoms (%d H2O, cubic %.1f A), "
+                                   "40 SFs/element (G2+G3), rc 6.35 A cos "
+                                   "cutoff, nsf-25-25-1 tanh, plain FP64 "
+                                   "path, neighbour list rebuilt every step"
+                                   % (n, args.nmol, args.box),
+                       "atoms": n, "path": "plain-fp64",
+                       "flops_per_atom_step": total / n,
+                       "kernel_ms": kt},
+            "roofline": roof,
+            "cpu_baseline": cpu,
+        }
+        print(json.dumps(out), flush=True)
+        ctx.close()
+
+
+if __name__ == "__main__":
+    main()

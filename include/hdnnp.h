@@ -200,10 +200,20 @@ int64_t hdnnp_sf_statistics(hdnnp_ctx* c, int e, double* out);
 int    hdnnp_profile_enable(hdnnp_ctx* c, int enable);
 double hdnnp_profile_avg_ms(hdnnp_ctx* c, const char* name, int64_t* calls);
 int    hdnnp_profile_reset(hdnnp_ctx* c);
-/* Algorithmic work of the last evaluation: counts of neighbour pairs,
- * element-matched angular pairs, triplets inside the cutoff, and SF member
- * evaluations (radial, angular), for roofline accounting. */
-int    hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts /* 6 */);
+/* Algorithmic work of the loaded structure and neighbour list, counted on
+ * the device with the filters of SymGrpExpRad/Angn/Angw::calculate, for
+ * roofline accounting (HDNNP_WORK_COUNTS values):
+ *   [0] neighbour-list entries          [1] radial member evaluations
+ *   [2] narrow angular pairs tested     [3] wide angular pairs tested
+ *       (element-matched, rij, rik < rc)
+ *   [4] narrow triplets (rjk < rc)      [5] wide triplets
+ *   [6] narrow member evaluations       [7] wide member evaluations
+ *   [8] multiplies of pow_int(1 + lambda cos, zeta - 1) over [6] + [7]
+ *   [9] dE/dG . dG/dr terms of Mode::calculateForces (per atom: SF count +
+ *       per neighbour slot the SF-derivative count of its element)
+ * Needs hdnnp_neighbor_list first. */
+#define HDNNP_WORK_COUNTS 10
+int    hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -1053,10 +1053,31 @@ int hdnnp_profile_reset(hdnnp_ctx* c)
 int hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts)
 {
     return guarded([&]() {
-        for (int k = 0; k < 6; ++k) counts[k] = 0;
-        counts[0] = c->M;
-        throw Error(HDNNP_E_UNSUPPORTED,
-                    "ERROR: Work counters are not available yet.\n");
+        requireNL(c);
+        c->setDevice();
+        int const cap = std::max(c->maxNeigh, 1);
+        if (sf_forward_lds_bytes(cap) > 65536)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: Too many neighbours.\n");
+        DBuf<unsigned long long> d;
+        d.ensure(HDNNP_WORK_COUNTS);
+        hipCheck(hipMemsetAsync(d.p, 0, HDNNP_WORK_COUNTS * sizeof(uint64_t),
+                                c->stream), "memset");
+        SFArgs a{};
+        a.T = c->T;
+        a.n = (int)c->n;
+        a.off = c->dOff.p;
+        a.geo = c->dGeo.p;
+        a.nbrJ = c->dNbrJ.p;
+        a.elem = c->dElemAtom.p;
+        a.ldG = c->ld;
+        a.cap = cap;
+        hipCheck(launch_sf_count(a, d.p, c->stream), "work counters");
+        unsigned long long h[HDNNP_WORK_COUNTS];
+        hipCheck(hipMemcpyAsync(h, d.p, sizeof h, hipMemcpyDeviceToHost,
+                                c->stream), "counters");
+        c->sync();
+        d.release();
+        for (int k = 0; k < HDNNP_WORK_COUNTS; ++k) counts[k] = (int64_t)h[k];
     });
 }
 

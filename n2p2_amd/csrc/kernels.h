@@ -58,6 +58,9 @@ struct SFArgs
 };
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
 size_t sf_forward_lds_bytes(int cap);
+// Work counters (10 x uint64, accumulated): see k_sf_count.
+hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
+                           hipStream_t s);
 
 struct ForceArgs
 {

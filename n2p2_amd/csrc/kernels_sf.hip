@@ -105,30 +105,13 @@ __device__ __forceinline__ void triplet_forward(
     }
 }
 
-template <int MAXM>
-__global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
+// Stages the atom's neighbours in LDS partitioned by element (stable ballot
+// compaction); sStart / sCnt receive each element's block.
+__device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
+                                 int n, int* sStart, int* sCnt)
 {
-    extern __shared__ double sm[];
-    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
-    int const cap = a.cap;
-    Stage S;
-    S.X = sm;
-    S.Y = S.X + cap;
-    S.Z = S.Y + cap;
-    S.D = S.Z + cap;
-    S.FC = S.D + cap;
-    int* Q = (int*)(S.FC + cap);
-
-    int const i = blockIdx.x;
     int const lane = threadIdx.x;
-    ModelTables const& T = a.T;
-    int const ne = T.ne;
-    int const ec = a.elem[i];
-    ElemDesc const& ed = T.elem[ec];
-    int const s0 = a.off[i];
-    int const n = a.off[i + 1] - s0;
-
-    // 1. element counts and stable placement.
+    int const ne = a.T.ne;
     int cnt[kMaxElements];
 #pragma unroll
     for (int q = 0; q < kMaxElements; ++q) cnt[q] = 0;
@@ -188,6 +171,88 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
         }
     }
     __syncthreads();
+}
+
+// Unordered pairs of a group's neighbour blocks, indexed 0..npairs-1.
+struct PairCursor
+{
+    bool same;
+    int a0, na, b0, nb, hh, npairs;
+    float inv;
+    __device__ PairCursor(bool same_, int a0_, int na_, int b0_, int nb_)
+        : same(same_), a0(a0_), na(na_), b0(b0_), nb(nb_)
+    {
+        hh = same ? (na - 1) / 2 : 0;
+        npairs = same ? na * (na - 1) / 2 : na * nb;
+        int const dv = same ? hh : nb;
+        inv = dv > 0 ? 1.0f / (float)dv : 0.0f;
+    }
+    __device__ bool decode(int p, int& ja, int& kb) const
+    {
+        if (p >= npairs) return false;
+        int r, c;
+        if (same)
+        {
+            if (p < na * hh)
+            {
+                divmod_small(p, hh, inv, r, c);
+                int v = r + 1 + c;
+                if (v >= na) v -= na;
+                ja = a0 + r;
+                kb = a0 + v;
+            }
+            else
+            {
+                r = p - na * hh;
+                ja = a0 + r;
+                kb = a0 + r + na / 2;
+            }
+        }
+        else
+        {
+            divmod_small(p, nb, inv, r, c);
+            ja = a0 + r;
+            kb = b0 + c;
+        }
+        return true;
+    }
+};
+
+// rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
+__device__ __forceinline__ bool pair_passes(Stage const& S, int ja, int kb,
+                                            bool wide, GroupDesc const& g)
+{
+    if (!(S.D[ja] < g.rc && S.D[kb] < g.rc)) return false;
+    if (wide) return true;
+    double const dx = S.X[kb] - S.X[ja];
+    double const dy = S.Y[kb] - S.Y[ja];
+    double const dz = S.Z[kb] - S.Z[ja];
+    return (dx * dx + dy * dy + dz * dz) < g.rc2;
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
+{
+    extern __shared__ double sm[];
+    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    int const cap = a.cap;
+    Stage S;
+    S.X = sm;
+    S.Y = S.X + cap;
+    S.Z = S.Y + cap;
+    S.D = S.Z + cap;
+    S.FC = S.D + cap;
+    int* Q = (int*)(S.FC + cap);
+
+    int const i = blockIdx.x;
+    int const lane = threadIdx.x;
+    ModelTables const& T = a.T;
+    int const ec = a.elem[i];
+    ElemDesc const& ed = T.elem[ec];
+    int const s0 = a.off[i];
+    int const n = a.off[i + 1] - s0;
+
+    stage_by_element(a, S, s0, n, sStart, sCnt);
 
     double rcCached = -1.0;
     double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
@@ -235,57 +300,14 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             bool const wide = (g.type == 9);
             int const a0 = sStart[g.e1], na = sCnt[g.e1];
             int const b0 = sStart[g.e2], nb = sCnt[g.e2];
-            bool const same = (g.e1 == g.e2);
-            int const hh = same ? (na - 1) / 2 : 0;
-            int const npairs = same ? na * (na - 1) / 2 : na * nb;
-            int const dv = same ? hh : nb;
-            float const inv = dv > 0 ? 1.0f / (float)dv : 0.0f;
+            PairCursor pc(g.e1 == g.e2, a0, na, b0, nb);
             int qn = 0;
-            for (int base = 0; base < npairs; base += 64)
+            for (int base = 0; base < pc.npairs; base += 64)
             {
-                int const p = base + lane;
-                bool pass = false;
                 int ja = 0, kb = 0;
-                if (p < npairs)
-                {
-                    if (same)
-                    {
-                        int r, c;
-                        if (p < na * hh)
-                        {
-                            divmod_small(p, hh, inv, r, c);
-                            int v = r + 1 + c;
-                            if (v >= na) v -= na;
-                            ja = a0 + r;
-                            kb = a0 + v;
-                        }
-                        else
-                        {
-                            r = p - na * hh;
-                            ja = a0 + r;
-                            kb = a0 + r + na / 2;
-                        }
-                    }
-                    else
-                    {
-                        int r, c;
-                        divmod_small(p, nb, inv, r, c);
-                        ja = a0 + r;
-                        kb = b0 + c;
-                    }
-                    double const rij = S.D[ja], rik = S.D[kb];
-                    if (rij < g.rc && rik < g.rc)
-                    {
-                        if (wide) pass = true;
-                        else
-                        {
-                            double const dx = S.X[kb] - S.X[ja];
-                            double const dy = S.Y[kb] - S.Y[ja];
-                            double const dz = S.Z[kb] - S.Z[ja];
-                            pass = (dx * dx + dy * dy + dz * dz) < g.rc2;
-                        }
-                    }
-                }
+                bool pass = false;
+                if (pc.decode(base + lane, ja, kb))
+                    pass = pair_passes(S, ja, kb, wide, g);
                 uint64_t const mask = __ballot(pass);
                 if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
                 qn += __popcll(mask);
@@ -323,6 +345,92 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
         }
     }
 }
+
+// Work counters of the algorithm for one structure (same enumeration and
+// filters as k_sf_forward, no arithmetic on the members):
+//   [0] neighbour-list entries
+//   [1] radial member evaluations        [2] narrow pairs tested (rij, rik < rc)
+//   [3] wide pairs tested                [4] narrow triplets (rjk < rc)
+//   [5] wide triplets                    [6] narrow member evaluations
+//   [7] wide member evaluations          [8] multiplies in pow_int(plambda,
+//                                            zeta - 1) over all evaluations
+//   [9] dE/dG . dG/dr contraction terms of Mode::calculateForces:
+//       per atom nsf + sum over slots of numSFD(element of the slot)
+__global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
+                                                 unsigned long long* out)
+{
+    extern __shared__ double sm[];
+    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    int const cap = a.cap;
+    Stage S;
+    S.X = sm;
+    S.Y = S.X + cap;
+    S.Z = S.Y + cap;
+    S.D = S.Z + cap;
+    S.FC = S.D + cap;
+    int const i = blockIdx.x;
+    int const lane = threadIdx.x;
+    ModelTables const& T = a.T;
+    int const ec = a.elem[i];
+    ElemDesc const& ed = T.elem[ec];
+    int const s0 = a.off[i];
+    int const n = a.off[i + 1] - s0;
+    stage_by_element(a, S, s0, n, sStart, sCnt);
+    unsigned long long c[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) c[k] = 0;
+    for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        if (g.type == 2)
+        {
+            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            for (int t = lane; t < na; t += 64)
+                if (S.D[a0 + t] < g.rc) c[1] += g.nmem;
+            continue;
+        }
+        bool const wide = (g.type == 9);
+        int powm = 0;   // pow_int multiplies of one evaluation of all members
+        for (int m = 0; m < g.nmem; ++m)
+        {
+            MemberDesc const& md = T.member[g.mem0 + m];
+            int e = md.useint ? md.zint - 1 : 0;
+            int it = 0, bits = 0;
+            do { bits += (e & 1); e >>= 1; ++it; } while (e);
+            powm += md.useint ? it + bits : 0;
+        }
+        PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
+                      sCnt[g.e2]);
+        for (int base = 0; base < pc.npairs; base += 64)
+        {
+            int ja = 0, kb = 0;
+            if (!pc.decode(base + lane, ja, kb)) continue;
+            if (!(S.D[ja] < g.rc && S.D[kb] < g.rc)) continue;
+            c[wide ? 3 : 2] += 1;
+            if (pair_passes(S, ja, kb, wide, g))
+            {
+                c[wide ? 5 : 4] += 1;
+                c[wide ? 7 : 6] += g.nmem;
+                c[8] += powm;
+            }
+        }
+    }
+    if (lane == 0)
+    {
+        c[0] = n;
+        unsigned long long t = ed.nsf;
+        for (int q = 0; q < T.ne; ++q) t += (unsigned long long)sCnt[q] * ed.nsfd[q];
+        c[9] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+    {
+        unsigned long long v = c[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        if (lane == 0 && v) atomicAdd(&out[k], v);
+    }
+}
+
 
 // Per SF of one element: min, max, sum, sum of squares over its atoms.  One
 // block per SF, fixed-order tree reduction.
@@ -406,6 +514,14 @@ hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
     k_sf_forward<kMaxMembers><<<a.n, 64, sf_forward_lds_bytes(a.cap), s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
+                           hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    k_sf_count<<<a.n, 64, sf_forward_lds_bytes(a.cap), s>>>(a, counters);
     return hipGetLastError();
 }
 

@@ -538,6 +538,7 @@ Model::Flat Model::flatten() const
         ElemDesc d{};
         d.nsf = (int)e.sf.size();
         d.sf0 = (int)F.scale.size();
+        for (int k = 0; k < ne; ++k) d.nsfd[k] = (int)e.table[k].size();
         for (auto const& s : e.sf)
         {
             ScaleDesc sd{};

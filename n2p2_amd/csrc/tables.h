@@ -80,6 +80,8 @@ struct ElemDesc
     int nrad;            // of which radial chunks: [g0, g0+nrad)
     int cfg0, ncfg;      // cutoff configurations
     int sf0;             // first ScaleDesc of this element
+    int nsfd[kMaxElements];   // SF derivatives per neighbour element
+                              // (Element::getSymmetryFunctionNumTable)
     // network
     int nlayers;         // including input and output layer
     int width[kMaxLayers];

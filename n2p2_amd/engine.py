@@ -308,6 +308,13 @@ class Context:
             _check(int(na))
         return na, out
 
+    def work_counts(self):
+        """hdnnp_work_counts: 10 exact counts of pairs, triplets and member
+        evaluations for the loaded structure (see include/hdnnp.h)."""
+        c = np.zeros(10, dtype=np.int64)
+        _check(_lib.hdnnp_work_counts(self._h, _ptr(c, ctypes.c_int64)))
+        return c
+
     # --- profiling
     def profile(self, enable=True):
         _check(_lib.hdnnp_profile_enable(self._h, int(enable)))

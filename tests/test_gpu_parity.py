@@ -153,10 +153,12 @@ def test_cutoff_goldens_gpu(workdir, ct):
     m, ctx = _engine(d, scaling=False)
     ctx.set_structure(np.array([[0.0, 0.0, 0.0], [g["r"], 0.0, 0.0]]), [0, 0])
     e = ctx.evaluate(0)
-    assert abs(ctx.G()[0, 0] - g["f"][ct]) <= 1e-14
-    assert abs(e - 2 * g["f"][ct]) <= 1e-14
+    # test_CutoffFunction.cpp tests each value at 100 eps absolute.
+    eps = np.finfo(float).eps
+    assert abs(ctx.G()[0, 0] - g["f"][ct]) <= 100 * eps
+    assert abs(e - 2 * g["f"][ct]) <= 200 * eps
     f = ctx.forces()
-    assert abs(f[0, 0] - 2 * g["df"][ct]) <= 1e-13
+    assert abs(f[0, 0] - 2 * g["df"][ct]) <= 200 * eps
 
 
 def test_symfunc_fixture_values_gpu(workdir):
