@@ -114,7 +114,7 @@ def main():
         ctx = engine.Context(model)
         ctx.set_structure(r, elem, box)
         ctx.calculate_neighbor_list(model.max_cutoff)
-        counts = ctx.work_counts()
+        counts = [int(x) for x in ctx.work_counts()]
         log("work counts", counts.tolist())
 
         for _ in range(args.warmup):
@@ -140,9 +140,10 @@ def main():
 
         fwd = costmodel.forward_flops(counts)
         bwd = costmodel.backward_flops(counts)
-        nnf = sum(np.count_nonzero(elem == e) * costmodel.network_flops(
+        nnf = sum(int(np.count_nonzero(elem == e)) * costmodel.network_flops(
                       [[model.num_sf(e)] + list(hidden) + [1]])
                   for e in range(model.num_elements))
+        fwd, bwd, nnf = int(fwd), int(bwd), int(nnf)
         total = fwd + bwd + nnf
         dom = max(kt, key=kt.get)
         if dom == "force":
@@ -198,6 +199,7 @@ def main():
                                    % (n, args.nmol, args.box),
                        "atoms": n, "path": "plain-fp64",
                        "flops_per_atom_step": total / n,
+                       "work_counts": counts,
                        "kernel_ms": kt},
             "roofline": roof,
             "cpu_baseline": cpu,
