@@ -79,6 +79,57 @@ __device__ __forceinline__ void cutoff_fdf(CutCfg const& c, double r,
     dfc = df * c.iw;
 }
 
+// Cutoff with the type fixed at compile time (CT >= 0) or read from the
+// configuration (CT < 0).  The specialised forms keep the heavy kernels
+// free of the other types' code and registers.
+template <int CT>
+__device__ __forceinline__ void cutoff_fdf_t(CutCfg const& c, double r,
+                                             double& fc, double& dfc)
+{
+    if (CT < 0)
+    {
+        cutoff_fdf(c, r, fc, dfc);
+        return;
+    }
+    if (r >= c.rc) { fc = 0.0; dfc = 0.0; return; }
+    if (CT == CT_TANHU)
+    {
+        double const th = tanh(1.0 - r * c.rcinv);
+        double const th2 = th * th;
+        fc = th * th2;
+        dfc = 3.0 * th2 * (th2 - 1.0) * c.rcinv;
+        return;
+    }
+    if (r < c.rci) { fc = 1.0; dfc = 0.0; return; }
+    double const x = (r - c.rci) * c.iw;
+    if (CT == CT_COS)
+    {
+        double const cs = cos(kPi * x);
+        fc = 0.5 * (cs + 1.0);
+        dfc = -0.5 * c.iw * kPi * sqrt(1.0 - cs * cs);
+    }
+    else   // CT_POLY2
+    {
+        double const x2 = x * x;
+        fc = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
+        dfc = x2 * ((60.0 - 30.0 * x) * x - 30.0) * c.iw;
+    }
+}
+
+// Launches KERNEL<CT, ...> for the cutoff types with specialised code and
+// KERNEL<-1, ...> (runtime switch) for the others.
+#define HDNNP_CUTOFF_DISPATCH(ct, CALL)                                    \
+    do                                                                     \
+    {                                                                      \
+        switch (ct)                                                        \
+        {                                                                  \
+        case CT_COS: { constexpr int kCT = CT_COS; CALL; } break;           \
+        case CT_TANHU: { constexpr int kCT = CT_TANHU; CALL; } break;       \
+        case CT_POLY2: { constexpr int kCT = CT_POLY2; CALL; } break;       \
+        default: { constexpr int kCT = -1; CALL; } break;                   \
+        }                                                                  \
+    } while (0)
+
 // Exponentiation by squaring (utility.cpp:292-317 semantics).
 __device__ __forceinline__ double pow_int(double x, int n)
 {
@@ -152,6 +203,28 @@ __device__ __forceinline__ void activate(int a, double x, double& v, double& d)
     case AF_HARMONIC: v = x * x; d = 2.0 * x; break;
     default: v = 0.0; d = 0.0; break;
     }
+}
+
+// Member parameters of one group, staged in LDS at the start of the group
+// and read with wave-uniform addresses (broadcast) inside the pair loops.
+struct MParam
+{
+    double lambda, zl, eta, rs, zeta, w;   // w: per-kernel weight
+    int    zint, flags;                    // flags: 1 integer power, 2 new exp
+};
+
+__device__ __forceinline__ MParam make_mparam(MemberDesc const& md, double w)
+{
+    MParam p;
+    p.lambda = md.lambda;
+    p.zl = md.zl;
+    p.eta = md.eta;
+    p.rs = md.rs;
+    p.zeta = md.zeta;
+    p.w = w;
+    p.zint = md.zint;
+    p.flags = (md.useint ? 1 : 0) | (md.calcexp ? 2 : 0);
+    return p;
 }
 
 // SymFnc::scale (SymFnc.cpp:169-195).

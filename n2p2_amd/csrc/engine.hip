@@ -224,6 +224,16 @@ struct hdnnp_ctx
         T.ne = (int)flat.elem.size();
         T.maxnsf = model->m.maxNumSF();
         T.maxCutoff = model->m.maxCutoffRadius;
+        T.cutType = model->m.cutoffType;
+        T.genpow = 0;
+        T.maxGroupMembers = 0;
+        for (auto const& g : flat.group)
+        {
+            T.maxGroupMembers = std::max(T.maxGroupMembers, g.nmem);
+            if (g.type != 2)
+                for (int m = g.mem0; m < g.mem0 + g.nmem; ++m)
+                    if (!flat.member[m].useint) T.genpow = 1;
+        }
         T.elem = dElem.p;
         T.cfg = dCfg.p;
         T.group = dGroup.p;

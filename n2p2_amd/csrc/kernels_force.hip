@@ -12,14 +12,29 @@
 // pointing at another atom j of cslot[j, partner(q)]), partner(q) being the
 // slot of j that points back at m with the opposite image.
 //
-// Backward kernel: one wavefront per atom, neighbours staged in LDS in slot
-// order.  Lane L owns slots j = L, L+64, ...; for its slot it evaluates
-// every ordered pair (j, k) that passes the angular filters (found with a
-// per-lane bitmask so the heavy loop runs only over passing partners), and
-// accumulates the slot-j part of the pair derivative (terms p1 and p3 of
-// SymGrpExpAngn.cpp:300-337 / SymGrpExpAngw.cpp:288-310) in registers.  The
-// partner slot k receives its part from the lane owning k, so every slot is
-// written by exactly one lane and no atomics are needed.
+// Backward kernel, one wavefront per atom:
+//  * neighbours staged in LDS partitioned by element (as the SF kernel);
+//  * groups run one after another, so member parameters and the atom's
+//    dE/dG weights are wave-uniform;
+//  * the member sum is contracted before the vector work.  With
+//    fg = exp(.) (1 + lambda cos)^(zeta-1) (SymGrpExpAngn.cpp:276-297) the
+//    reference's p1 and p3 (:300-337) are
+//      p1 = fg [pfc zeta lambda A + plambda c1],  A = 1/(rij rik) - cos/rij^2
+//      p3 = fg [pfc zeta lambda / (rij rik) + plambda c3]
+//    with c1 = pr1 - 2 eta pfc (rij - rs)/rij and c3 = 2 eta pfc (rjk -
+//    rs)/rjk - pr3 (rs > 0), c1 = pr1 - 2 eta pfc and c3 = 2 eta pfc - pr3
+//    (rs = 0), c3 = 0 for the wide type (SymGrpExpAngw.cpp:300-310); so
+//      P1 = sum_m w_m p1_m = pfc A T1 + T2,  P3 = pfc/(rij rik) T1 + T3,
+//      T1 = sum w fg zeta lambda, T2 = sum w fg plambda c1,
+//      T3 = sum w fg plambda c3,  w_m = dE/dG_m * factorNorm_m;
+//  * every ordered pair (j, k) of a group is evaluated by the lane that
+//    owns slot j and contributes -(P1 dr_ij + P3 dr_jk) to slot j and
+//    P1 dr_ij to the centre; slot k's part comes from the pair (k, j).  The
+//    group's ordered pairs are split into 64 equal contiguous chunks, one
+//    per lane; a lane adds the sum of every slot whose pairs start inside
+//    its chunk straight into the LDS accumulator and leaves the sum of the
+//    slot it inherited from the previous chunk in a per-lane head buffer,
+//    merged afterwards in lane order.  No atomics; fixed summation order.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -27,278 +42,460 @@ namespace hdnnp {
 
 namespace {
 
-// Per-member parameters staged in LDS (non-uniform access by lanes whose
-// pairs belong to different groups).
-struct MemLDS
+constexpr int kChunkWords = 8;   // pass bits per lane chunk: 8 x 32 pairs
+
+struct BLds
 {
-    double lambda, zl, fderiv, fnorm, eta, rs, zeta, w;   // w = dE/dG_sf
-    int    zint, useint, calcexp, pad;
+    double* X;
+    double* Y;
+    double* Z;
+    double* D;
+    double* INV;
+    double* FC;
+    double* DFC;
+    double* ACC;     // 3 per staged neighbour
+    int* SLOT;       // original slot of each staged neighbour
+    uint32_t* MASK;  // [word][64]
+    double* HEAD;    // 3 per lane
+    int* HEADJ;      // per lane
 };
 
-struct GrpLDS
+__device__ __forceinline__ void stage(ForceArgs const& a, BLds const& S,
+                                      int s0, int n, int* sStart, int* sCnt)
 {
-    double rc, rc2;
-    int    type, cfg, nmem, mem;   // mem: local member index
+    int const lane = threadIdx.x;
+    int const ne = a.T.ne;
+    int cnt[kMaxElements];
+#pragma unroll
+    for (int q = 0; q < kMaxElements; ++q) cnt[q] = 0;
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int const e = s < n ? a.elem[a.nbrJ[s0 + s]] : -1;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+            if (q < ne) cnt[q] += __popcll(__ballot(e == q));
+    }
+    int start[kMaxElements];
+    {
+        int acc = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            start[q] = acc;
+            acc += (q < ne) ? cnt[q] : 0;
+        }
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            sStart[q] = start[q];
+            sCnt[q] = (q < ne) ? cnt[q] : 0;
+        }
+    }
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int e = -1;
+        double4 g = make_double4(0, 0, 0, 1);
+        if (s < n)
+        {
+            e = a.elem[a.nbrJ[s0 + s]];
+            g = a.geo[s0 + s];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            if (q < ne)
+            {
+                uint64_t const mask = __ballot(e == q);
+                if (e == q)
+                {
+                    int const pos = start[q] + lanes_below(mask);
+                    S.X[pos] = g.x;
+                    S.Y[pos] = g.y;
+                    S.Z[pos] = g.z;
+                    S.D[pos] = g.w;
+                    S.INV[pos] = 1.0 / g.w;
+                    S.SLOT[pos] = s;
+                    S.ACC[3 * pos] = 0.0;
+                    S.ACC[3 * pos + 1] = 0.0;
+                    S.ACC[3 * pos + 2] = 0.0;
+                }
+                start[q] += __popcll(mask);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Ordered pairs (j, k) of a group, q in [0, total): for e1 != e2 first
+// j in A x k in B, then j in B x k in A; for e1 == e2 j in A, k in A \ {j}.
+struct OrderedPairs
+{
+    bool same;
+    int a0, na, b0, nb, total, split;
+    float invA, invB;
+    __device__ OrderedPairs(bool s, int a0_, int na_, int b0_, int nb_)
+        : same(s), a0(a0_), na(na_), b0(b0_), nb(nb_)
+    {
+        if (same)
+        {
+            total = na * (na - 1);
+            split = total;
+            invA = na > 1 ? 1.0f / (float)(na - 1) : 0.0f;
+            invB = 0.0f;
+        }
+        else
+        {
+            split = na * nb;
+            total = 2 * split;
+            invA = nb > 0 ? 1.0f / (float)nb : 0.0f;   // rows of A
+            invB = na > 0 ? 1.0f / (float)na : 0.0f;   // rows of B
+        }
+    }
+    // j, k as staged indices; first = q is the first pair of row j.
+    __device__ void decode(int q, int& j, int& k, bool& first) const
+    {
+        int r, c;
+        if (same)
+        {
+            int const d = na - 1;
+            r = (int)((float)q * invA);
+            c = q - r * d;
+            if (c < 0) { --r; c += d; } else if (c >= d) { ++r; c -= d; }
+            j = a0 + r;
+            k = a0 + (c < r ? c : c + 1);
+            first = (c == 0);
+        }
+        else if (q < split)
+        {
+            r = (int)((float)q * invA);
+            c = q - r * nb;
+            if (c < 0) { --r; c += nb; } else if (c >= nb) { ++r; c -= nb; }
+            j = a0 + r;
+            k = b0 + c;
+            first = (c == 0);
+        }
+        else
+        {
+            int const qq = q - split;
+            r = (int)((float)qq * invB);
+            c = qq - r * na;
+            if (c < 0) { --r; c += na; } else if (c >= na) { ++r; c -= na; }
+            j = b0 + r;
+            k = a0 + c;
+            first = (c == 0);
+        }
+    }
 };
 
+__device__ __forceinline__ void flush(BLds const& S, int j, bool owned,
+                                      double cx, double cy, double cz,
+                                      bool& headUsed)
+{
+    if (owned)
+    {
+        S.ACC[3 * j] += cx;
+        S.ACC[3 * j + 1] += cy;
+        S.ACC[3 * j + 2] += cz;
+    }
+    else
+    {
+        int const lane = threadIdx.x;
+        S.HEADJ[lane] = j;
+        S.HEAD[3 * lane] = cx;
+        S.HEAD[3 * lane + 1] = cy;
+        S.HEAD[3 * lane + 2] = cz;
+        headUsed = true;
+    }
+}
+
+template <int CT, bool GENPOW>
 __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
 {
     extern __shared__ double sm[];
+    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    __shared__ MParam MP[kMaxMembers];
     int const cap = a.cap;
-    int const ncfgMax = a.maxCfg;
-    ModelTables const& T = a.T;
-    int const ne = T.ne;
+    BLds S;
+    S.X = sm;
+    S.Y = S.X + cap;
+    S.Z = S.Y + cap;
+    S.D = S.Z + cap;
+    S.INV = S.D + cap;
+    S.FC = S.INV + cap;
+    S.DFC = S.FC + cap;
+    S.ACC = S.DFC + cap;
+    S.HEAD = S.ACC + 3 * cap;
+    S.SLOT = (int*)(S.HEAD + 3 * 64);
+    S.HEADJ = S.SLOT + cap;
+    S.MASK = (uint32_t*)(S.HEADJ + 64);
+
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
+    ModelTables const& T = a.T;
     int const ec = a.elem[i];
     ElemDesc const& ed = T.elem[ec];
     int const s0 = a.off[i];
     int const n = a.off[i + 1] - s0;
-    int const nw = (n + 31) >> 5;
-
-    double* X = sm;
-    double* Y = X + cap;
-    double* Z = Y + cap;
-    double* D = Z + cap;
-    double* FC = D + cap;                       // [cfg][cap]
-    double* DFC = FC + (size_t)ncfgMax * cap;   // [cfg][cap]
-    MemLDS* M = (MemLDS*)(DFC + (size_t)ncfgMax * cap);
-    GrpLDS* GR = (GrpLDS*)(M + a.maxMembers);
-    double* PR = (double*)(GR + kMaxElements * kMaxElements * kMaxPairGroups);
-    double* PJ = PR + kMaxElements * kMaxElements;   // pair rjk^2 limit
-    int* E = (int*)(PJ + kMaxElements * kMaxElements);
-    uint32_t* MASK = (uint32_t*)(E + cap);           // [word][64]
-
-    // Stage neighbours (slot order) and cutoff values per configuration.
-    for (int s = lane; s < n; s += 64)
-    {
-        double4 const g = a.geo[s0 + s];
-        X[s] = g.x;
-        Y[s] = g.y;
-        Z[s] = g.z;
-        D[s] = g.w;
-        E[s] = a.elem[a.nbrJ[s0 + s]];
-        for (int c = 0; c < ed.ncfg; ++c)
-        {
-            double f, df;
-            cutoff_fdf(T.cfg[ed.cfg0 + c], g.w, f, df);
-            FC[c * cap + s] = f;
-            DFC[c * cap + s] = df;
-        }
-    }
-    // Members of this element (contiguous) with this atom's dE/dG.
-    int const mbase = T.group[ed.g0].mem0;
-    int nmTotal = 0;
-    for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi) nmTotal += T.group[gi].nmem;
     double const* __restrict__ dEdGi = a.dEdG + (size_t)i * a.ldG;
-    for (int m = lane; m < nmTotal; m += 64)
-    {
-        MemberDesc const& md = T.member[mbase + m];
-        MemLDS o;
-        o.lambda = md.lambda;
-        o.zl = md.zl;
-        o.fderiv = md.fderiv;
-        o.fnorm = md.fnorm;
-        o.eta = md.eta;
-        o.rs = md.rs;
-        o.zeta = md.zeta;
-        o.w = dEdGi[md.sf];
-        o.zint = md.zint;
-        o.useint = md.useint;
-        o.calcexp = md.calcexp;
-        o.pad = 0;
-        M[m] = o;
-    }
-    // Angular group chunks per (e_j, e_k) and per-pair filter radii.
-    for (int pq = lane; pq < ne * ne; pq += 64)
-    {
-        int const ej = pq / ne, ek = pq % ne;
-        int const pb = (ec * ne + ej) * ne + ek;
-        int const pc = T.pairCount[pb];
-        double rmax = 0.0, rjk2 = 0.0;
-        for (int t = 0; t < pc; ++t)
-        {
-            GroupDesc const& g = T.group[T.pairGroups[pb * kMaxPairGroups + t]];
-            GrpLDS o;
-            o.rc = g.rc;
-            o.rc2 = g.rc2;
-            o.type = g.type;
-            o.cfg = g.cfg;
-            o.nmem = g.nmem;
-            o.mem = g.mem0 - mbase;
-            GR[pq * kMaxPairGroups + t] = o;
-            rmax = fmax(rmax, g.rc);
-            rjk2 = fmax(rjk2, g.type == 9 ? 1.0e300 : g.rc2);
-        }
-        PR[pq] = rmax;      // 0 if no angular group for this pair
-        PJ[pq] = rjk2;
-    }
-    __syncthreads();
 
-    double sx = 0.0, sy = 0.0, sz = 0.0;     // this lane's part of cself
-    for (int jr = 0; jr < n; jr += 64)
-    {
-        int const j = jr + lane;
-        bool const act = j < n;
-        int const jj = act ? j : 0;
-        double const xj = X[jj], yj = Y[jj], zj = Z[jj], dj = D[jj];
-        int const ej = E[jj];
-        double cx = 0.0, cy = 0.0, cz = 0.0;  // slot-j contribution
+    stage(a, S, s0, n, sStart, sCnt);
 
-        // Radial groups (SymGrpExpRad.cpp:194-233): uniform group loop.
-        for (int gi = ed.g0; gi < ed.g0 + ed.nrad; ++gi)
+    double sx = 0.0, sy = 0.0, sz = 0.0;    // this lane's share of cself
+    double rcCached = -1.0;
+    for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        if (g.rc != rcCached)
         {
-            GroupDesc const g = T.group[gi];
-            if (act && g.e1 == ej && dj < g.rc)
+            for (int t = lane; t < n; t += 64)
             {
-                double const fc = FC[g.cfg * cap + jj];
-                double const dfc = DFC[g.cfg * cap + jj];
+                double f, df;
+                cutoff_fdf_t<CT>(cc, S.D[t], f, df);
+                S.FC[t] = f;
+                S.DFC[t] = df;
+            }
+            rcCached = g.rc;
+            __syncthreads();
+        }
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        int const nmem = g.nmem;
+        // Member parameters with this atom's dE/dG weights, in LDS.
+        if (lane < nmem)
+        {
+            MemberDesc const& md = mem[lane];
+            double const w = dEdGi[md.sf]
+                           * (g.type == 2 ? md.sfac : md.fnorm);
+            MP[lane] = make_mparam(md, w);
+        }
+        __syncthreads();
+
+        if (g.type == 2)
+        {
+            // SymGrpExpRad.cpp:194-233: p1 = sf (f' - 2 eta (r-rs) f) e / r.
+            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            for (int t = lane; t < na; t += 64)
+            {
+                int const j = a0 + t;
+                double const r = S.D[j];
+                if (!(r < g.rc)) continue;
+                double const fc = S.FC[j], dfc = S.DFC[j], inv = S.INV[j];
                 double P = 0.0;
-                for (int m = 0; m < g.nmem; ++m)
+                for (int m = 0; m < nmem; ++m)
                 {
-                    MemberDesc const& md = T.member[g.mem0 + m];
-                    double const pexp = exp(-md.eta * (dj - md.rs) * (dj - md.rs));
-                    double const p1 = md.sfac * (dfc - 2.0 * md.eta
-                                    * (dj - md.rs) * fc) * pexp / dj;
-                    P += M[g.mem0 - mbase + m].w * p1;
+                    double const eta = MP[m].eta, rs = MP[m].rs;
+                    double const pexp = exp(-eta * (r - rs) * (r - rs));
+                    P += MP[m].w * ((dfc - 2.0 * eta * (r - rs) * fc) * pexp
+                                    * inv);
                 }
-                sx += P * xj; sy += P * yj; sz += P * zj;
-                cx -= P * xj; cy -= P * yj; cz -= P * zj;
+                double const x = S.X[j], y = S.Y[j], z = S.Z[j];
+                sx += P * x; sy += P * y; sz += P * z;
+                S.ACC[3 * j] -= P * x;
+                S.ACC[3 * j + 1] -= P * y;
+                S.ACC[3 * j + 2] -= P * z;
             }
+            __syncthreads();
+            continue;
         }
 
-        // Angular: bitmask of partners k passing the pair filters.
-        for (int w = 0; w < nw; ++w)
+        bool const wide = (g.type == 9);
+        OrderedPairs op(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
+                        sCnt[g.e2]);
+        int const total = op.total;
+        if (total == 0) continue;
+        // Chunk of this lane: [q0, q1), at most 32 * kChunkWords long.
+        int const per = (total + 63) / 64;
+        int nChunks = 1;
+        if (per > 32 * kChunkWords) nChunks = (per + 32 * kChunkWords - 1)
+                                              / (32 * kChunkWords);
+        for (int ch = 0; ch < nChunks; ++ch)
         {
-            uint32_t bits = 0;
-            int const kmax = min(32, n - w * 32);
-            for (int b = 0; b < kmax; ++b)
+            int const span = (total + 64 * nChunks - 1) / (64 * nChunks);
+            int const q0 = min(total, (ch * 64 + lane) * span);
+            int const q1 = min(total, q0 + span);
+            int const len = q1 - q0;
+            int const nw = (span + 31) >> 5;
+            // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow).
+            int lastJ = -1;
+            bool firstOwned = true;
+            for (int wd = 0; wd < nw; ++wd)
             {
-                int const k = w * 32 + b;
-                int const pq = ej * ne + E[k];
-                double const rmax = PR[pq];
-                double const dx = X[k] - xj, dy = Y[k] - yj, dz = Z[k] - zj;
-                double const rjk2 = dx * dx + dy * dy + dz * dz;
-                bool const pass = (k != jj) && dj < rmax && D[k] < rmax
-                                && rjk2 < PJ[pq];
-                bits |= (pass ? 1u : 0u) << b;
+                uint32_t bits = 0;
+                for (int b = 0; b < 32; ++b)
+                {
+                    int const t = wd * 32 + b;
+                    if (t < len)
+                    {
+                        int j, k;
+                        bool first;
+                        op.decode(q0 + t, j, k, first);
+                        if (t == 0) firstOwned = first;
+                        bool pass = S.D[j] < g.rc && S.D[k] < g.rc;
+                        if (pass && !wide)
+                        {
+                            double const dx = S.X[k] - S.X[j];
+                            double const dy = S.Y[k] - S.Y[j];
+                            double const dz = S.Z[k] - S.Z[j];
+                            pass = dx * dx + dy * dy + dz * dz < g.rc2;
+                        }
+                        bits |= (pass ? 1u : 0u) << b;
+                    }
+                }
+                S.MASK[wd * 64 + lane] = bits;
             }
-            MASK[w * 64 + lane] = act ? bits : 0u;
-        }
-        if (act)
-        {
-            int w = 0;
-            uint32_t bits = nw > 0 ? MASK[lane] : 0u;
+            if (len > 0)
+            {
+                int j0, k0;
+                bool f0;
+                op.decode(q0, j0, k0, f0);
+                lastJ = j0;
+            }
+            S.HEADJ[lane] = -1;
+            bool headUsed = false;
+            bool curOwned = firstOwned;
+            int curJ = lastJ;
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            bool curHas = false;
+            int wd = 0;
+            uint32_t bits = nw > 0 ? S.MASK[lane] : 0u;
             for (;;)
             {
-                while (bits == 0u && w < nw)
+                while (bits == 0u && wd < nw)
                 {
-                    ++w;
-                    if (w < nw) bits = MASK[w * 64 + lane];
+                    ++wd;
+                    if (wd < nw) bits = S.MASK[wd * 64 + lane];
                 }
-                if (w >= nw) break;
+                if (wd >= nw) break;
                 int const b = __builtin_ctz(bits);
                 bits &= bits - 1u;
-                int const k = w * 32 + b;
-                int const ek = E[k];
-                int const pq = ej * ne + ek;
-                int const pb = (ec * ne + ej) * ne + ek;
-                int const pc = T.pairCount[pb];
-                double const xk = X[k], yk = Y[k], zk = Z[k], dk = D[k];
-                double const dx = xk - xj, dy = yk - yj, dz = zk - zj;
-                double const rjk2 = dx * dx + dy * dy + dz * dz;
-                for (int t = 0; t < pc; ++t)
+                int j, k;
+                bool first;
+                op.decode(q0 + wd * 32 + b, j, k, first);
+                if (j != curJ)
                 {
-                    GrpLDS const g = GR[pq * kMaxPairGroups + t];
-                    bool const wide = (g.type == 9);
-                    if (!(dj < g.rc && dk < g.rc && (wide || rjk2 < g.rc2)))
-                        continue;
-                    double const rij = dj, rik = dk;
-                    double const fcij = FC[g.cfg * cap + jj];
-                    double const dfcij = DFC[g.cfg * cap + jj];
-                    double const fcik = FC[g.cfg * cap + k];
-                    double const r2ij = rij * rij, r2ik = rik * rik;
-                    double pfc, r2sum, pr1, pr3 = 0.0, rjk = 0.0;
-                    if (!wide)
+                    if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
+                    curJ = j;
+                    curOwned = true;     // rows after the chunk start
+                    cx = cy = cz = 0.0;
+                    curHas = false;
+                }
+                // ---- triplet (j as the first neighbour, k second) ----
+                double const rij = S.D[j], rik = S.D[k];
+                double const invj = S.INV[j], invk = S.INV[k];
+                double const xj = S.X[j], yj = S.Y[j], zj = S.Z[j];
+                double const dx = S.X[k] - xj, dy = S.Y[k] - yj,
+                             dz = S.Z[k] - zj;
+                double const fcij = S.FC[j], dfcij = S.DFC[j];
+                double const fcik = S.FC[k];
+                double const rinvijik = invj * invk;
+                double const costijk = (xj * S.X[k] + yj * S.Y[k]
+                                        + zj * S.Z[k]) * rinvijik;
+                double const r2ij = rij * rij, r2ik = rik * rik;
+                double pfc, r2sum, pr1, pr3 = 0.0, rjk = 0.0, invjk = 0.0;
+                if (!wide)
+                {
+                    double const rjk2 = dx * dx + dy * dy + dz * dz;
+                    rjk = sqrt(rjk2);
+                    invjk = 1.0 / rjk;
+                    double fcjk, dfcjk;
+                    cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
+                    pfc = fcij * fcik * fcjk;
+                    r2sum = r2ij + r2ik + rjk2;
+                    pr1 = fcik * fcjk * dfcij * invj;
+                    pr3 = fcij * fcik * dfcjk * invjk;
+                }
+                else
+                {
+                    pfc = fcij * fcik;
+                    r2sum = r2ij + r2ik;
+                    pr1 = fcik * dfcij * invj;
+                }
+                double T1 = 0.0, T2 = 0.0, T3 = 0.0;
+                double vexp = 0.0, c1 = 0.0, c3 = 0.0;
+                for (int m = 0; m < nmem; ++m)
+                {
+                    int const flags = MP[m].flags;
+                    if (flags & 2)
                     {
-                        rjk = sqrt(rjk2);
-                        double fcjk, dfcjk;
-                        cutoff_fdf(T.cfg[ed.cfg0 + g.cfg], rjk, fcjk, dfcjk);
-                        pfc = fcij * fcik * fcjk;
-                        r2sum = r2ij + r2ik + rjk * rjk;
-                        pr1 = fcik * fcjk * dfcij / rij;
-                        pr3 = fcij * fcik * dfcjk / rjk;
-                    }
-                    else
-                    {
-                        pfc = fcij * fcik;
-                        r2sum = r2ij + r2ik;
-                        pr1 = fcik * dfcij / rij;
-                    }
-                    double const rinvijik = 1.0 / rij / rik;
-                    double const costijk = (xj * xk + yj * yk + zj * zk)
-                                         * rinvijik;
-                    double vexp = 0.0, rijs = 0.0, rjks = 0.0;
-                    double P1 = 0.0, P3 = 0.0;
-                    for (int m = 0; m < g.nmem; ++m)
-                    {
-                        MemLDS const mp = M[g.mem + m];
-                        if (mp.calcexp)
+                        double const eta = MP[m].eta, rs = MP[m].rs;
+                        double const te = 2.0 * eta * pfc;
+                        if (rs > 0.0)
                         {
-                            if (mp.rs > 0.0)
+                            double const rijs = rij - rs;
+                            double const riks = rik - rs;
+                            double s = rijs * rijs + riks * riks;
+                            c1 = pr1 - te * rijs * invj;
+                            if (!wide)
                             {
-                                rijs = rij - mp.rs;
-                                double const riks = rik - mp.rs;
-                                double s = rijs * rijs + riks * riks;
-                                if (!wide)
-                                {
-                                    rjks = rjk - mp.rs;
-                                    s += rjks * rjks;
-                                }
-                                vexp = exp(-mp.eta * s);
+                                double const rjks = rjk - rs;
+                                s += rjks * rjks;
+                                c3 = te * rjks * invjk - pr3;
                             }
-                            else vexp = exp(-mp.eta * r2sum);
-                        }
-                        double const plambda = 1.0 + mp.lambda * costijk;
-                        double fg = vexp;
-                        if (plambda <= 0.0) fg = 0.0;
-                        else if (mp.useint) fg *= pow_int(plambda, mp.zint - 1);
-                        else fg *= pow(plambda, mp.zeta - 1.0);
-                        fg *= mp.fnorm;
-                        double const pfczl = pfc * mp.zl;
-                        double const p2etapl = plambda * mp.fderiv;
-                        double p1, p3;
-                        if (mp.rs > 0.0)
-                        {
-                            p1 = fg * (pfczl * (rinvijik - costijk / r2ij
-                               - p2etapl * rijs / rij) + pr1 * plambda);
-                            p3 = wide ? fg * pfczl * rinvijik
-                                      : fg * (pfczl * (rinvijik + p2etapl
-                                        * rjks / rjk) - pr3 * plambda);
+                            vexp = exp(-eta * s);
                         }
                         else
                         {
-                            p1 = fg * (pfczl * (rinvijik - costijk / r2ij
-                               - p2etapl) + pr1 * plambda);
-                            p3 = wide ? fg * pfczl * rinvijik
-                                      : fg * (pfczl * (rinvijik + p2etapl)
-                                        - pr3 * plambda);
+                            c1 = pr1 - te;
+                            if (!wide) c3 = te - pr3;
+                            vexp = exp(-eta * r2sum);
                         }
-                        P1 += mp.w * p1;
-                        P3 += mp.w * p3;
                     }
-                    sx += P1 * xj; sy += P1 * yj; sz += P1 * zj;
-                    cx -= P1 * xj + P3 * dx;
-                    cy -= P1 * yj + P3 * dy;
-                    cz -= P1 * zj + P3 * dz;
+                    double const pl = 1.0 + MP[m].lambda * costijk;
+                    double fgw = 0.0;
+                    if (pl > 0.0)
+                    {
+                        double const pw = (!GENPOW || (flags & 1))
+                            ? pow_int(pl, MP[m].zint - 1)
+                            : pow(pl, MP[m].zeta - 1.0);
+                        fgw = MP[m].w * vexp * pw;
+                    }
+                    T1 += fgw * MP[m].zl;
+                    double const fp = fgw * pl;
+                    T2 += fp * c1;
+                    T3 += fp * c3;
                 }
+                double const A = rinvijik - costijk * invj * invj;
+                double const P1 = pfc * A * T1 + T2;
+                double const P3 = pfc * rinvijik * T1 + T3;
+                sx += P1 * xj; sy += P1 * yj; sz += P1 * zj;
+                cx -= P1 * xj + P3 * dx;
+                cy -= P1 * yj + P3 * dy;
+                cz -= P1 * zj + P3 * dz;
+                curHas = true;
             }
-            size_t const q = (size_t)(s0 + j);
-            a.cslot[3 * q] = cx;
-            a.cslot[3 * q + 1] = cy;
-            a.cslot[3 * q + 2] = cz;
+            if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
+            __syncthreads();
+            // Merge head partials in lane order: the first lane of each run
+            // of equal HEADJ sums the run and adds it once.
+            int const hj = S.HEADJ[lane];
+            if (hj >= 0 && (lane == 0 || S.HEADJ[lane - 1] != hj))
+            {
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+                for (int L = lane; L < 64 && S.HEADJ[L] == hj; ++L)
+                {
+                    hx += S.HEAD[3 * L];
+                    hy += S.HEAD[3 * L + 1];
+                    hz += S.HEAD[3 * L + 2];
+                }
+                S.ACC[3 * hj] += hx;
+                S.ACC[3 * hj + 1] += hy;
+                S.ACC[3 * hj + 2] += hz;
+            }
+            __syncthreads();
+            (void)headUsed;
         }
-        __syncthreads();
+    }
+    for (int t = lane; t < n; t += 64)
+    {
+        size_t const q = (size_t)(s0 + S.SLOT[t]);
+        a.cslot[3 * q] = S.ACC[3 * t];
+        a.cslot[3 * q + 1] = S.ACC[3 * t + 1];
+        a.cslot[3 * q + 2] = S.ACC[3 * t + 2];
     }
     sx = wave_sum(sx);
     sy = wave_sum(sy);
@@ -353,15 +550,16 @@ __global__ __launch_bounds__(256) void k_gather(
 
 size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf)
 {
+    (void)maxMembers;
+    (void)maxCfg;
     (void)maxnsf;
     size_t b = 0;
-    b += (size_t)cap * 4 * sizeof(double);
-    b += (size_t)cap * 2 * maxCfg * sizeof(double);
-    b += (size_t)maxMembers * sizeof(MemLDS);
-    b += (size_t)kMaxElements * kMaxElements * kMaxPairGroups * sizeof(GrpLDS);
-    b += (size_t)kMaxElements * kMaxElements * 2 * sizeof(double);
-    b += (size_t)cap * sizeof(int);
-    b += (size_t)((cap + 31) / 32) * 64 * sizeof(uint32_t);
+    b += (size_t)cap * 7 * sizeof(double);        // X Y Z D INV FC DFC
+    b += (size_t)cap * 3 * sizeof(double);        // ACC
+    b += (size_t)64 * 3 * sizeof(double);         // HEAD
+    b += (size_t)cap * sizeof(int);               // SLOT
+    b += (size_t)64 * sizeof(int);                // HEADJ
+    b += (size_t)kChunkWords * 64 * sizeof(uint32_t);   // MASK
     return b;
 }
 
@@ -370,7 +568,12 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
     if (a.n <= 0) return hipSuccess;
     size_t const shm = force_backward_lds_bytes(a.cap, a.maxMembers, a.maxCfg,
                                                 a.T.maxnsf);
-    k_force_backward<<<a.n, 64, shm, s>>>(a);
+    if (a.T.genpow)
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
+                              (k_force_backward<kCT, true><<<a.n, 64, shm, s>>>(a)));
+    else
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
+                              (k_force_backward<kCT, false><<<a.n, 64, shm, s>>>(a)));
     return hipGetLastError();
 }
 

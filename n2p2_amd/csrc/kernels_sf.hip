@@ -7,12 +7,13 @@
 //   1. the atom's neighbours are staged in LDS, partitioned by element
 //      (stable ballot compaction), so every group reads exactly the
 //      neighbour blocks it needs;
-//   2. groups run one after another; a group's member parameters are
-//      wave-uniform (scalar loads) and its accumulators live in registers;
+//   2. groups run one after another; a group's member parameters are staged
+//      in LDS and read with wave-uniform addresses, its member sums live in
+//      registers;
 //   3. angular groups enumerate the unordered pairs of their blocks, test
 //      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
 //      with a ballot prefix; whenever 64 are queued every lane evaluates
-//      one, so the expensive triplet work runs without divergence.
+//      one, so the expensive triplet work runs without divergence;
 //   4. member sums are reduced with a fixed butterfly (deterministic).
 #include "device_common.h"
 #include "kernels.h"
@@ -29,13 +30,10 @@ struct Stage
     double* Y;
     double* Z;
     double* D;
+    double* INV;
     double* FC;
 };
 
-// Unordered-pair decoding.  Different elements: p -> (p / nb, p % nb).
-// Same element (n atoms): circle pairing, p -> {r, r + 1 + c mod n} for
-// p < n*h (h = (n-1)/2), and {r, r + n/2} for the n/2 remaining pairs when
-// n is even; every unordered pair appears exactly once.
 __device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
                                              int& r)
 {
@@ -43,66 +41,6 @@ __device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
     r = p - q * d;
     if (r < 0) { --q; r += d; }
     else if (r >= d) { ++q; r -= d; }
-}
-
-template <int MAXM>
-__device__ __forceinline__ void triplet_forward(
-    Stage const& S, int ja, int kb, bool wide, CutCfg const& cc,
-    MemberDesc const* __restrict__ mem, int nmem, double (&acc)[MAXM])
-{
-    double const rij = S.D[ja], rik = S.D[kb];
-    double const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
-    double const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
-    double const fcij = S.FC[ja], fcik = S.FC[kb];
-    double const r2ij = rij * rij, r2ik = rik * rik;
-    double pfc, r2sum, rjk = 0.0;
-    if (!wide)
-    {
-        double const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
-        double const rjk2 = dx * dx + dy * dy + dz * dz;
-        rjk = sqrt(rjk2);
-        double fcjk, dfcjk;
-        cutoff_fdf(cc, rjk, fcjk, dfcjk);
-        pfc = fcij * fcik * fcjk;
-        r2sum = r2ij + r2ik + rjk * rjk;
-    }
-    else
-    {
-        pfc = fcij * fcik;
-        r2sum = r2ij + r2ik;
-    }
-    double const rinvijik = 1.0 / rij / rik;
-    double const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
-    double vexp = 0.0;
-#pragma unroll
-    for (int m = 0; m < MAXM; ++m)
-    {
-        if (m < nmem)
-        {
-            MemberDesc const& md = mem[m];
-            if (md.calcexp)
-            {
-                if (md.rs > 0.0)
-                {
-                    double const a = rij - md.rs, b = rik - md.rs;
-                    double s = a * a + b * b;
-                    if (!wide)
-                    {
-                        double const c = rjk - md.rs;
-                        s += c * c;
-                    }
-                    vexp = exp(-md.eta * s);
-                }
-                else vexp = exp(-md.eta * r2sum);
-            }
-            double const plambda = 1.0 + md.lambda * costijk;
-            double fg = vexp;
-            if (plambda <= 0.0) fg = 0.0;
-            else if (md.useint) fg *= pow_int(plambda, md.zint - 1);
-            else fg *= pow(plambda, md.zeta - 1.0);
-            acc[m] += fg * plambda * pfc;
-        }
-    }
 }
 
 // Stages the atom's neighbours in LDS partitioned by element (stable ballot
@@ -146,7 +84,7 @@ __device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
     {
         int const s = base + lane;
         int e = -1;
-        double4 g = make_double4(0, 0, 0, 0);
+        double4 g = make_double4(0, 0, 0, 1);
         if (s < n)
         {
             e = a.elem[a.nbrJ[s0 + s]];
@@ -165,6 +103,7 @@ __device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
                     S.Y[pos] = g.y;
                     S.Z[pos] = g.z;
                     S.D[pos] = g.w;
+                    if (S.INV) S.INV[pos] = 1.0 / g.w;
                 }
                 start[q] += __popcll(mask);
             }
@@ -174,6 +113,10 @@ __device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
 }
 
 // Unordered pairs of a group's neighbour blocks, indexed 0..npairs-1.
+// Different elements: p -> (p / nb, p % nb).  Same element (n atoms):
+// circle pairing, p -> {r, r + 1 + c mod n} for p < n*h (h = (n-1)/2), and
+// {r, r + n/2} for the n/2 remaining pairs when n is even; every unordered
+// pair appears exactly once.
 struct PairCursor
 {
     bool same;
@@ -230,18 +173,86 @@ __device__ __forceinline__ bool pair_passes(Stage const& S, int ja, int kb,
     return (dx * dx + dy * dy + dz * dz) < g.rc2;
 }
 
-template <int MAXM>
+// One triplet, all members of the group: acc[m] += fg * plambda * pfc with
+// fg = exp(-eta sum r^2) plambda^(zeta - 1) (SymGrpExpAngn.cpp:276-297).
+template <int MAXM, int CT, bool GENPOW>
+__device__ __forceinline__ void triplet_forward(
+    Stage const& S, int ja, int kb, bool wide, CutCfg const& cc,
+    MParam const* MP, int nmem, double (&acc)[MAXM])
+{
+    double const rij = S.D[ja], rik = S.D[kb];
+    double const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
+    double const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
+    double const fcij = S.FC[ja], fcik = S.FC[kb];
+    double pfc, r2sum, rjk = 0.0;
+    if (!wide)
+    {
+        double const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+        double const rjk2 = dx * dx + dy * dy + dz * dz;
+        rjk = sqrt(rjk2);
+        double fcjk, dfcjk;
+        cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
+        pfc = fcij * fcik * fcjk;
+        r2sum = rij * rij + rik * rik + rjk2;
+    }
+    else
+    {
+        pfc = fcij * fcik;
+        r2sum = rij * rij + rik * rik;
+    }
+    double const rinvijik = S.INV[ja] * S.INV[kb];
+    double const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
+    double vexp = 0.0;
+#pragma unroll
+    for (int m = 0; m < MAXM; ++m)
+    {
+        if (m < nmem)
+        {
+            int const flags = MP[m].flags;
+            if (flags & 2)
+            {
+                double const eta = MP[m].eta, rs = MP[m].rs;
+                if (rs > 0.0)
+                {
+                    double const a = rij - rs, b = rik - rs;
+                    double s = a * a + b * b;
+                    if (!wide)
+                    {
+                        double const c = rjk - rs;
+                        s += c * c;
+                    }
+                    vexp = exp(-eta * s);
+                }
+                else vexp = exp(-eta * r2sum);
+            }
+            double const plambda = 1.0 + MP[m].lambda * costijk;
+            double fg = 0.0;
+            if (plambda > 0.0)
+            {
+                if (!GENPOW || (flags & 1))
+                    fg = vexp * pow_int(plambda, MP[m].zint - 1);
+                else
+                    fg = vexp * pow(plambda, MP[m].zeta - 1.0);
+            }
+            acc[m] += fg * plambda * pfc;
+        }
+    }
+}
+
+template <int MAXM, int CT, bool GENPOW>
 __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
 {
     extern __shared__ double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    __shared__ MParam MP[kMaxMembers];
     int const cap = a.cap;
     Stage S;
     S.X = sm;
     S.Y = S.X + cap;
     S.Z = S.Y + cap;
     S.D = S.Z + cap;
-    S.FC = S.D + cap;
+    S.INV = S.D + cap;
+    S.FC = S.INV + cap;
     int* Q = (int*)(S.FC + cap);
 
     int const i = blockIdx.x;
@@ -260,18 +271,20 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
     {
         GroupDesc const g = T.group[gi];
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        int const nmem = g.nmem;
+        if (lane < nmem) MP[lane] = make_mparam(mem[lane], 0.0);
         if (g.rc != rcCached)
         {
             for (int t = lane; t < n; t += 64)
             {
                 double f, df;
-                cutoff_fdf(cc, S.D[t], f, df);
+                cutoff_fdf_t<CT>(cc, S.D[t], f, df);
                 S.FC[t] = f;
             }
             rcCached = g.rc;
-            __syncthreads();
         }
-        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        __syncthreads();
         double acc[MAXM];
 #pragma unroll
         for (int m = 0; m < MAXM; ++m) acc[m] = 0.0;
@@ -287,9 +300,9 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
                     double const fc = S.FC[a0 + t];
 #pragma unroll
                     for (int m = 0; m < MAXM; ++m)
-                        if (m < g.nmem)
+                        if (m < nmem)
                         {
-                            double const eta = mem[m].eta, rs = mem[m].rs;
+                            double const eta = MP[m].eta, rs = MP[m].rs;
                             acc[m] += exp(-eta * (r - rs) * (r - rs)) * fc;
                         }
                 }
@@ -298,9 +311,8 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
         else
         {
             bool const wide = (g.type == 9);
-            int const a0 = sStart[g.e1], na = sCnt[g.e1];
-            int const b0 = sStart[g.e2], nb = sCnt[g.e2];
-            PairCursor pc(g.e1 == g.e2, a0, na, b0, nb);
+            PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                          sStart[g.e2], sCnt[g.e2]);
             int qn = 0;
             for (int base = 0; base < pc.npairs; base += 64)
             {
@@ -315,8 +327,8 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
                 if (qn >= 64)
                 {
                     int const e = Q[qn - 64 + lane];
-                    triplet_forward<MAXM>(S, e & 0xffff, e >> 16, wide, cc,
-                                          mem, g.nmem, acc);
+                    triplet_forward<MAXM, CT, GENPOW>(S, e & 0xffff, e >> 16,
+                                                      wide, cc, MP, nmem, acc);
                     qn -= 64;
                     __syncthreads();
                 }
@@ -324,25 +336,26 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             if (lane < qn)
             {
                 int const e = Q[lane];
-                triplet_forward<MAXM>(S, e & 0xffff, e >> 16, wide, cc, mem,
-                                      g.nmem, acc);
+                triplet_forward<MAXM, CT, GENPOW>(S, e & 0xffff, e >> 16, wide,
+                                                  cc, MP, nmem, acc);
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int m = 0; m < MAXM; ++m)
         {
-            if (m < g.nmem)
+            if (m < nmem)
             {
                 double const v = wave_sum(acc[m]);
                 if (lane == m)
                 {
-                    double const raw = (g.type == 2) ? v : v * mem[m].gfac;
-                    int const sf = mem[m].sf;
-                    Gi[sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + sf], raw);
+                    MemberDesc const& md = mem[m];
+                    double const raw = (g.type == 2) ? v : v * md.gfac;
+                    Gi[md.sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + md.sf],
+                                                       raw);
                 }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -367,7 +380,8 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     S.Y = S.X + cap;
     S.Z = S.Y + cap;
     S.D = S.Z + cap;
-    S.FC = S.D + cap;
+    S.INV = nullptr;
+    S.FC = nullptr;
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
     ModelTables const& T = a.T;
@@ -419,7 +433,8 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     {
         c[0] = n;
         unsigned long long t = ed.nsf;
-        for (int q = 0; q < T.ne; ++q) t += (unsigned long long)sCnt[q] * ed.nsfd[q];
+        for (int q = 0; q < T.ne; ++q)
+            t += (unsigned long long)sCnt[q] * ed.nsfd[q];
         c[9] = t;
     }
 #pragma unroll
@@ -431,13 +446,12 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     }
 }
 
-
 // Per SF of one element: min, max, sum, sum of squares over its atoms.  One
 // block per SF, fixed-order tree reduction.
 __global__ __launch_bounds__(256) void k_sf_stats(const double* __restrict__ G,
                                                   int ldG,
                                                   const int* __restrict__ atoms,
-                                                  int na, int nsf,
+                                                  int na,
                                                   double* __restrict__ out)
 {
     __shared__ double smin[256], smax[256], ssum[256], ssq[256];
@@ -472,7 +486,6 @@ __global__ __launch_bounds__(256) void k_sf_stats(const double* __restrict__ G,
         out[4 * k + 2] = ssum[0];
         out[4 * k + 3] = ssq[0];
     }
-    (void)nsf;
 }
 
 // Element::updateSymmetryFunctionStatistics (Element.cpp:462-541): count
@@ -503,17 +516,30 @@ __global__ void k_ew(ModelTables T, const double* __restrict__ G, int ldG,
     ew[i] = c;
 }
 
+template <int MAXM, int CT>
+void launch_fwd(SFArgs const& a, hipStream_t s)
+{
+    size_t const shm = sf_forward_lds_bytes(a.cap);
+    if (a.T.genpow)
+        k_sf_forward<MAXM, CT, true><<<a.n, 64, shm, s>>>(a);
+    else
+        k_sf_forward<MAXM, CT, false><<<a.n, 64, shm, s>>>(a);
+}
+
 }  // namespace
 
 size_t sf_forward_lds_bytes(int cap)
 {
-    return (size_t)cap * 5 * sizeof(double) + kQueue * sizeof(int);
+    return (size_t)cap * 6 * sizeof(double) + kQueue * sizeof(int);
 }
 
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    k_sf_forward<kMaxMembers><<<a.n, 64, sf_forward_lds_bytes(a.cap), s>>>(a);
+    if (a.T.maxGroupMembers <= 8)
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType, (launch_fwd<8, kCT>(a, s)));
+    else
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType, (launch_fwd<kMaxMembers, kCT>(a, s)));
     return hipGetLastError();
 }
 
@@ -529,7 +555,7 @@ hipError_t launch_sf_stats(const double* G, int ldG, const int* atoms, int na,
                            int nsf, double* out, hipStream_t s)
 {
     if (nsf <= 0) return hipSuccess;
-    k_sf_stats<<<nsf, 256, 0, s>>>(G, ldG, atoms, na, nsf, out);
+    k_sf_stats<<<nsf, 256, 0, s>>>(G, ldG, atoms, na, out);
     return hipGetLastError();
 }
 

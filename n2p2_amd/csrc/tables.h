@@ -96,6 +96,9 @@ struct ModelTables
 {
     int ne;
     int maxnsf;
+    int cutType;          // cutoff_type of the settings (global in 2G)
+    int genpow;           // some angular member has a non-integer zeta
+    int maxGroupMembers;  // largest group chunk
     double maxCutoff;
     const ElemDesc*   elem;
     const CutCfg*     cfg;
