@@ -177,6 +177,14 @@ int hdnnp_get_dEdG(hdnnp_ctx* c, double* dEdG, int ld);
 int hdnnp_get_atomic_energies(hdnnp_ctx* c, double* e);
 /* Atom::f (n x 3). */
 int hdnnp_get_forces(hdnnp_ctx* c, double* f);
+/* Per-atom self force -sum_s dE_i/dG_s dG_s/dr_i (n x 3) and per
+ * neighbour-list entry pair force -sum_s dE_i/dG_s dG_s/dr_(slot) (M x 3),
+ * i.e. Atom::calculateSelfForceShort and Atom::calculatePairForceShort for
+ * every slot (Atom.cpp:371-401), after hdnnp_forces.  Either pointer may
+ * be NULL.  These are what a caller scattering forces to ghost atoms needs
+ * (InterfaceLammps::getForces, InterfaceLammps.cpp:546-581). */
+int hdnnp_get_pair_forces(hdnnp_ctx* c, double* self_forces,
+                          double* pair_forces);
 /* Device pointer of the n x 3 force array (valid until the next
  * hdnnp_set_structure); for callers that keep MD state on the GPU. */
 const double* hdnnp_forces_device(hdnnp_ctx* c);

@@ -972,6 +972,27 @@ int hdnnp_get_forces(hdnnp_ctx* c, double* f)
 
 const double* hdnnp_forces_device(hdnnp_ctx* c) { return c->dF.p; }
 
+int hdnnp_get_pair_forces(hdnnp_ctx* c, double* self_forces,
+                          double* pair_forces)
+{
+    return guarded([&]() {
+        if (!c->dF.p || !c->haveNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
+        c->setDevice();
+        copyOut(c, self_forces, c->dCself.p, 3 * (size_t)c->n);
+        copyOut(c, pair_forces, c->dCslot.p, 3 * (size_t)c->M);
+        c->sync();
+        // Stored as +sum dE/dG dG/dr; the reference's self and pair forces
+        // carry the minus sign.
+        if (self_forces)
+            for (size_t k = 0; k < 3 * (size_t)c->n; ++k)
+                self_forces[k] = -self_forces[k];
+        if (pair_forces)
+            for (size_t k = 0; k < 3 * (size_t)c->M; ++k)
+                pair_forces[k] = -pair_forces[k];
+    });
+}
+
 int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom)
 {
     int64_t total = -1;

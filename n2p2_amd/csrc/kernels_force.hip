@@ -42,7 +42,13 @@ namespace hdnnp {
 
 namespace {
 
-constexpr int kChunkWords = 8;   // pass bits per lane chunk: 8 x 32 pairs
+// Pass-bit words per lane: a group has fewer than cap^2 ordered pairs, so a
+// lane's chunk holds at most ceil(cap^2 / 64) of them.
+__host__ __device__ inline int mask_words(int cap)
+{
+    int const span = (cap * cap + 63) / 64;
+    return (span + 31) / 32;
+}
 
 struct BLds
 {
@@ -156,10 +162,11 @@ struct OrderedPairs
             invB = na > 0 ? 1.0f / (float)na : 0.0f;   // rows of B
         }
     }
-    // j, k as staged indices; first = q is the first pair of row j.
-    __device__ void decode(int q, int& j, int& k, bool& first) const
+    // j, k as staged indices; c = position of q in row j (q - c is the
+    // row's first pair); len = the row's length.
+    __device__ void decode(int q, int& j, int& k, int& c, int& len) const
     {
-        int r, c;
+        int r;
         if (same)
         {
             int const d = na - 1;
@@ -168,7 +175,7 @@ struct OrderedPairs
             if (c < 0) { --r; c += d; } else if (c >= d) { ++r; c -= d; }
             j = a0 + r;
             k = a0 + (c < r ? c : c + 1);
-            first = (c == 0);
+            len = d;
         }
         else if (q < split)
         {
@@ -177,7 +184,7 @@ struct OrderedPairs
             if (c < 0) { --r; c += nb; } else if (c >= nb) { ++r; c -= nb; }
             j = a0 + r;
             k = b0 + c;
-            first = (c == 0);
+            len = nb;
         }
         else
         {
@@ -187,7 +194,7 @@ struct OrderedPairs
             if (c < 0) { --r; c += na; } else if (c >= na) { ++r; c -= na; }
             j = b0 + r;
             k = a0 + c;
-            first = (c == 0);
+            len = na;
         }
     }
 };
@@ -308,21 +315,15 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                         sCnt[g.e2]);
         int const total = op.total;
         if (total == 0) continue;
-        // Chunk of this lane: [q0, q1), at most 32 * kChunkWords long.
-        int const per = (total + 63) / 64;
-        int nChunks = 1;
-        if (per > 32 * kChunkWords) nChunks = (per + 32 * kChunkWords - 1)
-                                              / (32 * kChunkWords);
-        for (int ch = 0; ch < nChunks; ++ch)
+        // Chunk of this lane: [q0, q1); the mask buffer holds a.maskWords
+        // words per lane, enough for the largest possible chunk.
         {
-            int const span = (total + 64 * nChunks - 1) / (64 * nChunks);
-            int const q0 = min(total, (ch * 64 + lane) * span);
+            int const span = (total + 63) / 64;
+            int const q0 = min(total, lane * span);
             int const q1 = min(total, q0 + span);
             int const len = q1 - q0;
             int const nw = (span + 31) >> 5;
             // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow).
-            int lastJ = -1;
-            bool firstOwned = true;
             for (int wd = 0; wd < nw; ++wd)
             {
                 uint32_t bits = 0;
@@ -331,10 +332,8 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                     int const t = wd * 32 + b;
                     if (t < len)
                     {
-                        int j, k;
-                        bool first;
-                        op.decode(q0 + t, j, k, first);
-                        if (t == 0) firstOwned = first;
+                        int j, k, c, rl;
+                        op.decode(q0 + t, j, k, c, rl);
                         bool pass = S.D[j] < g.rc && S.D[k] < g.rc;
                         if (pass && !wide)
                         {
@@ -348,17 +347,29 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 }
                 S.MASK[wd * 64 + lane] = bits;
             }
+            // Row of the chunk's first pair; owned if it starts here.
+            int curJ = -1;
+            bool curOwned = true;
+            // Row of the chunk's last pair, if it starts in this chunk and
+            // runs past it: this lane merges its continuation.
+            int tailJ = -1, tailEnd = 0;
             if (len > 0)
             {
-                int j0, k0;
-                bool f0;
-                op.decode(q0, j0, k0, f0);
-                lastJ = j0;
+                int j0, k0, c0, l0;
+                op.decode(q0, j0, k0, c0, l0);
+                curJ = j0;
+                curOwned = (c0 == 0);
+                int jl, kl, cl, ll;
+                op.decode(q1 - 1, jl, kl, cl, ll);
+                int const rowStart = q1 - 1 - cl;
+                if (rowStart >= q0 && rowStart + ll > q1)
+                {
+                    tailJ = jl;
+                    tailEnd = rowStart + ll;
+                }
             }
             S.HEADJ[lane] = -1;
             bool headUsed = false;
-            bool curOwned = firstOwned;
-            int curJ = lastJ;
             double cx = 0.0, cy = 0.0, cz = 0.0;
             bool curHas = false;
             int wd = 0;
@@ -373,9 +384,8 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 if (wd >= nw) break;
                 int const b = __builtin_ctz(bits);
                 bits &= bits - 1u;
-                int j, k;
-                bool first;
-                op.decode(q0 + wd * 32 + b, j, k, first);
+                int j, k, cpos, rlen;
+                op.decode(q0 + wd * 32 + b, j, k, cpos, rlen);
                 if (j != curJ)
                 {
                     if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
@@ -470,21 +480,23 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             }
             if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
             __syncthreads();
-            // Merge head partials in lane order: the first lane of each run
-            // of equal HEADJ sums the run and adds it once.
-            int const hj = S.HEADJ[lane];
-            if (hj >= 0 && (lane == 0 || S.HEADJ[lane - 1] != hj))
+            // The lane owning the start of a row that runs past its chunk
+            // adds the head partials of the following chunks of that row,
+            // in lane order; one writer per row, chunks without passing
+            // pairs (HEADJ = -1) are skipped.
+            if (tailJ >= 0)
             {
                 double hx = 0.0, hy = 0.0, hz = 0.0;
-                for (int L = lane; L < 64 && S.HEADJ[L] == hj; ++L)
-                {
-                    hx += S.HEAD[3 * L];
-                    hy += S.HEAD[3 * L + 1];
-                    hz += S.HEAD[3 * L + 2];
-                }
-                S.ACC[3 * hj] += hx;
-                S.ACC[3 * hj + 1] += hy;
-                S.ACC[3 * hj + 2] += hz;
+                for (int L = lane + 1; L < 64 && L * span < tailEnd; ++L)
+                    if (S.HEADJ[L] == tailJ)
+                    {
+                        hx += S.HEAD[3 * L];
+                        hy += S.HEAD[3 * L + 1];
+                        hz += S.HEAD[3 * L + 2];
+                    }
+                S.ACC[3 * tailJ] += hx;
+                S.ACC[3 * tailJ + 1] += hy;
+                S.ACC[3 * tailJ + 2] += hz;
             }
             __syncthreads();
             (void)headUsed;
@@ -559,7 +571,7 @@ size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf)
     b += (size_t)64 * 3 * sizeof(double);         // HEAD
     b += (size_t)cap * sizeof(int);               // SLOT
     b += (size_t)64 * sizeof(int);                // HEADJ
-    b += (size_t)kChunkWords * 64 * sizeof(uint32_t);   // MASK
+    b += (size_t)mask_words(cap) * 64 * sizeof(uint32_t);   // MASK
     return b;
 }
 

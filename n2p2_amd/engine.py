@@ -77,6 +77,7 @@ _SIGNATURES = [
     ("hdnnp_get_atomic_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("hdnnp_get_forces", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("hdnnp_forces_device", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("hdnnp_get_pair_forces", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -290,6 +291,15 @@ class Context:
         out = np.zeros((self.n, 3))
         _check(_lib.hdnnp_get_forces(self._h, _ptr(out, ctypes.c_double)))
         return out
+
+    def pair_forces(self):
+        """(self forces n x 3, pair forces M x 3) as the reference's
+        Atom::calculateSelfForceShort / calculatePairForceShort."""
+        sf = np.zeros((self.n, 3))
+        pf = np.zeros((self.num_neighbor_entries(), 3))
+        _check(_lib.hdnnp_get_pair_forces(self._h, _ptr(sf, ctypes.c_double),
+                                          _ptr(pf, ctypes.c_double)))
+        return sf, pf
 
     def forces_device_ptr(self):
         return _lib.hdnnp_forces_device(self._h)

@@ -1842,6 +1842,35 @@ int orc_get_dGdr_neighbor(const orc_struct* s, int64_t i, int64_t k,
     return (int)(n.dGdr.size() / 3);
 }
 
+// Atom.cpp:371-401 for every atom and every neighbour slot.
+int orc_get_pair_forces(const orc_struct* s, const orc_model* m,
+                        double* self_forces, double* pair_forces)
+{
+    size_t p = 0;
+    for (size_t i = 0; i < s->atoms.size(); ++i)
+    {
+        Atom const& a = s->atoms[i];
+        double sf[3] = {0, 0, 0};
+        for (size_t k = 0; k < a.G.size(); ++k)
+            for (int c = 0; c < 3; ++c) sf[c] -= a.dEdG[k] * a.dGdr[3 * k + c];
+        if (self_forces)
+            for (int c = 0; c < 3; ++c) self_forces[3 * i + c] = sf[c];
+        for (auto const& n : a.neighbors)
+        {
+            auto const& table = m->el[a.element].table[n.element];
+            double pf[3] = {0, 0, 0};
+            size_t const nd = n.dGdr.size() / 3;
+            for (size_t t = 0; t < nd; ++t)
+                for (int c = 0; c < 3; ++c)
+                    pf[c] -= a.dEdG[table[t]] * n.dGdr[3 * t + c];
+            if (pair_forces)
+                for (int c = 0; c < 3; ++c) pair_forces[3 * p + c] = pf[c];
+            ++p;
+        }
+    }
+    return 0;
+}
+
 double orc_evaluate(orc_struct* s, orc_model* m, int cells, int force_mode)
 {
     if (cells) neighborListCells(*s, m->maxCutoffRadius);

@@ -94,6 +94,13 @@ int  orc_get_dGdr_self(const orc_struct* s, int64_t i, double* out);
 int  orc_get_dGdr_neighbor(const orc_struct* s, int64_t i, int64_t k,
                            double* out);
 
+/* Atom::calculateSelfForceShort per atom (n x 3) and
+ * Atom::calculatePairForceShort of every neighbour slot of every atom
+ * (M x 3, CSR order), after orc_symmetry_functions(derivatives=1) and
+ * orc_networks(derivatives=1). */
+int orc_get_pair_forces(const orc_struct* s, const orc_model* m,
+                        double* self_forces, double* pair_forces);
+
 /* Prediction::readStructureFromFile + predict in one call: full pipeline in
  * physical units with energy offsets.  Writes energy and forces (n x 3). */
 int orc_predict_file(orc_model* m, const char* path, double* energy,

@@ -52,6 +52,7 @@ _SIG = [
     ("orc_get_forces", None, [_V, _D]),
     ("orc_get_dGdr_self", ctypes.c_int, [_V, ctypes.c_int64, _D]),
     ("orc_get_dGdr_neighbor", ctypes.c_int, [_V, ctypes.c_int64, ctypes.c_int64, _D]),
+    ("orc_get_pair_forces", ctypes.c_int, [_V, _V, _D, _D]),
     ("orc_predict_file", ctypes.c_int, [_V, ctypes.c_char_p, _D, _D, ctypes.c_int64, _I64]),
     ("orc_evaluate", ctypes.c_double, [_V, _V, ctypes.c_int, ctypes.c_int]),
     ("orc_energy_only", ctypes.c_double, [_V, _V, ctypes.c_int]),
@@ -251,6 +252,13 @@ class OStruct:
         out = np.zeros((self.n, 3))
         lib().orc_get_forces(self.h, _p(out))
         return out
+
+    def pair_forces(self):
+        off, _, _, _ = self.neighbors()
+        sf = np.zeros((self.n, 3))
+        pf = np.zeros((int(off[-1]), 3))
+        lib().orc_get_pair_forces(self.h, self.model.h, _p(sf), _p(pf))
+        return sf, pf
 
     def dGdr_self(self, i):
         out = np.zeros((self.maxnsf, 3))

@@ -123,6 +123,46 @@ def _run_both(ctx, s, force_mode=0):
     return e, oe
 
 
+def _pair_force_check(ctx, s, r, e):
+    sf, pf = ctx.pair_forces()
+    osf, opf = s.pair_forces()
+    off, idx, _, _ = s.neighbors()
+    scale = max(np.abs(opf).max(), 1e-300)
+    bad = np.nonzero(np.abs(pf - opf).max(1) > 1e-12 * scale)[0]
+    if len(bad):
+        q = bad[0]
+        i = np.searchsorted(off, q, side="right") - 1
+        ne = [int(np.sum(e[idx[off[i]:off[i + 1]]] == k)) for k in range(e.max() + 1)]
+        msg = ("%d bad slots of %d; first: atom %d (element %d, %d neighbours, "
+               "per element %s), slot %d -> atom %d (element %d): gpu %s oracle %s"
+               % (len(bad), len(pf), i, e[i], off[i + 1] - off[i], ne, q - off[i],
+                  idx[q], e[idx[q]], pf[q], opf[q]))
+        raise AssertionError(msg)
+    assert np.abs(sf - osf).max() <= 1e-12 * max(np.abs(osf).max(), 1e-300)
+
+
+def test_pair_forces_per_slot(workdir):
+    """Atom::calculatePairForceShort of every slot and the self forces."""
+    cases_ = []
+    d = os.path.join(cases.EXAMPLES, "Cu2S_PBE")
+    p = Prediction(d)
+    p.setup()
+    p.read_structure_from_file()
+    r, box = nio.to_normalized(p.r_phys, p.box_phys, p.conv_length)
+    cases_.append(("Cu2S", d, r, p.elem, box))
+    dw, _, _, _ = cases.water_case(workdir)
+    r2, e2, b2 = W.water(8192, 62.1, 99)
+    cases_.append(("water24576", dw, r2, e2, b2))
+    for name, dd, r, e, box in cases_:
+        say("pair forces", name)
+        m, ctx = _engine(dd)
+        ctx.set_structure(r, e, box)
+        ctx.evaluate(0)
+        s = OStruct(OModel.from_dir(dd), r, e, box)
+        s.evaluate(cells=True)
+        _pair_force_check(ctx, s, r, np.asarray(e))
+
+
 def test_water192_parity(workdir):
     d, r, e, box = cases.water_case(workdir)
     m, ctx = _engine(d)
