@@ -115,7 +115,7 @@ def main():
         ctx.set_structure(r, elem, box)
         ctx.calculate_neighbor_list(model.max_cutoff)
         counts = [int(x) for x in ctx.work_counts()]
-        log("work counts", counts.tolist())
+        log("work counts", counts)
 
         for _ in range(args.warmup):
             ctx.evaluate(engine.FORCES_REFERENCE)
