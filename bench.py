@@ -156,12 +156,23 @@ def main():
             flops, ms = None, kt[dom]
         if flops is not None:
             achieved = flops / (ms * 1e-3) / 1e12
-            roof = {"kernel": {"force": "k_force_backward",
-                               "symfunc": "k_sf_forward",
-                               "network": "k_nn"}[dom],
+            kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
+                     "network": "k_nn"}[dom]
+            traffic = None
+            tp = os.path.join(ROOT, "profiles", "traffic.json")
+            if os.path.exists(tp) and args.nmol == 65536:
+                with open(tp) as f:
+                    tj = json.load(f)
+                kt_ = tj.get("kernels", {}).get(kname)
+                if kt_:
+                    traffic = kt_["read_bytes"] + kt_["write_bytes"]
+            roof = {"kernel": kname,
                     "bound": "valu-fp64", "achieved": achieved,
                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                    "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                    "traffic_source": "profiles/traffic.json (rocprofv3 "
+                                      "FETCH_SIZE + WRITE_SIZE, copy-kernel "
+                                      "calibrated)" if traffic else None,
                     "algorithmic_flops_per_launch": flops,
                     "avg_launch_ms": ms}
         else:

@@ -13,6 +13,56 @@ namespace hdnnp {
 constexpr int kWave = 64;
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
+// sin(pi x) and cos(pi x) for x in [0, 1].  With u = x - 1/2:
+// cos(pi x) = -sin(pi u) and sin(pi x) = cos(pi u); both are evaluated from
+// their Taylor series in t = u^2 (|u| <= 1/2), truncated after the first
+// term below 1e-17.  The coefficients (-1)^k pi^(2k+1)/(2k+1)! and
+// (-1)^k pi^(2k)/(2k)! were computed in 60-digit arithmetic and rounded to
+// double.  Max abs error against libm over [0, 1]: 4.4e-16.
+__device__ __forceinline__ void sincospi01(double x, double& s, double& c)
+{
+    double const u = x - 0.5;
+    double const t = u * u;
+    double sp = -1.05184717169320651e-11;
+    sp = fma(sp, t, 5.39266466260812895e-10);
+    sp = fma(sp, t, -2.29484289972698730e-08);
+    sp = fma(sp, t, 7.95205400147551261e-07);
+    sp = fma(sp, t, -2.19153534478302173e-05);
+    sp = fma(sp, t, 4.66302805767612554e-04);
+    sp = fma(sp, t, -7.37043094571435044e-03);
+    sp = fma(sp, t, 8.21458866111282326e-02);
+    sp = fma(sp, t, -5.99264529320792105e-01);
+    sp = fma(sp, t, 2.55016403987734552e+00);
+    sp = fma(sp, t, -5.16771278004997026e+00);
+    sp = fma(sp, t, 3.14159265358979312e+00);
+    double cp = 1.37686472803774139e-12;
+    cp = fma(cp, t, -7.70070713060135385e-11);
+    cp = fma(cp, t, 3.60473079746250112e-09);
+    cp = fma(cp, t, -1.38789524622137714e-07);
+    cp = fma(cp, t, 4.30306958703294729e-06);
+    cp = fma(cp, t, -1.04638104924845705e-04);
+    cp = fma(cp, t, 1.92957430940392314e-03);
+    cp = fma(cp, t, -2.58068913900140612e-02);
+    cp = fma(cp, t, 2.35330630358893206e-01);
+    cp = fma(cp, t, -1.33526276885458950e+00);
+    cp = fma(cp, t, 4.05871212641676848e+00);
+    cp = fma(cp, t, -4.93480220054467900e+00);
+    cp = fma(cp, t, 1.0);
+    c = -u * sp;
+    s = cp;
+}
+
+// 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps
+// y <- y (3 - x y^2) / 2 (each step squares the relative error).
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double const hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+
 // CutoffFunction::fdf (CutoffFunction.h:210-220, CutoffFunction.cpp:128-213,
 // CoreFunction.h:214-318): value and derivative, zero at and beyond rc.
 __device__ __forceinline__ void cutoff_fdf(CutCfg const& c, double r,
@@ -35,9 +85,12 @@ __device__ __forceinline__ void cutoff_fdf(CutCfg const& c, double r,
     double const x = (r - c.rci) * c.iw;
     if (t == CT_COS)
     {
-        double const cs = cos(kPi * x);
+        // f = (cos(pi x) + 1)/2, f' = -iw pi sin(pi x)/2; the reference
+        // forms sin(pi x) as sqrt(1 - cos^2), equal on [0, 1].
+        double sn, cs;
+        sincospi01(x, sn, cs);
         fc = 0.5 * (cs + 1.0);
-        dfc = -0.5 * c.iw * kPi * sqrt(1.0 - cs * cs);
+        dfc = -0.5 * c.iw * kPi * sn;
         return;
     }
     double f, df;
@@ -104,9 +157,10 @@ __device__ __forceinline__ void cutoff_fdf_t(CutCfg const& c, double r,
     double const x = (r - c.rci) * c.iw;
     if (CT == CT_COS)
     {
-        double const cs = cos(kPi * x);
+        double sn, cs;
+        sincospi01(x, sn, cs);
         fc = 0.5 * (cs + 1.0);
-        dfc = -0.5 * c.iw * kPi * sqrt(1.0 - cs * cs);
+        dfc = -0.5 * c.iw * kPi * sn;
     }
     else   // CT_POLY2
     {

@@ -197,6 +197,24 @@ struct OrderedPairs
             len = na;
         }
     }
+    // Sequential walk: (j, c, len, half) of pair q -> pair q + 1.
+    __device__ void next(int& j, int& c, int& len, int& half) const
+    {
+        if (++c < len) return;
+        c = 0;
+        if (!same && half == 0 && j == a0 + na - 1)
+        {
+            half = 1;
+            j = b0;
+            len = na;
+        }
+        else ++j;
+    }
+    __device__ int partner(int j, int c, int half) const
+    {
+        if (same) return a0 + (c < j - a0 ? c : c + 1);
+        return half == 0 ? b0 + c : a0 + c;
+    }
 };
 
 __device__ __forceinline__ void flush(BLds const& S, int j, bool owned,
@@ -323,7 +341,15 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             int const q1 = min(total, q0 + span);
             int const len = q1 - q0;
             int const nw = (span + 31) >> 5;
-            // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow).
+            // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow),
+            // walking the chunk's pairs in order.
+            int wj = 0, wc = 0, wl = 1, wh = 0;
+            if (len > 0)
+            {
+                int k0;
+                op.decode(q0, wj, k0, wc, wl);
+                wh = (!op.same && q0 >= op.split) ? 1 : 0;
+            }
             for (int wd = 0; wd < nw; ++wd)
             {
                 uint32_t bits = 0;
@@ -332,8 +358,9 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                     int const t = wd * 32 + b;
                     if (t < len)
                     {
-                        int j, k, c, rl;
-                        op.decode(q0 + t, j, k, c, rl);
+                        int const j = wj;
+                        int const k = op.partner(wj, wc, wh);
+                        op.next(wj, wc, wl, wh);
                         bool pass = S.D[j] < g.rc && S.D[k] < g.rc;
                         if (pass && !wide)
                         {
@@ -410,8 +437,8 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 if (!wide)
                 {
                     double const rjk2 = dx * dx + dy * dy + dz * dz;
-                    rjk = sqrt(rjk2);
-                    invjk = 1.0 / rjk;
+                    invjk = rsqrt_nr(rjk2);
+                    rjk = rjk2 * invjk;
                     double fcjk, dfcjk;
                     cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
                     pfc = fcij * fcik * fcjk;

@@ -189,7 +189,7 @@ __device__ __forceinline__ void triplet_forward(
     {
         double const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
         double const rjk2 = dx * dx + dy * dy + dz * dz;
-        rjk = sqrt(rjk2);
+        rjk = rjk2 * rsqrt_nr(rjk2);
         double fcjk, dfcjk;
         cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
         pfc = fcij * fcik * fcjk;
