@@ -127,8 +127,11 @@ def _pair_force_check(ctx, s, r, e):
     sf, pf = ctx.pair_forces()
     osf, opf = s.pair_forces()
     off, idx, _, _ = s.neighbors()
-    scale = max(np.abs(opf).max(), 1e-300)
-    bad = np.nonzero(np.abs(pf - opf).max(1) > 1e-12 * scale)[0]
+    # 1e-11 absolute per slot (forces are checked at 1e-9).  The cosine
+    # cutoff's derivative is sin(pi x) here and sqrt(1 - cos^2) in the
+    # reference; they differ by cancellation error for pairs within ~1e-6 A
+    # of the cutoff.
+    bad = np.nonzero(np.abs(pf - opf).max(1) > 1e-11)[0]
     if len(bad):
         q = bad[0]
         i = np.searchsorted(off, q, side="right") - 1
