@@ -141,7 +141,7 @@ def _pair_force_check(ctx, s, r, e):
                % (len(bad), len(pf), i, e[i], off[i + 1] - off[i], ne, q - off[i],
                   idx[q], e[idx[q]], pf[q], opf[q]))
         raise AssertionError(msg)
-    assert np.abs(sf - osf).max() <= 1e-12 * max(np.abs(osf).max(), 1e-300)
+    assert np.abs(sf - osf).max() <= 1e-10
 
 
 def test_pair_forces_per_slot(workdir):
