@@ -7,6 +7,9 @@ structure's SF statistics (computed on the GPU, the nnp-scaling
 definition), random-init nsf-25-25-1 tanh networks (Xavier-uniform, seeded).
 One step = one full Mode::evaluateNNP on the GPU: neighbour list rebuilt,
 symmetry functions, networks, energy, forces; inputs resident in HBM.
+Both paths are timed in the same run: the plain FP64 path and the mixed-
+precision path (the reported value), with the mixed path's force and
+energy error against the FP64 results.
 
 Prints one JSON line (see the driver contract) with a roofline object for
 the dominant kernel and a CPU baseline (the oracle, OpenMP on the host
@@ -27,6 +30,7 @@ import numpy as np  # noqa: E402
 from n2p2_amd import costmodel, engine, workloads as W  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X vector FP64
+PEAK_FP32_TFLOPS = 157.3    # MI355X vector FP32 (published; packed math)
 PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
 
 
@@ -91,6 +95,65 @@ def cpu_baseline(tmp, seconds_target=20.0):
                       % (len(times), t, cpu)}
 
 
+def run_path(ctx, precision, steps, warmup):
+    """Times `steps` full evaluations of one precision path after `warmup`
+    untimed ones; returns the step time, energy, forces and per-kernel
+    device times (HIP events on the engine's stream)."""
+    ctx.set_precision(precision)
+    for _ in range(warmup):
+        ctx.evaluate(engine.FORCES_REFERENCE)
+    # evaluate() returns after its stream has drained (the energy and the
+    # forces are complete), so each side of the timed region is synchronous.
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        energy = ctx.evaluate(engine.FORCES_REFERENCE)
+    dt = (time.perf_counter() - t0) / steps
+    forces = ctx.forces()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(max(3, min(steps, 10))):
+        ctx.evaluate(engine.FORCES_REFERENCE)
+    kt = {k: ctx.profile_avg_ms(k)[0]
+          for k in ("neighbor", "symfunc", "network", "force", "gather")}
+    ctx.profile(False)
+    return dt, energy, forces, kt
+
+
+def roofline(kt, counts, nnf, n, fp32):
+    """Roofline object for the dominant kernel of one path."""
+    fwd = int(costmodel.forward_flops(counts))
+    bwd = int(costmodel.backward_flops(counts))
+    peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
+    dom = max(kt, key=kt.get)
+    flops = {"force": bwd, "symfunc": fwd, "network": nnf}.get(dom)
+    ms = kt[dom]
+    if flops is None:
+        nb = costmodel.neighbor_bytes(n, int(counts[0]))
+        achieved = nb / (ms * 1e-3) / 1e9
+        return {"kernel": "neighbor build", "bound": "hbm",
+                "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                "avg_launch_ms": ms}
+    kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
+             "network": "k_nn"}[dom]
+    achieved = flops / (ms * 1e-3) / 1e12
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic.json")
+    key = kname + ("_fp32" if fp32 else "")
+    if os.path.exists(tp) and n == 196608:
+        with open(tp) as f:
+            tk = json.load(f).get("kernels", {}).get(key)
+        if tk:
+            traffic = tk["read_bytes"] + tk["write_bytes"]
+    return {"kernel": kname, "bound": "valu-fp32" if fp32 else "valu-fp64",
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic,
+            "traffic_source": ("profiles/traffic.json (rocprofv3 FETCH_SIZE "
+                               "+ WRITE_SIZE, copy-kernel calibrated)")
+                              if traffic else None,
+            "algorithmic_flops_per_launch": flops, "avg_launch_ms": ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,72 +179,29 @@ def main():
         ctx.calculate_neighbor_list(model.max_cutoff)
         counts = [int(x) for x in ctx.work_counts()]
         log("work counts", counts)
+        nnf = int(sum(int(np.count_nonzero(elem == e))
+                      * costmodel.network_flops(
+                          [[model.num_sf(e)] + list(hidden) + [1]])
+                      for e in range(model.num_elements)))
 
-        for _ in range(args.warmup):
-            ctx.evaluate(engine.FORCES_REFERENCE)
-        log("warmup done")
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            energy = ctx.evaluate(engine.FORCES_REFERENCE)
-        t1 = time.perf_counter()
-        dt = (t1 - t0) / args.steps
-        log("timed %d steps: %.3f ms/step, E=%.10f" % (args.steps, dt * 1e3,
-                                                        energy))
-
-        # Per-kernel device time (HIP events on the engine's stream).
-        ctx.profile(True)
-        ctx.profile_reset()
-        for _ in range(max(3, min(args.steps, 10))):
-            ctx.evaluate(engine.FORCES_REFERENCE)
-        kt = {k: ctx.profile_avg_ms(k)[0]
-              for k in ("neighbor", "symfunc", "network", "force", "gather")}
-        ctx.profile(False)
-        log("kernel ms", kt)
-
-        fwd = costmodel.forward_flops(counts)
-        bwd = costmodel.backward_flops(counts)
-        nnf = sum(int(np.count_nonzero(elem == e)) * costmodel.network_flops(
-                      [[model.num_sf(e)] + list(hidden) + [1]])
-                  for e in range(model.num_elements))
-        fwd, bwd, nnf = int(fwd), int(bwd), int(nnf)
-        total = fwd + bwd + nnf
-        dom = max(kt, key=kt.get)
-        if dom == "force":
-            flops, ms = bwd, kt["force"]
-        elif dom == "symfunc":
-            flops, ms = fwd, kt["symfunc"]
-        elif dom == "network":
-            flops, ms = nnf, kt["network"]
-        else:
-            flops, ms = None, kt[dom]
-        if flops is not None:
-            achieved = flops / (ms * 1e-3) / 1e12
-            kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
-                     "network": "k_nn"}[dom]
-            traffic = None
-            tp = os.path.join(ROOT, "profiles", "traffic.json")
-            if os.path.exists(tp) and args.nmol == 65536:
-                with open(tp) as f:
-                    tj = json.load(f)
-                kt_ = tj.get("kernels", {}).get(kname)
-                if kt_:
-                    traffic = kt_["read_bytes"] + kt_["write_bytes"]
-            roof = {"kernel": kname,
-                    "bound": "valu-fp64", "achieved": achieved,
-                    "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                    "traffic_source": "profiles/traffic.json (rocprofv3 "
-                                      "FETCH_SIZE + WRITE_SIZE, copy-kernel "
-                                      "calibrated)" if traffic else None,
-                    "algorithmic_flops_per_launch": flops,
-                    "avg_launch_ms": ms}
-        else:
-            nb = costmodel.neighbor_bytes(n, int(counts[0]))
-            achieved = nb / (ms * 1e-3) / 1e9
-            roof = {"kernel": "neighbor build", "bound": "hbm",
-                    "achieved": achieved, "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                    "traffic": None, "avg_launch_ms": ms}
+        dt64, e64, f64, kt64 = run_path(ctx, engine.PRECISION_FP64,
+                                        args.steps, args.warmup)
+        log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
+            % (dt64 * 1e3, e64, kt64))
+        dtm, em, fm, ktm = run_path(ctx, engine.PRECISION_MIXED, args.steps,
+                                    args.warmup)
+        log("mixed: %.3f ms/step, E=%.10f, kernel ms %s"
+            % (dtm * 1e3, em, ktm))
+        df = fm - f64
+        err = {"force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
+               "force_max_eV_A": float(np.abs(df).max()),
+               "force_rms_scale_eV_A": float(np.sqrt(np.mean(f64 ** 2))),
+               "energy_rel": float(abs(em - e64) / abs(e64)),
+               "reference": "plain FP64 GPU path of the same run (parity-"
+                            "tested against the FP64 oracle)"}
+        log("mixed vs FP64 errors", err)
+        fwd = int(costmodel.forward_flops(counts))
+        bwd = int(costmodel.backward_flops(counts))
 
         cpu = None
         if not args.no_cpu_baseline:
@@ -192,27 +212,37 @@ def main():
         out = {
             "metric": "E+F atom-steps/s, mixed-precision vs plain FP64 GPU, "
                       "at force err <=1 meV/A",
-            "value": n / dt,
+            "value": n / dtm,
             "unit": "atom-steps/s",
             "n_gpus": 1,
             "steps": args.steps,
             "warmup": args.warmup,
-            "ms_per_step": dt * 1e3,
+            "ms_per_step": dtm * 1e3,
             "higher_is_better": True,
             "scaling": "weak",
             "vs_baseline": None,
-            "dtype": "f64",
+            "dtype": "f32+f64",
             "data": "synthetic",
             "config": {"workload": "water %d atoms (%d H2O, cubic %.1f A), "
                                    "40 SFs/element (G2+G3), rc 6.35 A cos "
-                                   "cutoff, nsf-25-25-1 tanh, plain FP64 "
-                                   "path, neighbour list rebuilt every step"
+                                   "cutoff, nsf-25-25-1 tanh, neighbour "
+                                   "list rebuilt every step; value = mixed-"
+                                   "precision path (FP32 SFs and force "
+                                   "contraction, FP64 scaling, networks, "
+                                   "energy and force sums)"
                                    % (n, args.nmol, args.box),
-                       "atoms": n, "path": "plain-fp64",
-                       "flops_per_atom_step": total / n,
+                       "atoms": n, "path": "mixed",
+                       "plain_fp64": {"atom_steps_per_s": n / dt64,
+                                      "ms_per_step": dt64 * 1e3,
+                                      "kernel_ms": kt64,
+                                      "roofline": roofline(kt64, counts, nnf,
+                                                           n, False)},
+                       "mixed_vs_plain_fp64": dt64 / dtm,
+                       "mixed_error_vs_fp64": err,
+                       "flops_per_atom_step": (fwd + bwd + nnf) / n,
                        "work_counts": counts,
-                       "kernel_ms": kt},
-            "roofline": roof,
+                       "kernel_ms": ktm},
+            "roofline": roofline(ktm, counts, nnf, n, True),
             "cpu_baseline": cpu,
         }
         print(json.dumps(out), flush=True)

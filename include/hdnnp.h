@@ -46,10 +46,11 @@ extern "C" {
 
 /* Arithmetic of the descriptor and network kernels. */
 #define HDNNP_PRECISION_FP64  0   /* plain path: FP64 throughout */
-#define HDNNP_PRECISION_MIXED 1   /* FP32 descriptors with compensated sums,
-                                     16-bit MFMA network layers with FP32
-                                     accumulation; FP64 energy total, force
-                                     accumulation and SF scaling */
+#define HDNNP_PRECISION_MIXED 1   /* FP32 symmetry-function and force-
+                                     contraction arithmetic (per-lane FP32
+                                     partial sums combined in FP64); FP64
+                                     SF scaling, networks, energy total and
+                                     force accumulation */
 
 typedef struct hdnnp_model hdnnp_model;
 typedef struct hdnnp_ctx   hdnnp_ctx;

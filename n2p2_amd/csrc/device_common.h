@@ -52,6 +52,32 @@ __device__ __forceinline__ void sincospi01(double x, double& s, double& c)
     s = cp;
 }
 
+// FP32 form of the same two series, truncated after the first term below
+// 1e-8 (sine through u^13, cosine through u^14) and rounded to float.  Max
+// abs error against libm over [0, 1]: below 2.5e-7.
+__device__ __forceinline__ void sincospi01(float x, float& s, float& c)
+{
+    float const u = x - 0.5f;
+    float const t = u * u;
+    float sp = 4.66302805767612554e-04f;
+    sp = fmaf(sp, t, -7.37043094571435044e-03f);
+    sp = fmaf(sp, t, 8.21458866111282326e-02f);
+    sp = fmaf(sp, t, -5.99264529320792105e-01f);
+    sp = fmaf(sp, t, 2.55016403987734552e+00f);
+    sp = fmaf(sp, t, -5.16771278004997026e+00f);
+    sp = fmaf(sp, t, 3.14159265358979312e+00f);
+    float cp = -1.04638104924845705e-04f;
+    cp = fmaf(cp, t, 1.92957430940392314e-03f);
+    cp = fmaf(cp, t, -2.58068913900140612e-02f);
+    cp = fmaf(cp, t, 2.35330630358893206e-01f);
+    cp = fmaf(cp, t, -1.33526276885458950e+00f);
+    cp = fmaf(cp, t, 4.05871212641676848e+00f);
+    cp = fmaf(cp, t, -4.93480220054467900e+00f);
+    cp = fmaf(cp, t, 1.0f);
+    c = -u * sp;
+    s = cp;
+}
+
 // 1/sqrt(x) for x > 0: hardware estimate refined by two Newton steps
 // y <- y (3 - x y^2) / 2 (each step squares the relative error).
 __device__ __forceinline__ double rsqrt_nr(double x)
@@ -63,110 +89,135 @@ __device__ __forceinline__ double rsqrt_nr(double x)
     return y;
 }
 
+// FP32: the hardware estimate (1 ulp) as it is.
+__device__ __forceinline__ float rsqrt_nr(float x)
+{
+    return __builtin_amdgcn_rsqf(x);
+}
+
+// e^x.  FP64: the library function.  FP32: the hardware 2^x of x log2(e)
+// (relative error below 2e-6 for |x| < 30, the range of the SF exponents).
+__device__ __forceinline__ double exp_r(double x) { return exp(x); }
+__device__ __forceinline__ float exp_r(float x) { return __expf(x); }
+
+// 1 / x.  FP64: IEEE division.  FP32: the hardware reciprocal (1 ulp).
+__device__ __forceinline__ double rcp_r(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp_r(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
+__device__ __forceinline__ double tanh_r(double x) { return tanh(x); }
+__device__ __forceinline__ float tanh_r(float x) { return tanhf(x); }
+
 // CutoffFunction::fdf (CutoffFunction.h:210-220, CutoffFunction.cpp:128-213,
 // CoreFunction.h:214-318): value and derivative, zero at and beyond rc.
-__device__ __forceinline__ void cutoff_fdf(CutCfg const& c, double r,
-                                           double& fc, double& dfc)
+// R is the arithmetic type (double, or float on the mixed-precision path).
+template <typename R>
+__device__ __forceinline__ void cutoff_fdf(CutCfg const& c, R r, R& fc,
+                                           R& dfc)
 {
-    if (r >= c.rc) { fc = 0.0; dfc = 0.0; return; }
+    if (r >= (R)c.rc) { fc = R(0); dfc = R(0); return; }
     int const t = c.type;
     if (t == CT_TANHU || t == CT_TANH)
     {
-        double const th = tanh(1.0 - r * c.rcinv);
-        double const th2 = th * th;
+        R const th = tanh_r(R(1) - r * (R)c.rcinv);
+        R const th2 = th * th;
         // CT_TANH multiplies by tanh(1)^-3 so that f(0) = 1 (host-computed).
-        double const pre = (t == CT_TANH) ? c.tanhpre : 1.0;
+        R const pre = (t == CT_TANH) ? (R)c.tanhpre : R(1);
         fc = pre * th * th2;
-        dfc = 3.0 * pre * th2 * (th2 - 1.0) * c.rcinv;
+        dfc = R(3) * pre * th2 * (th2 - R(1)) * (R)c.rcinv;
         return;
     }
-    if (t == CT_HARD) { fc = 1.0; dfc = 0.0; return; }
-    if (r < c.rci) { fc = 1.0; dfc = 0.0; return; }
-    double const x = (r - c.rci) * c.iw;
+    if (t == CT_HARD) { fc = R(1); dfc = R(0); return; }
+    if (r < (R)c.rci) { fc = R(1); dfc = R(0); return; }
+    R const iw = (R)c.iw;
+    R const x = (r - (R)c.rci) * iw;
     if (t == CT_COS)
     {
         // f = (cos(pi x) + 1)/2, f' = -iw pi sin(pi x)/2; the reference
         // forms sin(pi x) as sqrt(1 - cos^2), equal on [0, 1].
-        double sn, cs;
+        R sn, cs;
         sincospi01(x, sn, cs);
-        fc = 0.5 * (cs + 1.0);
-        dfc = -0.5 * c.iw * kPi * sn;
+        fc = R(0.5) * (cs + R(1));
+        dfc = R(-0.5) * iw * (R)kPi * sn;
         return;
     }
-    double f, df;
+    R f, df;
     if (t == CT_POLY1)
     {
-        f = (2.0 * x - 3.0) * x * x + 1.0;
-        df = x * (6.0 * x - 6.0);
+        f = (R(2) * x - R(3)) * x * x + R(1);
+        df = x * (R(6) * x - R(6));
     }
     else if (t == CT_POLY2)
     {
-        double const x2 = x * x;
-        f = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
-        df = x2 * ((60.0 - 30.0 * x) * x - 30.0);
+        R const x2 = x * x;
+        f = ((R(15) - R(6) * x) * x - R(10)) * x * x2 + R(1);
+        df = x2 * ((R(60) - R(30) * x) * x - R(30));
     }
     else if (t == CT_POLY3)
     {
-        double const x2 = x * x;
-        f = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
-        df = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
+        R const x2 = x * x;
+        f = (x * (x * (R(20) * x - R(70)) + R(84)) - R(35)) * x2 * x2 + R(1);
+        df = x2 * x * (x * (x * (R(140) * x - R(420)) + R(420)) - R(140));
     }
     else if (t == CT_POLY4)
     {
-        double x4 = x * x;
+        R x4 = x * x;
         x4 *= x4;
-        f = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0)
-          * x * x4 + 1.0;
-        df = x4 * (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0)
-           - 630.0);
+        f = (x * (x * ((R(315) - R(70) * x) * x - R(540)) + R(420)) - R(126))
+          * x * x4 + R(1);
+        df = x4 * (x * (x * ((R(2520) - R(630) * x) * x - R(3780))
+                        + R(2520)) - R(630));
     }
     else  // CT_EXP: f(x) = e^(1 - 1/(1-x^2))
     {
-        double const e = 2.718281828459045090795598298427648842334747314453125;
-        double const tmp = 1.0 / (x * x - 1.0);
-        double const ex = exp(tmp);
+        R const e = (R)2.718281828459045090795598298427648842334747314453125;
+        R const tmp = R(1) / (x * x - R(1));
+        R const ex = exp_r(tmp);
         f = e * ex;
-        df = -2.0 * e * x * tmp * tmp * ex;
+        df = R(-2) * e * x * tmp * tmp * ex;
     }
     fc = f;
-    dfc = df * c.iw;
+    dfc = df * iw;
 }
 
 // Cutoff with the type fixed at compile time (CT >= 0) or read from the
 // configuration (CT < 0).  The specialised forms keep the heavy kernels
 // free of the other types' code and registers.
-template <int CT>
-__device__ __forceinline__ void cutoff_fdf_t(CutCfg const& c, double r,
-                                             double& fc, double& dfc)
+template <int CT, typename R>
+__device__ __forceinline__ void cutoff_fdf_t(CutCfg const& c, R r, R& fc,
+                                             R& dfc)
 {
     if (CT < 0)
     {
         cutoff_fdf(c, r, fc, dfc);
         return;
     }
-    if (r >= c.rc) { fc = 0.0; dfc = 0.0; return; }
+    if (r >= (R)c.rc) { fc = R(0); dfc = R(0); return; }
     if (CT == CT_TANHU)
     {
-        double const th = tanh(1.0 - r * c.rcinv);
-        double const th2 = th * th;
+        R const th = tanh_r(R(1) - r * (R)c.rcinv);
+        R const th2 = th * th;
         fc = th * th2;
-        dfc = 3.0 * th2 * (th2 - 1.0) * c.rcinv;
+        dfc = R(3) * th2 * (th2 - R(1)) * (R)c.rcinv;
         return;
     }
-    if (r < c.rci) { fc = 1.0; dfc = 0.0; return; }
-    double const x = (r - c.rci) * c.iw;
+    if (r < (R)c.rci) { fc = R(1); dfc = R(0); return; }
+    R const iw = (R)c.iw;
+    R const x = (r - (R)c.rci) * iw;
     if (CT == CT_COS)
     {
-        double sn, cs;
+        R sn, cs;
         sincospi01(x, sn, cs);
-        fc = 0.5 * (cs + 1.0);
-        dfc = -0.5 * c.iw * kPi * sn;
+        fc = R(0.5) * (cs + R(1));
+        dfc = R(-0.5) * iw * (R)kPi * sn;
     }
     else   // CT_POLY2
     {
-        double const x2 = x * x;
-        fc = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
-        dfc = x2 * ((60.0 - 30.0 * x) * x - 30.0) * c.iw;
+        R const x2 = x * x;
+        fc = ((R(15) - R(6) * x) * x - R(10)) * x * x2 + R(1);
+        dfc = x2 * ((R(60) - R(30) * x) * x - R(30)) * iw;
     }
 }
 
@@ -185,12 +236,13 @@ __device__ __forceinline__ void cutoff_fdf_t(CutCfg const& c, double r,
     } while (0)
 
 // Exponentiation by squaring (utility.cpp:292-317 semantics).
-__device__ __forceinline__ double pow_int(double x, int n)
+template <typename R>
+__device__ __forceinline__ R pow_int(R x, int n)
 {
     unsigned m;
-    if (n < 0) { x = 1.0 / x; m = (unsigned)(-n); }
+    if (n < 0) { x = R(1) / x; m = (unsigned)(-n); }
     else m = (unsigned)n;
-    double r = 1.0;
+    R r = R(1);
     do
     {
         if (m & 1u) r *= x;
@@ -198,6 +250,56 @@ __device__ __forceinline__ double pow_int(double x, int n)
         x *= x;
     } while (m);
     return r;
+}
+
+// x^N with the products pow_int forms for exponent N, N fixed at compile
+// time: straight-line code, bit-identical to pow_int(x, N).
+template <int N, typename R>
+__device__ __forceinline__ R pow_fixed(R x)
+{
+    R r = R(1);
+    bool first = true;
+    int m = N;
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+    {
+        if (m & 1)
+        {
+            r = first ? x : r * x;
+            first = false;
+        }
+        m >>= 1;
+        if (m) x *= x;
+    }
+    return r;
+}
+
+// pow_int(x, n) for a wave-uniform n: a scalar branch to straight-line
+// products for 0 <= n <= 16, the loop otherwise.
+template <typename R>
+__device__ __forceinline__ R pow_uniform(R x, int n)
+{
+    switch (n)
+    {
+    case 0: return R(1);
+    case 1: return x;
+    case 2: return pow_fixed<2>(x);
+    case 3: return pow_fixed<3>(x);
+    case 4: return pow_fixed<4>(x);
+    case 5: return pow_fixed<5>(x);
+    case 6: return pow_fixed<6>(x);
+    case 7: return pow_fixed<7>(x);
+    case 8: return pow_fixed<8>(x);
+    case 9: return pow_fixed<9>(x);
+    case 10: return pow_fixed<10>(x);
+    case 11: return pow_fixed<11>(x);
+    case 12: return pow_fixed<12>(x);
+    case 13: return pow_fixed<13>(x);
+    case 14: return pow_fixed<14>(x);
+    case 15: return pow_fixed<15>(x);
+    case 16: return pow_fixed<16>(x);
+    default: return pow_int(x, n);
+    }
 }
 
 // NeuralNetwork::propagateLayer activations (NeuralNetwork.cpp:761-896):
@@ -261,24 +363,40 @@ __device__ __forceinline__ void activate(int a, double x, double& v, double& d)
 
 // Member parameters of one group, staged in LDS at the start of the group
 // and read with wave-uniform addresses (broadcast) inside the pair loops.
-struct MParam
+// Integer fields are made scalar with readfirstlane where they steer
+// branches.
+enum MemberFlags
 {
-    double lambda, zl, eta, rs, zeta, w;   // w: per-kernel weight
-    int    zint, flags;                    // flags: 1 integer power, 2 new exp
+    MF_INTPOW = 1,   // integer zeta (pow_int)
+    MF_NEWEXP = 2    // first member of an (eta, rs) run: new exponential
 };
 
-__device__ __forceinline__ MParam make_mparam(MemberDesc const& md, double w)
+template <typename R>
+struct MParam
 {
-    MParam p;
-    p.lambda = md.lambda;
-    p.zl = md.zl;
-    p.eta = md.eta;
-    p.rs = md.rs;
-    p.zeta = md.zeta;
-    p.w = w;
+    R   lambda, zl, eta, rs, zeta, w;   // w: per-kernel weight
+    int zint, flags;
+};
+
+template <typename R>
+__device__ __forceinline__ MParam<R> make_mparam(MemberDesc const& md,
+                                                 double w)
+{
+    MParam<R> p;
+    p.lambda = (R)md.lambda;
+    p.zl = (R)md.zl;
+    p.eta = (R)md.eta;
+    p.rs = (R)md.rs;
+    p.zeta = (R)md.zeta;
+    p.w = (R)w;
     p.zint = md.zint;
-    p.flags = (md.useint ? 1 : 0) | (md.calcexp ? 2 : 0);
+    p.flags = (md.useint ? MF_INTPOW : 0) | (md.calcexp ? MF_NEWEXP : 0);
     return p;
+}
+
+__device__ __forceinline__ int uniform(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // SymFnc::scale (SymFnc.cpp:169-195).
@@ -296,7 +414,8 @@ __device__ __forceinline__ double scale_sf(ScaleDesc const& s, double v)
 }
 
 // Butterfly sum over the 64 lanes of a wave; every lane gets the total.
-// The pairing pattern is fixed, so the result is deterministic.
+// The pairing pattern is fixed, so the result is deterministic.  Float
+// partials are summed in FP64.
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

@@ -449,7 +449,8 @@ void computeSF(hdnnp_ctx* c, bool raw)
     requireNL(c);
     c->setDevice();
     int const cap = std::max(c->maxNeigh, 1);
-    if (sf_forward_lds_bytes(cap) > 65536)
+    int const fp32 = (!raw && c->precision == HDNNP_PRECISION_MIXED) ? 1 : 0;
+    if (sf_forward_lds_bytes(cap, fp32) > 65536)
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the symmetry-"
                     "function kernel's LDS staging.\n");
@@ -465,6 +466,7 @@ void computeSF(hdnnp_ctx* c, bool raw)
     a.ldG = c->ld;
     a.cap = cap;
     a.raw = raw ? 1 : 0;
+    a.fp32 = fp32;
     hipEvent_t ev = nullptr;
     c->begin(K_SYMFUNC, ev);
     hipCheck(launch_sf_forward(a, c->stream), "symmetry functions");
@@ -539,8 +541,8 @@ void computeForces(hdnnp_ctx* c, int mode)
         throw Error(HDNNP_E_INVALID, "ERROR: Unknown force mode.\n");
     c->setDevice();
     int const cap = std::max(c->maxNeigh, 1);
-    size_t const shm = force_backward_lds_bytes(cap, c->maxMembers, c->maxCfg,
-                                                c->T.maxnsf);
+    int const fp32 = c->precision == HDNNP_PRECISION_MIXED ? 1 : 0;
+    size_t const shm = force_backward_lds_bytes(cap, fp32);
     if (shm > 65536)
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the force "
@@ -560,8 +562,7 @@ void computeForces(hdnnp_ctx* c, int mode)
     a.cslot = c->dCslot.p;
     a.cself = c->dCself.p;
     a.cap = cap;
-    a.maxMembers = std::max(c->maxMembers, 1);
-    a.maxCfg = c->maxCfg;
+    a.fp32 = fp32;
     hipEvent_t ev = nullptr, ev2 = nullptr;
     c->begin(K_FORCE, ev);
     hipCheck(launch_force_backward(a, c->stream), "force backward");
@@ -712,12 +713,12 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
 int hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision)
 {
     return guarded([&]() {
-        if (precision == HDNNP_PRECISION_FP64) { c->precision = precision; return; }
-        if (precision == HDNNP_PRECISION_MIXED)
-            throw Error(HDNNP_E_UNSUPPORTED,
-                        "ERROR: Mixed precision is not available in this "
-                        "build.\n");
-        throw Error(HDNNP_E_INVALID, "ERROR: Unknown precision.\n");
+        if (precision != HDNNP_PRECISION_FP64
+            && precision != HDNNP_PRECISION_MIXED)
+            throw Error(HDNNP_E_INVALID, "ERROR: Unknown precision.\n");
+        if (precision != c->precision)
+            c->haveSF = c->haveNN = c->haveNNDeriv = false;
+        c->precision = precision;
     });
 }
 
@@ -1087,7 +1088,7 @@ int hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts)
         requireNL(c);
         c->setDevice();
         int const cap = std::max(c->maxNeigh, 1);
-        if (sf_forward_lds_bytes(cap) > 65536)
+        if (sf_forward_lds_bytes(cap, 0) > 65536)
             throw Error(HDNNP_E_CAPACITY, "ERROR: Too many neighbours.\n");
         DBuf<unsigned long long> d;
         d.ensure(HDNNP_WORK_COUNTS);

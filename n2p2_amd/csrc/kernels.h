@@ -42,7 +42,8 @@ hipError_t launch_nl_fill(NLParams const& p, const double* r, const int* elem,
 hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
                              int* partner, hipStream_t s);
 
-// Symmetry functions (FP64).  raw != 0 skips scaling (statistics).
+// Symmetry functions.  raw != 0 skips scaling (statistics); fp32 != 0
+// selects the mixed-precision kernel (FP32 geometry and member sums).
 struct SFArgs
 {
     ModelTables T;
@@ -55,9 +56,10 @@ struct SFArgs
     int ldG;
     int cap;                     // LDS neighbour capacity per wave
     int raw;
+    int fp32;
 };
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
-size_t sf_forward_lds_bytes(int cap);
+size_t sf_forward_lds_bytes(int cap, int fp32);
 // Work counters (10 x uint64, accumulated): see k_sf_count.
 hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
                            hipStream_t s);
@@ -75,11 +77,10 @@ struct ForceArgs
     double* cslot;               // 3 x M (x, y, z per entry)
     double* cself;               // 3 x n
     int cap;
-    int maxMembers;              // members of the largest element
-    int maxCfg;
+    int fp32;                    // mixed-precision contraction
 };
 hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
-size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf);
+size_t force_backward_lds_bytes(int cap, int fp32);
 
 hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
                                const int* partner, const double* cslot,

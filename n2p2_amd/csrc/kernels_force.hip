@@ -1,5 +1,8 @@
-// Force kernels (FP64): contraction dE/dG . dG/dr with dG/dr recomputed,
-// then a deterministic per-atom gather.
+// Force kernels: contraction dE/dG . dG/dr with dG/dr recomputed, then a
+// deterministic per-atom gather.  The contraction runs in FP64 (plain path)
+// or in FP32 (mixed path: geometry, member sums, row partials and the
+// per-slot sums of a handful of row partials); the centre sums, cslot,
+// cself and the gather are FP64 on both.
 //
 // The reference stores dG/dr per SF and per neighbour slot
 // (SymGrp*::calculate with derivatives, Atom::allocate) and contracts them
@@ -26,15 +29,20 @@
 //    (rs = 0), c3 = 0 for the wide type (SymGrpExpAngw.cpp:300-310); so
 //      P1 = sum_m w_m p1_m = pfc A T1 + T2,  P3 = pfc/(rij rik) T1 + T3,
 //      T1 = sum w fg zeta lambda, T2 = sum w fg plambda c1,
-//      T3 = sum w fg plambda c3,  w_m = dE/dG_m * factorNorm_m;
+//      T3 = sum w fg plambda c3,  w_m = dE/dG_m * factorNorm_m.
+//    c1, c3 and exp(.) are shared by an (eta, rs) run of members, so T2 and
+//    T3 are formed once per run from U = sum w pw plambda; the powers come
+//    from pow_uniform (straight-line products per uniform exponent);
 //  * every ordered pair (j, k) of a group is evaluated by the lane that
 //    owns slot j and contributes -(P1 dr_ij + P3 dr_jk) to slot j and
-//    P1 dr_ij to the centre; slot k's part comes from the pair (k, j).  The
-//    group's ordered pairs are split into 64 equal contiguous chunks, one
-//    per lane; a lane adds the sum of every slot whose pairs start inside
-//    its chunk straight into the LDS accumulator and leaves the sum of the
-//    slot it inherited from the previous chunk in a per-lane head buffer,
-//    merged afterwards in lane order.  No atomics; fixed summation order.
+//    P1 dr_ij to the centre; slot k's part comes from the pair (k, j).  A
+//    lane sums P1 and P3 dr_jk over its run of pairs of one row j and
+//    applies dr_ij once per run.  The group's ordered pairs are split into
+//    64 equal contiguous chunks, one per lane; a lane adds the sum of every
+//    slot whose pairs start inside its chunk straight into the LDS
+//    accumulator and leaves the sum of the slot it inherited from the
+//    previous chunk in a per-lane head buffer, merged afterwards in lane
+//    order.  No atomics; fixed summation order.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -50,23 +58,31 @@ __host__ __device__ inline int mask_words(int cap)
     return (span + 31) / 32;
 }
 
+// LDS of one wave.  1/r is staged in FP64 only (FP32 takes the hardware
+// reciprocal); f_c'(r) is formed per row, not staged.
+template <typename R>
 struct BLds
 {
-    double* X;
-    double* Y;
-    double* Z;
-    double* D;
-    double* INV;
-    double* FC;
-    double* DFC;
-    double* ACC;     // 3 per staged neighbour
+    R* ACC;          // 3 per staged neighbour
+    R* HEAD;         // 3 per lane
+    R* X;
+    R* Y;
+    R* Z;
+    R* D;
+    R* INV;          // FP64 only
+    R* FC;
     int* SLOT;       // original slot of each staged neighbour
-    uint32_t* MASK;  // [word][64]
-    double* HEAD;    // 3 per lane
     int* HEADJ;      // per lane
+    uint32_t* MASK;  // [word][64]
+    __device__ R inv(int j) const
+    {
+        if (sizeof(R) == 8) return INV[j];
+        return rcp_r(D[j]);
+    }
 };
 
-__device__ __forceinline__ void stage(ForceArgs const& a, BLds const& S,
+template <typename R>
+__device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
                                       int s0, int n, int* sStart, int* sCnt)
 {
     int const lane = threadIdx.x;
@@ -120,15 +136,15 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds const& S,
                 if (e == q)
                 {
                     int const pos = start[q] + lanes_below(mask);
-                    S.X[pos] = g.x;
-                    S.Y[pos] = g.y;
-                    S.Z[pos] = g.z;
-                    S.D[pos] = g.w;
-                    S.INV[pos] = 1.0 / g.w;
+                    S.X[pos] = (R)g.x;
+                    S.Y[pos] = (R)g.y;
+                    S.Z[pos] = (R)g.z;
+                    S.D[pos] = (R)g.w;
+                    if (sizeof(R) == 8) S.INV[pos] = R(1) / (R)g.w;
                     S.SLOT[pos] = s;
-                    S.ACC[3 * pos] = 0.0;
-                    S.ACC[3 * pos + 1] = 0.0;
-                    S.ACC[3 * pos + 2] = 0.0;
+                    S.ACC[3 * pos] = R(0);
+                    S.ACC[3 * pos + 1] = R(0);
+                    S.ACC[3 * pos + 2] = R(0);
                 }
                 start[q] += __popcll(mask);
             }
@@ -217,10 +233,33 @@ struct OrderedPairs
     }
 };
 
-__device__ __forceinline__ void flush(BLds const& S, int j, bool owned,
-                                      double cx, double cy, double cz,
-                                      bool& headUsed)
+// A lane's partial sums over its run of passing pairs of one row j:
+// sum P1 and sum P3 dr_jk.
+template <typename R>
+struct RowSum
 {
+    R p1, vx, vy, vz;
+    __device__ void clear() { p1 = vx = vy = vz = R(0); }
+};
+
+// Ends a lane's run of row j (dr_ij = (xj, yj, zj)): slot j receives
+// -(sum P1 dr_ij + sum P3 dr_jk) (straight into the accumulator if the row
+// starts in this lane's chunk, else into the lane's head buffer), the
+// centre's FP64 sum P1 dr_ij.
+template <typename R>
+__device__ __forceinline__ void flush(BLds<R> const& S, int j, bool owned,
+                                      RowSum<R> const& rs, R xj, R yj, R zj,
+                                      double& sx, double& sy, double& sz)
+{
+    R const ax = rs.p1 * xj;
+    R const ay = rs.p1 * yj;
+    R const az = rs.p1 * zj;
+    sx += (double)ax;
+    sy += (double)ay;
+    sz += (double)az;
+    R const cx = -(ax + rs.vx);
+    R const cy = -(ay + rs.vy);
+    R const cz = -(az + rs.vz);
     if (owned)
     {
         S.ACC[3 * j] += cx;
@@ -234,28 +273,26 @@ __device__ __forceinline__ void flush(BLds const& S, int j, bool owned,
         S.HEAD[3 * lane] = cx;
         S.HEAD[3 * lane + 1] = cy;
         S.HEAD[3 * lane + 2] = cz;
-        headUsed = true;
     }
 }
 
-template <int CT, bool GENPOW>
+template <int CT, bool GENPOW, typename R>
 __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
 {
     extern __shared__ double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
-    __shared__ MParam MP[kMaxMembers];
+    __shared__ MParam<R> MP[kMaxMembers];
     int const cap = a.cap;
-    BLds S;
-    S.X = sm;
+    BLds<R> S;
+    S.ACC = (R*)sm;
+    S.HEAD = S.ACC + 3 * cap;
+    S.X = S.HEAD + 3 * 64;
     S.Y = S.X + cap;
     S.Z = S.Y + cap;
     S.D = S.Z + cap;
     S.INV = S.D + cap;
-    S.FC = S.INV + cap;
-    S.DFC = S.FC + cap;
-    S.ACC = S.DFC + cap;
-    S.HEAD = S.ACC + 3 * cap;
-    S.SLOT = (int*)(S.HEAD + 3 * 64);
+    S.FC = S.INV + (sizeof(R) == 8 ? cap : 0);
+    S.SLOT = (int*)(S.FC + cap);
     S.HEADJ = S.SLOT + cap;
     S.MASK = (uint32_t*)(S.HEADJ + 64);
 
@@ -280,10 +317,9 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
         {
             for (int t = lane; t < n; t += 64)
             {
-                double f, df;
+                R f, df;
                 cutoff_fdf_t<CT>(cc, S.D[t], f, df);
                 S.FC[t] = f;
-                S.DFC[t] = df;
             }
             rcCached = g.rc;
             __syncthreads();
@@ -296,9 +332,10 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             MemberDesc const& md = mem[lane];
             double const w = dEdGi[md.sf]
                            * (g.type == 2 ? md.sfac : md.fnorm);
-            MP[lane] = make_mparam(md, w);
+            MP[lane] = make_mparam<R>(md, w);
         }
         __syncthreads();
+        R const grc = (R)g.rc;
 
         if (g.type == 2)
         {
@@ -307,22 +344,26 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             for (int t = lane; t < na; t += 64)
             {
                 int const j = a0 + t;
-                double const r = S.D[j];
-                if (!(r < g.rc)) continue;
-                double const fc = S.FC[j], dfc = S.DFC[j], inv = S.INV[j];
-                double P = 0.0;
+                R const r = S.D[j];
+                if (!(r < grc)) continue;
+                R fc, dfc;
+                cutoff_fdf_t<CT>(cc, r, fc, dfc);
+                R const inv = S.inv(j);
+                R P = R(0);
                 for (int m = 0; m < nmem; ++m)
                 {
-                    double const eta = MP[m].eta, rs = MP[m].rs;
-                    double const pexp = exp(-eta * (r - rs) * (r - rs));
-                    P += MP[m].w * ((dfc - 2.0 * eta * (r - rs) * fc) * pexp
+                    R const eta = MP[m].eta, rs = MP[m].rs;
+                    R const pexp = exp_r(-eta * (r - rs) * (r - rs));
+                    P += MP[m].w * ((dfc - R(2) * eta * (r - rs) * fc) * pexp
                                     * inv);
                 }
-                double const x = S.X[j], y = S.Y[j], z = S.Z[j];
-                sx += P * x; sy += P * y; sz += P * z;
-                S.ACC[3 * j] -= P * x;
-                S.ACC[3 * j + 1] -= P * y;
-                S.ACC[3 * j + 2] -= P * z;
+                R const px = P * S.X[j];
+                R const py = P * S.Y[j];
+                R const pz = P * S.Z[j];
+                sx += (double)px; sy += (double)py; sz += (double)pz;
+                S.ACC[3 * j] -= px;
+                S.ACC[3 * j + 1] -= py;
+                S.ACC[3 * j + 2] -= pz;
             }
             __syncthreads();
             continue;
@@ -333,14 +374,15 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                         sCnt[g.e2]);
         int const total = op.total;
         if (total == 0) continue;
-        // Chunk of this lane: [q0, q1); the mask buffer holds a.maskWords
-        // words per lane, enough for the largest possible chunk.
+        // Chunk of this lane: [q0, q1); the mask buffer holds
+        // mask_words(cap) words per lane, enough for the largest chunk.
         {
             int const span = (total + 63) / 64;
             int const q0 = min(total, lane * span);
             int const q1 = min(total, q0 + span);
             int const len = q1 - q0;
             int const nw = (span + 31) >> 5;
+            R const grc2 = (R)g.rc2;
             // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow),
             // walking the chunk's pairs in order.
             int wj = 0, wc = 0, wl = 1, wh = 0;
@@ -361,31 +403,31 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                         int const j = wj;
                         int const k = op.partner(wj, wc, wh);
                         op.next(wj, wc, wl, wh);
-                        bool pass = S.D[j] < g.rc && S.D[k] < g.rc;
+                        bool pass = S.D[j] < grc && S.D[k] < grc;
                         if (pass && !wide)
                         {
-                            double const dx = S.X[k] - S.X[j];
-                            double const dy = S.Y[k] - S.Y[j];
-                            double const dz = S.Z[k] - S.Z[j];
-                            pass = dx * dx + dy * dy + dz * dz < g.rc2;
+                            R const dx = S.X[k] - S.X[j];
+                            R const dy = S.Y[k] - S.Y[j];
+                            R const dz = S.Z[k] - S.Z[j];
+                            pass = dx * dx + dy * dy + dz * dz < grc2;
                         }
                         bits |= (pass ? 1u : 0u) << b;
                     }
                 }
                 S.MASK[wd * 64 + lane] = bits;
             }
-            // Row of the chunk's first pair; owned if it starts here.
-            int curJ = -1;
-            bool curOwned = true;
+            // Row of the chunk's first pair (j0); owned if it starts here.
+            // Rows starting later are always owned.
+            int j0 = -1;
+            bool firstOwned = true;
             // Row of the chunk's last pair, if it starts in this chunk and
             // runs past it: this lane merges its continuation.
             int tailJ = -1, tailEnd = 0;
             if (len > 0)
             {
-                int j0, k0, c0, l0;
+                int k0, c0, l0;
                 op.decode(q0, j0, k0, c0, l0);
-                curJ = j0;
-                curOwned = (c0 == 0);
+                firstOwned = (c0 == 0);
                 int jl, kl, cl, ll;
                 op.decode(q1 - 1, jl, kl, cl, ll);
                 int const rowStart = q1 - 1 - cl;
@@ -396,9 +438,14 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 }
             }
             S.HEADJ[lane] = -1;
-            bool headUsed = false;
-            double cx = 0.0, cy = 0.0, cz = 0.0;
+            RowSum<R> rsum;
+            rsum.clear();
             bool curHas = false;
+            int curJ = -1;
+            bool curOwned = true;
+            // Quantities of the current row j, held in registers.
+            R rij = R(0), invj = R(0), xj = R(0), yj = R(0), zj = R(0);
+            R fcij = R(0), dfcij = R(0);
             int wd = 0;
             uint32_t bits = nw > 0 ? S.MASK[lane] : 0u;
             for (;;)
@@ -415,31 +462,36 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 op.decode(q0 + wd * 32 + b, j, k, cpos, rlen);
                 if (j != curJ)
                 {
-                    if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
+                    if (curHas)
+                        flush(S, curJ, curOwned, rsum, xj, yj, zj, sx, sy,
+                              sz);
+                    curOwned = (j != j0) || firstOwned;
                     curJ = j;
-                    curOwned = true;     // rows after the chunk start
-                    cx = cy = cz = 0.0;
+                    rsum.clear();
                     curHas = false;
+                    rij = S.D[j];
+                    invj = S.inv(j);
+                    xj = S.X[j];
+                    yj = S.Y[j];
+                    zj = S.Z[j];
+                    cutoff_fdf_t<CT>(cc, rij, fcij, dfcij);
                 }
                 // ---- triplet (j as the first neighbour, k second) ----
-                double const rij = S.D[j], rik = S.D[k];
-                double const invj = S.INV[j], invk = S.INV[k];
-                double const xj = S.X[j], yj = S.Y[j], zj = S.Z[j];
-                double const dx = S.X[k] - xj, dy = S.Y[k] - yj,
-                             dz = S.Z[k] - zj;
-                double const fcij = S.FC[j], dfcij = S.DFC[j];
-                double const fcik = S.FC[k];
-                double const rinvijik = invj * invk;
-                double const costijk = (xj * S.X[k] + yj * S.Y[k]
-                                        + zj * S.Z[k]) * rinvijik;
-                double const r2ij = rij * rij, r2ik = rik * rik;
-                double pfc, r2sum, pr1, pr3 = 0.0, rjk = 0.0, invjk = 0.0;
+                R const rik = S.D[k];
+                R const invk = S.inv(k);
+                R const xk = S.X[k], yk = S.Y[k], zk = S.Z[k];
+                R const dx = xk - xj, dy = yk - yj, dz = zk - zj;
+                R const fcik = S.FC[k];
+                R const rinvijik = invj * invk;
+                R const costijk = (xj * xk + yj * yk + zj * zk) * rinvijik;
+                R const r2ij = rij * rij, r2ik = rik * rik;
+                R pfc, r2sum, pr1, pr3 = R(0), rjk = R(0), invjk = R(0);
                 if (!wide)
                 {
-                    double const rjk2 = dx * dx + dy * dy + dz * dz;
+                    R const rjk2 = dx * dx + dy * dy + dz * dz;
                     invjk = rsqrt_nr(rjk2);
                     rjk = rjk2 * invjk;
-                    double fcjk, dfcjk;
+                    R fcjk, dfcjk;
                     cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
                     pfc = fcij * fcik * fcjk;
                     r2sum = r2ij + r2ik + rjk2;
@@ -452,60 +504,71 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                     r2sum = r2ij + r2ik;
                     pr1 = fcik * dfcij * invj;
                 }
-                double T1 = 0.0, T2 = 0.0, T3 = 0.0;
-                double vexp = 0.0, c1 = 0.0, c3 = 0.0;
+                R T1 = R(0), T2 = R(0), T3 = R(0);
+                R vexp = R(0), c1 = R(0), c3 = R(0), S1 = R(0), U = R(0);
                 for (int m = 0; m < nmem; ++m)
                 {
-                    int const flags = MP[m].flags;
-                    if (flags & 2)
+                    int const flags = uniform(MP[m].flags);
+                    if (flags & MF_NEWEXP)
                     {
-                        double const eta = MP[m].eta, rs = MP[m].rs;
-                        double const te = 2.0 * eta * pfc;
-                        if (rs > 0.0)
+                        // Close the previous (eta, rs) run.
+                        T1 += vexp * S1;
+                        R const vu = vexp * U;
+                        T2 += vu * c1;
+                        T3 += vu * c3;
+                        S1 = U = R(0);
+                        R const eta = MP[m].eta, rs = MP[m].rs;
+                        R const te = R(2) * eta * pfc;
+                        if (rs > R(0))
                         {
-                            double const rijs = rij - rs;
-                            double const riks = rik - rs;
-                            double s = rijs * rijs + riks * riks;
+                            R const rijs = rij - rs;
+                            R const riks = rik - rs;
+                            R s = rijs * rijs + riks * riks;
                             c1 = pr1 - te * rijs * invj;
                             if (!wide)
                             {
-                                double const rjks = rjk - rs;
+                                R const rjks = rjk - rs;
                                 s += rjks * rjks;
                                 c3 = te * rjks * invjk - pr3;
                             }
-                            vexp = exp(-eta * s);
+                            vexp = exp_r(-eta * s);
                         }
                         else
                         {
                             c1 = pr1 - te;
                             if (!wide) c3 = te - pr3;
-                            vexp = exp(-eta * r2sum);
+                            vexp = exp_r(-eta * r2sum);
                         }
                     }
-                    double const pl = 1.0 + MP[m].lambda * costijk;
-                    double fgw = 0.0;
-                    if (pl > 0.0)
+                    R const pl = R(1) + MP[m].lambda * costijk;
+                    if (pl > R(0))
                     {
-                        double const pw = (!GENPOW || (flags & 1))
-                            ? pow_int(pl, MP[m].zint - 1)
-                            : pow(pl, MP[m].zeta - 1.0);
-                        fgw = MP[m].w * vexp * pw;
+                        R pw;
+                        if (!GENPOW || (flags & MF_INTPOW))
+                            pw = pow_uniform(pl, uniform(MP[m].zint) - 1);
+                        else pw = pow(pl, MP[m].zeta - R(1));
+                        R const fgw = MP[m].w * pw;
+                        S1 += fgw * MP[m].zl;
+                        U += fgw * pl;
                     }
-                    T1 += fgw * MP[m].zl;
-                    double const fp = fgw * pl;
-                    T2 += fp * c1;
-                    T3 += fp * c3;
                 }
-                double const A = rinvijik - costijk * invj * invj;
-                double const P1 = pfc * A * T1 + T2;
-                double const P3 = pfc * rinvijik * T1 + T3;
-                sx += P1 * xj; sy += P1 * yj; sz += P1 * zj;
-                cx -= P1 * xj + P3 * dx;
-                cy -= P1 * yj + P3 * dy;
-                cz -= P1 * zj + P3 * dz;
+                T1 += vexp * S1;
+                {
+                    R const vu = vexp * U;
+                    T2 += vu * c1;
+                    T3 += vu * c3;
+                }
+                R const A = rinvijik - costijk * invj * invj;
+                R const P1 = pfc * A * T1 + T2;
+                R const P3 = pfc * rinvijik * T1 + T3;
+                rsum.p1 += P1;
+                rsum.vx += P3 * dx;
+                rsum.vy += P3 * dy;
+                rsum.vz += P3 * dz;
                 curHas = true;
             }
-            if (curHas) flush(S, curJ, curOwned, cx, cy, cz, headUsed);
+            if (curHas)
+                flush(S, curJ, curOwned, rsum, xj, yj, zj, sx, sy, sz);
             __syncthreads();
             // The lane owning the start of a row that runs past its chunk
             // adds the head partials of the following chunks of that row,
@@ -513,7 +576,7 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             // pairs (HEADJ = -1) are skipped.
             if (tailJ >= 0)
             {
-                double hx = 0.0, hy = 0.0, hz = 0.0;
+                R hx = R(0), hy = R(0), hz = R(0);
                 for (int L = lane + 1; L < 64 && L * span < tailEnd; ++L)
                     if (S.HEADJ[L] == tailJ)
                     {
@@ -526,7 +589,6 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 S.ACC[3 * tailJ + 2] += hz;
             }
             __syncthreads();
-            (void)headUsed;
         }
     }
     for (int t = lane; t < n; t += 64)
@@ -585,17 +647,26 @@ __global__ __launch_bounds__(256) void k_gather(
     }
 }
 
+template <typename R>
+void launch_bwd(ForceArgs const& a, size_t shm, hipStream_t s)
+{
+    if (a.T.genpow)
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
+            (k_force_backward<kCT, true, R><<<a.n, 64, shm, s>>>(a)));
+    else
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
+            (k_force_backward<kCT, false, R><<<a.n, 64, shm, s>>>(a)));
+}
+
 }  // namespace
 
-size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf)
+size_t force_backward_lds_bytes(int cap, int fp32)
 {
-    (void)maxMembers;
-    (void)maxCfg;
-    (void)maxnsf;
+    size_t const r = fp32 ? sizeof(float) : sizeof(double);
     size_t b = 0;
-    b += (size_t)cap * 7 * sizeof(double);        // X Y Z D INV FC DFC
-    b += (size_t)cap * 3 * sizeof(double);        // ACC
-    b += (size_t)64 * 3 * sizeof(double);         // HEAD
+    b += (size_t)cap * 3 * r;                     // ACC
+    b += (size_t)64 * 3 * r;                      // HEAD
+    b += (size_t)cap * (fp32 ? 5 : 6) * r;        // X Y Z D [INV] FC
     b += (size_t)cap * sizeof(int);               // SLOT
     b += (size_t)64 * sizeof(int);                // HEADJ
     b += (size_t)mask_words(cap) * 64 * sizeof(uint32_t);   // MASK
@@ -605,14 +676,9 @@ size_t force_backward_lds_bytes(int cap, int maxMembers, int maxCfg, int maxnsf)
 hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    size_t const shm = force_backward_lds_bytes(a.cap, a.maxMembers, a.maxCfg,
-                                                a.T.maxnsf);
-    if (a.T.genpow)
-        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
-                              (k_force_backward<kCT, true><<<a.n, 64, shm, s>>>(a)));
-    else
-        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
-                              (k_force_backward<kCT, false><<<a.n, 64, shm, s>>>(a)));
+    size_t const shm = force_backward_lds_bytes(a.cap, a.fp32);
+    if (a.fp32) launch_bwd<float>(a, shm, s);
+    else launch_bwd<double>(a, shm, s);
     return hipGetLastError();
 }
 

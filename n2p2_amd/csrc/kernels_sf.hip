@@ -1,4 +1,6 @@
-// Symmetry-function forward kernel (FP64): G values of every atom.
+// Symmetry-function forward kernel: G values of every atom, in FP64 (plain
+// path) or with FP32 geometry and member sums (mixed path; lane partials
+// summed in FP64, scaling in FP64).
 //
 // Restates SymGrpExpRad::calculate (SymGrpExpRad.cpp:144-239),
 // SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403) and
@@ -14,7 +16,10 @@
 //      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
 //      with a ballot prefix; whenever 64 are queued every lane evaluates
 //      one, so the expensive triplet work runs without divergence;
-//   4. member sums are reduced with a fixed butterfly (deterministic).
+//   4. inside a triplet the members are walked as (eta, rs) runs sharing one
+//      exponential; integer powers branch on the wave-uniform exponent to
+//      straight-line products (bit-identical to pow_int);
+//   5. member sums are reduced with a fixed butterfly (deterministic).
 #include "device_common.h"
 #include "kernels.h"
 
@@ -24,14 +29,15 @@ namespace {
 
 constexpr int kQueue = 128;
 
+template <typename R>
 struct Stage
 {
-    double* X;
-    double* Y;
-    double* Z;
-    double* D;
-    double* INV;
-    double* FC;
+    R* X;
+    R* Y;
+    R* Z;
+    R* D;
+    R* INV;
+    R* FC;
 };
 
 __device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
@@ -45,7 +51,8 @@ __device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
 
 // Stages the atom's neighbours in LDS partitioned by element (stable ballot
 // compaction); sStart / sCnt receive each element's block.
-__device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
+template <typename R>
+__device__ void stage_by_element(SFArgs const& a, Stage<R> const& S, int s0,
                                  int n, int* sStart, int* sCnt)
 {
     int const lane = threadIdx.x;
@@ -99,11 +106,11 @@ __device__ void stage_by_element(SFArgs const& a, Stage const& S, int s0,
                 if (e == q)
                 {
                     int const pos = start[q] + lanes_below(mask);
-                    S.X[pos] = g.x;
-                    S.Y[pos] = g.y;
-                    S.Z[pos] = g.z;
-                    S.D[pos] = g.w;
-                    if (S.INV) S.INV[pos] = 1.0 / g.w;
+                    S.X[pos] = (R)g.x;
+                    S.Y[pos] = (R)g.y;
+                    S.Z[pos] = (R)g.z;
+                    S.D[pos] = (R)g.w;
+                    if (S.INV) S.INV[pos] = R(1) / (R)g.w;
                 }
                 start[q] += __popcll(mask);
             }
@@ -162,35 +169,37 @@ struct PairCursor
 };
 
 // rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
-__device__ __forceinline__ bool pair_passes(Stage const& S, int ja, int kb,
+template <typename R>
+__device__ __forceinline__ bool pair_passes(Stage<R> const& S, int ja, int kb,
                                             bool wide, GroupDesc const& g)
 {
-    if (!(S.D[ja] < g.rc && S.D[kb] < g.rc)) return false;
+    R const rc = (R)g.rc;
+    if (!(S.D[ja] < rc && S.D[kb] < rc)) return false;
     if (wide) return true;
-    double const dx = S.X[kb] - S.X[ja];
-    double const dy = S.Y[kb] - S.Y[ja];
-    double const dz = S.Z[kb] - S.Z[ja];
-    return (dx * dx + dy * dy + dz * dz) < g.rc2;
+    R const dx = S.X[kb] - S.X[ja];
+    R const dy = S.Y[kb] - S.Y[ja];
+    R const dz = S.Z[kb] - S.Z[ja];
+    return (dx * dx + dy * dy + dz * dz) < (R)g.rc2;
 }
 
 // One triplet, all members of the group: acc[m] += fg * plambda * pfc with
 // fg = exp(-eta sum r^2) plambda^(zeta - 1) (SymGrpExpAngn.cpp:276-297).
-template <int MAXM, int CT, bool GENPOW>
+template <int MAXM, int CT, bool GENPOW, typename R>
 __device__ __forceinline__ void triplet_forward(
-    Stage const& S, int ja, int kb, bool wide, CutCfg const& cc,
-    MParam const* MP, int nmem, double (&acc)[MAXM])
+    Stage<R> const& S, int ja, int kb, bool wide, CutCfg const& cc,
+    MParam<R> const* MP, int nmem, R (&acc)[MAXM])
 {
-    double const rij = S.D[ja], rik = S.D[kb];
-    double const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
-    double const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
-    double const fcij = S.FC[ja], fcik = S.FC[kb];
-    double pfc, r2sum, rjk = 0.0;
+    R const rij = S.D[ja], rik = S.D[kb];
+    R const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
+    R const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
+    R const fcij = S.FC[ja], fcik = S.FC[kb];
+    R pfc, r2sum, rjk = R(0);
     if (!wide)
     {
-        double const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
-        double const rjk2 = dx * dx + dy * dy + dz * dz;
+        R const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+        R const rjk2 = dx * dx + dy * dy + dz * dz;
         rjk = rjk2 * rsqrt_nr(rjk2);
-        double fcjk, dfcjk;
+        R fcjk, dfcjk;
         cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
         pfc = fcij * fcik * fcjk;
         r2sum = rij * rij + rik * rik + rjk2;
@@ -200,54 +209,55 @@ __device__ __forceinline__ void triplet_forward(
         pfc = fcij * fcik;
         r2sum = rij * rij + rik * rik;
     }
-    double const rinvijik = S.INV[ja] * S.INV[kb];
-    double const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
-    double vexp = 0.0;
+    R const rinvijik = S.INV[ja] * S.INV[kb];
+    R const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
+    R vp = R(0);             // exp(.) * pfc of the current (eta, rs) run
 #pragma unroll
     for (int m = 0; m < MAXM; ++m)
     {
         if (m < nmem)
         {
-            int const flags = MP[m].flags;
-            if (flags & 2)
+            int const flags = uniform(MP[m].flags);
+            if (flags & MF_NEWEXP)
             {
-                double const eta = MP[m].eta, rs = MP[m].rs;
-                if (rs > 0.0)
+                R const eta = MP[m].eta, rs = MP[m].rs;
+                R vexp;
+                if (rs > R(0))
                 {
-                    double const a = rij - rs, b = rik - rs;
-                    double s = a * a + b * b;
+                    R const a = rij - rs, b = rik - rs;
+                    R s = a * a + b * b;
                     if (!wide)
                     {
-                        double const c = rjk - rs;
+                        R const c = rjk - rs;
                         s += c * c;
                     }
-                    vexp = exp(-eta * s);
+                    vexp = exp_r(-eta * s);
                 }
-                else vexp = exp(-eta * r2sum);
+                else vexp = exp_r(-eta * r2sum);
+                vp = vexp * pfc;
             }
-            double const plambda = 1.0 + MP[m].lambda * costijk;
-            double fg = 0.0;
-            if (plambda > 0.0)
+            R const pl = R(1) + MP[m].lambda * costijk;
+            if (pl > R(0))
             {
-                if (!GENPOW || (flags & 1))
-                    fg = vexp * pow_int(plambda, MP[m].zint - 1);
-                else
-                    fg = vexp * pow(plambda, MP[m].zeta - 1.0);
+                R pw;
+                if (!GENPOW || (flags & MF_INTPOW))
+                    pw = pow_uniform(pl, uniform(MP[m].zint) - 1);
+                else pw = pow(pl, MP[m].zeta - R(1));
+                acc[m] += vp * (pw * pl);
             }
-            acc[m] += fg * plambda * pfc;
         }
     }
 }
 
-template <int MAXM, int CT, bool GENPOW>
+template <int MAXM, int CT, bool GENPOW, typename R>
 __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
 {
     extern __shared__ double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
-    __shared__ MParam MP[kMaxMembers];
+    __shared__ MParam<R> MP[kMaxMembers];
     int const cap = a.cap;
-    Stage S;
-    S.X = sm;
+    Stage<R> S;
+    S.X = (R*)sm;
     S.Y = S.X + cap;
     S.Z = S.Y + cap;
     S.D = S.Z + cap;
@@ -273,37 +283,38 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         int const nmem = g.nmem;
-        if (lane < nmem) MP[lane] = make_mparam(mem[lane], 0.0);
+        if (lane < nmem) MP[lane] = make_mparam<R>(mem[lane], 0.0);
         if (g.rc != rcCached)
         {
             for (int t = lane; t < n; t += 64)
             {
-                double f, df;
+                R f, df;
                 cutoff_fdf_t<CT>(cc, S.D[t], f, df);
                 S.FC[t] = f;
             }
             rcCached = g.rc;
         }
         __syncthreads();
-        double acc[MAXM];
+        R acc[MAXM];
 #pragma unroll
-        for (int m = 0; m < MAXM; ++m) acc[m] = 0.0;
+        for (int m = 0; m < MAXM; ++m) acc[m] = R(0);
 
         if (g.type == 2)
         {
             int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            R const rc = (R)g.rc;
             for (int t = lane; t < na; t += 64)
             {
-                double const r = S.D[a0 + t];
-                if (r < g.rc)
+                R const r = S.D[a0 + t];
+                if (r < rc)
                 {
-                    double const fc = S.FC[a0 + t];
+                    R const fc = S.FC[a0 + t];
 #pragma unroll
                     for (int m = 0; m < MAXM; ++m)
                         if (m < nmem)
                         {
-                            double const eta = MP[m].eta, rs = MP[m].rs;
-                            acc[m] += exp(-eta * (r - rs) * (r - rs)) * fc;
+                            R const eta = MP[m].eta, rs = MP[m].rs;
+                            acc[m] += exp_r(-eta * (r - rs) * (r - rs)) * fc;
                         }
                 }
             }
@@ -345,7 +356,7 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
         {
             if (m < nmem)
             {
-                double const v = wave_sum(acc[m]);
+                double const v = wave_sum((double)acc[m]);
                 if (lane == m)
                 {
                     MemberDesc const& md = mem[m];
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
 }
 
 // Work counters of the algorithm for one structure (same enumeration and
-// filters as k_sf_forward, no arithmetic on the members):
+// filters as the FP64 k_sf_forward, no arithmetic on the members):
 //   [0] neighbour-list entries
 //   [1] radial member evaluations        [2] narrow pairs tested (rij, rik < rc)
 //   [3] wide pairs tested                [4] narrow triplets (rjk < rc)
@@ -375,7 +386,7 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     extern __shared__ double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     int const cap = a.cap;
-    Stage S;
+    Stage<double> S;
     S.X = sm;
     S.Y = S.X + cap;
     S.Z = S.Y + cap;
@@ -516,30 +527,39 @@ __global__ void k_ew(ModelTables T, const double* __restrict__ G, int ldG,
     ew[i] = c;
 }
 
-template <int MAXM, int CT>
+template <int MAXM, int CT, typename R>
 void launch_fwd(SFArgs const& a, hipStream_t s)
 {
-    size_t const shm = sf_forward_lds_bytes(a.cap);
+    size_t const shm = sf_forward_lds_bytes(a.cap, a.fp32);
     if (a.T.genpow)
-        k_sf_forward<MAXM, CT, true><<<a.n, 64, shm, s>>>(a);
+        k_sf_forward<MAXM, CT, true, R><<<a.n, 64, shm, s>>>(a);
     else
-        k_sf_forward<MAXM, CT, false><<<a.n, 64, shm, s>>>(a);
+        k_sf_forward<MAXM, CT, false, R><<<a.n, 64, shm, s>>>(a);
+}
+
+template <typename R>
+void launch_fwd_r(SFArgs const& a, hipStream_t s)
+{
+    if (a.T.maxGroupMembers <= 8)
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType, (launch_fwd<8, kCT, R>(a, s)));
+    else
+        HDNNP_CUTOFF_DISPATCH(a.T.cutType,
+                              (launch_fwd<kMaxMembers, kCT, R>(a, s)));
 }
 
 }  // namespace
 
-size_t sf_forward_lds_bytes(int cap)
+size_t sf_forward_lds_bytes(int cap, int fp32)
 {
-    return (size_t)cap * 6 * sizeof(double) + kQueue * sizeof(int);
+    size_t const r = fp32 ? sizeof(float) : sizeof(double);
+    return (size_t)cap * 6 * r + kQueue * sizeof(int);
 }
 
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    if (a.T.maxGroupMembers <= 8)
-        HDNNP_CUTOFF_DISPATCH(a.T.cutType, (launch_fwd<8, kCT>(a, s)));
-    else
-        HDNNP_CUTOFF_DISPATCH(a.T.cutType, (launch_fwd<kMaxMembers, kCT>(a, s)));
+    if (a.fp32) launch_fwd_r<float>(a, s);
+    else launch_fwd_r<double>(a, s);
     return hipGetLastError();
 }
 
@@ -547,7 +567,7 @@ hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
                            hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    k_sf_count<<<a.n, 64, sf_forward_lds_bytes(a.cap), s>>>(a, counters);
+    k_sf_count<<<a.n, 64, sf_forward_lds_bytes(a.cap, 0), s>>>(a, counters);
     return hipGetLastError();
 }
 

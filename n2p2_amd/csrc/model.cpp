@@ -579,9 +579,20 @@ Model::Flat Model::flatten() const
             for (auto const& g : e.groups)
             {
                 if ((pass == 0) != (g.type == 2)) continue;
-                // Members sorted as SymGrp*::sortMembers (SF order).
+                // Members in SF order (SymGrp*::sortMembers), then, for the
+                // angular types, stably grouped into (eta, rs) runs: a run
+                // shares one exponential and its derivative factors.  Each
+                // member keeps its own SF slot, so the order is free.
                 std::vector<int> mem = g.members;
                 std::sort(mem.begin(), mem.end());
+                if (g.type != 2)
+                    std::stable_sort(mem.begin(), mem.end(),
+                                     [&](int a, int b) {
+                        SymFn const& x = e.sf[a];
+                        SymFn const& y = e.sf[b];
+                        if (x.eta != y.eta) return x.eta < y.eta;
+                        return x.rs < y.rs;
+                    });
                 for (size_t c0 = 0; c0 < mem.size(); c0 += kMaxMembers)
                 {
                     GroupDesc gd{};
