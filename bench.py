@@ -162,6 +162,10 @@ def main():
     ap.add_argument("--nmol", type=int, default=65536)
     ap.add_argument("--box", type=float, default=124.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=("both", "mixed", "fp64"),
+                    default="both",
+                    help="time one path only (for profiling runs; the "
+                         "reported line needs both)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the multi-GPU path "
@@ -184,14 +188,20 @@ def main():
                           [[model.num_sf(e)] + list(hidden) + [1]])
                       for e in range(model.num_elements)))
 
-        dt64, e64, f64, kt64 = run_path(ctx, engine.PRECISION_FP64,
+        if args.path != "mixed":
+            dt64, e64, f64, kt64 = run_path(ctx, engine.PRECISION_FP64,
+                                            args.steps, args.warmup)
+            log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
+                % (dt64 * 1e3, e64, kt64))
+        if args.path != "fp64":
+            dtm, em, fm, ktm = run_path(ctx, engine.PRECISION_MIXED,
                                         args.steps, args.warmup)
-        log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
-            % (dt64 * 1e3, e64, kt64))
-        dtm, em, fm, ktm = run_path(ctx, engine.PRECISION_MIXED, args.steps,
-                                    args.warmup)
-        log("mixed: %.3f ms/step, E=%.10f, kernel ms %s"
-            % (dtm * 1e3, em, ktm))
+            log("mixed: %.3f ms/step, E=%.10f, kernel ms %s"
+                % (dtm * 1e3, em, ktm))
+        if args.path != "both":
+            log("single-path run: no JSON line")
+            ctx.close()
+            return
         df = fm - f64
         err = {"force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
                "force_max_eV_A": float(np.abs(df).max()),

@@ -274,6 +274,33 @@ __device__ __forceinline__ R pow_fixed(R x)
     return r;
 }
 
+// Powers of x = plambda shared by the members of an (eta, rs, lambda) run
+// sorted by zeta: after l steps P = x^(2^l - 1) and X = x^(2^l), the
+// running product and square of pow_int's loop, so at(l) for zeta = 2^l is
+// bit-identical to pow_int(x, zeta - 1).  l is wave-uniform.
+template <typename R>
+struct PowChain
+{
+    R P, X;
+    int l;
+    __device__ __forceinline__ void reset(R x)
+    {
+        P = R(1);
+        X = x;
+        l = 0;
+    }
+    __device__ __forceinline__ R at(int level)
+    {
+        while (l < level)
+        {
+            P *= X;
+            X *= X;
+            ++l;
+        }
+        return P;
+    }
+};
+
 // pow_int(x, n) for a wave-uniform n: a scalar branch to straight-line
 // products for 0 <= n <= 16, the loop otherwise.
 template <typename R>
@@ -368,14 +395,16 @@ __device__ __forceinline__ void activate(int a, double x, double& v, double& d)
 enum MemberFlags
 {
     MF_INTPOW = 1,   // integer zeta (pow_int)
-    MF_NEWEXP = 2    // first member of an (eta, rs) run: new exponential
+    MF_NEWEXP = 2,   // first member of an (eta, rs) run: new exponential
+    MF_NEWLAM = 4,   // first member of an (eta, rs, lambda) run
+    MF_PREFIX = 8    // zeta = 2^level: power from the run's prefix chain
 };
 
 template <typename R>
 struct MParam
 {
     R   lambda, zl, eta, rs, zeta, w;   // w: per-kernel weight
-    int zint, flags;
+    int zint, flags;                    // flags | level << 8
 };
 
 template <typename R>
@@ -390,13 +419,72 @@ __device__ __forceinline__ MParam<R> make_mparam(MemberDesc const& md,
     p.zeta = (R)md.zeta;
     p.w = (R)w;
     p.zint = md.zint;
-    p.flags = (md.useint ? MF_INTPOW : 0) | (md.calcexp ? MF_NEWEXP : 0);
+    p.flags = (md.useint ? MF_INTPOW : 0) | (md.calcexp ? MF_NEWEXP : 0)
+            | (md.newlam ? MF_NEWLAM : 0)
+            | (md.level >= 0 ? MF_PREFIX | (md.level << 8) : 0);
     return p;
 }
 
 __device__ __forceinline__ int uniform(int v)
 {
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Wave-uniform copies of floating-point values (scalar registers).
+__device__ __forceinline__ float uniform_r(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ double uniform_r(double v)
+{
+    long long const b = __double_as_longlong(v);
+    int const lo = __builtin_amdgcn_readfirstlane((int)b);
+    int const hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// exp(-eta s) from the exponent factor of exp_factor(eta): FP64 takes the
+// library exp of -eta s; FP32 the hardware 2^x of (-eta log2 e) s.
+__device__ __forceinline__ double exp_factor(double eta) { return -eta; }
+__device__ __forceinline__ float exp_factor(float eta)
+{
+    return -eta * 1.44269504088896340736f;
+}
+__device__ __forceinline__ double exp_scaled(double a) { return exp(a); }
+__device__ __forceinline__ float exp_scaled(float a)
+{
+    return __builtin_amdgcn_exp2f(a);
+}
+
+// Cutoff value and derivative for rc > r >= 0 without branches for the
+// specialised types: x = (r - rci) iw clamped to [0, 1] gives f = 1, f' = 0
+// below rci exactly as the branch does (the cosine form to rounding).  Other
+// types take the general form.
+template <int CT, typename R>
+__device__ __forceinline__ void cutoff_inside(CutCfg const& c, R r, R& fc,
+                                              R& dfc)
+{
+    if (CT == CT_COS)
+    {
+        R const iw = (R)c.iw;
+        R x = (r - (R)c.rci) * iw;
+        x = fmin(fmax(x, R(0)), R(1));
+        R sn, cs;
+        sincospi01(x, sn, cs);
+        fc = R(0.5) * (cs + R(1));
+        dfc = R(-0.5) * iw * (R)kPi * sn;
+    }
+    else if (CT == CT_POLY2)
+    {
+        R const iw = (R)c.iw;
+        R x = (r - (R)c.rci) * iw;
+        x = fmin(fmax(x, R(0)), R(1));
+        R const x2 = x * x;
+        fc = ((R(15) - R(6) * x) * x - R(10)) * x * x2 + R(1);
+        dfc = x2 * ((R(60) - R(30) * x) * x - R(30)) * iw;
+    }
+    else cutoff_fdf_t<CT>(c, r, fc, dfc);
 }
 
 // SymFnc::scale (SymFnc.cpp:169-195).

@@ -16,9 +16,10 @@
 // slot of j that points back at m with the opposite image.
 //
 // Backward kernel, one wavefront per atom:
-//  * neighbours staged in LDS partitioned by element (as the SF kernel);
-//  * groups run one after another, so member parameters and the atom's
-//    dE/dG weights are wave-uniform;
+//  * neighbours staged in LDS partitioned by element (as the SF kernel), as
+//    packed records (x, y, z, r) and (f_c(r), 1/r);
+//  * groups run one after another; a group's member parameters, with this
+//    atom's dE/dG weights, are held in scalar registers for the pair loop;
 //  * the member sum is contracted before the vector work.  With
 //    fg = exp(.) (1 + lambda cos)^(zeta-1) (SymGrpExpAngn.cpp:276-297) the
 //    reference's p1 and p3 (:300-337) are
@@ -31,18 +32,22 @@
 //      T1 = sum w fg zeta lambda, T2 = sum w fg plambda c1,
 //      T3 = sum w fg plambda c3,  w_m = dE/dG_m * factorNorm_m.
 //    c1, c3 and exp(.) are shared by an (eta, rs) run of members, so T2 and
-//    T3 are formed once per run from U = sum w pw plambda; the powers come
-//    from pow_uniform (straight-line products per uniform exponent);
+//    T3 are formed once per run from U = sum w pw plambda; an (eta, rs,
+//    lambda) run sorted by zeta shares one chain of powers (PowChain);
 //  * every ordered pair (j, k) of a group is evaluated by the lane that
 //    owns slot j and contributes -(P1 dr_ij + P3 dr_jk) to slot j and
 //    P1 dr_ij to the centre; slot k's part comes from the pair (k, j).  A
 //    lane sums P1 and P3 dr_jk over its run of pairs of one row j and
 //    applies dr_ij once per run.  The group's ordered pairs are split into
-//    64 equal contiguous chunks, one per lane; a lane adds the sum of every
-//    slot whose pairs start inside its chunk straight into the LDS
-//    accumulator and leaves the sum of the slot it inherited from the
-//    previous chunk in a per-lane head buffer, merged afterwards in lane
-//    order.  No atomics; fixed summation order.
+//    64 equal contiguous chunks, one per lane; a lane first marks its
+//    chunk's passing pairs in an LDS bit mask, then walks the set bits with
+//    an incremental (row, column) cursor.  It adds the sum of every slot
+//    whose pairs start inside its chunk straight into the LDS accumulator
+//    and leaves the sum of the slot it inherited from the previous chunk in
+//    a per-lane head buffer, merged afterwards in lane order.  No atomics;
+//    fixed summation order.
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -58,27 +63,22 @@ __host__ __device__ inline int mask_words(int cap)
     return (span + 31) / 32;
 }
 
-// LDS of one wave.  1/r is staged in FP64 only (FP32 takes the hardware
-// reciprocal); f_c'(r) is formed per row, not staged.
+template <typename R>
+using Vec4 = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+template <typename R>
+using Vec2 = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+
+// LDS of one wave.
 template <typename R>
 struct BLds
 {
+    Vec4<R>* P;      // (x, y, z, r) per staged neighbour
+    Vec2<R>* Q;      // (f_c(r) for the current cutoff, 1/r)
     R* ACC;          // 3 per staged neighbour
     R* HEAD;         // 3 per lane
-    R* X;
-    R* Y;
-    R* Z;
-    R* D;
-    R* INV;          // FP64 only
-    R* FC;
     int* SLOT;       // original slot of each staged neighbour
     int* HEADJ;      // per lane
     uint32_t* MASK;  // [word][64]
-    __device__ R inv(int j) const
-    {
-        if (sizeof(R) == 8) return INV[j];
-        return rcp_r(D[j]);
-    }
 };
 
 template <typename R>
@@ -136,11 +136,13 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
                 if (e == q)
                 {
                     int const pos = start[q] + lanes_below(mask);
-                    S.X[pos] = (R)g.x;
-                    S.Y[pos] = (R)g.y;
-                    S.Z[pos] = (R)g.z;
-                    S.D[pos] = (R)g.w;
-                    if (sizeof(R) == 8) S.INV[pos] = R(1) / (R)g.w;
+                    Vec4<R> p;
+                    p.x = (R)g.x;
+                    p.y = (R)g.y;
+                    p.z = (R)g.z;
+                    p.w = (R)g.w;
+                    S.P[pos] = p;
+                    S.Q[pos].y = R(1) / (R)g.w;
                     S.SLOT[pos] = s;
                     S.ACC[3 * pos] = R(0);
                     S.ACC[3 * pos + 1] = R(0);
@@ -178,9 +180,9 @@ struct OrderedPairs
             invB = na > 0 ? 1.0f / (float)na : 0.0f;   // rows of B
         }
     }
-    // j, k as staged indices; c = position of q in row j (q - c is the
-    // row's first pair); len = the row's length.
-    __device__ void decode(int q, int& j, int& k, int& c, int& len) const
+    // j (staged index of the row), c = position of q in row j, len = the
+    // row's length, half = 1 for the B x A part.
+    __device__ void decode(int q, int& j, int& c, int& len, int& half) const
     {
         int r;
         if (same)
@@ -190,8 +192,8 @@ struct OrderedPairs
             c = q - r * d;
             if (c < 0) { --r; c += d; } else if (c >= d) { ++r; c -= d; }
             j = a0 + r;
-            k = a0 + (c < r ? c : c + 1);
             len = d;
+            half = 0;
         }
         else if (q < split)
         {
@@ -199,8 +201,8 @@ struct OrderedPairs
             c = q - r * nb;
             if (c < 0) { --r; c += nb; } else if (c >= nb) { ++r; c -= nb; }
             j = a0 + r;
-            k = b0 + c;
             len = nb;
+            half = 0;
         }
         else
         {
@@ -209,15 +211,13 @@ struct OrderedPairs
             c = qq - r * na;
             if (c < 0) { --r; c += na; } else if (c >= na) { ++r; c -= na; }
             j = b0 + r;
-            k = a0 + c;
             len = na;
+            half = 1;
         }
     }
-    // Sequential walk: (j, c, len, half) of pair q -> pair q + 1.
-    __device__ void next(int& j, int& c, int& len, int& half) const
+    // The row after row j.
+    __device__ void next_row(int& j, int& len, int& half) const
     {
-        if (++c < len) return;
-        c = 0;
         if (!same && half == 0 && j == a0 + na - 1)
         {
             half = 1;
@@ -226,12 +226,54 @@ struct OrderedPairs
         }
         else ++j;
     }
+    // Sequential walk: (j, c, len, half) of pair q -> pair q + 1.
+    __device__ void next(int& j, int& c, int& len, int& half) const
+    {
+        if (++c < len) return;
+        c = 0;
+        next_row(j, len, half);
+    }
+    // Advances the cursor by d >= 0 pairs.
+    __device__ void advance(int d, int& j, int& c, int& len, int& half) const
+    {
+        c += d;
+        while (c >= len)
+        {
+            c -= len;
+            next_row(j, len, half);
+        }
+    }
     __device__ int partner(int j, int c, int half) const
     {
         if (same) return a0 + (c < j - a0 ? c : c + 1);
         return half == 0 ? b0 + c : a0 + c;
     }
 };
+
+// A group's member parameters in wave-uniform (scalar) registers: lambda,
+// zeta lambda, the weight and the flags word.  The exponential's parameters
+// are read from LDS at the start of each (eta, rs) run.
+template <typename R, int MAXM>
+struct Members
+{
+    R lam[MAXM], zl[MAXM], w[MAXM];
+    int code[MAXM];
+};
+
+template <typename R, int MAXM>
+__device__ __forceinline__ void load_members(MParam<R> const* MP, int nmem,
+                                             Members<R, MAXM>& M)
+{
+#pragma unroll
+    for (int m = 0; m < MAXM; ++m)
+    {
+        MParam<R> const& p = MP[m < nmem ? m : 0];
+        M.lam[m] = uniform_r(p.lambda);
+        M.zl[m] = uniform_r(p.zl);
+        M.w[m] = uniform_r(p.w);
+        M.code[m] = uniform(p.flags);
+    }
+}
 
 // A lane's partial sums over its run of passing pairs of one row j:
 // sum P1 and sum P3 dr_jk.
@@ -276,23 +318,135 @@ __device__ __forceinline__ void flush(BLds<R> const& S, int j, bool owned,
     }
 }
 
-template <int CT, bool GENPOW, typename R>
-__global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
+// Quantities of row j held in registers while a lane walks the row.
+template <typename R>
+struct Row
 {
-    extern __shared__ double sm[];
+    R x, y, z, r, inv, fc, dfcinv, r2, inv2;
+};
+
+// P1 and P3 of one triplet (j first neighbour, k second), all members.
+template <int CT, bool GENPOW, int MAXM, typename R>
+__device__ __forceinline__ void triplet_backward(
+    Row<R> const& J, Vec4<R> const& pk, Vec2<R> const& qk, bool wide,
+    CutCfg const& cc, Members<R, MAXM> const& M, MParam<R> const* MP,
+    int nmem, R& P1, R& P3, R& dx, R& dy, R& dz)
+{
+    R const rik = pk.w, fcik = qk.x, invk = qk.y;
+    dx = pk.x - J.x;
+    dy = pk.y - J.y;
+    dz = pk.z - J.z;
+    R const rinvijik = J.inv * invk;
+    R const costijk = (J.x * pk.x + J.y * pk.y + J.z * pk.z) * rinvijik;
+    R pfc, r2sum, pr1, pr3 = R(0), rjk = R(0), invjk = R(0);
+    if (!wide)
+    {
+        R const rjk2 = dx * dx + dy * dy + dz * dz;
+        invjk = rsqrt_nr(rjk2);
+        rjk = rjk2 * invjk;
+        R fcjk, dfcjk;
+        cutoff_inside<CT>(cc, rjk, fcjk, dfcjk);
+        R const fij_ik = J.fc * fcik;
+        pfc = fij_ik * fcjk;
+        r2sum = J.r2 + rik * rik + rjk2;
+        pr1 = fcik * fcjk * J.dfcinv;
+        pr3 = fij_ik * dfcjk * invjk;
+    }
+    else
+    {
+        pfc = J.fc * fcik;
+        r2sum = J.r2 + rik * rik;
+        pr1 = fcik * J.dfcinv;
+    }
+    R T1 = R(0), T2 = R(0), T3 = R(0);
+    R vexp = R(0), c1 = R(0), c3 = R(0), S1 = R(0), U = R(0);
+    R pl = R(0);
+    PowChain<R> chain;
+    chain.reset(R(0));
+#pragma unroll
+    for (int m = 0; m < MAXM; ++m)
+    {
+        if (m < nmem)
+        {
+            int const fl = M.code[m];
+            if (fl & MF_NEWEXP)
+            {
+                if (m > 0)
+                {
+                    // Close the previous (eta, rs) run.
+                    T1 += vexp * S1;
+                    R const vu = vexp * U;
+                    T2 += vu * c1;
+                    T3 += vu * c3;
+                    S1 = U = R(0);
+                }
+                R const rs = MP[m].rs, eta = MP[m].eta;
+                R const neta = exp_factor(eta);
+                R const te = R(2) * eta * pfc;
+                if (rs > R(0))
+                {
+                    R const rijs = J.r - rs;
+                    R const riks = rik - rs;
+                    R s = rijs * rijs + riks * riks;
+                    c1 = pr1 - te * rijs * J.inv;
+                    if (!wide)
+                    {
+                        R const rjks = rjk - rs;
+                        s += rjks * rjks;
+                        c3 = te * rjks * invjk - pr3;
+                    }
+                    vexp = exp_scaled(neta * s);
+                }
+                else
+                {
+                    c1 = pr1 - te;
+                    if (!wide) c3 = te - pr3;
+                    vexp = exp_scaled(neta * r2sum);
+                }
+            }
+            if (fl & MF_NEWLAM)
+            {
+                pl = R(1) + M.lam[m] * costijk;
+                chain.reset(pl);
+            }
+            R pw;
+            if (fl & MF_PREFIX) pw = chain.at((fl >> 8) & 15);
+            else if (!GENPOW || (fl & MF_INTPOW))
+                pw = pow_int(pl, MP[m].zint - 1);
+            else pw = pow(pl, MP[m].zeta - R(1));
+            R const fgw = pl > R(0) ? M.w[m] * pw : R(0);
+            S1 += fgw * M.zl[m];
+            U += fgw * pl;
+        }
+    }
+    T1 += vexp * S1;
+    {
+        R const vu = vexp * U;
+        T2 += vu * c1;
+        T3 += vu * c3;
+    }
+    R const A = rinvijik - costijk * J.inv2;
+    P1 = pfc * A * T1 + T2;
+    P3 = pfc * rinvijik * T1 + T3;
+}
+
+// FP32 with up to 8 members per group: at most 128 VGPRs so that four
+// waves fit per SIMD (measured faster than three waves without the bound).
+template <int CT, bool GENPOW, int MAXM, typename R>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && MAXM <= 8 ? 4 : 1, 8)))
+void k_force_backward(ForceArgs a)
+{
+    extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     __shared__ MParam<R> MP[kMaxMembers];
     int const cap = a.cap;
     BLds<R> S;
-    S.ACC = (R*)sm;
+    S.P = (Vec4<R>*)sm;
+    S.Q = (Vec2<R>*)(S.P + cap);
+    S.ACC = (R*)(S.Q + cap);
     S.HEAD = S.ACC + 3 * cap;
-    S.X = S.HEAD + 3 * 64;
-    S.Y = S.X + cap;
-    S.Z = S.Y + cap;
-    S.D = S.Z + cap;
-    S.INV = S.D + cap;
-    S.FC = S.INV + (sizeof(R) == 8 ? cap : 0);
-    S.SLOT = (int*)(S.FC + cap);
+    S.SLOT = (int*)(S.HEAD + 3 * 64);
     S.HEADJ = S.SLOT + cap;
     S.MASK = (uint32_t*)(S.HEADJ + 64);
 
@@ -318,8 +472,8 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             for (int t = lane; t < n; t += 64)
             {
                 R f, df;
-                cutoff_fdf_t<CT>(cc, S.D[t], f, df);
-                S.FC[t] = f;
+                cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
+                S.Q[t].x = f;
             }
             rcCached = g.rc;
             __syncthreads();
@@ -344,11 +498,12 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             for (int t = lane; t < na; t += 64)
             {
                 int const j = a0 + t;
-                R const r = S.D[j];
+                Vec4<R> const pj = S.P[j];
+                R const r = pj.w;
                 if (!(r < grc)) continue;
                 R fc, dfc;
                 cutoff_fdf_t<CT>(cc, r, fc, dfc);
-                R const inv = S.inv(j);
+                R const inv = S.Q[j].y;
                 R P = R(0);
                 for (int m = 0; m < nmem; ++m)
                 {
@@ -357,9 +512,9 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                     P += MP[m].w * ((dfc - R(2) * eta * (r - rs) * fc) * pexp
                                     * inv);
                 }
-                R const px = P * S.X[j];
-                R const py = P * S.Y[j];
-                R const pz = P * S.Z[j];
+                R const px = P * pj.x;
+                R const py = P * pj.y;
+                R const pz = P * pj.z;
                 sx += (double)px; sy += (double)py; sz += (double)pz;
                 S.ACC[3 * j] -= px;
                 S.ACC[3 * j + 1] -= py;
@@ -374,6 +529,8 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                         sCnt[g.e2]);
         int const total = op.total;
         if (total == 0) continue;
+        Members<R, MAXM> M;
+        load_members(MP, nmem, M);
         // Chunk of this lane: [q0, q1); the mask buffer holds
         // mask_words(cap) words per lane, enough for the largest chunk.
         {
@@ -383,53 +540,49 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             int const len = q1 - q0;
             int const nw = (span + 31) >> 5;
             R const grc2 = (R)g.rc2;
+            // Cursor at the chunk's first pair.
+            int cj = 0, cc0 = 0, clen = 1, chalf = 0;
+            if (len > 0) op.decode(q0, cj, cc0, clen, chalf);
+            int const j0 = cj;
+            bool const firstOwned = (cc0 == 0);
             // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow),
             // walking the chunk's pairs in order.
-            int wj = 0, wc = 0, wl = 1, wh = 0;
-            if (len > 0)
             {
-                int k0;
-                op.decode(q0, wj, k0, wc, wl);
-                wh = (!op.same && q0 >= op.split) ? 1 : 0;
-            }
-            for (int wd = 0; wd < nw; ++wd)
-            {
-                uint32_t bits = 0;
-                for (int b = 0; b < 32; ++b)
+                int wj = cj, wc = cc0, wl = clen, wh = chalf;
+                for (int wd = 0; wd < nw; ++wd)
                 {
-                    int const t = wd * 32 + b;
-                    if (t < len)
+                    uint32_t bits = 0;
+                    for (int b = 0; b < 32; ++b)
                     {
-                        int const j = wj;
-                        int const k = op.partner(wj, wc, wh);
-                        op.next(wj, wc, wl, wh);
-                        bool pass = S.D[j] < grc && S.D[k] < grc;
-                        if (pass && !wide)
+                        int const t = wd * 32 + b;
+                        if (t < len)
                         {
-                            R const dx = S.X[k] - S.X[j];
-                            R const dy = S.Y[k] - S.Y[j];
-                            R const dz = S.Z[k] - S.Z[j];
-                            pass = dx * dx + dy * dy + dz * dz < grc2;
+                            int const k = op.partner(wj, wc, wh);
+                            Vec4<R> const pj = S.P[wj];
+                            Vec4<R> const pk = S.P[k];
+                            op.next(wj, wc, wl, wh);
+                            bool pass = pj.w < grc && pk.w < grc;
+                            if (!wide)
+                            {
+                                R const dx = pk.x - pj.x;
+                                R const dy = pk.y - pj.y;
+                                R const dz = pk.z - pj.z;
+                                pass = pass
+                                    && dx * dx + dy * dy + dz * dz < grc2;
+                            }
+                            bits |= (pass ? 1u : 0u) << b;
                         }
-                        bits |= (pass ? 1u : 0u) << b;
                     }
+                    S.MASK[wd * 64 + lane] = bits;
                 }
-                S.MASK[wd * 64 + lane] = bits;
             }
-            // Row of the chunk's first pair (j0); owned if it starts here.
-            // Rows starting later are always owned.
-            int j0 = -1;
-            bool firstOwned = true;
             // Row of the chunk's last pair, if it starts in this chunk and
             // runs past it: this lane merges its continuation.
             int tailJ = -1, tailEnd = 0;
             if (len > 0)
             {
-                int k0, c0, l0;
-                op.decode(q0, j0, k0, c0, l0);
-                firstOwned = (c0 == 0);
-                int jl, kl, cl, ll;
-                op.decode(q1 - 1, jl, kl, cl, ll);
+                int jl, cl, ll, hl;
+                op.decode(q1 - 1, jl, cl, ll, hl);
                 int const rowStart = q1 - 1 - cl;
                 if (rowStart >= q0 && rowStart + ll > q1)
                 {
@@ -443,9 +596,10 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
             bool curHas = false;
             int curJ = -1;
             bool curOwned = true;
-            // Quantities of the current row j, held in registers.
-            R rij = R(0), invj = R(0), xj = R(0), yj = R(0), zj = R(0);
-            R fcij = R(0), dfcij = R(0);
+            Row<R> J;
+            J.x = J.y = J.z = J.r = J.inv = J.fc = J.dfcinv = J.r2 = J.inv2
+                = R(0);
+            int qc = q0;            // position of the cursor
             int wd = 0;
             uint32_t bits = nw > 0 ? S.MASK[lane] : 0u;
             for (;;)
@@ -458,109 +612,38 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 if (wd >= nw) break;
                 int const b = __builtin_ctz(bits);
                 bits &= bits - 1u;
-                int j, k, cpos, rlen;
-                op.decode(q0 + wd * 32 + b, j, k, cpos, rlen);
+                int const q = q0 + wd * 32 + b;
+                op.advance(q - qc, cj, cc0, clen, chalf);
+                qc = q;
+                int const j = cj;
+                int const k = op.partner(cj, cc0, chalf);
                 if (j != curJ)
                 {
                     if (curHas)
-                        flush(S, curJ, curOwned, rsum, xj, yj, zj, sx, sy,
+                        flush(S, curJ, curOwned, rsum, J.x, J.y, J.z, sx, sy,
                               sz);
                     curOwned = (j != j0) || firstOwned;
                     curJ = j;
                     rsum.clear();
                     curHas = false;
-                    rij = S.D[j];
-                    invj = S.inv(j);
-                    xj = S.X[j];
-                    yj = S.Y[j];
-                    zj = S.Z[j];
-                    cutoff_fdf_t<CT>(cc, rij, fcij, dfcij);
+                    Vec4<R> const pj = S.P[j];
+                    Vec2<R> const qj = S.Q[j];
+                    J.x = pj.x;
+                    J.y = pj.y;
+                    J.z = pj.z;
+                    J.r = pj.w;
+                    J.fc = qj.x;
+                    J.inv = qj.y;
+                    J.r2 = J.r * J.r;
+                    J.inv2 = J.inv * J.inv;
+                    R fcj, dfcj;
+                    cutoff_fdf_t<CT>(cc, J.r, fcj, dfcj);
+                    J.dfcinv = dfcj * J.inv;
                 }
-                // ---- triplet (j as the first neighbour, k second) ----
-                R const rik = S.D[k];
-                R const invk = S.inv(k);
-                R const xk = S.X[k], yk = S.Y[k], zk = S.Z[k];
-                R const dx = xk - xj, dy = yk - yj, dz = zk - zj;
-                R const fcik = S.FC[k];
-                R const rinvijik = invj * invk;
-                R const costijk = (xj * xk + yj * yk + zj * zk) * rinvijik;
-                R const r2ij = rij * rij, r2ik = rik * rik;
-                R pfc, r2sum, pr1, pr3 = R(0), rjk = R(0), invjk = R(0);
-                if (!wide)
-                {
-                    R const rjk2 = dx * dx + dy * dy + dz * dz;
-                    invjk = rsqrt_nr(rjk2);
-                    rjk = rjk2 * invjk;
-                    R fcjk, dfcjk;
-                    cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
-                    pfc = fcij * fcik * fcjk;
-                    r2sum = r2ij + r2ik + rjk2;
-                    pr1 = fcik * fcjk * dfcij * invj;
-                    pr3 = fcij * fcik * dfcjk * invjk;
-                }
-                else
-                {
-                    pfc = fcij * fcik;
-                    r2sum = r2ij + r2ik;
-                    pr1 = fcik * dfcij * invj;
-                }
-                R T1 = R(0), T2 = R(0), T3 = R(0);
-                R vexp = R(0), c1 = R(0), c3 = R(0), S1 = R(0), U = R(0);
-                for (int m = 0; m < nmem; ++m)
-                {
-                    int const flags = uniform(MP[m].flags);
-                    if (flags & MF_NEWEXP)
-                    {
-                        // Close the previous (eta, rs) run.
-                        T1 += vexp * S1;
-                        R const vu = vexp * U;
-                        T2 += vu * c1;
-                        T3 += vu * c3;
-                        S1 = U = R(0);
-                        R const eta = MP[m].eta, rs = MP[m].rs;
-                        R const te = R(2) * eta * pfc;
-                        if (rs > R(0))
-                        {
-                            R const rijs = rij - rs;
-                            R const riks = rik - rs;
-                            R s = rijs * rijs + riks * riks;
-                            c1 = pr1 - te * rijs * invj;
-                            if (!wide)
-                            {
-                                R const rjks = rjk - rs;
-                                s += rjks * rjks;
-                                c3 = te * rjks * invjk - pr3;
-                            }
-                            vexp = exp_r(-eta * s);
-                        }
-                        else
-                        {
-                            c1 = pr1 - te;
-                            if (!wide) c3 = te - pr3;
-                            vexp = exp_r(-eta * r2sum);
-                        }
-                    }
-                    R const pl = R(1) + MP[m].lambda * costijk;
-                    if (pl > R(0))
-                    {
-                        R pw;
-                        if (!GENPOW || (flags & MF_INTPOW))
-                            pw = pow_uniform(pl, uniform(MP[m].zint) - 1);
-                        else pw = pow(pl, MP[m].zeta - R(1));
-                        R const fgw = MP[m].w * pw;
-                        S1 += fgw * MP[m].zl;
-                        U += fgw * pl;
-                    }
-                }
-                T1 += vexp * S1;
-                {
-                    R const vu = vexp * U;
-                    T2 += vu * c1;
-                    T3 += vu * c3;
-                }
-                R const A = rinvijik - costijk * invj * invj;
-                R const P1 = pfc * A * T1 + T2;
-                R const P3 = pfc * rinvijik * T1 + T3;
+                R P1, P3, dx, dy, dz;
+                triplet_backward<CT, GENPOW, MAXM>(J, S.P[k], S.Q[k], wide, cc,
+                                                   M, MP, nmem, P1, P3, dx,
+                                                   dy, dz);
                 rsum.p1 += P1;
                 rsum.vx += P3 * dx;
                 rsum.vy += P3 * dy;
@@ -568,7 +651,7 @@ __global__ __launch_bounds__(64) void k_force_backward(ForceArgs a)
                 curHas = true;
             }
             if (curHas)
-                flush(S, curJ, curOwned, rsum, xj, yj, zj, sx, sy, sz);
+                flush(S, curJ, curOwned, rsum, J.x, J.y, J.z, sx, sy, sz);
             __syncthreads();
             // The lane owning the start of a row that runs past its chunk
             // adds the head partials of the following chunks of that row,
@@ -647,15 +730,22 @@ __global__ __launch_bounds__(256) void k_gather(
     }
 }
 
-template <typename R>
+template <int MAXM, typename R>
 void launch_bwd(ForceArgs const& a, size_t shm, hipStream_t s)
 {
     if (a.T.genpow)
         HDNNP_CUTOFF_DISPATCH(a.T.cutType,
-            (k_force_backward<kCT, true, R><<<a.n, 64, shm, s>>>(a)));
+            (k_force_backward<kCT, true, MAXM, R><<<a.n, 64, shm, s>>>(a)));
     else
         HDNNP_CUTOFF_DISPATCH(a.T.cutType,
-            (k_force_backward<kCT, false, R><<<a.n, 64, shm, s>>>(a)));
+            (k_force_backward<kCT, false, MAXM, R><<<a.n, 64, shm, s>>>(a)));
+}
+
+template <typename R>
+void launch_bwd_r(ForceArgs const& a, size_t shm, hipStream_t s)
+{
+    if (a.T.maxGroupMembers <= 8) launch_bwd<8, R>(a, shm, s);
+    else launch_bwd<kMaxMembers, R>(a, shm, s);
 }
 
 }  // namespace
@@ -664,9 +754,9 @@ size_t force_backward_lds_bytes(int cap, int fp32)
 {
     size_t const r = fp32 ? sizeof(float) : sizeof(double);
     size_t b = 0;
+    b += (size_t)cap * 6 * r;                     // P (4) and Q (2)
     b += (size_t)cap * 3 * r;                     // ACC
     b += (size_t)64 * 3 * r;                      // HEAD
-    b += (size_t)cap * (fp32 ? 5 : 6) * r;        // X Y Z D [INV] FC
     b += (size_t)cap * sizeof(int);               // SLOT
     b += (size_t)64 * sizeof(int);                // HEADJ
     b += (size_t)mask_words(cap) * 64 * sizeof(uint32_t);   // MASK
@@ -677,8 +767,8 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
     size_t const shm = force_backward_lds_bytes(a.cap, a.fp32);
-    if (a.fp32) launch_bwd<float>(a, shm, s);
-    else launch_bwd<double>(a, shm, s);
+    if (a.fp32) launch_bwd_r<float>(a, shm, s);
+    else launch_bwd_r<double>(a, shm, s);
     return hipGetLastError();
 }
 

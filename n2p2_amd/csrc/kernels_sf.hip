@@ -10,16 +10,18 @@
 //      (stable ballot compaction), so every group reads exactly the
 //      neighbour blocks it needs;
 //   2. groups run one after another; a group's member parameters are staged
-//      in LDS and read with wave-uniform addresses, its member sums live in
-//      registers;
+//      in LDS and the ones read per member held in scalar registers, its
+//      member sums live in registers;
 //   3. angular groups enumerate the unordered pairs of their blocks, test
 //      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
 //      with a ballot prefix; whenever 64 are queued every lane evaluates
 //      one, so the expensive triplet work runs without divergence;
 //   4. inside a triplet the members are walked as (eta, rs) runs sharing one
-//      exponential; integer powers branch on the wave-uniform exponent to
-//      straight-line products (bit-identical to pow_int);
+//      exponential and (eta, rs, lambda) runs sorted by zeta sharing one
+//      chain of powers of plambda (PowChain, bit-identical to pow_int);
 //   5. member sums are reduced with a fixed butterfly (deterministic).
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -30,14 +32,16 @@ namespace {
 constexpr int kQueue = 128;
 
 template <typename R>
+using Vec4 = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+template <typename R>
+using Vec2 = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+
+// Staged neighbours: (x, y, z, r) and (f_c(r) for the current cutoff, 1/r).
+template <typename R>
 struct Stage
 {
-    R* X;
-    R* Y;
-    R* Z;
-    R* D;
-    R* INV;
-    R* FC;
+    Vec4<R>* P;
+    Vec2<R>* Q;      // null when only geometry is needed
 };
 
 __device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
@@ -106,11 +110,13 @@ __device__ void stage_by_element(SFArgs const& a, Stage<R> const& S, int s0,
                 if (e == q)
                 {
                     int const pos = start[q] + lanes_below(mask);
-                    S.X[pos] = (R)g.x;
-                    S.Y[pos] = (R)g.y;
-                    S.Z[pos] = (R)g.z;
-                    S.D[pos] = (R)g.w;
-                    if (S.INV) S.INV[pos] = R(1) / (R)g.w;
+                    Vec4<R> p;
+                    p.x = (R)g.x;
+                    p.y = (R)g.y;
+                    p.z = (R)g.z;
+                    p.w = (R)g.w;
+                    S.P[pos] = p;
+                    if (S.Q) S.Q[pos].y = R(1) / (R)g.w;
                 }
                 start[q] += __popcll(mask);
             }
@@ -174,77 +180,94 @@ __device__ __forceinline__ bool pair_passes(Stage<R> const& S, int ja, int kb,
                                             bool wide, GroupDesc const& g)
 {
     R const rc = (R)g.rc;
-    if (!(S.D[ja] < rc && S.D[kb] < rc)) return false;
-    if (wide) return true;
-    R const dx = S.X[kb] - S.X[ja];
-    R const dy = S.Y[kb] - S.Y[ja];
-    R const dz = S.Z[kb] - S.Z[ja];
-    return (dx * dx + dy * dy + dz * dz) < (R)g.rc2;
+    Vec4<R> const pj = S.P[ja], pk = S.P[kb];
+    bool pass = pj.w < rc && pk.w < rc;
+    if (!wide)
+    {
+        R const dx = pk.x - pj.x;
+        R const dy = pk.y - pj.y;
+        R const dz = pk.z - pj.z;
+        pass = pass && (dx * dx + dy * dy + dz * dz) < (R)g.rc2;
+    }
+    return pass;
 }
+
+// A group's member parameters in wave-uniform (scalar) registers: lambda
+// and the flags word; the exponential's parameters are read from LDS at the
+// start of each (eta, rs) run.
+template <typename R, int MAXM>
+struct Members
+{
+    R lam[MAXM];
+    int code[MAXM];
+};
 
 // One triplet, all members of the group: acc[m] += fg * plambda * pfc with
 // fg = exp(-eta sum r^2) plambda^(zeta - 1) (SymGrpExpAngn.cpp:276-297).
 template <int MAXM, int CT, bool GENPOW, typename R>
 __device__ __forceinline__ void triplet_forward(
     Stage<R> const& S, int ja, int kb, bool wide, CutCfg const& cc,
-    MParam<R> const* MP, int nmem, R (&acc)[MAXM])
+    Members<R, MAXM> const& M, MParam<R> const* MP, int nmem,
+    R (&acc)[MAXM])
 {
-    R const rij = S.D[ja], rik = S.D[kb];
-    R const x1 = S.X[ja], y1 = S.Y[ja], z1 = S.Z[ja];
-    R const x2 = S.X[kb], y2 = S.Y[kb], z2 = S.Z[kb];
-    R const fcij = S.FC[ja], fcik = S.FC[kb];
+    Vec4<R> const pj = S.P[ja], pk = S.P[kb];
+    Vec2<R> const qj = S.Q[ja], qk = S.Q[kb];
+    R const rij = pj.w, rik = pk.w;
     R pfc, r2sum, rjk = R(0);
     if (!wide)
     {
-        R const dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+        R const dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
         R const rjk2 = dx * dx + dy * dy + dz * dz;
         rjk = rjk2 * rsqrt_nr(rjk2);
         R fcjk, dfcjk;
-        cutoff_fdf_t<CT>(cc, rjk, fcjk, dfcjk);
-        pfc = fcij * fcik * fcjk;
+        cutoff_inside<CT>(cc, rjk, fcjk, dfcjk);
+        pfc = qj.x * qk.x * fcjk;
         r2sum = rij * rij + rik * rik + rjk2;
     }
     else
     {
-        pfc = fcij * fcik;
+        pfc = qj.x * qk.x;
         r2sum = rij * rij + rik * rik;
     }
-    R const rinvijik = S.INV[ja] * S.INV[kb];
-    R const costijk = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
+    R const rinvijik = qj.y * qk.y;
+    R const costijk = (pj.x * pk.x + pj.y * pk.y + pj.z * pk.z) * rinvijik;
     R vp = R(0);             // exp(.) * pfc of the current (eta, rs) run
+    R pl = R(0);             // plambda of the current lambda run
+    PowChain<R> chain;
+    chain.reset(R(0));
 #pragma unroll
     for (int m = 0; m < MAXM; ++m)
     {
         if (m < nmem)
         {
-            int const flags = uniform(MP[m].flags);
-            if (flags & MF_NEWEXP)
+            int const fl = M.code[m];
+            if (fl & MF_NEWEXP)
             {
                 R const eta = MP[m].eta, rs = MP[m].rs;
-                R vexp;
+                R s = r2sum;
                 if (rs > R(0))
                 {
                     R const a = rij - rs, b = rik - rs;
-                    R s = a * a + b * b;
+                    s = a * a + b * b;
                     if (!wide)
                     {
                         R const c = rjk - rs;
                         s += c * c;
                     }
-                    vexp = exp_r(-eta * s);
                 }
-                else vexp = exp_r(-eta * r2sum);
-                vp = vexp * pfc;
+                vp = exp_scaled(exp_factor(eta) * s) * pfc;
             }
-            R const pl = R(1) + MP[m].lambda * costijk;
-            if (pl > R(0))
+            if (fl & MF_NEWLAM)
             {
-                R pw;
-                if (!GENPOW || (flags & MF_INTPOW))
-                    pw = pow_uniform(pl, uniform(MP[m].zint) - 1);
-                else pw = pow(pl, MP[m].zeta - R(1));
-                acc[m] += vp * (pw * pl);
+                pl = R(1) + M.lam[m] * costijk;
+                chain.reset(pl);
             }
+            R pw;
+            if (fl & MF_PREFIX) pw = chain.at((fl >> 8) & 15);
+            else if (!GENPOW || (fl & MF_INTPOW))
+                pw = pow_int(pl, MP[m].zint - 1);
+            else pw = pow(pl, MP[m].zeta - R(1));
+            acc[m] += pl > R(0) ? vp * (pw * pl) : R(0);
         }
     }
 }
@@ -252,18 +275,14 @@ __device__ __forceinline__ void triplet_forward(
 template <int MAXM, int CT, bool GENPOW, typename R>
 __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
 {
-    extern __shared__ double sm[];
+    extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     __shared__ MParam<R> MP[kMaxMembers];
     int const cap = a.cap;
     Stage<R> S;
-    S.X = (R*)sm;
-    S.Y = S.X + cap;
-    S.Z = S.Y + cap;
-    S.D = S.Z + cap;
-    S.INV = S.D + cap;
-    S.FC = S.INV + cap;
-    int* Q = (int*)(S.FC + cap);
+    S.P = (Vec4<R>*)sm;
+    S.Q = (Vec2<R>*)(S.P + cap);
+    int* Q = (int*)(S.Q + cap);
 
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
@@ -289,8 +308,8 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             for (int t = lane; t < n; t += 64)
             {
                 R f, df;
-                cutoff_fdf_t<CT>(cc, S.D[t], f, df);
-                S.FC[t] = f;
+                cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
+                S.Q[t].x = f;
             }
             rcCached = g.rc;
         }
@@ -305,10 +324,10 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             R const rc = (R)g.rc;
             for (int t = lane; t < na; t += 64)
             {
-                R const r = S.D[a0 + t];
+                R const r = S.P[a0 + t].w;
                 if (r < rc)
                 {
-                    R const fc = S.FC[a0 + t];
+                    R const fc = S.Q[a0 + t].x;
 #pragma unroll
                     for (int m = 0; m < MAXM; ++m)
                         if (m < nmem)
@@ -324,6 +343,14 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             bool const wide = (g.type == 9);
             PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
                           sStart[g.e2], sCnt[g.e2]);
+            Members<R, MAXM> M;
+#pragma unroll
+            for (int m = 0; m < MAXM; ++m)
+            {
+                MParam<R> const& p = MP[m < nmem ? m : 0];
+                M.lam[m] = uniform_r(p.lambda);
+                M.code[m] = uniform(p.flags);
+            }
             int qn = 0;
             for (int base = 0; base < pc.npairs; base += 64)
             {
@@ -339,7 +366,8 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
                 {
                     int const e = Q[qn - 64 + lane];
                     triplet_forward<MAXM, CT, GENPOW>(S, e & 0xffff, e >> 16,
-                                                      wide, cc, MP, nmem, acc);
+                                                      wide, cc, M, MP, nmem,
+                                                      acc);
                     qn -= 64;
                     __syncthreads();
                 }
@@ -348,7 +376,7 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
             {
                 int const e = Q[lane];
                 triplet_forward<MAXM, CT, GENPOW>(S, e & 0xffff, e >> 16, wide,
-                                                  cc, MP, nmem, acc);
+                                                  cc, M, MP, nmem, acc);
             }
         }
 #pragma unroll
@@ -383,16 +411,12 @@ __global__ __launch_bounds__(64) void k_sf_forward(SFArgs a)
 __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
                                                  unsigned long long* out)
 {
-    extern __shared__ double sm[];
+    extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     int const cap = a.cap;
     Stage<double> S;
-    S.X = sm;
-    S.Y = S.X + cap;
-    S.Z = S.Y + cap;
-    S.D = S.Z + cap;
-    S.INV = nullptr;
-    S.FC = nullptr;
+    S.P = (double4*)sm;
+    S.Q = nullptr;
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
     ModelTables const& T = a.T;
@@ -411,7 +435,7 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
         {
             int const a0 = sStart[g.e1], na = sCnt[g.e1];
             for (int t = lane; t < na; t += 64)
-                if (S.D[a0 + t] < g.rc) c[1] += g.nmem;
+                if (S.P[a0 + t].w < g.rc) c[1] += g.nmem;
             continue;
         }
         bool const wide = (g.type == 9);
@@ -430,7 +454,7 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
         {
             int ja = 0, kb = 0;
             if (!pc.decode(base + lane, ja, kb)) continue;
-            if (!(S.D[ja] < g.rc && S.D[kb] < g.rc)) continue;
+            if (!(S.P[ja].w < g.rc && S.P[kb].w < g.rc)) continue;
             c[wide ? 3 : 2] += 1;
             if (pair_passes(S, ja, kb, wide, g))
             {

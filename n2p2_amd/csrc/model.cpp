@@ -580,9 +580,10 @@ Model::Flat Model::flatten() const
             {
                 if ((pass == 0) != (g.type == 2)) continue;
                 // Members in SF order (SymGrp*::sortMembers), then, for the
-                // angular types, stably grouped into (eta, rs) runs: a run
-                // shares one exponential and its derivative factors.  Each
-                // member keeps its own SF slot, so the order is free.
+                // angular types, stably ordered by (eta, rs, lambda, zeta):
+                // an (eta, rs) run shares one exponential, an (eta, rs,
+                // lambda) run one chain of powers of plambda.  Each member
+                // keeps its own SF slot, so the order is free.
                 std::vector<int> mem = g.members;
                 std::sort(mem.begin(), mem.end());
                 if (g.type != 2)
@@ -591,7 +592,9 @@ Model::Flat Model::flatten() const
                         SymFn const& x = e.sf[a];
                         SymFn const& y = e.sf[b];
                         if (x.eta != y.eta) return x.eta < y.eta;
-                        return x.rs < y.rs;
+                        if (x.rs != y.rs) return x.rs < y.rs;
+                        if (x.lambda != y.lambda) return x.lambda < y.lambda;
+                        return x.zeta < y.zeta;
                     });
                 for (size_t c0 = 0; c0 < mem.size(); c0 += kMaxMembers)
                 {
@@ -641,6 +644,14 @@ Model::Flat Model::flatten() const
                             md.calcexp = (p.eta != s.eta || p.rs != s.rs);
                         }
                         md.sf = s.index;
+                        md.newlam = md.calcexp
+                                 || e.sf[mem[i - 1]].lambda != s.lambda;
+                        // zeta = 2^l: plambda^(zeta-1) is the l-th prefix
+                        // product x x^2 x^4 ... of pow_int's squaring chain.
+                        md.level = -1;
+                        if (s.useIntegerPow)
+                            for (int l = 0; l < 16; ++l)
+                                if (s.zetaInt == (1 << l)) md.level = l;
                         F.member.push_back(md);
                     }
                     if (pass == 0) d.nrad++;

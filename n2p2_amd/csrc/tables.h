@@ -64,6 +64,8 @@ struct MemberDesc
     int    useint;    // integer power path (SymFncBaseExpAng.cpp:82-90)
     int    calcexp;   // recompute the exponential for this member
     int    sf;        // SF index within the centre element (NN input slot)
+    int    newlam;    // first member of an (eta, rs, lambda) run
+    int    level;     // zeta = 2^level (integer power), else -1
 };
 
 struct ScaleDesc

@@ -91,11 +91,15 @@ EXPORTED = [s[0] for s in _SIGNATURES]
 _lib = None
 
 
-def load_library(path=LIB_PATH):
-    """Loads libhdnnp.so (raises if it was not built)."""
+def load_library(path=None):
+    """Loads libhdnnp.so (raises if it was not built).  The environment
+    variable N2P2_AMD_LIBRARY names another build of the same library (for
+    A/B timing of kernel variants)."""
     global _lib
     if _lib is not None:
         return _lib
+    if path is None:
+        path = os.environ.get("N2P2_AMD_LIBRARY", LIB_PATH)
     if not os.path.exists(path):
         raise ImportError(
             "n2p2_amd: HIP engine library %s is missing; build it with "
