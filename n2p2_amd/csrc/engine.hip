@@ -396,30 +396,49 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     c->dCellAtoms.ensure(n);
     c->dCount.ensure(n);
     c->dOff.ensure(n + 1);
-    c->dStats.ensure(2);
+    c->dStats.ensure(3);
     c->dBlockSums.ensure((std::max(n, ncell) + 1023) / 1024 + 1);
     c->dTotal.ensure(1);
     hipCheck(hipMemsetAsync(c->dCellCount.p, 0, ncell * sizeof(int), s), "memset");
-    hipCheck(hipMemsetAsync(c->dStats.p, 0, 2 * sizeof(int), s), "memset");
+    hipCheck(hipMemsetAsync(c->dStats.p, 0, 3 * sizeof(int), s), "memset");
     hipCheck(launch_nl_bin(p, c->dR.p, c->dCellOfAtom.p, c->dRank.p,
                            c->dCellCount.p, s), "nl bin");
     hipCheck(scan_exclusive(c->dCellCount.p, c->dCellStart.p, ncell,
                             c->dBlockSums.p, c->dTotal.p, s), "cell scan");
     hipCheck(launch_nl_scatter(p, c->dCellOfAtom.p, c->dRank.p, c->dCellStart.p,
                                c->dCellAtoms.p, s), "nl scatter");
-    hipCheck(launch_nl_count(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
-                             c->dCellStart.p, c->dCellCount.p, c->dCellAtoms.p,
-                             c->dCount.p, c->dStats.p, s), "nl count");
-    hipCheck(scan_exclusive(c->dCount.p, c->dOff.p, n, c->dBlockSums.p,
-                            c->dTotal.p, s), "count scan");
-    int hstat[3];
-    hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 2 * sizeof(int),
-                            hipMemcpyDeviceToHost, s), "stats");
-    hipCheck(hipMemcpyAsync(hstat + 2, c->dTotal.p, sizeof(int),
-                            hipMemcpyDeviceToHost, s), "total");
-    hipCheck(hipStreamSynchronize(s), "sync");
+    // Cell-per-workgroup kernels when at most 64 neighbour cells are searched
+    // and every cell's candidates fit their LDS; the per-atom kernels
+    // otherwise.  Both produce the same list.
+    bool cellKernels = (2 * p.m[0] + 1) * (2 * p.m[1] + 1) * (2 * p.m[2] + 1)
+                     <= 64;
+    int hstat[4];
+    for (int attempt = 0; attempt < 2; ++attempt)
+    {
+        if (cellKernels)
+            hipCheck(launch_nl_count_cell(p, ncell, c->dR.p, c->dCellStart.p,
+                                          c->dCellCount.p, c->dCellAtoms.p,
+                                          c->dCount.p, c->dStats.p, s),
+                     "nl count");
+        else
+            hipCheck(launch_nl_count(p, c->dR.p, c->dElemAtom.p,
+                                     c->dCellOfAtom.p, c->dCellStart.p,
+                                     c->dCellCount.p, c->dCellAtoms.p,
+                                     c->dCount.p, c->dStats.p, s), "nl count");
+        hipCheck(scan_exclusive(c->dCount.p, c->dOff.p, n, c->dBlockSums.p,
+                                c->dTotal.p, s), "count scan");
+        hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "count max");
+        hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 3 * sizeof(int),
+                                hipMemcpyDeviceToHost, s), "stats");
+        hipCheck(hipMemcpyAsync(hstat + 3, c->dTotal.p, sizeof(int),
+                                hipMemcpyDeviceToHost, s), "total");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        if (!(cellKernels && hstat[2] > 0)) break;
+        cellKernels = false;     // a cell overflowed the staging: redo
+        hipCheck(hipMemsetAsync(c->dStats.p, 0, 3 * sizeof(int), s), "memset");
+    }
     int const maxc = hstat[0];
-    int64_t const M = hstat[2];
+    int64_t const M = hstat[3];
     if (M < 0 || M > (int64_t)2000000000)
         throw Error(HDNNP_E_CAPACITY, "ERROR: Neighbour list too large.\n");
     int keyCap = 64;
@@ -431,10 +450,17 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     c->dNbrJ.ensure(M);
     c->dGeo.ensure(M);
     c->dPartner.ensure(M);
-    hipCheck(launch_nl_fill(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
-                            c->dCellStart.p, c->dCellCount.p, c->dCellAtoms.p,
-                            c->dStats.p, c->dOff.p, keyCap, c->dKeys.p,
-                            c->dNbrJ.p, c->dGeo.p, s), "nl fill");
+    if (cellKernels)
+        hipCheck(launch_nl_fill_cell(p, ncell, c->dR.p, c->dCellStart.p,
+                                     c->dCellCount.p, c->dCellAtoms.p,
+                                     c->dStats.p, c->dOff.p, c->dKeys.p,
+                                     c->dNbrJ.p, c->dGeo.p, s), "nl fill");
+    else
+        hipCheck(launch_nl_fill(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
+                                c->dCellStart.p, c->dCellCount.p,
+                                c->dCellAtoms.p, c->dStats.p, c->dOff.p, keyCap,
+                                c->dKeys.p, c->dNbrJ.p, c->dGeo.p, s),
+                 "nl fill");
     hipCheck(launch_nl_partner(n, c->dOff.p, c->dKeys.p, c->dPartner.p, s),
              "nl partner");
     c->end(K_NEIGHBOR, ev0);

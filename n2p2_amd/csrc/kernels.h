@@ -24,6 +24,8 @@ struct NLParams
 
 hipError_t scan_exclusive(const int* in, int* out, int n, int* blockSums,
                           int* total, hipStream_t s);
+// out[0] = maximum of in[0..n).
+hipError_t launch_max_int(const int* in, int n, int* out, hipStream_t s);
 hipError_t launch_nl_bin(NLParams const& p, const double* r, int* cellOfAtom,
                          int* rank, int* cellCount, hipStream_t s);
 hipError_t launch_nl_scatter(NLParams const& p, const int* cellOfAtom,
@@ -39,6 +41,18 @@ hipError_t launch_nl_fill(NLParams const& p, const double* r, const int* elem,
                           int* stats, const int* off, int keyCap,
                           uint64_t* keys, int* nbrJ, double4* geo,
                           hipStream_t s);
+// Cell-per-workgroup variants (at most 64 neighbour cells searched):
+// stats[2] receives the largest staged candidate count when a cell exceeds
+// the kernel's LDS capacity (the caller then uses the per-atom kernels).
+hipError_t launch_nl_count_cell(NLParams const& p, int ncell, const double* r,
+                                const int* cellStart, const int* cellCount,
+                                const int* cellAtoms, int* count, int* stats,
+                                hipStream_t s);
+hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
+                               const int* cellStart, const int* cellCount,
+                               const int* cellAtoms, int* stats,
+                               const int* off, uint64_t* keys, int* nbrJ,
+                               double4* geo, hipStream_t s);
 hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
                              int* partner, hipStream_t s);
 

@@ -212,11 +212,7 @@ __global__ __launch_bounds__(64) void k_nl(
     }
     if (MODE == 0)
     {
-        if (lane == 0)
-        {
-            count[i] = hits;
-            atomicMax(&stats[0], hits);
-        }
+        if (lane == 0) count[i] = hits;
         return;
     }
     // MODE 1: bitonic sort of keys[0, hits) padded to a power of two.
@@ -268,6 +264,197 @@ __global__ __launch_bounds__(64) void k_nl(
     (void)elem;
 }
 
+// Cell kernels: one workgroup of four waves per cell.  The atoms of the
+// cell's (2m+1)^3 neighbour cells (m = 1 when cells are at least rc wide)
+// are staged once in LDS with their image vectors; each wave then takes one
+// atom of the cell at a time and tests all staged candidates, 64 per step.
+// MODE 0 counts hits.  MODE 1 first sorts the staged candidates by
+// (j, image), the reference's loop order, so that the ballot compaction of
+// an atom's hits is already in list order and they are written directly.
+// The host uses these kernels when at most 64 neighbour cells are searched
+// and falls back to k_nl when a cell has more than kCellCand candidates
+// (stats[2]).
+constexpr int kCellCand = 1024;
+constexpr int kCellWaves = 4;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_nl_cell(
+    NLParams p, const double* __restrict__ r,
+    const int* __restrict__ cellStart, const int* __restrict__ cellCount,
+    const int* __restrict__ cellAtoms, int* __restrict__ count,
+    int* __restrict__ stats, const int* __restrict__ off,
+    uint64_t* __restrict__ keysOut, int* __restrict__ nbrJ,
+    double4* __restrict__ geo)
+{
+    __shared__ double cX[kCellCand], cY[kCellCand], cZ[kCellCand];
+    // MODE 0: (j << 21 | image).  MODE 1: (j << 31 | image << 10 | t), t the
+    // staging slot, sorted.
+    __shared__ uint64_t cKey[kCellCand];
+    __shared__ int sStart[64], sCnt[64], sPre[64], sImg[64];
+    __shared__ int sTotal;
+    int const cell = blockIdx.x;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const w = tid >> 6;
+    int const nAt = cellCount[cell];
+    if (nAt == 0) return;
+    int const ci[3] = {cell / (p.nc[1] * p.nc[2]), (cell / p.nc[2]) % p.nc[1],
+                       cell % p.nc[2]};
+    int const w0 = 2 * p.m[0] + 1, w1 = 2 * p.m[1] + 1, w2 = 2 * p.m[2] + 1;
+    int const ncand = w0 * w1 * w2;
+    int const zeroImg = (kBias << 14) | (kBias << 7) | kBias;
+    if (tid < 64)
+    {
+        int cnt = 0, st = 0, img = zeroImg;
+        if (tid < ncand)
+        {
+            int const d[3] = {tid / (w1 * w2) - p.m[0], (tid / w2) % w1 - p.m[1],
+                              tid % w2 - p.m[2]};
+            int cw[3], sh[3];
+            bool ok = true;
+            for (int k = 0; k < 3; ++k)
+            {
+                int const a = ci[k] + d[k];
+                int const wr = ((a % p.nc[k]) + p.nc[k]) % p.nc[k];
+                sh[k] = (a - wr) / p.nc[k];
+                cw[k] = wr;
+                if (!p.periodic && sh[k] != 0) ok = false;
+            }
+            if (ok)
+            {
+                int const cid = (cw[0] * p.nc[1] + cw[1]) * p.nc[2] + cw[2];
+                st = cellStart[cid];
+                cnt = cellCount[cid];
+                // The image of j at r_j + sh*box has bc = -sh.
+                img = ((kBias - sh[0]) << 14) | ((kBias - sh[1]) << 7)
+                    | (kBias - sh[2]);
+            }
+        }
+        int pre = cnt;
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            int const v = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += v;
+        }
+        sStart[tid] = st;
+        sCnt[tid] = cnt;
+        sPre[tid] = pre - cnt;
+        sImg[tid] = img;
+        if (tid == 63) sTotal = pre;
+    }
+    __syncthreads();
+    int const total = sTotal;
+    if (total > kCellCand)
+    {
+        if (tid == 0) atomicMax(&stats[2], total);
+        return;
+    }
+    int P = 1;
+    while (P < total) P <<= 1;
+    for (int t = tid; t < (MODE == 1 ? P : total); t += 256)
+    {
+        if (t >= total)
+        {
+            cKey[t] = ~0ull;
+            continue;
+        }
+        // last candidate cell whose exclusive prefix <= t
+        int lo = 0, hi = 63;
+        while (lo < hi)
+        {
+            int const mid = (lo + hi + 1) >> 1;
+            if (sPre[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        while (lo > 0 && (sCnt[lo] == 0 || sPre[lo] + sCnt[lo] <= t)) --lo;
+        int const j = cellAtoms[sStart[lo] + (t - sPre[lo])];
+        cX[t] = r[3 * (size_t)j];
+        cY[t] = r[3 * (size_t)j + 1];
+        cZ[t] = r[3 * (size_t)j + 2];
+        uint64_t const img = (uint64_t)sImg[lo];
+        cKey[t] = MODE == 1
+                ? ((uint64_t)(uint32_t)j << 31) | (img << 10) | (uint64_t)t
+                : ((uint64_t)(uint32_t)j << 21) | img;
+    }
+    __syncthreads();
+    if (MODE == 1)
+    {
+        // bitonic sort of cKey[0, P)
+        for (int k = 2; k <= P; k <<= 1)
+        {
+            for (int jj = k >> 1; jj > 0; jj >>= 1)
+            {
+                for (int t = tid; t < P; t += 256)
+                {
+                    int const q = t ^ jj;
+                    if (q > t)
+                    {
+                        uint64_t const x = cKey[t], y = cKey[q];
+                        bool const up = (t & k) == 0;
+                        if ((x > y) == up)
+                        {
+                            cKey[t] = y;
+                            cKey[q] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    int const a0 = cellStart[cell];
+    for (int a = w; a < nAt; a += kCellWaves)
+    {
+        int const i = cellAtoms[a0 + a];
+        double const ri[3] = {r[3 * (size_t)i], r[3 * (size_t)i + 1],
+                              r[3 * (size_t)i + 2]};
+        int const o = MODE == 1 ? off[i] : 0;
+        int hits = 0;
+        for (int u0 = 0; u0 < total; u0 += 64)
+        {
+            int const u = u0 + lane;
+            bool hit = false;
+            int j = 0, img = 0, t = 0;
+            double dr[3] = {0.0, 0.0, 0.0};
+            if (u < total)
+            {
+                uint64_t const ck = cKey[u];
+                if (MODE == 1)
+                {
+                    j = (int)(ck >> 31);
+                    img = (int)((ck >> 10) & 0x1fffff);
+                    t = (int)(ck & 1023);
+                }
+                else
+                {
+                    j = (int)(ck >> 21);
+                    img = (int)(ck & 0x1fffff);
+                    t = u;
+                }
+                if (!(j == i && img == zeroImg))
+                {
+                    int const b0 = ((img >> 14) & 127) - kBias;
+                    int const b1 = ((img >> 7) & 127) - kBias;
+                    int const b2 = (img & 127) - kBias;
+                    double const rj[3] = {cX[t], cY[t], cZ[t]};
+                    image_dr(p, ri, rj, b0, b1, b2, dr);
+                    hit = norm2_pinned(dr) <= p.rc2;
+                }
+            }
+            uint64_t const mask = __ballot(hit);
+            if (MODE == 1 && hit)
+            {
+                int const q = o + hits + lanes_below(mask);
+                keysOut[q] = ((uint64_t)(uint32_t)j << 21) | (uint64_t)img;
+                nbrJ[q] = j;
+                geo[q] = make_double4(dr[0], dr[1], dr[2],
+                                      sqrt(norm2_pinned(dr)));
+            }
+            hits += __popcll(mask);
+        }
+        if (MODE == 0 && lane == 0) count[i] = hits;
+    }
+}
+
 // Partner slot: for entry q of atom i pointing at (j, bc) the entry of j
 // pointing at (i, -bc); binary search in j's sorted keys.  One wave per atom.
 __global__ __launch_bounds__(64) void k_partner(
@@ -293,6 +480,25 @@ __global__ __launch_bounds__(64) void k_partner(
         }
         partner[q] = found;
     }
+}
+
+// out[0] = max(in[0..n)); one block, fixed-order tree (the per-atom
+// neighbour counts, whose maximum sizes the fill's LDS).
+__global__ __launch_bounds__(1024) void k_max(const int* __restrict__ in,
+                                              int n, int* __restrict__ out)
+{
+    __shared__ int s[1024];
+    int const t = threadIdx.x;
+    int m = 0;
+    for (int q = t; q < n; q += 1024) m = max(m, in[q]);
+    s[t] = m;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1)
+    {
+        if (t < o) s[t] = max(s[t], s[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) out[0] = s[0];
 }
 
 // Exclusive scan helpers (block-level, then block offsets).
@@ -358,6 +564,12 @@ hipError_t scan_exclusive(const int* in, int* out, int n, int* blockSums,
     return hipGetLastError();
 }
 
+hipError_t launch_max_int(const int* in, int n, int* out, hipStream_t s)
+{
+    k_max<<<1, 1024, 0, s>>>(in, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_nl_bin(NLParams const& p, const double* r, int* cellOfAtom,
                          int* rank, int* cellCount, hipStream_t s)
 {
@@ -398,6 +610,31 @@ hipError_t launch_nl_fill(NLParams const& p, const double* r, const int* elem,
                                  nbrJ, geo);
     return hipGetLastError();
 }
+
+hipError_t launch_nl_count_cell(NLParams const& p, int ncell, const double* r,
+                                const int* cellStart, const int* cellCount,
+                                const int* cellAtoms, int* count, int* stats,
+                                hipStream_t s)
+{
+    if (ncell <= 0) return hipSuccess;
+    k_nl_cell<0><<<ncell, 256, 0, s>>>(p, r, cellStart, cellCount, cellAtoms,
+                                       count, stats, nullptr, nullptr, nullptr,
+                                       nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
+                               const int* cellStart, const int* cellCount,
+                               const int* cellAtoms, int* stats,
+                               const int* off, uint64_t* keys, int* nbrJ,
+                               double4* geo, hipStream_t s)
+{
+    if (ncell <= 0) return hipSuccess;
+    k_nl_cell<1><<<ncell, 256, 0, s>>>(p, r, cellStart, cellCount, cellAtoms,
+                                       nullptr, stats, off, keys, nbrJ, geo);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
                              int* partner, hipStream_t s)
