@@ -202,6 +202,10 @@ def main():
             log("single-path run: no JSON line")
             ctx.close()
             return
+        dt16, e16, f16, kt16 = run_path(ctx, engine.PRECISION_MIXED16,
+                                        args.steps, args.warmup)
+        log("mixed with FP16 networks: %.3f ms/step, E=%.10f, kernel ms %s"
+            % (dt16 * 1e3, e16, kt16))
         df = fm - f64
         err = {"force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
                "force_max_eV_A": float(np.abs(df).max()),
@@ -210,6 +214,11 @@ def main():
                "reference": "plain FP64 GPU path of the same run (parity-"
                             "tested against the FP64 oracle)"}
         log("mixed vs FP64 errors", err)
+        d16 = f16 - f64
+        err16 = {"force_rmse_eV_A": float(np.sqrt(np.mean(d16 ** 2))),
+                 "force_max_eV_A": float(np.abs(d16).max()),
+                 "energy_rel": float(abs(e16 - e64) / abs(e64))}
+        log("mixed16 vs FP64 errors", err16)
         fwd = int(costmodel.forward_flops(counts))
         bwd = int(costmodel.backward_flops(counts))
 
@@ -238,8 +247,8 @@ def main():
                                    "cutoff, nsf-25-25-1 tanh, neighbour "
                                    "list rebuilt every step; value = mixed-"
                                    "precision path (FP32 SFs and force "
-                                   "contraction, FP64 scaling, networks, "
-                                   "energy and force sums)"
+                                   "contraction, FP32 MFMA networks, FP64 "
+                                   "scaling, energy and force sums)"
                                    % (n, args.nmol, args.box),
                        "atoms": n, "path": "mixed",
                        "plain_fp64": {"atom_steps_per_s": n / dt64,
@@ -247,6 +256,13 @@ def main():
                                       "kernel_ms": kt64,
                                       "roofline": roofline(kt64, counts, nnf,
                                                            n, False)},
+                       "mixed16": {"atom_steps_per_s": n / dt16,
+                                   "ms_per_step": dt16 * 1e3,
+                                   "kernel_ms": kt16,
+                                   "error_vs_fp64": err16,
+                                   "what": "as the mixed path with FP16 "
+                                           "MFMA network layers (FP32 "
+                                           "accumulation)"},
                        "mixed_vs_plain_fp64": dt64 / dtm,
                        "mixed_error_vs_fp64": err,
                        "flops_per_atom_step": (fwd + bwd + nnf) / n,

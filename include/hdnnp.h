@@ -48,9 +48,12 @@ extern "C" {
 #define HDNNP_PRECISION_FP64  0   /* plain path: FP64 throughout */
 #define HDNNP_PRECISION_MIXED 1   /* FP32 symmetry-function and force-
                                      contraction arithmetic (per-lane FP32
-                                     partial sums combined in FP64); FP64
-                                     SF scaling, networks, energy total and
-                                     force accumulation */
+                                     partial sums combined in FP64), FP32
+                                     MFMA network layers; FP64 SF scaling,
+                                     biases and activations, energy total
+                                     and force accumulation */
+#define HDNNP_PRECISION_MIXED16 2 /* as MIXED with FP16 MFMA network layers
+                                     (FP32 accumulation) */
 
 typedef struct hdnnp_model hdnnp_model;
 typedef struct hdnnp_ctx   hdnnp_ctx;
@@ -108,7 +111,7 @@ int    hdnnp_model_set_scaling(hdnnp_model* m, int e, int scale_type,
  * structure and work buffers, one HIP stream.  device = HIP ordinal. */
 int  hdnnp_ctx_create(const hdnnp_model* m, int device, hdnnp_ctx** out);
 void hdnnp_ctx_destroy(hdnnp_ctx* c);
-/* HDNNP_PRECISION_FP64 (default) or HDNNP_PRECISION_MIXED. */
+/* HDNNP_PRECISION_FP64 (default), _MIXED or _MIXED16. */
 int  hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision);
 /* Propagates a model change made after hdnnp_ctx_create (e.g.
  * hdnnp_model_set_scaling) to the context's device tables. */

@@ -112,6 +112,7 @@ struct hdnnp_ctx
     DBuf<ScaleDesc> dScale;
     DBuf<int> dPairGroups, dPairCount;
     DBuf<double> dWeights;
+    DBuf<float> dWeights32;
     int maxMembers = 0, maxCfg = 1, ld = 1;
 
     // structure
@@ -221,6 +222,10 @@ struct hdnnp_ctx
         up(dPairGroups, flat.pairGroups);
         up(dPairCount, flat.pairCount);
         up(dWeights, flat.weights);
+        {
+            std::vector<float> w32(flat.weights.begin(), flat.weights.end());
+            up(dWeights32, w32);
+        }
         T.ne = (int)flat.elem.size();
         T.maxnsf = model->m.maxNumSF();
         T.maxCutoff = model->m.maxCutoffRadius;
@@ -242,6 +247,7 @@ struct hdnnp_ctx
         T.pairGroups = dPairGroups.p;
         T.pairCount = dPairCount.p;
         T.weights = dWeights.p;
+        T.weights32 = dWeights32.p;
         ld = std::max(1, T.maxnsf);
         maxMembers = 0;
         maxCfg = 1;
@@ -475,7 +481,7 @@ void computeSF(hdnnp_ctx* c, bool raw)
     requireNL(c);
     c->setDevice();
     int const cap = std::max(c->maxNeigh, 1);
-    int const fp32 = (!raw && c->precision == HDNNP_PRECISION_MIXED) ? 1 : 0;
+    int const fp32 = (!raw && c->precision != HDNNP_PRECISION_FP64) ? 1 : 0;
     if (sf_forward_lds_bytes(cap, fp32) > 65536)
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the symmetry-"
@@ -525,6 +531,10 @@ void computeNN(hdnnp_ctx* c, bool deriv)
     {
         NNArgs a{};
         a.T = c->T;
+        a.ed = c->flat.elem[e];
+        a.mode = c->precision == HDNNP_PRECISION_MIXED16 ? 2
+               : c->precision == HDNNP_PRECISION_MIXED ? 1 : 0;
+        if (nn_mfma_lds_bytes(a.ed, a.mode) > 65536) a.mode = -1;
         a.e = (int)e;
         a.atoms = c->dAtomsOfElem[e].p;
         a.na = (int)c->hAtomsOfElem[e].size();
@@ -567,7 +577,7 @@ void computeForces(hdnnp_ctx* c, int mode)
         throw Error(HDNNP_E_INVALID, "ERROR: Unknown force mode.\n");
     c->setDevice();
     int const cap = std::max(c->maxNeigh, 1);
-    int const fp32 = c->precision == HDNNP_PRECISION_MIXED ? 1 : 0;
+    int const fp32 = c->precision != HDNNP_PRECISION_FP64 ? 1 : 0;
     size_t const shm = force_backward_lds_bytes(cap, fp32);
     if (shm > 65536)
         throw Error(HDNNP_E_CAPACITY,
@@ -722,6 +732,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
     c->dElem.release(); c->dCfg.release(); c->dGroup.release();
     c->dMember.release(); c->dScale.release(); c->dPairGroups.release();
     c->dPairCount.release(); c->dWeights.release();
+    c->dWeights32.release();
     for (auto& b : c->dAtomsOfElem) b.release();
     c->dR.release(); c->dElemAtom.release();
     c->dCellOfAtom.release(); c->dRank.release(); c->dCellCount.release();
@@ -740,7 +751,8 @@ int hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision)
 {
     return guarded([&]() {
         if (precision != HDNNP_PRECISION_FP64
-            && precision != HDNNP_PRECISION_MIXED)
+            && precision != HDNNP_PRECISION_MIXED
+            && precision != HDNNP_PRECISION_MIXED16)
             throw Error(HDNNP_E_INVALID, "ERROR: Unknown precision.\n");
         if (precision != c->precision)
             c->haveSF = c->haveNN = c->haveNNDeriv = false;

@@ -101,10 +101,15 @@ hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
                                const double* cself, int mode, double* F,
                                hipStream_t s);
 
-// Atomic networks (FP64) for the atoms of one element.
+// Atomic networks for the atoms of one element.  mode -1: per-atom FP64
+// kernel (any width); 0: FP64 MFMA tiles; 1: FP32 MFMA tiles; 2: FP16 MFMA
+// tiles with FP32 accumulation.  The MFMA kernels need
+// nn_mfma_lds_bytes(ed, mode) of LDS.
 struct NNArgs
 {
     ModelTables T;
+    ElemDesc ed;                 // host copy of T.elem[e]
+    int mode;
     int e;                       // element
     const int* atoms;            // atom indices of element e
     int na;
@@ -116,6 +121,7 @@ struct NNArgs
     int deriv;
 };
 hipError_t launch_nn(NNArgs const& a, hipStream_t s);
+size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode);
 
 hipError_t launch_energy_sum(const double* E, int n, double* out, hipStream_t s);
 

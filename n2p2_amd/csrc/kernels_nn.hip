@@ -1,4 +1,4 @@
-// Atomic neural networks (FP64) and the energy sum.
+// Atomic neural networks and the energy sum.
 //
 // NeuralNetwork::propagate (NeuralNetwork.cpp:386-394, :761-896) for all
 // atoms of one element at once: thread t handles atom atoms[t]; layer
@@ -102,6 +102,233 @@ __global__ __launch_bounds__(256) void k_nn(NNArgs a)
     }
 }
 
+// Matrix-core (MFMA) form of the same networks: one wavefront per tile of
+// 16 atoms of one element; every layer is the GEMM Z = X W^T over the tile
+// (K = fan-in), the backward pass the GEMMs Delta_(l-1) = Delta_l W_l
+// (K = fan-out).  Activations and derivatives live in LDS per tile, each
+// layer's output is written back in the operand layout of the next GEMM.
+// MODE 0: v_mfma_f64_16x16x4f64 (plain FP64 path).  MODE 1:
+// v_mfma_f32_16x16x4f32 (mixed path).  MODE 2: v_mfma_f32_16x16x16f16,
+// FP16 operands with FP32 accumulation.  Biases, activations and
+// normalisation in FP64 in every mode.
+// Operand layouts (16x16 tiles, lane l, v < 4): A[i = l % 16][k = K0 (l /
+// 16) + v'], B[k = K0 (l / 16) + v'][j = l % 16] with K0 = 1 (FP64, FP32;
+// one value per lane) or 4 (FP16, v' < 4); D[i][j = l % 16] in register v
+// with i = l / 16 + 4 v for FP64 and i = 4 (l / 16) + v for the FP32
+// accumulators (measured with a lane-layout probe on gfx950).
+typedef double nn_d4 __attribute__((ext_vector_type(4)));
+typedef float nn_f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 nn_h4 __attribute__((ext_vector_type(4)));
+
+template <int MODE> struct NNMma;
+
+template <> struct NNMma<0>
+{
+    typedef double Act;
+    typedef nn_d4 Acc;
+    static constexpr int K = 4;
+    // A tile step: X[m][k0 + l/16] (zero beyond nk).
+    __device__ static double a(Act const* X, int ld, int nk, int k0, int lane)
+    {
+        int const m = lane & 15, k = k0 + (lane >> 4);
+        return k < nk ? X[m * ld + k] : 0.0;
+    }
+    // B element (k, j) = W[j * ldw + k] (forward: W^T) or W[k * ldw + j]
+    // (backward); zero outside [0, nk) x [0, nj).
+    __device__ static double b(ModelTables const& T, int off, bool trans,
+                               int ldw, int nk, int nj, int k0, int j0,
+                               int lane)
+    {
+        int const j = j0 + (lane & 15), k = k0 + (lane >> 4);
+        if (k >= nk || j >= nj) return 0.0;
+        return T.weights[off + (trans ? j * ldw + k : k * ldw + j)];
+    }
+    __device__ static Acc mma(double a, double b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <> struct NNMma<1>
+{
+    typedef float Act;
+    typedef nn_f4 Acc;
+    static constexpr int K = 4;
+    __device__ static float a(Act const* X, int ld, int nk, int k0, int lane)
+    {
+        int const m = lane & 15, k = k0 + (lane >> 4);
+        return k < nk ? X[m * ld + k] : 0.0f;
+    }
+    __device__ static float b(ModelTables const& T, int off, bool trans,
+                              int ldw, int nk, int nj, int k0, int j0,
+                              int lane)
+    {
+        int const j = j0 + (lane & 15), k = k0 + (lane >> 4);
+        if (k >= nk || j >= nj) return 0.0f;
+        return T.weights32[off + (trans ? j * ldw + k : k * ldw + j)];
+    }
+    __device__ static Acc mma(float a, float b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <> struct NNMma<2>
+{
+    typedef float Act;
+    typedef nn_f4 Acc;
+    static constexpr int K = 16;
+    __device__ static nn_h4 a(Act const* X, int ld, int nk, int k0, int lane)
+    {
+        int const m = lane & 15, k = k0 + 4 * (lane >> 4);
+        nn_h4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = (_Float16)(k + q < nk ? X[m * ld + k + q] : 0.0f);
+        return v;
+    }
+    __device__ static nn_h4 b(ModelTables const& T, int off, bool trans,
+                              int ldw, int nk, int nj, int k0, int j0,
+                              int lane)
+    {
+        int const j = j0 + (lane & 15), k = k0 + 4 * (lane >> 4);
+        nn_h4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            float w = 0.0f;
+            if (k + q < nk && j < nj)
+                w = T.weights32[off + (trans ? j * ldw + k + q
+                                             : (k + q) * ldw + j)];
+            v[q] = (_Float16)w;
+        }
+        return v;
+    }
+    __device__ static Acc mma(nn_h4 a, nn_h4 b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_nn_mfma(NNArgs a)
+{
+    typedef NNMma<MODE> MM;
+    typedef typename MM::Act Act;
+    typedef typename MM::Acc Acc;
+    extern __shared__ double nnsm[];
+    Act* const act = (Act*)nnsm;
+    ModelTables const& T = a.T;
+    ElemDesc const& ed = T.elem[a.e];
+    int const L = ed.nlayers;
+    int const lane = threadIdx.x;
+    int const tile0 = blockIdx.x * 16;
+    // LDS: value planes of layers 0..L-1, then derivative planes of 1..L-1,
+    // each [16][width].
+    int voff[kMaxLayers], doff[kMaxLayers];
+    {
+        int o = 0;
+        for (int l = 0; l < L; ++l) { voff[l] = o; o += 16 * ed.width[l]; }
+        for (int l = 1; l < L; ++l) { doff[l] = o; o += 16 * ed.width[l]; }
+    }
+    // Input layer: the tile's scaled SFs (zero rows past the last atom).
+    {
+        int const nin = ed.width[0];
+        for (int q = lane; q < 16 * nin; q += 64)
+        {
+            int const m = q / nin, k = q - m * nin;
+            int const t = tile0 + m;
+            double g = 0.0;
+            if (t < a.na) g = a.G[(size_t)a.atoms[t] * a.ldG + k];
+            act[voff[0] + q] = (Act)g;
+        }
+    }
+    __syncthreads();
+    for (int l = 1; l < L; ++l)
+    {
+        int const nin = ed.width[l - 1], nout = ed.width[l];
+        Act const* X = act + voff[l - 1];
+        for (int j0 = 0; j0 < nout; j0 += 16)
+        {
+            Acc acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] = 0;
+            for (int k0 = 0; k0 < nin; k0 += MM::K)
+                acc = MM::mma(MM::a(X, nin, nin, k0, lane),
+                              MM::b(T, ed.woff[l], true, nin, nin, nout, k0,
+                                    j0, lane),
+                              acc);
+            int const j = j0 + (lane & 15);
+            if (j < nout)
+            {
+                double const bias = T.weights[ed.boff[l] + j];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    int const m = MM::row(lane, v);
+                    double z = (double)acc[v] + bias;
+                    if (ed.normalize) z /= nin;
+                    double val, d;
+                    activate(ed.act[l], z, val, d);
+                    act[voff[l] + m * nout + j] = (Act)val;
+                    act[doff[l] + m * nout + j] = (Act)d;
+                    if (l == L - 1 && tile0 + m < a.na)
+                        a.E[a.atoms[tile0 + m]] = val;   // one output neuron
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!a.deriv) return;
+    // Output delta (one neuron): f' / fan-in with normalize_nodes.
+    if (lane < 16)
+    {
+        double g = (double)act[doff[L - 1] + lane];
+        if (ed.normalize) g /= ed.width[L - 2];
+        act[doff[L - 1] + lane] = (Act)g;
+    }
+    __syncthreads();
+    for (int l = L - 1; l >= 1; --l)
+    {
+        int const nin = ed.width[l - 1], nout = ed.width[l];
+        Act const* Dl = act + doff[l];
+        for (int j0 = 0; j0 < nin; j0 += 16)
+        {
+            Acc acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] = 0;
+            for (int k0 = 0; k0 < nout; k0 += MM::K)
+                acc = MM::mma(MM::a(Dl, nout, nout, k0, lane),
+                              MM::b(T, ed.woff[l], false, nin, nout, nin, k0,
+                                    j0, lane),
+                              acc);
+            int const j = j0 + (lane & 15);
+            if (j < nin)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    int const m = MM::row(lane, v);
+                    if (l >= 2)
+                    {
+                        int const q = doff[l - 1] + m * nin + j;
+                        double g = (double)act[q];
+                        if (ed.normalize) g /= ed.width[l - 2];
+                        act[q] = (Act)((double)acc[v] * g);
+                    }
+                    else if (tile0 + m < a.na)
+                        a.dEdG[(size_t)a.atoms[tile0 + m] * a.ldG + j]
+                            = (double)acc[v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Energy = sum of atomic energies: each thread sums a strided slice in
 // order, then a fixed tree; deterministic.
 __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ E,
@@ -123,10 +350,27 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ E,
 
 }  // namespace
 
+size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode)
+{
+    size_t w = 0;
+    for (int l = 0; l < ed.nlayers; ++l) w += ed.width[l];
+    for (int l = 1; l < ed.nlayers; ++l) w += ed.width[l];
+    return 16 * w * (mode >= 1 ? sizeof(float) : sizeof(double));
+}
+
 hipError_t launch_nn(NNArgs const& a, hipStream_t s)
 {
     if (a.na <= 0) return hipSuccess;
-    k_nn<<<(a.na + 255) / 256, 256, 0, s>>>(a);
+    if (a.mode < 0)
+        k_nn<<<(a.na + 255) / 256, 256, 0, s>>>(a);
+    else
+    {
+        size_t const shm = nn_mfma_lds_bytes(a.ed, a.mode);
+        int const nb = (a.na + 15) / 16;
+        if (a.mode == 2) k_nn_mfma<2><<<nb, 64, shm, s>>>(a);
+        else if (a.mode == 1) k_nn_mfma<1><<<nb, 64, shm, s>>>(a);
+        else k_nn_mfma<0><<<nb, 64, shm, s>>>(a);
+    }
     return hipGetLastError();
 }
 

@@ -111,6 +111,7 @@ struct ModelTables
     const int*        pairGroups;   // [pb * kMaxPairGroups + t], global ids
     const int*        pairCount;    // [pb]
     const double*     weights;
+    const float*      weights32;    // the same weights rounded to FP32
 };
 
 }  // namespace hdnnp
