@@ -113,6 +113,16 @@ int  hdnnp_ctx_create(const hdnnp_model* m, int device, hdnnp_ctx** out);
 void hdnnp_ctx_destroy(hdnnp_ctx* c);
 /* HDNNP_PRECISION_FP64 (default), _MIXED or _MIXED16. */
 int  hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision);
+/* 1 (default unless the environment sets N2P2_AMD_RTC=0): the symmetry-
+ * function and force kernels are compiled for this model at first use
+ * (member constants folded in, hipRTC) and kept for the process; 0: the
+ * generic kernels that read the member tables.  Same results to rounding.
+ * No counterpart in the reference (an engine control like the precision). */
+int  hdnnp_ctx_set_specialized(hdnnp_ctx* c, int enable);
+/* *active = 1 when the specialised kernels for the current precision are
+ * enabled and compiled (compiling them now if they were not yet), 0 when
+ * the generic kernels run.  No counterpart in the reference. */
+int  hdnnp_ctx_specialized_active(hdnnp_ctx* c, int* active);
 /* Propagates a model change made after hdnnp_ctx_create (e.g.
  * hdnnp_model_set_scaling) to the context's device tables. */
 int  hdnnp_ctx_update_model(hdnnp_ctx* c);

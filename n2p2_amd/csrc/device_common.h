@@ -1,10 +1,12 @@
 // Device helpers shared by the HIP kernels: cutoff functions, integer
-// powers, activations and 64-lane wave reductions.  All FP64.
+// powers, activations and 64-lane wave reductions, templated on the
+// arithmetic type (double, or float on the mixed-precision path).
 #ifndef HDNNP_DEVICE_COMMON_H
 #define HDNNP_DEVICE_COMMON_H
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
-#include <stdint.h>
+#endif
 
 #include "tables.h"
 
@@ -12,6 +14,13 @@ namespace hdnnp {
 
 constexpr int kWave = 64;
 constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Four- and two-component vectors of the arithmetic type R (LDS records).
+template <typename R> struct VecT;
+template <> struct VecT<float> { typedef float4 V4; typedef float2 V2; };
+template <> struct VecT<double> { typedef double4 V4; typedef double2 V2; };
+template <typename R> using Vec4 = typename VecT<R>::V4;
+template <typename R> using Vec2 = typename VecT<R>::V2;
 
 // sin(pi x) and cos(pi x) for x in [0, 1].  With u = x - 1/2:
 // cos(pi x) = -sin(pi u) and sin(pi x) = cos(pi u); both are evaluated from

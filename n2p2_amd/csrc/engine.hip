@@ -8,6 +8,7 @@
 
 #include "kernels.h"
 #include "model.h"
+#include "specialize.h"
 
 #include <hip/hip_runtime.h>
 
@@ -109,6 +110,7 @@ struct hdnnp_ctx
     DBuf<CutCfg> dCfg;
     DBuf<GroupDesc> dGroup;
     DBuf<MemberDesc> dMember;
+    DBuf<LamRun> dRun;
     DBuf<ScaleDesc> dScale;
     DBuf<int> dPairGroups, dPairCount;
     DBuf<double> dWeights;
@@ -147,6 +149,21 @@ struct hdnnp_ctx
     bool haveSF = false, haveNN = false, haveNNDeriv = false;
     double energy = 0.0;
     int precision = HDNNP_PRECISION_FP64;
+    // Model-specialised SF / force kernels per arithmetic (0 FP64, 1 FP32),
+    // compiled on first use (specialize.hip); generic kernels otherwise.
+    bool specialized = plan_enabled_by_default();
+    PlanKernels plan[2];
+    bool planTried[2] = {false, false};
+    PlanKernels planFor(int fp32)
+    {
+        if (!specialized) return PlanKernels();
+        if (!planTried[fp32])
+        {
+            plan[fp32] = plan_kernels(flat, T, fp32, device);
+            planTried[fp32] = true;
+        }
+        return plan[fp32];
+    }
 
     // profiling
     bool prof = false;
@@ -207,6 +224,7 @@ struct hdnnp_ctx
     void uploadModel()
     {
         flat = model->m.flatten();
+        planTried[0] = planTried[1] = false;
         auto up = [&](auto& buf, auto const& v) {
             buf.ensure(v.size());
             if (!v.empty())
@@ -218,6 +236,7 @@ struct hdnnp_ctx
         up(dCfg, flat.cfg);
         up(dGroup, flat.group);
         up(dMember, flat.member);
+        up(dRun, flat.run);
         up(dScale, flat.scale);
         up(dPairGroups, flat.pairGroups);
         up(dPairCount, flat.pairCount);
@@ -243,6 +262,7 @@ struct hdnnp_ctx
         T.cfg = dCfg.p;
         T.group = dGroup.p;
         T.member = dMember.p;
+        T.run = dRun.p;
         T.scale = dScale.p;
         T.pairGroups = dPairGroups.p;
         T.pairCount = dPairCount.p;
@@ -501,7 +521,17 @@ void computeSF(hdnnp_ctx* c, bool raw)
     a.fp32 = fp32;
     hipEvent_t ev = nullptr;
     c->begin(K_SYMFUNC, ev);
-    hipCheck(launch_sf_forward(a, c->stream), "symmetry functions");
+    PlanKernels const pk = raw ? PlanKernels() : c->planFor(fp32);
+    if (pk.sf && c->n > 0)
+    {
+        void* params[] = {&a};
+        hipCheck(hipModuleLaunchKernel(pk.sf, (unsigned)c->n, 1, 1, 64, 1, 1,
+                                       (unsigned)sf_forward_lds_bytes(cap,
+                                                                      fp32),
+                                       c->stream, params, nullptr),
+                 "symmetry functions (specialised)");
+    }
+    else hipCheck(launch_sf_forward(a, c->stream), "symmetry functions");
     c->end(K_SYMFUNC, ev);
     c->haveSF = !raw;
     c->haveNN = c->haveNNDeriv = false;
@@ -601,7 +631,16 @@ void computeForces(hdnnp_ctx* c, int mode)
     a.fp32 = fp32;
     hipEvent_t ev = nullptr, ev2 = nullptr;
     c->begin(K_FORCE, ev);
-    hipCheck(launch_force_backward(a, c->stream), "force backward");
+    PlanKernels const pk = c->planFor(fp32);
+    if (pk.force && c->n > 0)
+    {
+        void* params[] = {&a};
+        hipCheck(hipModuleLaunchKernel(pk.force, (unsigned)c->n, 1, 1, 64, 1,
+                                       1, (unsigned)shm, c->stream, params,
+                                       nullptr),
+                 "force backward (specialised)");
+    }
+    else hipCheck(launch_force_backward(a, c->stream), "force backward");
     c->end(K_FORCE, ev);
     c->begin(K_GATHER, ev2);
     hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dNbrJ.p,
@@ -730,7 +769,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
     }
     for (auto e : c->eventPool) (void)hipEventDestroy(e);
     c->dElem.release(); c->dCfg.release(); c->dGroup.release();
-    c->dMember.release(); c->dScale.release(); c->dPairGroups.release();
+    c->dMember.release(); c->dRun.release(); c->dScale.release(); c->dPairGroups.release();
     c->dPairCount.release(); c->dWeights.release();
     c->dWeights32.release();
     for (auto& b : c->dAtomsOfElem) b.release();
@@ -757,6 +796,26 @@ int hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision)
         if (precision != c->precision)
             c->haveSF = c->haveNN = c->haveNNDeriv = false;
         c->precision = precision;
+    });
+}
+
+int hdnnp_ctx_set_specialized(hdnnp_ctx* c, int enable)
+{
+    return guarded([&]() {
+        if (!c) throw Error(HDNNP_E_INVALID, "ERROR: Null context.\n");
+        if ((enable != 0) != c->specialized)
+            c->haveSF = c->haveNN = c->haveNNDeriv = false;
+        c->specialized = enable != 0;
+    });
+}
+
+int hdnnp_ctx_specialized_active(hdnnp_ctx* c, int* active)
+{
+    return guarded([&]() {
+        if (!c || !active) throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        c->setDevice();
+        PlanKernels const pk = c->planFor(c->precision != HDNNP_PRECISION_FP64);
+        *active = (pk.sf && pk.force) ? 1 : 0;
     });
 }
 

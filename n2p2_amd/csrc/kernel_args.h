@@ -1,0 +1,45 @@
+// Argument blocks of the symmetry-function and force kernels, shared by the
+// ahead-of-time kernels and the model-specialised ones compiled at run time.
+#ifndef HDNNP_KERNEL_ARGS_H
+#define HDNNP_KERNEL_ARGS_H
+
+#include "tables.h"
+
+namespace hdnnp {
+
+// Symmetry functions.  raw != 0 skips scaling (statistics); fp32 != 0
+// selects the mixed-precision kernel (FP32 geometry and member sums).
+struct SFArgs
+{
+    ModelTables T;
+    int n;                       // atoms
+    const int* off;              // CSR offsets (n+1)
+    const double4* geo;          // dx, dy, dz, d per entry
+    const int* nbrJ;             // neighbour atom per entry
+    const int* elem;             // element per atom
+    double* G;                   // n x ldG
+    int ldG;
+    int cap;                     // LDS neighbour capacity per wave
+    int raw;
+    int fp32;
+};
+
+struct ForceArgs
+{
+    ModelTables T;
+    int n;
+    const int* off;
+    const double4* geo;
+    const int* nbrJ;
+    const int* elem;
+    const double* dEdG;          // n x ldG
+    int ldG;
+    double* cslot;               // 3 x M (x, y, z per entry)
+    double* cself;               // 3 x n
+    int cap;
+    int fp32;                    // mixed-precision contraction
+};
+
+}  // namespace hdnnp
+
+#endif

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernel_args.h"
 #include "tables.h"
 
 namespace hdnnp {
@@ -56,43 +57,12 @@ hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
 hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
                              int* partner, hipStream_t s);
 
-// Symmetry functions.  raw != 0 skips scaling (statistics); fp32 != 0
-// selects the mixed-precision kernel (FP32 geometry and member sums).
-struct SFArgs
-{
-    ModelTables T;
-    int n;                       // atoms
-    const int* off;              // CSR offsets (n+1)
-    const double4* geo;          // dx, dy, dz, d per entry
-    const int* nbrJ;             // neighbour atom per entry
-    const int* elem;             // element per atom
-    double* G;                   // n x ldG
-    int ldG;
-    int cap;                     // LDS neighbour capacity per wave
-    int raw;
-    int fp32;
-};
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
 size_t sf_forward_lds_bytes(int cap, int fp32);
 // Work counters (10 x uint64, accumulated): see k_sf_count.
 hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
                            hipStream_t s);
 
-struct ForceArgs
-{
-    ModelTables T;
-    int n;
-    const int* off;
-    const double4* geo;
-    const int* nbrJ;
-    const int* elem;
-    const double* dEdG;          // n x ldG
-    int ldG;
-    double* cslot;               // 3 x M (x, y, z per entry)
-    double* cself;               // 3 x n
-    int cap;
-    int fp32;                    // mixed-precision contraction
-};
 hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
 size_t force_backward_lds_bytes(int cap, int fp32);
 

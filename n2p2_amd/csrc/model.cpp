@@ -609,9 +609,40 @@ Model::Flat Model::flatten() const
                     gd.mem0 = (int)F.member.size();
                     size_t const c1 = std::min(mem.size(), c0 + kMaxMembers);
                     gd.nmem = (int)(c1 - c0);
-                    for (size_t i = c0; i < c1; ++i)
+                    // zeta = 2^level: plambda^(zeta-1) is the level-th
+                    // prefix product x x^2 x^4 ... of pow_int's squaring
+                    // chain; -1 for any other zeta.
+                    auto levelOf = [&](SymFn const& s) {
+                        if (!s.useIntegerPow) return -1;
+                        for (int l = 0; l <= kMaxLevel; ++l)
+                            if (s.zetaInt == (1 << l)) return l;
+                        return -1;
+                    };
+                    // Within each (eta, rs, lambda) run: chain members by
+                    // level, then generic members.
+                    std::vector<int> cm(mem.begin() + c0, mem.begin() + c1);
+                    if (g.type != 2)
                     {
-                        SymFn const& s = e.sf[mem[i]];
+                        size_t r0 = 0;
+                        while (r0 < cm.size())
+                        {
+                            size_t r1 = r0 + 1;
+                            SymFn const& f = e.sf[cm[r0]];
+                            while (r1 < cm.size()
+                                   && e.sf[cm[r1]].eta == f.eta
+                                   && e.sf[cm[r1]].rs == f.rs
+                                   && e.sf[cm[r1]].lambda == f.lambda)
+                                ++r1;
+                            std::stable_partition(
+                                cm.begin() + r0, cm.begin() + r1,
+                                [&](int a) { return levelOf(e.sf[a]) >= 0; });
+                            r0 = r1;
+                        }
+                    }
+                    gd.run0 = (int)F.run.size();
+                    for (size_t q = 0; q < cm.size(); ++q)
+                    {
+                        SymFn const& s = e.sf[cm[q]];
                         MemberDesc md{};
                         md.eta = s.eta;
                         md.rs = s.rs;
@@ -637,21 +668,41 @@ Model::Flat Model::flatten() const
                         md.zint = s.zetaInt;
                         md.useint = s.useIntegerPow ? 1 : 0;
                         // calculateExp: new eta or rs, or first of a chunk.
-                        if (i == c0) md.calcexp = 1;
+                        if (q == 0) md.calcexp = 1;
                         else
                         {
-                            SymFn const& p = e.sf[mem[i - 1]];
+                            SymFn const& p = e.sf[cm[q - 1]];
                             md.calcexp = (p.eta != s.eta || p.rs != s.rs);
                         }
                         md.sf = s.index;
                         md.newlam = md.calcexp
-                                 || e.sf[mem[i - 1]].lambda != s.lambda;
-                        // zeta = 2^l: plambda^(zeta-1) is the l-th prefix
-                        // product x x^2 x^4 ... of pow_int's squaring chain.
-                        md.level = -1;
-                        if (s.useIntegerPow)
-                            for (int l = 0; l < 16; ++l)
-                                if (s.zetaInt == (1 << l)) md.level = l;
+                                 || e.sf[cm[q - 1]].lambda != s.lambda;
+                        md.level = levelOf(s);
+                        if (g.type != 2)
+                        {
+                            if (md.newlam)
+                            {
+                                LamRun lr{};
+                                lr.lambda = s.lambda;
+                                lr.eta = s.eta;
+                                lr.rs = s.rs;
+                                lr.newexp = md.calcexp;
+                                lr.m0 = (int)q;
+                                lr.lmax = -1;
+                                for (int l = 0; l <= kMaxLevel; ++l)
+                                    lr.lvEnd[l] = (int)q;
+                                F.run.push_back(lr);
+                                gd.nrun++;
+                            }
+                            LamRun& lr = F.run.back();
+                            lr.mend = (int)q + 1;
+                            if (md.level >= 0)
+                            {
+                                lr.lmax = std::max(lr.lmax, md.level);
+                                for (int l = md.level; l <= kMaxLevel; ++l)
+                                    lr.lvEnd[l] = (int)q + 1;
+                            }
+                        }
                         F.member.push_back(md);
                     }
                     if (pass == 0) d.nrad++;

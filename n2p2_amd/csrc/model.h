@@ -86,6 +86,7 @@ public:
         std::vector<CutCfg>     cfg;
         std::vector<GroupDesc>  group;
         std::vector<MemberDesc> member;
+        std::vector<LamRun>     run;
         std::vector<ScaleDesc>  scale;
         std::vector<int>        pairGroups, pairCount;
         std::vector<double>     weights;

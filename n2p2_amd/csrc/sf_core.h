@@ -1,0 +1,417 @@
+// Device code of the symmetry-function forward kernel, shared by the
+// ahead-of-time build (generic member loop reading the group tables) and
+// the model-specialised build compiled at run time (HDNNP_PLAN defined and
+// plan_forward_members provided before this header).
+//
+// Symmetry-function forward kernel: G values of every atom, in FP64 (plain
+// path) or with FP32 geometry and member sums (mixed path; lane partials
+// summed in FP64, scaling in FP64).
+//
+// Restates SymGrpExpRad::calculate (SymGrpExpRad.cpp:144-239),
+// SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403) and
+// SymGrpExpAngw::calculate (SymGrpExpAngw.cpp:156-376) without derivatives
+// (the force kernel recomputes those).  One wavefront per atom:
+//   1. the atom's neighbours are staged in LDS, partitioned by element
+//      (stable ballot compaction), so every group reads exactly the
+//      neighbour blocks it needs;
+//   2. groups run one after another; a group's member parameters are staged
+//      in LDS and the ones read per member held in scalar registers, its
+//      member sums live in registers;
+//   3. angular groups enumerate the unordered pairs of their blocks, test
+//      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
+//      with a ballot prefix; whenever 64 are queued every lane evaluates
+//      one, so the expensive triplet work runs without divergence;
+//   4. inside a triplet the members are walked as (eta, rs) runs sharing one
+//      exponential and (eta, rs, lambda) runs sorted by zeta sharing one
+//      chain of powers of plambda (PowChain, bit-identical to pow_int);
+//   5. member sums are reduced with a fixed butterfly (deterministic).
+#ifndef HDNNP_SF_CORE_H
+#define HDNNP_SF_CORE_H
+
+#include "device_common.h"
+#include "kernel_args.h"
+
+namespace hdnnp {
+
+namespace {
+
+constexpr int kQueue = 128;
+
+// Staged neighbours: (x, y, z, r) and (f_c(r) for the current cutoff, 1/r).
+template <typename R>
+struct Stage
+{
+    Vec4<R>* P;
+    Vec2<R>* Q;      // null when only geometry is needed
+};
+
+__device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
+                                             int& r)
+{
+    q = (int)((float)p * inv);
+    r = p - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+}
+
+// Stages the atom's neighbours in LDS partitioned by element (stable ballot
+// compaction); sStart / sCnt receive each element's block.
+template <typename R>
+__device__ void stage_by_element(SFArgs const& a, Stage<R> const& S, int s0,
+                                 int n, int* sStart, int* sCnt)
+{
+    int const lane = threadIdx.x;
+    int const ne = a.T.ne;
+    int cnt[kMaxElements];
+#pragma unroll
+    for (int q = 0; q < kMaxElements; ++q) cnt[q] = 0;
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int const e = s < n ? a.elem[a.nbrJ[s0 + s]] : -1;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+            if (q < ne) cnt[q] += __popcll(__ballot(e == q));
+    }
+    int start[kMaxElements];
+    {
+        int acc = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            start[q] = acc;
+            acc += (q < ne) ? cnt[q] : 0;
+        }
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            sStart[q] = start[q];
+            sCnt[q] = (q < ne) ? cnt[q] : 0;
+        }
+    }
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int e = -1;
+        double4 g = make_double4(0, 0, 0, 1);
+        if (s < n)
+        {
+            e = a.elem[a.nbrJ[s0 + s]];
+            g = a.geo[s0 + s];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            if (q < ne)
+            {
+                uint64_t const mask = __ballot(e == q);
+                if (e == q)
+                {
+                    int const pos = start[q] + lanes_below(mask);
+                    Vec4<R> p;
+                    p.x = (R)g.x;
+                    p.y = (R)g.y;
+                    p.z = (R)g.z;
+                    p.w = (R)g.w;
+                    S.P[pos] = p;
+                    if (S.Q) S.Q[pos].y = R(1) / (R)g.w;
+                }
+                start[q] += __popcll(mask);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Unordered pairs of a group's neighbour blocks, indexed 0..npairs-1.
+// Different elements: p -> (p / nb, p % nb).  Same element (n atoms):
+// circle pairing, p -> {r, r + 1 + c mod n} for p < n*h (h = (n-1)/2), and
+// {r, r + n/2} for the n/2 remaining pairs when n is even; every unordered
+// pair appears exactly once.
+struct PairCursor
+{
+    bool same;
+    int a0, na, b0, nb, hh, npairs;
+    float inv;
+    __device__ PairCursor(bool same_, int a0_, int na_, int b0_, int nb_)
+        : same(same_), a0(a0_), na(na_), b0(b0_), nb(nb_)
+    {
+        hh = same ? (na - 1) / 2 : 0;
+        npairs = same ? na * (na - 1) / 2 : na * nb;
+        int const dv = same ? hh : nb;
+        inv = dv > 0 ? 1.0f / (float)dv : 0.0f;
+    }
+    __device__ bool decode(int p, int& ja, int& kb) const
+    {
+        if (p >= npairs) return false;
+        int r, c;
+        if (same)
+        {
+            if (p < na * hh)
+            {
+                divmod_small(p, hh, inv, r, c);
+                int v = r + 1 + c;
+                if (v >= na) v -= na;
+                ja = a0 + r;
+                kb = a0 + v;
+            }
+            else
+            {
+                r = p - na * hh;
+                ja = a0 + r;
+                kb = a0 + r + na / 2;
+            }
+        }
+        else
+        {
+            divmod_small(p, nb, inv, r, c);
+            ja = a0 + r;
+            kb = b0 + c;
+        }
+        return true;
+    }
+};
+
+// rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
+template <typename R>
+__device__ __forceinline__ bool pair_passes(Stage<R> const& S, int ja, int kb,
+                                            bool wide, GroupDesc const& g)
+{
+    R const rc = (R)g.rc;
+    Vec4<R> const pj = S.P[ja], pk = S.P[kb];
+    bool pass = pj.w < rc && pk.w < rc;
+    if (!wide)
+    {
+        R const dx = pk.x - pj.x;
+        R const dy = pk.y - pj.y;
+        R const dz = pk.z - pj.z;
+        pass = pass && (dx * dx + dy * dy + dz * dz) < (R)g.rc2;
+    }
+    return pass;
+}
+
+// A group's member parameters in wave-uniform (scalar) registers: lambda
+// and the flags word; the exponential's parameters are read from LDS at the
+// start of each (eta, rs) run.
+template <typename R, int MAXM>
+struct FwdMembers
+{
+    R lam[MAXM];
+    int code[MAXM];
+};
+
+// One triplet, all members of the group: acc[m] += fg * plambda * pfc with
+// fg = exp(-eta sum r^2) plambda^(zeta - 1) (SymGrpExpAngn.cpp:276-297).
+template <int MAXM, int CT, bool GENPOW, typename R>
+__device__ __forceinline__ void triplet_forward(
+    int gi, Stage<R> const& S, int ja, int kb, bool wide, CutCfg const& cc,
+    FwdMembers<R, MAXM> const& M, MParam<R> const* MP, int nmem,
+    R (&acc)[MAXM])
+{
+    Vec4<R> const pj = S.P[ja], pk = S.P[kb];
+    Vec2<R> const qj = S.Q[ja], qk = S.Q[kb];
+    R const rij = pj.w, rik = pk.w;
+    R pfc, r2sum, rjk = R(0);
+    if (!wide)
+    {
+        R const dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
+        R const rjk2 = dx * dx + dy * dy + dz * dz;
+        rjk = rjk2 * rsqrt_nr(rjk2);
+        R fcjk, dfcjk;
+        cutoff_inside<CT>(cc, rjk, fcjk, dfcjk);
+        pfc = qj.x * qk.x * fcjk;
+        r2sum = rij * rij + rik * rik + rjk2;
+    }
+    else
+    {
+        pfc = qj.x * qk.x;
+        r2sum = rij * rij + rik * rik;
+    }
+    R const rinvijik = qj.y * qk.y;
+    R const costijk = (pj.x * pk.x + pj.y * pk.y + pj.z * pk.z) * rinvijik;
+#ifdef HDNNP_PLAN
+    // Member sums of the model-specialised build (straight-line code per
+    // group chunk with the member constants folded in).
+    (void)M;
+    (void)MP;
+    (void)nmem;
+    plan_forward_members<R, MAXM>(gi, wide, costijk, pfc, r2sum, rij, rik,
+                                  rjk, acc);
+#else
+    (void)gi;
+    R vp = R(0);             // exp(.) * pfc of the current (eta, rs) run
+    R pl = R(0);             // plambda of the current lambda run
+    PowChain<R> chain;
+    chain.reset(R(0));
+#pragma unroll
+    for (int m = 0; m < MAXM; ++m)
+    {
+        if (m < nmem)
+        {
+            int const fl = M.code[m];
+            if (fl & MF_NEWEXP)
+            {
+                R const eta = MP[m].eta, rs = MP[m].rs;
+                R s = r2sum;
+                if (rs > R(0))
+                {
+                    R const a = rij - rs, b = rik - rs;
+                    s = a * a + b * b;
+                    if (!wide)
+                    {
+                        R const c = rjk - rs;
+                        s += c * c;
+                    }
+                }
+                vp = exp_scaled(exp_factor(eta) * s) * pfc;
+            }
+            if (fl & MF_NEWLAM)
+            {
+                pl = R(1) + M.lam[m] * costijk;
+                chain.reset(pl);
+            }
+            R pw;
+            if (fl & MF_PREFIX) pw = chain.at((fl >> 8) & 15);
+            else if (!GENPOW || (fl & MF_INTPOW))
+                pw = pow_int(pl, MP[m].zint - 1);
+            else pw = pow(pl, MP[m].zeta - R(1));
+            acc[m] += pl > R(0) ? vp * (pw * pl) : R(0);
+        }
+    }
+#endif
+}
+
+// Body of the forward kernel (one 64-lane workgroup per atom).
+template <int MAXM, int CT, bool GENPOW, typename R>
+__device__ __forceinline__ void sf_forward_body(SFArgs const& a)
+{
+    extern __shared__ __align__(16) double sm[];
+    __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
+    __shared__ MParam<R> MP[kMaxMembers];
+    int const cap = a.cap;
+    Stage<R> S;
+    S.P = (Vec4<R>*)sm;
+    S.Q = (Vec2<R>*)(S.P + cap);
+    int* Q = (int*)(S.Q + cap);
+
+    int const i = blockIdx.x;
+    int const lane = threadIdx.x;
+    ModelTables const& T = a.T;
+    int const ec = a.elem[i];
+    ElemDesc const& ed = T.elem[ec];
+    int const s0 = a.off[i];
+    int const n = a.off[i + 1] - s0;
+
+    stage_by_element(a, S, s0, n, sStart, sCnt);
+
+    double rcCached = -1.0;
+    double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
+    for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        int const nmem = g.nmem;
+        if (lane < nmem) MP[lane] = make_mparam<R>(mem[lane], 0.0);
+        if (g.rc != rcCached)
+        {
+            for (int t = lane; t < n; t += 64)
+            {
+                R f, df;
+                cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
+                S.Q[t].x = f;
+            }
+            rcCached = g.rc;
+        }
+        __syncthreads();
+        R acc[MAXM];
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m) acc[m] = R(0);
+
+        if (g.type == 2)
+        {
+            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            R const rc = (R)g.rc;
+            for (int t = lane; t < na; t += 64)
+            {
+                R const r = S.P[a0 + t].w;
+                if (r < rc)
+                {
+                    R const fc = S.Q[a0 + t].x;
+#pragma unroll
+                    for (int m = 0; m < MAXM; ++m)
+                        if (m < nmem)
+                        {
+                            R const eta = MP[m].eta, rs = MP[m].rs;
+                            acc[m] += exp_r(-eta * (r - rs) * (r - rs)) * fc;
+                        }
+                }
+            }
+        }
+        else
+        {
+            bool const wide = (g.type == 9);
+            PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                          sStart[g.e2], sCnt[g.e2]);
+            FwdMembers<R, MAXM> M;
+#pragma unroll
+            for (int m = 0; m < MAXM; ++m)
+            {
+                MParam<R> const& p = MP[m < nmem ? m : 0];
+                M.lam[m] = uniform_r(p.lambda);
+                M.code[m] = uniform(p.flags);
+            }
+            int qn = 0;
+            for (int base = 0; base < pc.npairs; base += 64)
+            {
+                int ja = 0, kb = 0;
+                bool pass = false;
+                if (pc.decode(base + lane, ja, kb))
+                    pass = pair_passes(S, ja, kb, wide, g);
+                uint64_t const mask = __ballot(pass);
+                if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
+                qn += __popcll(mask);
+                __syncthreads();
+                if (qn >= 64)
+                {
+                    int const e = Q[qn - 64 + lane];
+                    triplet_forward<MAXM, CT, GENPOW>(gi, S, e & 0xffff,
+                                                      e >> 16, wide, cc, M, MP,
+                                                      nmem, acc);
+                    qn -= 64;
+                    __syncthreads();
+                }
+            }
+            if (lane < qn)
+            {
+                int const e = Q[lane];
+                triplet_forward<MAXM, CT, GENPOW>(gi, S, e & 0xffff, e >> 16,
+                                                  wide, cc, M, MP, nmem, acc);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MAXM; ++m)
+        {
+            if (m < nmem)
+            {
+                double const v = wave_sum((double)acc[m]);
+                if (lane == m)
+                {
+                    MemberDesc const& md = mem[m];
+                    double const raw = (g.type == 2) ? v : v * md.gfac;
+                    Gi[md.sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + md.sf],
+                                                       raw);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+}  // namespace hdnnp
+
+#endif

@@ -1,0 +1,37 @@
+// Model-specialised kernels compiled at run time (see specialize.hip).
+#ifndef HDNNP_SPECIALIZE_H
+#define HDNNP_SPECIALIZE_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "model.h"
+#include "tables.h"
+
+namespace hdnnp {
+
+// Kernels taking SFArgs / ForceArgs by value, launched with 64 threads per
+// atom and the same dynamic LDS as the generic kernels.  Null if the
+// specialisation is unavailable (then the generic kernels are used).
+struct PlanKernels
+{
+    hipFunction_t sf = nullptr;
+    hipFunction_t force = nullptr;
+};
+
+// Specialised kernels for the model (fp32: mixed-precision arithmetic) on
+// the given device, compiled on first request and kept for the process.
+PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T, int fp32,
+                         int device);
+
+// The generated member-loop code (the part of the kernel source that
+// depends on the model).
+std::string plan_source(Model::Flat const& F);
+
+// False when the environment sets N2P2_AMD_RTC=0.
+bool plan_enabled_by_default();
+
+}  // namespace hdnnp
+
+#endif

@@ -1,0 +1,333 @@
+// Model-specialised symmetry-function and force kernels.
+//
+// The member loops of the two heavy kernels are generic: they read each
+// member's lambda, zeta and exponential parameters from the group tables
+// and branch on them per triplet.  For a given model these are constants,
+// so this file writes them out as straight-line C++ (one switch case per
+// angular group chunk: one exponential per (eta, rs) run, one chain of
+// powers plambda^(2^l - 1) per (eta, rs, lambda) run, every member's
+// contribution with its lambda zeta folded in), compiles the kernels
+// together with that code through hipRTC for the context's device, and
+// keeps the loaded module for the process.  The kernel bodies are the same
+// device code as the ahead-of-time kernels (force_core.h, sf_core.h).
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "model.h"
+#include "specialize.h"
+
+namespace hdnnp {
+
+namespace {
+
+// Sources handed to hipRTC as named headers (generated at build time).
+#include "rtc_sources.inc"
+
+std::string lit(double v)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "R(%a)", v);
+    return b;
+}
+
+// plambda = 1 + lambda cos.
+std::string plambda(double lambda)
+{
+    if (lambda == 1.0) return "R(1) + costijk";
+    if (lambda == -1.0) return "R(1) - costijk";
+    return "R(1) + " + lit(lambda) + " * costijk";
+}
+
+// Exponent sum of an (eta, rs) run into s_.
+void emit_exp_sum(std::ostringstream& o, LamRun const& r, bool wide)
+{
+    if (r.rs > 0.0)
+    {
+        o << "        R const a_ = rij - " << lit(r.rs) << ";\n"
+          << "        R const b_ = rik - " << lit(r.rs) << ";\n"
+          << "        R s_ = a_ * a_ + b_ * b_;\n";
+        if (!wide)
+            o << "        R const c_ = rjk - " << lit(r.rs) << ";\n"
+              << "        s_ += c_ * c_;\n";
+    }
+    else o << "        R const s_ = r2sum;\n";
+}
+
+void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
+                        Model::Flat const& F)
+{
+    bool const wide = (g.type == 9);
+    o << "    case " << gi << ":\n    {\n"
+      << "        R S1 = R(0), U = R(0), vexp = R(0), c1 = R(0), c3 = R(0);\n";
+    bool first = true;
+    for (int ri = 0; ri < g.nrun; ++ri)
+    {
+        LamRun const& r = F.run[g.run0 + ri];
+        if (r.newexp)
+        {
+            if (!first)
+                o << "        T1 += vexp * S1;\n"
+                  << "        { R const vu = vexp * U; T2 += vu * c1;"
+                     " T3 += vu * c3; }\n"
+                  << "        S1 = U = R(0);\n";
+            first = false;
+            o << "        {\n";
+            emit_exp_sum(o, r, wide);
+            o << "        R const te = " << lit(2.0 * r.eta) << " * pfc;\n";
+            if (r.rs > 0.0)
+            {
+                o << "        c1 = pr1 - te * a_ * invj;\n";
+                if (!wide) o << "        c3 = te * c_ * invjk - pr3;\n";
+            }
+            else
+            {
+                o << "        c1 = pr1 - te;\n";
+                if (!wide) o << "        c3 = te - pr3;\n";
+            }
+            o << "        vexp = exp_scaled(exp_factor(" << lit(r.eta)
+              << ") * s_);\n        }\n";
+        }
+        o << "        {\n        R const pl = " << plambda(r.lambda) << ";\n"
+          << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n";
+        int level = 0;
+        int m = r.m0;
+        for (int l = 0; l <= r.lmax; ++l)
+            for (; m < r.lvEnd[l]; ++m)
+            {
+                MemberDesc const& md = F.member[g.mem0 + m];
+                for (; level < md.level; ++level)
+                    o << "        P *= X; X *= X;\n";
+                o << "        { R const f_ = w[" << m << "] * P; S1 += f_ * "
+                  << lit(md.zl) << "; U += f_ * pl; }\n";
+            }
+        for (; m < r.mend; ++m)
+        {
+            MemberDesc const& md = F.member[g.mem0 + m];
+            o << "        { R const pw_ = ";
+            if (md.useint) o << "pow_int(pl, " << md.zint - 1 << ")";
+            else o << "pow(pl, " << lit(md.zeta - 1.0) << ")";
+            o << "; R const f_ = pl > R(0) ? w[" << m << "] * pw_ : R(0);"
+              << " S1 += f_ * " << lit(md.zl) << "; U += f_ * pl; }\n";
+        }
+        o << "        }\n";
+    }
+    o << "        T1 += vexp * S1;\n"
+      << "        { R const vu = vexp * U; T2 += vu * c1; T3 += vu * c3; }\n"
+      << "        break;\n    }\n";
+}
+
+void emit_forward_case(std::ostringstream& o, int gi, GroupDesc const& g,
+                       Model::Flat const& F)
+{
+    bool const wide = (g.type == 9);
+    o << "    case " << gi << ":\n    {\n        R vp = R(0);\n";
+    for (int ri = 0; ri < g.nrun; ++ri)
+    {
+        LamRun const& r = F.run[g.run0 + ri];
+        if (r.newexp)
+        {
+            o << "        {\n";
+            emit_exp_sum(o, r, wide);
+            o << "        vp = exp_scaled(exp_factor(" << lit(r.eta)
+              << ") * s_) * pfc;\n        }\n";
+        }
+        o << "        {\n        R const pl = " << plambda(r.lambda) << ";\n"
+          << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n"
+          << "        R const vpl = vp * pl;\n";
+        int level = 0;
+        int m = r.m0;
+        for (int l = 0; l <= r.lmax; ++l)
+            for (; m < r.lvEnd[l]; ++m)
+            {
+                MemberDesc const& md = F.member[g.mem0 + m];
+                for (; level < md.level; ++level)
+                    o << "        P *= X; X *= X;\n";
+                o << "        acc[" << m << "] += vpl * P;\n";
+            }
+        for (; m < r.mend; ++m)
+        {
+            MemberDesc const& md = F.member[g.mem0 + m];
+            o << "        { R const pw_ = ";
+            if (md.useint) o << "pow_int(pl, " << md.zint - 1 << ")";
+            else o << "pow(pl, " << lit(md.zeta - 1.0) << ")";
+            o << "; acc[" << m << "] += pl > R(0) ? vpl * pw_ : R(0); }\n";
+        }
+        o << "        }\n";
+    }
+    o << "        break;\n    }\n";
+}
+
+int plan_maxm(ModelTables const& T) { return T.maxGroupMembers <= 8 ? 8 : 16; }
+
+int plan_cut(ModelTables const& T)
+{
+    int const c = T.cutType;
+    return (c == CT_COS || c == CT_TANHU || c == CT_POLY2) ? c : -1;
+}
+
+std::string full_source(std::string const& plan, ModelTables const& T,
+                        int fp32)
+{
+    std::ostringstream o;
+    char const* R = fp32 ? "float" : "double";
+    int const maxm = plan_maxm(T);
+    int const ct = plan_cut(T);
+    o << "#define HDNNP_PLAN 1\n"
+      << "#include \"device_common.h\"\n"
+      << "#include \"kernel_args.h\"\n"
+      << "namespace hdnnp {\nnamespace {\n" << plan << "}\n}\n"
+      << "#include \"force_core.h\"\n"
+      << "#include \"sf_core.h\"\n"
+      << "extern \"C\" __global__ __launch_bounds__(64)\n"
+      << "__attribute__((amdgpu_waves_per_eu("
+      << (fp32 && maxm <= 8 ? 4 : 1) << ", 8)))\n"
+      << "void hdnnp_plan_force(hdnnp::ForceArgs a)\n"
+      << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
+      << R << ">(a); }\n"
+      << "extern \"C\" __global__ __launch_bounds__(64)\n"
+      << "void hdnnp_plan_sf(hdnnp::SFArgs a)\n"
+      << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
+      << R << ">(a); }\n";
+    return o.str();
+}
+
+struct Module
+{
+    hipModule_t mod = nullptr;
+    PlanKernels k;
+};
+
+std::mutex g_mutex;
+std::map<std::string, std::shared_ptr<Module>> g_cache;
+bool g_warned = false;
+
+bool verbose()
+{
+    char const* v = std::getenv("N2P2_AMD_RTC_VERBOSE");
+    return v && *v && *v != '0';
+}
+
+std::shared_ptr<Module> compile(std::string const& src, int device)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hdnnp_plan.hip", kRtcNumHeaders,
+                            kRtcHeaders, kRtcHeaderNames) != HIPRTC_SUCCESS)
+        throw std::runtime_error("hiprtcCreateProgram failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        throw std::runtime_error("hipGetDeviceProperties failed");
+    std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+    // Packed FP32 instructions stay disabled, as for the ahead-of-time build.
+    char const* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-Xclang",
+                          "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult const rc = hiprtcCompileProgram(prog, 7, opts);
+    if (rc != HIPRTC_SUCCESS)
+    {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        throw std::runtime_error("hipRTC compilation failed:\n" + log);
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    std::vector<char> code(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    auto m = std::make_shared<Module>();
+    if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess)
+        throw std::runtime_error("hipModuleLoadData failed");
+    if (hipModuleGetFunction(&m->k.force, m->mod, "hdnnp_plan_force")
+            != hipSuccess
+        || hipModuleGetFunction(&m->k.sf, m->mod, "hdnnp_plan_sf")
+            != hipSuccess)
+        throw std::runtime_error("hipModuleGetFunction failed");
+    if (verbose())
+    {
+        double const s = std::chrono::duration<double>(
+            std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "[n2p2_amd] specialised kernels compiled in "
+                             "%.2f s (%zu bytes)\n", s, n);
+    }
+    return m;
+}
+
+}  // namespace
+
+std::string plan_source(Model::Flat const& F)
+{
+    std::ostringstream b, f;
+    b << "template <typename R>\n"
+         "__device__ __forceinline__ void plan_backward_members(\n"
+         "    int gi, bool wide, R costijk, R pfc, R r2sum, R pr1, R pr3,\n"
+         "    R rij, R rik, R rjk, R invj, R invjk, R const* w, R& T1,\n"
+         "    R& T2, R& T3)\n{\n"
+         "    (void)wide; (void)rij; (void)rik; (void)rjk; (void)invj;\n"
+         "    (void)invjk; (void)r2sum;\n"
+         "    switch (gi)\n    {\n";
+    f << "template <typename R, int MAXM>\n"
+         "__device__ __forceinline__ void plan_forward_members(\n"
+         "    int gi, bool wide, R costijk, R pfc, R r2sum, R rij, R rik,\n"
+         "    R rjk, R (&acc)[MAXM])\n{\n"
+         "    (void)wide; (void)rij; (void)rik; (void)rjk; (void)r2sum;\n"
+         "    switch (gi)\n    {\n";
+    for (size_t gi = 0; gi < F.group.size(); ++gi)
+    {
+        GroupDesc const& g = F.group[gi];
+        if (g.type == 2) continue;
+        emit_backward_case(b, (int)gi, g, F);
+        emit_forward_case(f, (int)gi, g, F);
+    }
+    b << "    default: break;\n    }\n}\n\n";
+    f << "    default: break;\n    }\n}\n\n";
+    return b.str() + f.str();
+}
+
+bool plan_enabled_by_default()
+{
+    char const* v = std::getenv("N2P2_AMD_RTC");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T,
+                         int fp32, int device)
+{
+    std::string const src = full_source(plan_source(F), T, fp32);
+    std::string const key = std::to_string(device) + ":"
+                          + std::to_string(std::hash<std::string>()(src))
+                          + ":" + std::to_string(src.size());
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second ? it->second->k : PlanKernels();
+    std::shared_ptr<Module> m;
+    try
+    {
+        m = compile(src, device);
+    }
+    catch (std::exception const& e)
+    {
+        if (!g_warned)
+            std::fprintf(stderr, "[n2p2_amd] model-specialised kernels "
+                                 "unavailable, using the generic kernels: "
+                                 "%s\n", e.what());
+        g_warned = true;
+    }
+    g_cache[key] = m;
+    return m ? m->k : PlanKernels();
+}
+
+}  // namespace hdnnp

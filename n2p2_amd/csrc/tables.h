@@ -12,7 +12,14 @@
 #ifndef HDNNP_TABLES_H
 #define HDNNP_TABLES_H
 
+#ifdef __HIPCC_RTC__
+// Runtime-compiled (hipRTC) builds have no C headers.
+typedef unsigned int uint32_t;
+typedef unsigned long long uint64_t;
+typedef long long int64_t;
+#else
 #include <stdint.h>
+#endif
 
 namespace hdnnp {
 
@@ -48,7 +55,27 @@ struct GroupDesc
     int    nmem;      // members in this chunk (<= kMaxMembers)
     int    mem0;      // first member in the member table
     int    cfg;       // index into the element's CutCfg range (local)
+    int    run0;      // angular: first lambda run in the run table
+    int    nrun;      // angular: lambda runs of this chunk
     double rc, rc2;
+};
+
+// Highest level l of the shared power chain (zeta = 2^l, i.e. zeta <= 64).
+constexpr int kMaxLevel = 6;
+
+// An (eta, rs, lambda) run of an angular group chunk.  Its members are
+// stored chain members first, by level (zeta = 2^level, level <= kMaxLevel),
+// then the members with any other zeta ("generic", pow_int or pow).
+// Member indices are chunk-local.
+struct LamRun
+{
+    double lambda, eta, rs;
+    int    newexp;                 // first run of an (eta, rs) run
+    int    m0;                     // first member
+    int    lmax;                   // highest level with members (-1: none)
+    int    mend;                   // one past the last member
+    int    lvEnd[kMaxLevel + 1];   // end of the members of each level
+    int    pad;
 };
 
 struct MemberDesc
@@ -106,6 +133,7 @@ struct ModelTables
     const CutCfg*     cfg;
     const GroupDesc*  group;
     const MemberDesc* member;
+    const LamRun*     run;
     const ScaleDesc*  scale;
     // angular group chunks per (centre c, e_j, e_k): block pb = (c*ne+ej)*ne+ek
     const int*        pairGroups;   // [pb * kMaxPairGroups + t], global ids

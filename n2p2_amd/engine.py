@@ -57,6 +57,9 @@ _SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("hdnnp_ctx_destroy", None, [ctypes.c_void_p]),
     ("hdnnp_ctx_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_ctx_set_specialized", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_ctx_specialized_active", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     ("hdnnp_ctx_update_model", ctypes.c_int, [ctypes.c_void_p]),
     ("hdnnp_set_structure", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_int32_p, _c_double_p]),
@@ -209,6 +212,17 @@ class Context:
 
     def set_precision(self, p):
         _check(_lib.hdnnp_ctx_set_precision(self._h, p))
+
+    def set_specialized(self, on):
+        """Model-specialised kernels (default) or the generic ones."""
+        _check(_lib.hdnnp_ctx_set_specialized(self._h, 1 if on else 0))
+
+    def specialized_active(self):
+        """True when the specialised kernels of the current precision are
+        enabled and compiled (compiles them if needed)."""
+        v = ctypes.c_int(0)
+        _check(_lib.hdnnp_ctx_specialized_active(self._h, ctypes.byref(v)))
+        return bool(v.value)
 
     def update_model(self):
         _check(_lib.hdnnp_ctx_update_model(self._h))
