@@ -100,27 +100,29 @@ def run_path(ctx, precision, steps, warmup):
     untimed ones; returns the step time, energy, forces and per-kernel
     device times (HIP events on the engine's stream)."""
     ctx.set_precision(precision)
+    spec = ctx.specialized_active()
     for _ in range(warmup):
         ctx.evaluate(engine.FORCES_REFERENCE)
     # evaluate() returns after its stream has drained (the energy and the
     # forces are complete), so each side of the timed region is synchronous.
+    # The per-kernel device times are HIP events recorded on the engine's
+    # stream around each kernel class during the same timed steps.
+    ctx.profile(True)
+    ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
         energy = ctx.evaluate(engine.FORCES_REFERENCE)
     dt = (time.perf_counter() - t0) / steps
-    forces = ctx.forces()
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(max(3, min(steps, 10))):
-        ctx.evaluate(engine.FORCES_REFERENCE)
     kt = {k: ctx.profile_avg_ms(k)[0]
           for k in ("neighbor", "symfunc", "network", "force", "gather")}
     ctx.profile(False)
-    return dt, energy, forces, kt
+    forces = ctx.forces()
+    return dt, energy, forces, kt, spec
 
 
-def roofline(kt, counts, nnf, n, fp32):
-    """Roofline object for the dominant kernel of one path."""
+def roofline(kt, counts, nnf, n, fp32, spec):
+    """Roofline object for the dominant kernel of one path; spec: the
+    model-specialised SF and force kernels ran (their names differ)."""
     fwd = int(costmodel.forward_flops(counts))
     bwd = int(costmodel.backward_flops(counts))
     peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
@@ -134,12 +136,17 @@ def roofline(kt, counts, nnf, n, fp32):
                 "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                 "avg_launch_ms": ms}
-    kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
-             "network": "k_nn"}[dom]
+    if spec:
+        sfx = "_f32" if fp32 else "_f64"
+        kname = {"force": "hdnnp_plan_force" + sfx,
+                 "symfunc": "hdnnp_plan_sf" + sfx, "network": "k_nn_mfma"}[dom]
+    else:
+        kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
+                 "network": "k_nn_mfma"}[dom]
     achieved = flops / (ms * 1e-3) / 1e12
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
-    key = kname + ("_fp32" if fp32 else "")
+    key = kname if spec else kname + ("_fp32" if fp32 else "")
     if os.path.exists(tp) and n == 196608:
         with open(tp) as f:
             tk = json.load(f).get("kernels", {}).get(key)
@@ -189,21 +196,21 @@ def main():
                       for e in range(model.num_elements)))
 
         if args.path != "mixed":
-            dt64, e64, f64, kt64 = run_path(ctx, engine.PRECISION_FP64,
-                                            args.steps, args.warmup)
+            dt64, e64, f64, kt64, sp64 = run_path(
+                ctx, engine.PRECISION_FP64, args.steps, args.warmup)
             log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
                 % (dt64 * 1e3, e64, kt64))
         if args.path != "fp64":
-            dtm, em, fm, ktm = run_path(ctx, engine.PRECISION_MIXED,
-                                        args.steps, args.warmup)
+            dtm, em, fm, ktm, spm = run_path(
+                ctx, engine.PRECISION_MIXED, args.steps, args.warmup)
             log("mixed: %.3f ms/step, E=%.10f, kernel ms %s"
                 % (dtm * 1e3, em, ktm))
         if args.path != "both":
             log("single-path run: no JSON line")
             ctx.close()
             return
-        dt16, e16, f16, kt16 = run_path(ctx, engine.PRECISION_MIXED16,
-                                        args.steps, args.warmup)
+        dt16, e16, f16, kt16, _ = run_path(ctx, engine.PRECISION_MIXED16,
+                                           args.steps, args.warmup)
         log("mixed with FP16 networks: %.3f ms/step, E=%.10f, kernel ms %s"
             % (dt16 * 1e3, e16, kt16))
         df = fm - f64
@@ -255,7 +262,7 @@ def main():
                                       "ms_per_step": dt64 * 1e3,
                                       "kernel_ms": kt64,
                                       "roofline": roofline(kt64, counts, nnf,
-                                                           n, False)},
+                                                           n, False, sp64)},
                        "mixed16": {"atom_steps_per_s": n / dt16,
                                    "ms_per_step": dt16 * 1e3,
                                    "kernel_ms": kt16,
@@ -268,7 +275,7 @@ def main():
                        "flops_per_atom_step": (fwd + bwd + nnf) / n,
                        "work_counts": counts,
                        "kernel_ms": ktm},
-            "roofline": roofline(ktm, counts, nnf, n, True),
+            "roofline": roofline(ktm, counts, nnf, n, True, spm),
             "cpu_baseline": cpu,
         }
         print(json.dumps(out), flush=True)

@@ -15,6 +15,20 @@ namespace hdnnp {
 constexpr int kWave = 64;
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
+// Entries of the LDS queue of passing neighbour pairs (two batches of 64).
+constexpr int kQueue = 128;
+
+// q = p / d, r = p % d for 0 <= p < 2^22 and d > 0 from inv = 1 / (float)d:
+// the float quotient is off by at most one, which the correction fixes.
+__device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
+                                             int& r)
+{
+    q = (int)((float)p * inv);
+    r = p - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+}
+
 // Four- and two-component vectors of the arithmetic type R (LDS records).
 template <typename R> struct VecT;
 template <> struct VecT<float> { typedef float4 V4; typedef float2 V2; };

@@ -4,53 +4,55 @@
 // plan_backward_members provided before this header).
 //
 // Force kernels: contraction dE/dG . dG/dr with dG/dr recomputed, then a
-// deterministic per-atom gather (the gather lives in kernels_force.hip).  The contraction runs in FP64 (plain path)
-// or in FP32 (mixed path: geometry, member sums, row partials and the
-// per-slot sums of a handful of row partials); the centre sums, cslot,
-// cself and the gather are FP64 on both.
+// deterministic per-atom gather (the gather lives in kernels_force.hip).
+// The contraction runs in FP64 (plain path) or in FP32 (mixed path:
+// geometry, member sums and the per-triplet contributions); the per-slot
+// accumulators, cslot, cself and the gather are FP64 on both.
 //
 // The reference stores dG/dr per SF and per neighbour slot
 // (SymGrp*::calculate with derivatives, Atom::allocate) and contracts them
 // in Mode::calculateForces (Mode.cpp:1849-1913) with
 // Atom::calculateSelfForceShort / calculatePairForceShort (Atom.cpp:371-401).
 // Here nothing per SF is stored: for atom i the kernel produces
-//   cself[i]     = sum_s dE_i/dG_s * dG_s/dr_i                 (3 doubles)
 //   cslot[i, q]  = sum_s dE_i/dG_s * dG_s/dr_(neighbour slot q) (3 doubles)
-// and the gather kernel forms F_m = -(cself[m] + sum over m's slots q
-// pointing at another atom j of cslot[j, partner(q)]), partner(q) being the
-// slot of j that points back at m with the opposite image.
+//   cself[i]     = sum_s dE_i/dG_s * dG_s/dr_i = -sum_q cslot[i, q]
+// (every G_s depends on relative positions only, so its gradients over the
+// centre and all neighbour slots sum to zero), and the gather kernel forms
+// F_m = -(cself[m] + sum over m's slots q pointing at another atom j of
+// cslot[j, partner(q)]), partner(q) being the slot of j that points back at
+// m with the opposite image.
 //
 // Backward kernel, one wavefront per atom:
 //  * neighbours staged in LDS partitioned by element (as the SF kernel), as
-//    packed records (x, y, z, r) and (f_c(r), 1/r);
+//    packed records (x, y, z, r) and (f_c(r), 1/r, f_c'(r)/r) for the
+//    current cutoff;
 //  * groups run one after another; a group's member parameters, with this
 //    atom's dE/dG weights, are held in scalar registers for the pair loop;
+//  * angular groups enumerate the unordered pairs {j, k} of their blocks in
+//    diagonal order (below), test rij, rik < rc and rjk < rc (narrow) and
+//    push passing pairs into an LDS queue with a ballot prefix; whenever 64
+//    are queued every lane evaluates one triplet, so the work is divergence
+//    free and each triplet is evaluated once for both of its slots;
 //  * the member sum is contracted before the vector work.  With
 //    fg = exp(.) (1 + lambda cos)^(zeta-1) (SymGrpExpAngn.cpp:276-297) the
-//    reference's p1 and p3 (:300-337) are
-//      p1 = fg [pfc zeta lambda A + plambda c1],  A = 1/(rij rik) - cos/rij^2
-//      p3 = fg [pfc zeta lambda / (rij rik) + plambda c3]
-//    with c1 = pr1 - 2 eta pfc (rij - rs)/rij and c3 = 2 eta pfc (rjk -
-//    rs)/rjk - pr3 (rs > 0), c1 = pr1 - 2 eta pfc and c3 = 2 eta pfc - pr3
-//    (rs = 0), c3 = 0 for the wide type (SymGrpExpAngw.cpp:300-310); so
-//      P1 = sum_m w_m p1_m = pfc A T1 + T2,  P3 = pfc/(rij rik) T1 + T3,
-//      T1 = sum w fg zeta lambda, T2 = sum w fg plambda c1,
-//      T3 = sum w fg plambda c3,  w_m = dE/dG_m * factorNorm_m.
-//    c1, c3 and exp(.) are shared by an (eta, rs) run of members, so T2 and
-//    T3 are formed once per run from U = sum w pw plambda; an (eta, rs,
-//    lambda) run sorted by zeta shares one chain of powers (PowChain);
-//  * every ordered pair (j, k) of a group is evaluated by the lane that
-//    owns slot j and contributes -(P1 dr_ij + P3 dr_jk) to slot j and
-//    P1 dr_ij to the centre; slot k's part comes from the pair (k, j).  A
-//    lane sums P1 and P3 dr_jk over its run of pairs of one row j and
-//    applies dr_ij once per run.  The group's ordered pairs are split into
-//    64 equal contiguous chunks, one per lane; a lane first marks its
-//    chunk's passing pairs in an LDS bit mask, then walks the set bits with
-//    an incremental (row, column) cursor.  It adds the sum of every slot
-//    whose pairs start inside its chunk straight into the LDS accumulator
-//    and leaves the sum of the slot it inherited from the previous chunk in
-//    a per-lane head buffer, merged afterwards in lane order.  No atomics;
-//    fixed summation order.
+//    reference's p1 (slot j), its mirror for slot k and p3 (:300-337) are
+//      P1j = pfc Aj T1 + pr1j V - pfc Ej,  Aj = 1/(rij rik) - cos/rij^2
+//      P1k = pfc Ak T1 + pr1k V - pfc Ek,  Ak = 1/(rij rik) - cos/rik^2
+//      P3  = pfc/(rij rik) T1 + pfc E3 - pr3 V  (last two terms narrow only)
+//    with T1 = sum w fg zeta lambda, V = sum w fg plambda,
+//    Ej = sum w fg plambda 2 eta (rij - rs)/rij, Ek and E3 alike with rik
+//    and rjk (rs = 0: all 2 eta), w_m = dE/dG_m * factorNorm_m,
+//    pr1j = f'c(rij)/rij fc(rik) fc(rjk) and pr3 = fc(rij) fc(rik)
+//    f'c(rjk)/rjk (SymGrpExpAngw.cpp:300-310 for the wide type).  exp(.)
+//    is shared by an (eta, rs) run of members and an (eta, rs, lambda) run
+//    sorted by zeta shares one chain of powers (PowChain);
+//  * slot j receives -(P1j dr_ij + P3 dr_jk), slot k -(P1k dr_ik - P3
+//    dr_jk), dr_jk = dr_ik - dr_ij, through LDS atomic adds.  The
+//    accumulators belong to this one wave, so the order of the adds is the
+//    program order of the wave (lanes of one instruction in hardware
+//    order): the result does not depend on timing and repeats bit for bit.
+//    The diagonal pair order keeps the slots of a batch of 64 mostly
+//    distinct, so the adds rarely collide.
 #ifndef HDNNP_FORCE_CORE_H
 #define HDNNP_FORCE_CORE_H
 
@@ -61,25 +63,15 @@ namespace hdnnp {
 
 namespace {
 
-// Pass-bit words per lane: a group has fewer than cap^2 ordered pairs, so a
-// lane's chunk holds at most ceil(cap^2 / 64) of them.
-__host__ __device__ inline int mask_words(int cap)
-{
-    int const span = (cap * cap + 63) / 64;
-    return (span + 31) / 32;
-}
-
 // LDS of one wave.
 template <typename R>
 struct BLds
 {
     Vec4<R>* P;      // (x, y, z, r) per staged neighbour
-    Vec2<R>* Q;      // (f_c(r) for the current cutoff, 1/r)
-    R* ACC;          // 3 per staged neighbour
-    R* HEAD;         // 3 per lane
+    Vec4<R>* Q;      // (f_c(r), 1/r, f_c'(r)/r, 0) for the current cutoff
+    double* ACC;     // 3 per staged neighbour (FP64 on both paths)
     int* SLOT;       // original slot of each staged neighbour
-    int* HEADJ;      // per lane
-    uint32_t* MASK;  // [word][64]
+    int* QUEUE;      // kQueue passing pairs, j | k << 16
 };
 
 template <typename R>
@@ -143,11 +135,16 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
                     p.z = (R)g.z;
                     p.w = (R)g.w;
                     S.P[pos] = p;
-                    S.Q[pos].y = R(1) / (R)g.w;
+                    Vec4<R> qq;
+                    qq.x = R(0);
+                    qq.y = R(1) / (R)g.w;
+                    qq.z = R(0);
+                    qq.w = R(0);
+                    S.Q[pos] = qq;
                     S.SLOT[pos] = s;
-                    S.ACC[3 * pos] = R(0);
-                    S.ACC[3 * pos + 1] = R(0);
-                    S.ACC[3 * pos + 2] = R(0);
+                    S.ACC[3 * pos] = 0.0;
+                    S.ACC[3 * pos + 1] = 0.0;
+                    S.ACC[3 * pos + 2] = 0.0;
                 }
                 start[q] += __popcll(mask);
             }
@@ -156,100 +153,82 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
     __syncthreads();
 }
 
-// Ordered pairs (j, k) of a group, q in [0, total): for e1 != e2 first
-// j in A x k in B, then j in B x k in A; for e1 == e2 j in A, k in A \ {j}.
-struct OrderedPairs
+// Unordered pairs {j, k} of a group's neighbour blocks in diagonal order,
+// p in [0, npairs).  Same element (block of n): p = c n + r gives the pair
+// {r, (r + 1 + c) mod n} at circular distance 1 + c.  The (n - 1) / 2 full
+// diagonals (distance below n/2) hold n pairs each; for n even the half
+// diagonal at distance n/2 follows with r < n/2.  Every unordered pair
+// appears once (n (n - 1) / 2 in all) and one diagonal holds each slot at
+// most twice.  Different elements, smaller block s (ns) and larger block l
+// (nl): p = c ns + r gives {s + r, l + (r + c) mod nl}, c < nl; one
+// diagonal holds each slot at most once.
+struct DiagPairs
 {
     bool same;
-    int a0, na, b0, nb, total, split;
-    float invA, invB;
-    __device__ OrderedPairs(bool s, int a0_, int na_, int b0_, int nb_)
-        : same(s), a0(a0_), na(na_), b0(b0_), nb(nb_)
+    int s0, ns, l0, nl, d, npairs;
+    float inv;
+    __device__ DiagPairs(bool same_, int a0, int na, int b0, int nb)
+        : same(same_)
     {
         if (same)
         {
-            total = na * (na - 1);
-            split = total;
-            invA = na > 1 ? 1.0f / (float)(na - 1) : 0.0f;
-            invB = 0.0f;
+            s0 = l0 = a0;
+            ns = nl = na;
+            d = na;
+            npairs = na * (na - 1) / 2;
         }
         else
         {
-            split = na * nb;
-            total = 2 * split;
-            invA = nb > 0 ? 1.0f / (float)nb : 0.0f;   // rows of A
-            invB = na > 0 ? 1.0f / (float)na : 0.0f;   // rows of B
+            bool const aSmall = na <= nb;
+            s0 = aSmall ? a0 : b0;
+            ns = aSmall ? na : nb;
+            l0 = aSmall ? b0 : a0;
+            nl = aSmall ? nb : na;
+            d = ns;
+            npairs = na * nb;
         }
+        inv = d > 0 ? 1.0f / (float)d : 0.0f;
     }
-    // j (staged index of the row), c = position of q in row j, len = the
-    // row's length, half = 1 for the B x A part.
-    __device__ void decode(int q, int& j, int& c, int& len, int& half) const
+    __device__ bool decode(int p, int& j, int& k) const
     {
-        int r;
+        if (p >= npairs) return false;
+        int c, r;
+        divmod_small(p, d, inv, c, r);
         if (same)
         {
-            int const d = na - 1;
-            r = (int)((float)q * invA);
-            c = q - r * d;
-            if (c < 0) { --r; c += d; } else if (c >= d) { ++r; c -= d; }
-            j = a0 + r;
-            len = d;
-            half = 0;
-        }
-        else if (q < split)
-        {
-            r = (int)((float)q * invA);
-            c = q - r * nb;
-            if (c < 0) { --r; c += nb; } else if (c >= nb) { ++r; c -= nb; }
-            j = a0 + r;
-            len = nb;
-            half = 0;
+            int v = r + 1 + c;
+            if (v >= ns) v -= ns;
+            j = s0 + r;
+            k = s0 + v;
         }
         else
         {
-            int const qq = q - split;
-            r = (int)((float)qq * invB);
-            c = qq - r * na;
-            if (c < 0) { --r; c += na; } else if (c >= na) { ++r; c -= na; }
-            j = b0 + r;
-            len = na;
-            half = 1;
+            int v = r + c;
+            // c < nl, r < ns <= nl: one subtraction brings v below nl.
+            if (v >= nl) v -= nl;
+            j = s0 + r;
+            k = l0 + v;
         }
-    }
-    // The row after row j.
-    __device__ void next_row(int& j, int& len, int& half) const
-    {
-        if (!same && half == 0 && j == a0 + na - 1)
-        {
-            half = 1;
-            j = b0;
-            len = na;
-        }
-        else ++j;
-    }
-    // Sequential walk: (j, c, len, half) of pair q -> pair q + 1.
-    __device__ void next(int& j, int& c, int& len, int& half) const
-    {
-        if (++c < len) return;
-        c = 0;
-        next_row(j, len, half);
-    }
-    // Advances the cursor by d >= 0 pairs.
-    __device__ void advance(int d, int& j, int& c, int& len, int& half) const
-    {
-        c += d;
-        while (c >= len)
-        {
-            c -= len;
-            next_row(j, len, half);
-        }
-    }
-    __device__ int partner(int j, int c, int half) const
-    {
-        if (same) return a0 + (c < j - a0 ? c : c + 1);
-        return half == 0 ? b0 + c : a0 + c;
+        return true;
     }
 };
+
+// rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
+template <typename R>
+__device__ __forceinline__ bool pair_in(BLds<R> const& S, int j, int k,
+                                        bool wide, R rc, R rc2)
+{
+    Vec4<R> const pj = S.P[j], pk = S.P[k];
+    bool pass = pj.w < rc && pk.w < rc;
+    if (!wide)
+    {
+        R const dx = pk.x - pj.x;
+        R const dy = pk.y - pj.y;
+        R const dz = pk.z - pj.z;
+        pass = pass && (dx * dx + dy * dy + dz * dz) < rc2;
+    }
+    return pass;
+}
 
 // A group's member parameters in wave-uniform (scalar) registers: lambda,
 // zeta lambda, the weight and the flags word.  The exponential's parameters
@@ -276,70 +255,32 @@ __device__ __forceinline__ void load_members(MParam<R> const* MP, int nmem,
     }
 }
 
-// A lane's partial sums over its run of passing pairs of one row j:
-// sum P1 and sum P3 dr_jk.
-template <typename R>
-struct RowSum
+// Atomic add to an LDS double of this wave's accumulators.
+__device__ __forceinline__ void lds_add(double* p, double v)
 {
-    R p1, vx, vy, vz;
-    __device__ void clear() { p1 = vx = vy = vz = R(0); }
-};
-
-// Ends a lane's run of row j (dr_ij = (xj, yj, zj)): slot j receives
-// -(sum P1 dr_ij + sum P3 dr_jk) (straight into the accumulator if the row
-// starts in this lane's chunk, else into the lane's head buffer), the
-// centre's FP64 sum P1 dr_ij.
-template <typename R>
-__device__ __forceinline__ void flush(BLds<R> const& S, int j, bool owned,
-                                      RowSum<R> const& rs, R xj, R yj, R zj,
-                                      double& sx, double& sy, double& sz)
-{
-    R const ax = rs.p1 * xj;
-    R const ay = rs.p1 * yj;
-    R const az = rs.p1 * zj;
-    sx += (double)ax;
-    sy += (double)ay;
-    sz += (double)az;
-    R const cx = -(ax + rs.vx);
-    R const cy = -(ay + rs.vy);
-    R const cz = -(az + rs.vz);
-    if (owned)
-    {
-        S.ACC[3 * j] += cx;
-        S.ACC[3 * j + 1] += cy;
-        S.ACC[3 * j + 2] += cz;
-    }
-    else
-    {
-        int const lane = threadIdx.x;
-        S.HEADJ[lane] = j;
-        S.HEAD[3 * lane] = cx;
-        S.HEAD[3 * lane + 1] = cy;
-        S.HEAD[3 * lane + 2] = cz;
-    }
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Quantities of row j held in registers while a lane walks the row.
-template <typename R>
-struct Row
-{
-    R x, y, z, r, inv, fc, dfcinv, r2, inv2;
-};
-
-// P1 and P3 of one triplet (j first neighbour, k second), all members.
+// One triplet {j, k} of angular group gi, all members: adds the slot-j and
+// slot-k contributions to the LDS accumulators.
 template <int CT, bool GENPOW, int MAXM, typename R>
 __device__ __forceinline__ void triplet_backward(
-    int gi, Row<R> const& J, Vec4<R> const& pk, Vec2<R> const& qk, bool wide,
-    CutCfg const& cc, Members<R, MAXM> const& M, MParam<R> const* MP,
-    int nmem, R& P1, R& P3, R& dx, R& dy, R& dz)
+    int gi, BLds<R> const& S, int j, int k, bool wide, CutCfg const& cc,
+    Members<R, MAXM> const& M, MParam<R> const* MP, int nmem)
 {
-    R const rik = pk.w, fcik = qk.x, invk = qk.y;
-    dx = pk.x - J.x;
-    dy = pk.y - J.y;
-    dz = pk.z - J.z;
-    R const rinvijik = J.inv * invk;
-    R const costijk = (J.x * pk.x + J.y * pk.y + J.z * pk.z) * rinvijik;
-    R pfc, r2sum, pr1, pr3 = R(0), rjk = R(0), invjk = R(0);
+    Vec4<R> const pj = S.P[j], pk = S.P[k];
+    Vec4<R> const qj = S.Q[j], qk = S.Q[k];
+    R const rij = pj.w, rik = pk.w;
+    R const fcij = qj.x, invj = qj.y, dfcij = qj.z;
+    R const fcik = qk.x, invk = qk.y, dfcik = qk.z;
+    R const dx = pk.x - pj.x;
+    R const dy = pk.y - pj.y;
+    R const dz = pk.z - pj.z;
+    R const rinvijik = invj * invk;
+    R const costijk = (pj.x * pk.x + pj.y * pk.y + pj.z * pk.z) * rinvijik;
+    R pfc, r2sum, pr1j, pr1k, pr3 = R(0), rjk = R(0), invjk = R(0);
+    R const fij_ik = fcij * fcik;
     if (!wide)
     {
         R const rjk2 = dx * dx + dy * dy + dz * dz;
@@ -347,29 +288,30 @@ __device__ __forceinline__ void triplet_backward(
         rjk = rjk2 * invjk;
         R fcjk, dfcjk;
         cutoff_inside<CT>(cc, rjk, fcjk, dfcjk);
-        R const fij_ik = J.fc * fcik;
         pfc = fij_ik * fcjk;
-        r2sum = J.r2 + rik * rik + rjk2;
-        pr1 = fcik * fcjk * J.dfcinv;
+        r2sum = rij * rij + rik * rik + rjk2;
+        pr1j = fcik * fcjk * dfcij;
+        pr1k = fcij * fcjk * dfcik;
         pr3 = fij_ik * dfcjk * invjk;
     }
     else
     {
-        pfc = J.fc * fcik;
-        r2sum = J.r2 + rik * rik;
-        pr1 = fcik * J.dfcinv;
+        pfc = fij_ik;
+        r2sum = rij * rij + rik * rik;
+        pr1j = fcik * dfcij;
+        pr1k = fcij * dfcik;
     }
-    R T1 = R(0), T2 = R(0), T3 = R(0);
+    R T1 = R(0), V = R(0), Ej = R(0), Ek = R(0), E3 = R(0);
 #ifdef HDNNP_PLAN
     // Member sums of the model-specialised build: straight-line code per
     // group chunk with the member constants folded in.
     (void)MP;
     (void)nmem;
-    plan_backward_members<R>(gi, wide, costijk, pfc, r2sum, pr1, pr3, J.r,
-                             rik, rjk, J.inv, invjk, M.w, T1, T2, T3);
+    plan_backward_members<R>(gi, wide, costijk, r2sum, rij, rik, rjk, invj,
+                             invk, invjk, M.w, T1, V, Ej, Ek, E3);
 #else
     (void)gi;
-    R vexp = R(0), c1 = R(0), c3 = R(0), S1 = R(0), U = R(0);
+    R vexp = R(0), S1 = R(0), U = R(0), tj = R(0), tk = R(0), t3 = R(0);
     R pl = R(0);
     PowChain<R> chain;
     chain.reset(R(0));
@@ -386,31 +328,33 @@ __device__ __forceinline__ void triplet_backward(
                     // Close the previous (eta, rs) run.
                     T1 += vexp * S1;
                     R const vu = vexp * U;
-                    T2 += vu * c1;
-                    T3 += vu * c3;
+                    V += vu;
+                    Ej += vu * tj;
+                    Ek += vu * tk;
+                    E3 += vu * t3;
                     S1 = U = R(0);
                 }
                 R const rs = MP[m].rs, eta = MP[m].eta;
                 R const neta = exp_factor(eta);
-                R const te = R(2) * eta * pfc;
+                R const te = R(2) * eta;
                 if (rs > R(0))
                 {
-                    R const rijs = J.r - rs;
+                    R const rijs = rij - rs;
                     R const riks = rik - rs;
                     R s = rijs * rijs + riks * riks;
-                    c1 = pr1 - te * rijs * J.inv;
+                    tj = te * rijs * invj;
+                    tk = te * riks * invk;
                     if (!wide)
                     {
                         R const rjks = rjk - rs;
                         s += rjks * rjks;
-                        c3 = te * rjks * invjk - pr3;
+                        t3 = te * rjks * invjk;
                     }
                     vexp = exp_scaled(neta * s);
                 }
                 else
                 {
-                    c1 = pr1 - te;
-                    if (!wide) c3 = te - pr3;
+                    tj = tk = t3 = te;
                     vexp = exp_scaled(neta * r2sum);
                 }
             }
@@ -432,13 +376,29 @@ __device__ __forceinline__ void triplet_backward(
     T1 += vexp * S1;
     {
         R const vu = vexp * U;
-        T2 += vu * c1;
-        T3 += vu * c3;
+        V += vu;
+        Ej += vu * tj;
+        Ek += vu * tk;
+        E3 += vu * t3;
     }
 #endif
-    R const A = rinvijik - costijk * J.inv2;
-    P1 = pfc * A * T1 + T2;
-    P3 = pfc * rinvijik * T1 + T3;
+    R const pt = pfc * T1;
+    R const P1j = pt * (rinvijik - costijk * invj * invj) + pr1j * V
+                - pfc * Ej;
+    R const P1k = pt * (rinvijik - costijk * invk * invk) + pr1k * V
+                - pfc * Ek;
+    R const P3 = pt * rinvijik + (wide ? R(0) : pfc * E3 - pr3 * V);
+    // FP64 LDS adds on both paths (the FP32 LDS add is much slower on this
+    // hardware, and the FP64 sums keep the mixed path's slot totals exact
+    // to FP64 rounding).
+    double* const aj = S.ACC + 3 * j;
+    double* const ak = S.ACC + 3 * k;
+    lds_add(aj, (double)(-(P1j * pj.x + P3 * dx)));
+    lds_add(aj + 1, (double)(-(P1j * pj.y + P3 * dy)));
+    lds_add(aj + 2, (double)(-(P1j * pj.z + P3 * dz)));
+    lds_add(ak, (double)(P3 * dx - P1k * pk.x));
+    lds_add(ak + 1, (double)(P3 * dy - P1k * pk.y));
+    lds_add(ak + 2, (double)(P3 * dz - P1k * pk.z));
 }
 
 // Body of the backward kernel (one 64-lane workgroup per atom).
@@ -451,12 +411,10 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     int const cap = a.cap;
     BLds<R> S;
     S.P = (Vec4<R>*)sm;
-    S.Q = (Vec2<R>*)(S.P + cap);
-    S.ACC = (R*)(S.Q + cap);
-    S.HEAD = S.ACC + 3 * cap;
-    S.SLOT = (int*)(S.HEAD + 3 * 64);
-    S.HEADJ = S.SLOT + cap;
-    S.MASK = (uint32_t*)(S.HEADJ + 64);
+    S.Q = S.P + cap;
+    S.ACC = (double*)(S.Q + cap);
+    S.SLOT = (int*)(S.ACC + 3 * cap);
+    S.QUEUE = S.SLOT + cap;
 
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
@@ -469,22 +427,23 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
 
     stage(a, S, s0, n, sStart, sCnt);
 
-    double sx = 0.0, sy = 0.0, sz = 0.0;    // this lane's share of cself
-    double rcCached = -1.0;
+    int cfgCached = -1;
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
         GroupDesc const g = T.group[gi];
-        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
-        if (g.rc != rcCached)
+        int const cfg = ed.cfg0 + g.cfg;
+        CutCfg const cc = T.cfg[cfg];
+        if (cfg != cfgCached)
         {
+            // f_c and f_c'/r of every staged neighbour for this cutoff.
             for (int t = lane; t < n; t += 64)
             {
                 R f, df;
                 cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
                 S.Q[t].x = f;
+                S.Q[t].z = df * S.Q[t].y;
             }
-            rcCached = g.rc;
-            __syncthreads();
+            cfgCached = cfg;
         }
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         int const nmem = g.nmem;
@@ -501,7 +460,8 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
 
         if (g.type == 2)
         {
-            // SymGrpExpRad.cpp:194-233: p1 = sf (f' - 2 eta (r-rs) f) e / r.
+            // SymGrpExpRad.cpp:194-233: p1 = sf (f' - 2 eta (r-rs) f) e / r;
+            // slot j receives -p1 dr_ij.  One lane per slot.
             int const a0 = sStart[g.e1], na = sCnt[g.e1];
             for (int t = lane; t < na; t += 64)
             {
@@ -509,194 +469,87 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
                 Vec4<R> const pj = S.P[j];
                 R const r = pj.w;
                 if (!(r < grc)) continue;
-                R fc, dfc;
-                cutoff_fdf_t<CT>(cc, r, fc, dfc);
-                R const inv = S.Q[j].y;
+                Vec4<R> const qj = S.Q[j];
+                R const fcinv = qj.x * qj.y, dfcinv = qj.z;
                 R P = R(0);
                 for (int m = 0; m < nmem; ++m)
                 {
                     R const eta = MP[m].eta, rs = MP[m].rs;
-                    R const pexp = exp_r(-eta * (r - rs) * (r - rs));
-                    P += MP[m].w * ((dfc - R(2) * eta * (r - rs) * fc) * pexp
-                                    * inv);
+                    R const d = r - rs;
+                    R const pexp = exp_r(-eta * d * d);
+                    P += MP[m].w * ((dfcinv - R(2) * eta * d * fcinv) * pexp);
                 }
-                R const px = P * pj.x;
-                R const py = P * pj.y;
-                R const pz = P * pj.z;
-                sx += (double)px; sy += (double)py; sz += (double)pz;
-                S.ACC[3 * j] -= px;
-                S.ACC[3 * j + 1] -= py;
-                S.ACC[3 * j + 2] -= pz;
+                S.ACC[3 * j] -= (double)(P * pj.x);
+                S.ACC[3 * j + 1] -= (double)(P * pj.y);
+                S.ACC[3 * j + 2] -= (double)(P * pj.z);
             }
             __syncthreads();
             continue;
         }
 
         bool const wide = (g.type == 9);
-        OrderedPairs op(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
-                        sCnt[g.e2]);
-        int const total = op.total;
-        if (total == 0) continue;
+        DiagPairs dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
+                     sCnt[g.e2]);
+        if (dp.npairs == 0)
+        {
+            __syncthreads();
+            continue;
+        }
         Members<R, MAXM> M;
         load_members(MP, nmem, M);
-        // Chunk of this lane: [q0, q1); the mask buffer holds
-        // mask_words(cap) words per lane, enough for the largest chunk.
+        R const grc2 = (R)g.rc2;
+        int qn = 0;
+        for (int base = 0; base < dp.npairs; base += 64)
         {
-            int const span = (total + 63) / 64;
-            int const q0 = min(total, lane * span);
-            int const q1 = min(total, q0 + span);
-            int const len = q1 - q0;
-            int const nw = (span + 31) >> 5;
-            R const grc2 = (R)g.rc2;
-            // Cursor at the chunk's first pair.
-            int cj = 0, cc0 = 0, clen = 1, chalf = 0;
-            if (len > 0) op.decode(q0, cj, cc0, clen, chalf);
-            int const j0 = cj;
-            bool const firstOwned = (cc0 == 0);
-            // Pass bits for the chunk (rij, rik < rc; rjk < rc if narrow),
-            // walking the chunk's pairs in order.
-            {
-                int wj = cj, wc = cc0, wl = clen, wh = chalf;
-                for (int wd = 0; wd < nw; ++wd)
-                {
-                    uint32_t bits = 0;
-                    for (int b = 0; b < 32; ++b)
-                    {
-                        int const t = wd * 32 + b;
-                        if (t < len)
-                        {
-                            int const k = op.partner(wj, wc, wh);
-                            Vec4<R> const pj = S.P[wj];
-                            Vec4<R> const pk = S.P[k];
-                            op.next(wj, wc, wl, wh);
-                            bool pass = pj.w < grc && pk.w < grc;
-                            if (!wide)
-                            {
-                                R const dx = pk.x - pj.x;
-                                R const dy = pk.y - pj.y;
-                                R const dz = pk.z - pj.z;
-                                pass = pass
-                                    && dx * dx + dy * dy + dz * dz < grc2;
-                            }
-                            bits |= (pass ? 1u : 0u) << b;
-                        }
-                    }
-                    S.MASK[wd * 64 + lane] = bits;
-                }
-            }
-            // Row of the chunk's last pair, if it starts in this chunk and
-            // runs past it: this lane merges its continuation.
-            int tailJ = -1, tailEnd = 0;
-            if (len > 0)
-            {
-                int jl, cl, ll, hl;
-                op.decode(q1 - 1, jl, cl, ll, hl);
-                int const rowStart = q1 - 1 - cl;
-                if (rowStart >= q0 && rowStart + ll > q1)
-                {
-                    tailJ = jl;
-                    tailEnd = rowStart + ll;
-                }
-            }
-            S.HEADJ[lane] = -1;
-            RowSum<R> rsum;
-            rsum.clear();
-            bool curHas = false;
-            int curJ = -1;
-            bool curOwned = true;
-            Row<R> J;
-            J.x = J.y = J.z = J.r = J.inv = J.fc = J.dfcinv = J.r2 = J.inv2
-                = R(0);
-            int qc = q0;            // position of the cursor
-            int wd = 0;
-            uint32_t bits = nw > 0 ? S.MASK[lane] : 0u;
-            for (;;)
-            {
-                while (bits == 0u && wd < nw)
-                {
-                    ++wd;
-                    if (wd < nw) bits = S.MASK[wd * 64 + lane];
-                }
-                if (wd >= nw) break;
-                int const b = __builtin_ctz(bits);
-                bits &= bits - 1u;
-                int const q = q0 + wd * 32 + b;
-                op.advance(q - qc, cj, cc0, clen, chalf);
-                qc = q;
-                int const j = cj;
-                int const k = op.partner(cj, cc0, chalf);
-                if (j != curJ)
-                {
-                    if (curHas)
-                        flush(S, curJ, curOwned, rsum, J.x, J.y, J.z, sx, sy,
-                              sz);
-                    curOwned = (j != j0) || firstOwned;
-                    curJ = j;
-                    rsum.clear();
-                    curHas = false;
-                    Vec4<R> const pj = S.P[j];
-                    Vec2<R> const qj = S.Q[j];
-                    J.x = pj.x;
-                    J.y = pj.y;
-                    J.z = pj.z;
-                    J.r = pj.w;
-                    J.fc = qj.x;
-                    J.inv = qj.y;
-                    J.r2 = J.r * J.r;
-                    J.inv2 = J.inv * J.inv;
-                    R fcj, dfcj;
-                    cutoff_fdf_t<CT>(cc, J.r, fcj, dfcj);
-                    J.dfcinv = dfcj * J.inv;
-                }
-                R P1, P3, dx, dy, dz;
-                triplet_backward<CT, GENPOW, MAXM>(gi, J, S.P[k], S.Q[k], wide,
-                                                   cc, M, MP, nmem, P1, P3,
-                                                   dx, dy, dz);
-                rsum.p1 += P1;
-                rsum.vx += P3 * dx;
-                rsum.vy += P3 * dy;
-                rsum.vz += P3 * dz;
-                curHas = true;
-            }
-            if (curHas)
-                flush(S, curJ, curOwned, rsum, J.x, J.y, J.z, sx, sy, sz);
+            int j = 0, k = 0;
+            bool pass = false;
+            if (dp.decode(base + lane, j, k))
+                pass = pair_in(S, j, k, wide, grc, grc2);
+            uint64_t const mask = __ballot(pass);
+            if (pass) S.QUEUE[qn + lanes_below(mask)] = j | (k << 16);
+            qn += __popcll(mask);
             __syncthreads();
-            // The lane owning the start of a row that runs past its chunk
-            // adds the head partials of the following chunks of that row,
-            // in lane order; one writer per row, chunks without passing
-            // pairs (HEADJ = -1) are skipped.
-            if (tailJ >= 0)
+            if (qn >= 64)
             {
-                R hx = R(0), hy = R(0), hz = R(0);
-                for (int L = lane + 1; L < 64 && L * span < tailEnd; ++L)
-                    if (S.HEADJ[L] == tailJ)
-                    {
-                        hx += S.HEAD[3 * L];
-                        hy += S.HEAD[3 * L + 1];
-                        hz += S.HEAD[3 * L + 2];
-                    }
-                S.ACC[3 * tailJ] += hx;
-                S.ACC[3 * tailJ + 1] += hy;
-                S.ACC[3 * tailJ + 2] += hz;
+                int const e = S.QUEUE[qn - 64 + lane];
+                triplet_backward<CT, GENPOW, MAXM>(gi, S, e & 0xffff, e >> 16,
+                                                   wide, cc, M, MP, nmem);
+                qn -= 64;
+                __syncthreads();
             }
-            __syncthreads();
         }
+        if (lane < qn)
+        {
+            int const e = S.QUEUE[lane];
+            triplet_backward<CT, GENPOW, MAXM>(gi, S, e & 0xffff, e >> 16,
+                                               wide, cc, M, MP, nmem);
+        }
+        __syncthreads();
     }
+    // cslot in FP64; cself = -(sum of all slots), summed per lane in slot
+    // order and then by the fixed butterfly.
+    double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int t = lane; t < n; t += 64)
     {
         size_t const q = (size_t)(s0 + S.SLOT[t]);
-        a.cslot[3 * q] = S.ACC[3 * t];
-        a.cslot[3 * q + 1] = S.ACC[3 * t + 1];
-        a.cslot[3 * q + 2] = S.ACC[3 * t + 2];
+        double const cx = S.ACC[3 * t];
+        double const cy = S.ACC[3 * t + 1];
+        double const cz = S.ACC[3 * t + 2];
+        a.cslot[3 * q] = cx;
+        a.cslot[3 * q + 1] = cy;
+        a.cslot[3 * q + 2] = cz;
+        sx += cx;
+        sy += cy;
+        sz += cz;
     }
     sx = wave_sum(sx);
     sy = wave_sum(sy);
     sz = wave_sum(sz);
     if (lane == 0)
     {
-        a.cself[3 * (size_t)i] = sx;
-        a.cself[3 * (size_t)i + 1] = sy;
-        a.cself[3 * (size_t)i + 2] = sz;
+        a.cself[3 * (size_t)i] = -sx;
+        a.cself[3 * (size_t)i + 1] = -sy;
+        a.cself[3 * (size_t)i + 2] = -sz;
     }
 }
 

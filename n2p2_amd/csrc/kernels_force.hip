@@ -80,12 +80,10 @@ size_t force_backward_lds_bytes(int cap, int fp32)
 {
     size_t const r = fp32 ? sizeof(float) : sizeof(double);
     size_t b = 0;
-    b += (size_t)cap * 6 * r;                     // P (4) and Q (2)
-    b += (size_t)cap * 3 * r;                     // ACC
-    b += (size_t)64 * 3 * r;                      // HEAD
+    b += (size_t)cap * 8 * r;                     // P (4) and Q (4)
+    b += (size_t)cap * 3 * sizeof(double);        // ACC
     b += (size_t)cap * sizeof(int);               // SLOT
-    b += (size_t)64 * sizeof(int);                // HEADJ
-    b += (size_t)mask_words(cap) * 64 * sizeof(uint32_t);   // MASK
+    b += (size_t)kQueue * sizeof(int);            // QUEUE
     return b;
 }
 

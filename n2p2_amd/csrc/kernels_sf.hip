@@ -30,7 +30,6 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
 {
     extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
-    int const cap = a.cap;
     Stage<double> S;
     S.P = (double4*)sm;
     S.Q = nullptr;

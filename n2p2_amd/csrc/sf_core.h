@@ -35,8 +35,6 @@ namespace hdnnp {
 
 namespace {
 
-constexpr int kQueue = 128;
-
 // Staged neighbours: (x, y, z, r) and (f_c(r) for the current cutoff, 1/r).
 template <typename R>
 struct Stage
@@ -44,15 +42,6 @@ struct Stage
     Vec4<R>* P;
     Vec2<R>* Q;      // null when only geometry is needed
 };
-
-__device__ __forceinline__ void divmod_small(int p, int d, float inv, int& q,
-                                             int& r)
-{
-    q = (int)((float)p * inv);
-    r = p - q * d;
-    if (r < 0) { --q; r += d; }
-    else if (r >= d) { ++q; r -= d; }
-}
 
 // Stages the atom's neighbours in LDS partitioned by element (stable ballot
 // compaction); sStart / sCnt receive each element's block.

@@ -70,34 +70,39 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
 {
     bool const wide = (g.type == 9);
     o << "    case " << gi << ":\n    {\n"
-      << "        R S1 = R(0), U = R(0), vexp = R(0), c1 = R(0), c3 = R(0);\n";
-    bool first = true;
+      << "        R S1 = R(0), U = R(0), vexp = R(0);\n"
+      << "        R E0 = R(0), Ejx = R(0), Ekx = R(0), E3x = R(0);\n";
+    bool anyZero = false, anyShift = false;
+    double eta = 0.0, rs = 0.0;
+    auto close_run = [&]() {
+        o << "        T1 += vexp * S1;\n"
+          << "        { R const vu = vexp * U; V += vu;";
+        if (rs > 0.0)
+        {
+            o << " R const te = " << lit(2.0 * eta) << " * vu;"
+              << " Ejx += te * a_ * invj; Ekx += te * b_ * invk;";
+            if (!wide) o << " E3x += te * c_ * invjk;";
+        }
+        else o << " E0 += " << lit(2.0 * eta) << " * vu;";
+        o << " }\n        S1 = U = R(0);\n        }\n";
+    };
+    bool open = false;
     for (int ri = 0; ri < g.nrun; ++ri)
     {
         LamRun const& r = F.run[g.run0 + ri];
         if (r.newexp)
         {
-            if (!first)
-                o << "        T1 += vexp * S1;\n"
-                  << "        { R const vu = vexp * U; T2 += vu * c1;"
-                     " T3 += vu * c3; }\n"
-                  << "        S1 = U = R(0);\n";
-            first = false;
+            if (open) close_run();
+            open = true;
+            eta = r.eta;
+            rs = r.rs;
+            (rs > 0.0 ? anyShift : anyZero) = true;
+            // The run's block stays open until it is closed, so that the
+            // shifted distances a_, b_, c_ remain in scope.
             o << "        {\n";
             emit_exp_sum(o, r, wide);
-            o << "        R const te = " << lit(2.0 * r.eta) << " * pfc;\n";
-            if (r.rs > 0.0)
-            {
-                o << "        c1 = pr1 - te * a_ * invj;\n";
-                if (!wide) o << "        c3 = te * c_ * invjk - pr3;\n";
-            }
-            else
-            {
-                o << "        c1 = pr1 - te;\n";
-                if (!wide) o << "        c3 = te - pr3;\n";
-            }
             o << "        vexp = exp_scaled(exp_factor(" << lit(r.eta)
-              << ") * s_);\n        }\n";
+              << ") * s_);\n";
         }
         o << "        {\n        R const pl = " << plambda(r.lambda) << ";\n"
           << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n";
@@ -123,8 +128,14 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
         }
         o << "        }\n";
     }
-    o << "        T1 += vexp * S1;\n"
-      << "        { R const vu = vexp * U; T2 += vu * c1; T3 += vu * c3; }\n"
+    if (open) close_run();
+    if (anyZero && !anyShift)
+        o << "        Ej = E0; Ek = E0; E3 = E0;\n";
+    else if (!anyZero)
+        o << "        Ej = Ejx; Ek = Ekx; E3 = E3x;\n";
+    else
+        o << "        Ej = E0 + Ejx; Ek = E0 + Ekx; E3 = E0 + E3x;\n";
+    o << "        (void)E0; (void)Ejx; (void)Ekx; (void)E3x;\n"
       << "        break;\n    }\n";
 }
 
@@ -193,11 +204,13 @@ std::string full_source(std::string const& plan, ModelTables const& T,
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
       << "__attribute__((amdgpu_waves_per_eu("
       << (fp32 && maxm <= 8 ? 4 : 1) << ", 8)))\n"
-      << "void hdnnp_plan_force(hdnnp::ForceArgs a)\n"
+      << "void hdnnp_plan_force_" << (fp32 ? "f32" : "f64")
+      << "(hdnnp::ForceArgs a)\n"
       << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
       << R << ">(a); }\n"
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
-      << "void hdnnp_plan_sf(hdnnp::SFArgs a)\n"
+      << "void hdnnp_plan_sf_" << (fp32 ? "f32" : "f64")
+      << "(hdnnp::SFArgs a)\n"
       << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
       << R << ">(a); }\n";
     return o.str();
@@ -219,8 +232,9 @@ bool verbose()
     return v && *v && *v != '0';
 }
 
-std::shared_ptr<Module> compile(std::string const& src, int device)
+std::shared_ptr<Module> compile(std::string const& src, int fp32, int device)
 {
+    std::string const sfx = fp32 ? "_f32" : "_f64";
     auto t0 = std::chrono::steady_clock::now();
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "hdnnp_plan.hip", kRtcNumHeaders,
@@ -251,10 +265,10 @@ std::shared_ptr<Module> compile(std::string const& src, int device)
     auto m = std::make_shared<Module>();
     if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess)
         throw std::runtime_error("hipModuleLoadData failed");
-    if (hipModuleGetFunction(&m->k.force, m->mod, "hdnnp_plan_force")
-            != hipSuccess
-        || hipModuleGetFunction(&m->k.sf, m->mod, "hdnnp_plan_sf")
-            != hipSuccess)
+    if (hipModuleGetFunction(&m->k.force, m->mod,
+                             ("hdnnp_plan_force" + sfx).c_str()) != hipSuccess
+        || hipModuleGetFunction(&m->k.sf, m->mod,
+                                ("hdnnp_plan_sf" + sfx).c_str()) != hipSuccess)
         throw std::runtime_error("hipModuleGetFunction failed");
     if (verbose())
     {
@@ -273,11 +287,11 @@ std::string plan_source(Model::Flat const& F)
     std::ostringstream b, f;
     b << "template <typename R>\n"
          "__device__ __forceinline__ void plan_backward_members(\n"
-         "    int gi, bool wide, R costijk, R pfc, R r2sum, R pr1, R pr3,\n"
-         "    R rij, R rik, R rjk, R invj, R invjk, R const* w, R& T1,\n"
-         "    R& T2, R& T3)\n{\n"
+         "    int gi, bool wide, R costijk, R r2sum, R rij, R rik, R rjk,\n"
+         "    R invj, R invk, R invjk, R const* w, R& T1, R& V, R& Ej,\n"
+         "    R& Ek, R& E3)\n{\n"
          "    (void)wide; (void)rij; (void)rik; (void)rjk; (void)invj;\n"
-         "    (void)invjk; (void)r2sum;\n"
+         "    (void)invk; (void)invjk; (void)r2sum;\n"
          "    switch (gi)\n    {\n";
     f << "template <typename R, int MAXM>\n"
          "__device__ __forceinline__ void plan_forward_members(\n"
@@ -307,16 +321,15 @@ PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T,
                          int fp32, int device)
 {
     std::string const src = full_source(plan_source(F), T, fp32);
-    std::string const key = std::to_string(device) + ":"
-                          + std::to_string(std::hash<std::string>()(src))
-                          + ":" + std::to_string(src.size());
+    // Keyed by the full source (a few tens of KB): no collisions.
+    std::string const key = std::to_string(device) + ":" + src;
     std::lock_guard<std::mutex> lock(g_mutex);
     auto it = g_cache.find(key);
     if (it != g_cache.end()) return it->second ? it->second->k : PlanKernels();
     std::shared_ptr<Module> m;
     try
     {
-        m = compile(src, device);
+        m = compile(src, fp32, device);
     }
     catch (std::exception const& e)
     {
