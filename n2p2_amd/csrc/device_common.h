@@ -543,6 +543,113 @@ __device__ __forceinline__ int wave_sum_int(int v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// Sums M per-lane values over the wave by a transposing butterfly: at each
+// of the first log2(M) steps a lane keeps half of its values, sends the
+// other half to the partner lane (xor 32, 16, ...) and adds what it
+// receives, then the remaining steps are a plain butterfly.  Returns the
+// total of member ((lane >> (6 - log2 M)) & (M - 1)); every lane of that
+// group of 64/M lanes holds it.  M - 1 + 6 - log2(M) exchanges and adds
+// instead of 6 M for M separate sums; the order of the additions is fixed.
+template <int M>
+__device__ __forceinline__ double wave_sum_transposed(double (&v)[M])
+{
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16
+                  || M == 32, "power of two up to 32");
+    int const lane = lane_id();
+#pragma unroll
+    for (int h = M / 2, o = 32; h >= 1; h >>= 1, o >>= 1)
+    {
+        bool const upper = (lane & o) != 0;
+#pragma unroll
+        for (int m = 0; m < h; ++m)
+        {
+            double const send = upper ? v[m] : v[m + h];
+            double const keep = upper ? v[m + h] : v[m];
+            v[m] = keep + __shfl_xor(send, o, kWave);
+        }
+    }
+    double t = v[0];
+#pragma unroll
+    for (int o = 32 / M; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+    return t;
+}
+
+// Unordered pairs {j, k} of a group's neighbour blocks in diagonal order.
+// Same element (block of n at s0): pair p = c n + r is {r, (r + 1 + c) mod
+// n}, at circular distance 1 + c.  The (n - 1) / 2 full diagonals
+// (distance below n/2) hold n pairs each; for n even the half diagonal at
+// distance n/2 follows with r < n/2.  Every unordered pair appears once
+// (n (n - 1) / 2 in all) and one diagonal holds each slot at most twice.
+// Different elements, smaller block s0 (ns) and larger block l0 (nl):
+// p = c ns + r is {s0 + r, l0 + (r + c) mod nl}, c < nl; one diagonal holds
+// each slot at most once.  A lane walks p = lane, lane + 64, ... with an
+// incremental (c, r) (no division per step).
+struct DiagPairs
+{
+    bool same;
+    int s0, ns, l0, nl, d, npairs;
+    int q64, r64;     // 64 = q64 d + r64
+    __device__ DiagPairs(bool same_, int a0, int na, int b0, int nb)
+        : same(same_)
+    {
+        if (same)
+        {
+            s0 = l0 = a0;
+            ns = nl = na;
+            d = na;
+            npairs = na * (na - 1) / 2;
+        }
+        else
+        {
+            bool const aSmall = na <= nb;
+            s0 = aSmall ? a0 : b0;
+            ns = aSmall ? na : nb;
+            l0 = aSmall ? b0 : a0;
+            nl = aSmall ? nb : na;
+            d = ns;
+            npairs = na * nb;
+        }
+        q64 = d > 0 ? 64 / d : 0;
+        r64 = 64 - q64 * d;
+    }
+    // Cursor (c, r) of pair p0 (one division, at the start).
+    __device__ void start(int p0, int& c, int& r) const
+    {
+        c = 0;
+        r = 0;
+        if (d > 0) divmod_small(p0, d, 1.0f / (float)d, c, r);
+    }
+    // (c, r) of pair p -> pair p + 64.
+    __device__ void advance(int& c, int& r) const
+    {
+        r += r64;
+        c += q64;
+        if (r >= d) { r -= d; ++c; }
+    }
+    // Staged indices of pair (c, r); only for p < npairs.
+    __device__ void pair(int c, int r, int& j, int& k) const
+    {
+        int v = r + c + (same ? 1 : 0);
+        // r < d and c < nl (c < n/2 for one block): one subtraction.
+        if (v >= nl) v -= nl;
+        j = s0 + r;
+        k = l0 + v;
+    }
+};
+
+// rij, rik < rc and (narrow only) rjk^2 < rc^2 from the staged records
+// (SymGrpExpAngn.cpp:182-232; the wide type skips rjk), branch free.
+template <typename R, typename V4>
+__device__ __forceinline__ bool pair_in_range(V4 const& pj, V4 const& pk,
+                                              bool wide, R rc, R rc2)
+{
+    R const dx = pk.x - pj.x;
+    R const dy = pk.y - pj.y;
+    R const dz = pk.z - pj.z;
+    R const d2 = dx * dx + dy * dy + dz * dz;
+    return (pj.w < rc) & (pk.w < rc) & (wide | (d2 < rc2));
+}
+
 // Number of set bits of a 64-lane ballot below this lane.
 __device__ __forceinline__ int lanes_below(uint64_t mask)
 {

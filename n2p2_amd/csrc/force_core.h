@@ -29,7 +29,7 @@
 //  * groups run one after another; a group's member parameters, with this
 //    atom's dE/dG weights, are held in scalar registers for the pair loop;
 //  * angular groups enumerate the unordered pairs {j, k} of their blocks in
-//    diagonal order (below), test rij, rik < rc and rjk < rc (narrow) and
+//    diagonal order (DiagPairs), test rij, rik < rc and rjk < rc (narrow) and
 //    push passing pairs into an LDS queue with a ballot prefix; whenever 64
 //    are queued every lane evaluates one triplet, so the work is divergence
 //    free and each triplet is evaluated once for both of its slots;
@@ -153,90 +153,13 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
     __syncthreads();
 }
 
-// Unordered pairs {j, k} of a group's neighbour blocks in diagonal order,
-// p in [0, npairs).  Same element (block of n): p = c n + r gives the pair
-// {r, (r + 1 + c) mod n} at circular distance 1 + c.  The (n - 1) / 2 full
-// diagonals (distance below n/2) hold n pairs each; for n even the half
-// diagonal at distance n/2 follows with r < n/2.  Every unordered pair
-// appears once (n (n - 1) / 2 in all) and one diagonal holds each slot at
-// most twice.  Different elements, smaller block s (ns) and larger block l
-// (nl): p = c ns + r gives {s + r, l + (r + c) mod nl}, c < nl; one
-// diagonal holds each slot at most once.
-struct DiagPairs
-{
-    bool same;
-    int s0, ns, l0, nl, d, npairs;
-    float inv;
-    __device__ DiagPairs(bool same_, int a0, int na, int b0, int nb)
-        : same(same_)
-    {
-        if (same)
-        {
-            s0 = l0 = a0;
-            ns = nl = na;
-            d = na;
-            npairs = na * (na - 1) / 2;
-        }
-        else
-        {
-            bool const aSmall = na <= nb;
-            s0 = aSmall ? a0 : b0;
-            ns = aSmall ? na : nb;
-            l0 = aSmall ? b0 : a0;
-            nl = aSmall ? nb : na;
-            d = ns;
-            npairs = na * nb;
-        }
-        inv = d > 0 ? 1.0f / (float)d : 0.0f;
-    }
-    __device__ bool decode(int p, int& j, int& k) const
-    {
-        if (p >= npairs) return false;
-        int c, r;
-        divmod_small(p, d, inv, c, r);
-        if (same)
-        {
-            int v = r + 1 + c;
-            if (v >= ns) v -= ns;
-            j = s0 + r;
-            k = s0 + v;
-        }
-        else
-        {
-            int v = r + c;
-            // c < nl, r < ns <= nl: one subtraction brings v below nl.
-            if (v >= nl) v -= nl;
-            j = s0 + r;
-            k = l0 + v;
-        }
-        return true;
-    }
-};
-
-// rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
-template <typename R>
-__device__ __forceinline__ bool pair_in(BLds<R> const& S, int j, int k,
-                                        bool wide, R rc, R rc2)
-{
-    Vec4<R> const pj = S.P[j], pk = S.P[k];
-    bool pass = pj.w < rc && pk.w < rc;
-    if (!wide)
-    {
-        R const dx = pk.x - pj.x;
-        R const dy = pk.y - pj.y;
-        R const dz = pk.z - pj.z;
-        pass = pass && (dx * dx + dy * dy + dz * dz) < rc2;
-    }
-    return pass;
-}
-
 // A group's member parameters in wave-uniform (scalar) registers: lambda,
 // zeta lambda, the weight and the flags word.  The exponential's parameters
 // are read from LDS at the start of each (eta, rs) run.
 template <typename R, int MAXM>
 struct Members
 {
-    R lam[MAXM], zl[MAXM], w[MAXM];
+    R lam[MAXM], zl[MAXM], w[MAXM], wz[MAXM];   // wz = w zeta lambda
     int code[MAXM];
 };
 
@@ -251,6 +174,7 @@ __device__ __forceinline__ void load_members(MParam<R> const* MP, int nmem,
         M.lam[m] = uniform_r(p.lambda);
         M.zl[m] = uniform_r(p.zl);
         M.w[m] = uniform_r(p.w);
+        M.wz[m] = uniform_r(p.w * p.zl);
         M.code[m] = uniform(p.flags);
     }
 }
@@ -308,7 +232,7 @@ __device__ __forceinline__ void triplet_backward(
     (void)MP;
     (void)nmem;
     plan_backward_members<R>(gi, wide, costijk, r2sum, rij, rik, rjk, invj,
-                             invk, invjk, M.w, T1, V, Ej, Ek, E3);
+                             invk, invjk, M.w, M.wz, T1, V, Ej, Ek, E3);
 #else
     (void)gi;
     R vexp = R(0), S1 = R(0), U = R(0), tj = R(0), tk = R(0), t3 = R(0);
@@ -488,8 +412,8 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         }
 
         bool const wide = (g.type == 9);
-        DiagPairs dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
-                     sCnt[g.e2]);
+        DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                           sStart[g.e2], sCnt[g.e2]);
         if (dp.npairs == 0)
         {
             __syncthreads();
@@ -499,12 +423,17 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         load_members(MP, nmem, M);
         R const grc2 = (R)g.rc2;
         int qn = 0;
+        int pc, pr;
+        dp.start(lane, pc, pr);
         for (int base = 0; base < dp.npairs; base += 64)
         {
-            int j = 0, k = 0;
-            bool pass = false;
-            if (dp.decode(base + lane, j, k))
-                pass = pair_in(S, j, k, wide, grc, grc2);
+            bool const valid = base + lane < dp.npairs;
+            int j, k;
+            dp.pair(pc, pr, j, k);
+            if (!valid) j = k = dp.s0;
+            dp.advance(pc, pr);
+            bool const pass = valid
+                && pair_in_range<R>(S.P[j], S.P[k], wide, grc, grc2);
             uint64_t const mask = __ballot(pass);
             if (pass) S.QUEUE[qn + lanes_below(mask)] = j | (k << 16);
             qn += __popcll(mask);

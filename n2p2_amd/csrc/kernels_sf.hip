@@ -64,15 +64,19 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
             do { bits += (e & 1); e >>= 1; ++it; } while (e);
             powm += md.useint ? it + bits : 0;
         }
-        PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1], sStart[g.e2],
-                      sCnt[g.e2]);
-        for (int base = 0; base < pc.npairs; base += 64)
+        DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                           sStart[g.e2], sCnt[g.e2]);
+        int pc = 0, pr = 0;
+        dp.start(lane, pc, pr);
+        for (int base = 0; base < dp.npairs; base += 64)
         {
-            int ja = 0, kb = 0;
-            if (!pc.decode(base + lane, ja, kb)) continue;
+            int ja, kb;
+            dp.pair(pc, pr, ja, kb);
+            dp.advance(pc, pr);
+            if (base + lane >= dp.npairs) continue;
             if (!(S.P[ja].w < g.rc && S.P[kb].w < g.rc)) continue;
             c[wide ? 3 : 2] += 1;
-            if (pair_passes(S, ja, kb, wide, g))
+            if (pair_in_range<double>(S.P[ja], S.P[kb], wide, g.rc, g.rc2))
             {
                 c[wide ? 5 : 4] += 1;
                 c[wide ? 7 : 6] += g.nmem;

@@ -17,7 +17,8 @@
 //   2. groups run one after another; a group's member parameters are staged
 //      in LDS and the ones read per member held in scalar registers, its
 //      member sums live in registers;
-//   3. angular groups enumerate the unordered pairs of their blocks, test
+//   3. angular groups enumerate the unordered pairs of their blocks in
+//      diagonal order (DiagPairs), test
 //      rij, rik < rc and rjk < rc, and push passing pairs into an LDS queue
 //      with a ballot prefix; whenever 64 are queued every lane evaluates
 //      one, so the expensive triplet work runs without divergence;
@@ -115,73 +116,6 @@ __device__ void stage_by_element(SFArgs const& a, Stage<R> const& S, int s0,
     __syncthreads();
 }
 
-// Unordered pairs of a group's neighbour blocks, indexed 0..npairs-1.
-// Different elements: p -> (p / nb, p % nb).  Same element (n atoms):
-// circle pairing, p -> {r, r + 1 + c mod n} for p < n*h (h = (n-1)/2), and
-// {r, r + n/2} for the n/2 remaining pairs when n is even; every unordered
-// pair appears exactly once.
-struct PairCursor
-{
-    bool same;
-    int a0, na, b0, nb, hh, npairs;
-    float inv;
-    __device__ PairCursor(bool same_, int a0_, int na_, int b0_, int nb_)
-        : same(same_), a0(a0_), na(na_), b0(b0_), nb(nb_)
-    {
-        hh = same ? (na - 1) / 2 : 0;
-        npairs = same ? na * (na - 1) / 2 : na * nb;
-        int const dv = same ? hh : nb;
-        inv = dv > 0 ? 1.0f / (float)dv : 0.0f;
-    }
-    __device__ bool decode(int p, int& ja, int& kb) const
-    {
-        if (p >= npairs) return false;
-        int r, c;
-        if (same)
-        {
-            if (p < na * hh)
-            {
-                divmod_small(p, hh, inv, r, c);
-                int v = r + 1 + c;
-                if (v >= na) v -= na;
-                ja = a0 + r;
-                kb = a0 + v;
-            }
-            else
-            {
-                r = p - na * hh;
-                ja = a0 + r;
-                kb = a0 + r + na / 2;
-            }
-        }
-        else
-        {
-            divmod_small(p, nb, inv, r, c);
-            ja = a0 + r;
-            kb = b0 + c;
-        }
-        return true;
-    }
-};
-
-// rij, rik < rc and (narrow only) rjk < rc: SymGrpExpAngn.cpp:182-232.
-template <typename R>
-__device__ __forceinline__ bool pair_passes(Stage<R> const& S, int ja, int kb,
-                                            bool wide, GroupDesc const& g)
-{
-    R const rc = (R)g.rc;
-    Vec4<R> const pj = S.P[ja], pk = S.P[kb];
-    bool pass = pj.w < rc && pk.w < rc;
-    if (!wide)
-    {
-        R const dx = pk.x - pj.x;
-        R const dy = pk.y - pj.y;
-        R const dz = pk.z - pj.z;
-        pass = pass && (dx * dx + dy * dy + dz * dz) < (R)g.rc2;
-    }
-    return pass;
-}
-
 // A group's member parameters in wave-uniform (scalar) registers: lambda
 // and the flags word; the exponential's parameters are read from LDS at the
 // start of each (eta, rs) run.
@@ -277,6 +211,7 @@ __device__ __forceinline__ void triplet_forward(
 template <int MAXM, int CT, bool GENPOW, typename R>
 __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
 {
+    static_assert(MAXM == 8 || MAXM == 16, "member chunks of 8 or 16");
     extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     __shared__ MParam<R> MP[kMaxMembers];
@@ -343,8 +278,9 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
         else
         {
             bool const wide = (g.type == 9);
-            PairCursor pc(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
-                          sStart[g.e2], sCnt[g.e2]);
+            DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                               sStart[g.e2], sCnt[g.e2]);
+            R const grc = (R)g.rc, grc2 = (R)g.rc2;
             FwdMembers<R, MAXM> M;
 #pragma unroll
             for (int m = 0; m < MAXM; ++m)
@@ -354,12 +290,17 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                 M.code[m] = uniform(p.flags);
             }
             int qn = 0;
-            for (int base = 0; base < pc.npairs; base += 64)
+            int pc = 0, pr = 0;
+            dp.start(lane, pc, pr);
+            for (int base = 0; base < dp.npairs; base += 64)
             {
-                int ja = 0, kb = 0;
-                bool pass = false;
-                if (pc.decode(base + lane, ja, kb))
-                    pass = pair_passes(S, ja, kb, wide, g);
+                bool const valid = base + lane < dp.npairs;
+                int ja, kb;
+                dp.pair(pc, pr, ja, kb);
+                if (!valid) ja = kb = dp.s0;
+                dp.advance(pc, pr);
+                bool const pass = valid
+                    && pair_in_range<R>(S.P[ja], S.P[kb], wide, grc, grc2);
                 uint64_t const mask = __ballot(pass);
                 if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
                 qn += __popcll(mask);
@@ -381,20 +322,19 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                                                   wide, cc, M, MP, nmem, acc);
             }
         }
+        // Member totals by the transposing butterfly: member m ends in the
+        // lanes (m << (6 - log2 MAXM)) + t; the first of them writes it.
+        double dv[MAXM];
 #pragma unroll
-        for (int m = 0; m < MAXM; ++m)
+        for (int m = 0; m < MAXM; ++m) dv[m] = (double)acc[m];
+        double const v = wave_sum_transposed<MAXM>(dv);
+        constexpr int kSh = (MAXM == 8) ? 3 : 2;   // 6 - log2(MAXM)
+        int const m = lane >> kSh;
+        if ((lane & ((1 << kSh) - 1)) == 0 && m < nmem)
         {
-            if (m < nmem)
-            {
-                double const v = wave_sum((double)acc[m]);
-                if (lane == m)
-                {
-                    MemberDesc const& md = mem[m];
-                    double const raw = (g.type == 2) ? v : v * md.gfac;
-                    Gi[md.sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + md.sf],
-                                                       raw);
-                }
-            }
+            MemberDesc const& md = mem[m];
+            double const raw = (g.type == 2) ? v : v * md.gfac;
+            Gi[md.sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + md.sf], raw);
         }
         __syncthreads();
     }

@@ -104,8 +104,11 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
             o << "        vexp = exp_scaled(exp_factor(" << lit(r.eta)
               << ") * s_);\n";
         }
+        // Per lambda run: W_ = sum w P, Z_ = sum (w zeta lambda) P, two
+        // FMAs per member; U += plambda W_ and S1 += Z_ at the run's end.
         o << "        {\n        R const pl = " << plambda(r.lambda) << ";\n"
-          << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n";
+          << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n"
+          << "        R W_ = R(0), Z_ = R(0);\n";
         int level = 0;
         int m = r.m0;
         for (int l = 0; l <= r.lmax; ++l)
@@ -114,8 +117,8 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
                 MemberDesc const& md = F.member[g.mem0 + m];
                 for (; level < md.level; ++level)
                     o << "        P *= X; X *= X;\n";
-                o << "        { R const f_ = w[" << m << "] * P; S1 += f_ * "
-                  << lit(md.zl) << "; U += f_ * pl; }\n";
+                o << "        W_ += w[" << m << "] * P; Z_ += wz[" << m
+                  << "] * P;\n";
             }
         for (; m < r.mend; ++m)
         {
@@ -123,10 +126,10 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
             o << "        { R const pw_ = ";
             if (md.useint) o << "pow_int(pl, " << md.zint - 1 << ")";
             else o << "pow(pl, " << lit(md.zeta - 1.0) << ")";
-            o << "; R const f_ = pl > R(0) ? w[" << m << "] * pw_ : R(0);"
-              << " S1 += f_ * " << lit(md.zl) << "; U += f_ * pl; }\n";
+            o << "; R const f_ = pl > R(0) ? pw_ : R(0); W_ += w[" << m
+              << "] * f_; Z_ += wz[" << m << "] * f_; }\n";
         }
-        o << "        }\n";
+        o << "        S1 += Z_; U += W_ * pl;\n        }\n";
     }
     if (open) close_run();
     if (anyZero && !anyShift)
@@ -288,8 +291,8 @@ std::string plan_source(Model::Flat const& F)
     b << "template <typename R>\n"
          "__device__ __forceinline__ void plan_backward_members(\n"
          "    int gi, bool wide, R costijk, R r2sum, R rij, R rik, R rjk,\n"
-         "    R invj, R invk, R invjk, R const* w, R& T1, R& V, R& Ej,\n"
-         "    R& Ek, R& E3)\n{\n"
+         "    R invj, R invk, R invjk, R const* w, R const* wz, R& T1,\n"
+         "    R& V, R& Ej, R& Ek, R& E3)\n{\n"
          "    (void)wide; (void)rij; (void)rik; (void)rjk; (void)invj;\n"
          "    (void)invk; (void)invjk; (void)r2sum;\n"
          "    switch (gi)\n    {\n";
