@@ -16,33 +16,40 @@ __global__ void k_copy(const double4* __restrict__ src, double4* __restrict__ ds
         dst[i] = src[i];
 }
 
+static void ok(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+    {
+        std::fprintf(stderr, "copy_probe: %s: %s\n", what, hipGetErrorString(e));
+        std::exit(1);
+    }
+}
+
 int main(int argc, char** argv)
 {
     size_t const mib = argc > 1 ? (size_t)std::atol(argv[1]) : 1024;
     size_t const bytes = mib << 20;
     size_t const n = bytes / sizeof(double4);
     double4 *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess)
-    {
-        std::fprintf(stderr, "copy_probe: allocation failed\n");
-        return 1;
-    }
-    hipMemset(a, 1, bytes);
-    hipMemset(b, 0, bytes);
-    hipDeviceSynchronize();
+    ok(hipMalloc(&a, bytes), "hipMalloc");
+    ok(hipMalloc(&b, bytes), "hipMalloc");
+    ok(hipMemset(a, 1, bytes), "hipMemset");
+    ok(hipMemset(b, 0, bytes), "hipMemset");
+    ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0);
+    ok(hipEventCreate(&e0), "hipEventCreate");
+    ok(hipEventCreate(&e1), "hipEventCreate");
+    ok(hipEventRecord(e0), "hipEventRecord");
     k_copy<<<2048, 256>>>(a, b, n);
-    hipEventRecord(e1);
-    hipEventSynchronize(e1);
+    ok(hipGetLastError(), "k_copy launch");
+    ok(hipEventRecord(e1), "hipEventRecord");
+    ok(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
+    ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
     std::printf("copy_probe: %zu bytes read, %zu bytes written, %.3f ms, "
                 "%.1f GB/s (read+write)\n", bytes, bytes, ms,
                 2.0 * bytes / (ms * 1e-3) / 1e9);
-    hipFree(a);
-    hipFree(b);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    ok(hipFree(a), "hipFree");
+    ok(hipFree(b), "hipFree");
+    return 0;
 }
