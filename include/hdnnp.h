@@ -4,8 +4,9 @@
  * This is the drop-in boundary under n2p2's libnnp Mode API.  Every entry
  * point replaces one reference interface, cited as file:line relative to the
  * n2p2 tree (src/libnnp unless stated).  The header-compatible C++ layer in
- * n2p2_amd/csrc/nnp (nnp::Mode, nnp::Structure, nnp::Prediction) calls
- * these functions; ctypes / cgo / JNI bindings can call them directly.
+ * n2p2_amd/nnp (nnp::Mode, nnp::Prediction, nnp::Structure, nnp::Atom, ...,
+ * built as libnnp_amd.so) calls these functions; ctypes / cgo / JNI
+ * bindings can call them directly.
  *
  * Conventions
  *   - plain C types only; arrays are caller-owned host memory unless the
