@@ -543,6 +543,17 @@ __device__ __forceinline__ int wave_sum_int(int v)
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// Work index of workgroup b of nb.  The hardware deals workgroups to the 8
+// XCDs round-robin (XCD = b % 8); this mapping gives each XCD one contiguous
+// range of work indices, so work items that touch the same data (atoms of
+// neighbouring cells) share that XCD's L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_work_index(int b, int nb)
+{
+    int const x = b & 7, k = b >> 3;
+    int const q = nb >> 3, r = nb & 7;   // XCD x holds q + (x < r) groups
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Sums M per-lane values over the wave by a transposing butterfly: at each
 // of the first log2(M) steps a lane keeps half of its values, sends the
 // other half to the partner lane (xor 32, 16, ...) and adds what it

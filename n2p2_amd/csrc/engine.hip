@@ -487,8 +487,8 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
                                 c->dCellAtoms.p, c->dStats.p, c->dOff.p, keyCap,
                                 c->dKeys.p, c->dNbrJ.p, c->dGeo.p, s),
                  "nl fill");
-    hipCheck(launch_nl_partner(n, c->dOff.p, c->dKeys.p, c->dPartner.p, s),
-             "nl partner");
+    hipCheck(launch_nl_partner(n, c->dOff.p, c->dKeys.p, c->dCellAtoms.p,
+                               c->dPartner.p, s), "nl partner");
     c->end(K_NEIGHBOR, ev0);
     c->M = M;
     c->maxNeigh = maxc;
@@ -644,8 +644,9 @@ void computeForces(hdnnp_ctx* c, int mode)
     c->end(K_FORCE, ev);
     c->begin(K_GATHER, ev2);
     hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dNbrJ.p,
-                                 c->dPartner.p, c->dCslot.p, c->dCself.p, mode,
-                                 c->dF.p, c->stream), "force gather");
+                                 c->dPartner.p, c->dCslot.p, c->dCself.p,
+                                 c->dCellAtoms.p, mode, c->dF.p, c->stream),
+             "force gather");
     c->end(K_GATHER, ev2);
     c->sync();
 }

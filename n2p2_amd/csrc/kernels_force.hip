@@ -21,15 +21,20 @@ void k_force_backward(ForceArgs a)
 // F_m = -(cself[m] + sum_q cslot[partner(q)]) over m's slots q; mode 0
 // skips slots pointing at m itself (Mode.cpp:1885 starts the unique list
 // after the centre atom), mode 1 keeps them.  One wave per atom, fixed
-// lane assignment and butterfly: deterministic.
+// lane assignment and butterfly: deterministic.  Atoms are taken in cell
+// order with the XCD-aware mapping, so the partner entries read by
+// neighbouring atoms share an L2.
 __global__ __launch_bounds__(256) void k_gather(
     int n, const int* __restrict__ off, const int* __restrict__ nbrJ,
     const int* __restrict__ partner, const double* __restrict__ cslot,
-    const double* __restrict__ cself, int mode, double* __restrict__ F)
+    const double* __restrict__ cself, const int* __restrict__ order,
+    int mode, double* __restrict__ F)
 {
-    int const i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int const t = xcd_work_index(blockIdx.x, gridDim.x) * 4
+                + (int)(threadIdx.x >> 6);
     int const lane = threadIdx.x & 63;
-    if (i >= n) return;
+    if (t >= n) return;
+    int const i = order[t];
     int const s0 = off[i], s1 = off[i + 1];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int q = s0 + lane; q < s1; q += 64)
@@ -98,12 +103,12 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
 
 hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
                                const int* partner, const double* cslot,
-                               const double* cself, int mode, double* F,
-                               hipStream_t s)
+                               const double* cself, const int* order,
+                               int mode, double* F, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_gather<<<(n + 3) / 4, 256, 0, s>>>(n, off, nbrJ, partner, cslot, cself,
-                                         mode, F);
+                                         order, mode, F);
     return hipGetLastError();
 }
 

@@ -292,7 +292,9 @@ __global__ __launch_bounds__(256) void k_nl_cell(
     __shared__ uint64_t cKey[kCellCand];
     __shared__ int sStart[64], sCnt[64], sPre[64], sImg[64];
     __shared__ int sTotal;
-    int const cell = blockIdx.x;
+    // Cells in XCD-aware order: each XCD takes a contiguous range of cells,
+    // whose candidate regions overlap in its L2.
+    int const cell = xcd_work_index(blockIdx.x, gridDim.x);
     int const tid = threadIdx.x;
     int const lane = tid & 63;
     int const w = tid >> 6;
@@ -456,13 +458,17 @@ __global__ __launch_bounds__(256) void k_nl_cell(
 }
 
 // Partner slot: for entry q of atom i pointing at (j, bc) the entry of j
-// pointing at (i, -bc); binary search in j's sorted keys.  One wave per atom.
+// pointing at (i, -bc); binary search in j's sorted keys.  One wave per
+// atom, atoms taken in cell order (order = atoms sorted by cell) with the
+// XCD-aware mapping: the lists searched by neighbouring atoms overlap and
+// stay in one L2.
 __global__ __launch_bounds__(64) void k_partner(
     int n, const int* __restrict__ off, const uint64_t* __restrict__ keys,
-    int* __restrict__ partner)
+    const int* __restrict__ order, int* __restrict__ partner)
 {
-    int const i = blockIdx.x;
-    if (i >= n) return;
+    int const w = xcd_work_index(blockIdx.x, gridDim.x);
+    if (w >= n) return;
+    int const i = order[w];
     int const s0 = off[i], s1 = off[i + 1];
     for (int q = s0 + (int)threadIdx.x; q < s1; q += 64)
     {
@@ -637,9 +643,10 @@ hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
 
 
 hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
-                             int* partner, hipStream_t s)
+                             const int* order, int* partner, hipStream_t s)
 {
-    k_partner<<<n, 64, 0, s>>>(n, off, keys, partner);
+    if (n <= 0) return hipSuccess;
+    k_partner<<<n, 64, 0, s>>>(n, off, keys, order, partner);
     return hipGetLastError();
 }
 
