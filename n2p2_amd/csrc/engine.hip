@@ -134,7 +134,16 @@ struct hdnnp_ctx
     NLParams nl{};
     int pbc[3] = {0, 0, 0};
     int ncell = 0;
+    // Entries of atom i: [off[i], off[i] + count[i]).  Cell builds use a
+    // fixed range of `slab` slots per atom (off[i] = i * slab, no count
+    // pass); the per-atom fallback builds a compact CSR.  entryCap = slots
+    // allocated; M = total entries (summed on demand, Mvalid).
+    int slab = 0;
+    int slabAtoms = -1;
+    bool slabLayout = false;
+    int64_t entryCap = 0;
     int64_t M = 0;
+    bool Mvalid = false;
     int maxNeigh = 0;
     bool haveNL = false;
     DBuf<int> dCellOfAtom, dRank, dCellCount, dCellStart, dCellAtoms, dCount,
@@ -422,11 +431,11 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     c->dCellAtoms.ensure(n);
     c->dCount.ensure(n);
     c->dOff.ensure(n + 1);
-    c->dStats.ensure(3);
+    c->dStats.ensure(4);
     c->dBlockSums.ensure((std::max(n, ncell) + 1023) / 1024 + 1);
     c->dTotal.ensure(1);
     hipCheck(hipMemsetAsync(c->dCellCount.p, 0, ncell * sizeof(int), s), "memset");
-    hipCheck(hipMemsetAsync(c->dStats.p, 0, 3 * sizeof(int), s), "memset");
+    hipCheck(hipMemsetAsync(c->dStats.p, 0, 4 * sizeof(int), s), "memset");
     hipCheck(launch_nl_bin(p, c->dR.p, c->dCellOfAtom.p, c->dRank.p,
                            c->dCellCount.p, s), "nl bin");
     hipCheck(scan_exclusive(c->dCellCount.p, c->dCellStart.p, ncell,
@@ -438,62 +447,142 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     // otherwise.  Both produce the same list.
     bool cellKernels = (2 * p.m[0] + 1) * (2 * p.m[1] + 1) * (2 * p.m[2] + 1)
                      <= 64;
-    int hstat[4];
-    for (int attempt = 0; attempt < 2; ++attempt)
-    {
-        if (cellKernels)
-            hipCheck(launch_nl_count_cell(p, ncell, c->dR.p, c->dCellStart.p,
-                                          c->dCellCount.p, c->dCellAtoms.p,
-                                          c->dCount.p, c->dStats.p, s),
-                     "nl count");
-        else
-            hipCheck(launch_nl_count(p, c->dR.p, c->dElemAtom.p,
-                                     c->dCellOfAtom.p, c->dCellStart.p,
-                                     c->dCellCount.p, c->dCellAtoms.p,
-                                     c->dCount.p, c->dStats.p, s), "nl count");
-        hipCheck(scan_exclusive(c->dCount.p, c->dOff.p, n, c->dBlockSums.p,
-                                c->dTotal.p, s), "count scan");
-        hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "count max");
-        hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 3 * sizeof(int),
+    int hstat[4] = {0, 0, 0, 0};
+    auto readStats = [&]() {
+        hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 4 * sizeof(int),
                                 hipMemcpyDeviceToHost, s), "stats");
-        hipCheck(hipMemcpyAsync(hstat + 3, c->dTotal.p, sizeof(int),
-                                hipMemcpyDeviceToHost, s), "total");
         hipCheck(hipStreamSynchronize(s), "sync");
-        if (!(cellKernels && hstat[2] > 0)) break;
-        cellKernels = false;     // a cell overflowed the staging: redo
-        hipCheck(hipMemsetAsync(c->dStats.p, 0, 3 * sizeof(int), s), "memset");
-    }
-    int const maxc = hstat[0];
-    int64_t const M = hstat[3];
-    if (M < 0 || M > (int64_t)2000000000)
-        throw Error(HDNNP_E_CAPACITY, "ERROR: Neighbour list too large.\n");
-    int keyCap = 64;
-    while (keyCap < maxc) keyCap <<= 1;
-    if (keyCap > 8192)
-        throw Error(HDNNP_E_CAPACITY,
-                    "ERROR: More than 8192 neighbours for one atom.\n");
-    c->dKeys.ensure(M);
-    c->dNbrJ.ensure(M);
-    c->dGeo.ensure(M);
-    c->dPartner.ensure(M);
-    if (cellKernels)
+    };
+    auto clearStats = [&]() {
+        hipCheck(hipMemsetAsync(c->dStats.p, 0, 4 * sizeof(int), s),
+                 "memset");
+    };
+    auto ensureEntries = [&](int64_t E) {
+        if (E < 0 || E > (int64_t)2000000000)
+            throw Error(HDNNP_E_CAPACITY,
+                        "ERROR: Neighbour list too large.\n");
+        c->dKeys.ensure(E);
+        c->dNbrJ.ensure(E);
+        c->dGeo.ensure(E);
+        c->dPartner.ensure(E);
+        c->entryCap = E;
+    };
+    auto slabFor = [](int maxc) {
+        // 1/8 headroom over the largest count plus 16, a multiple of 16.
+        return ((maxc + maxc / 8 + 16) + 15) / 16 * 16;
+    };
+    bool built = false;
+    if (cellKernels && c->slab > 0 && c->slabAtoms == n)
+    {
+        // Slab build: one pass, no count kernel.  The counts, their maximum
+        // and an overflow flag come back with the fill; on overflow the
+        // list is rebuilt below with a range sized by the new maximum.
+        ensureEntries((int64_t)n * c->slab);
+        hipCheck(launch_slab_starts(n, c->slab, c->dOff.p, s), "slab starts");
         hipCheck(launch_nl_fill_cell(p, ncell, c->dR.p, c->dCellStart.p,
                                      c->dCellCount.p, c->dCellAtoms.p,
-                                     c->dStats.p, c->dOff.p, c->dKeys.p,
-                                     c->dNbrJ.p, c->dGeo.p, s), "nl fill");
-    else
-        hipCheck(launch_nl_fill(p, c->dR.p, c->dElemAtom.p, c->dCellOfAtom.p,
-                                c->dCellStart.p, c->dCellCount.p,
-                                c->dCellAtoms.p, c->dStats.p, c->dOff.p, keyCap,
-                                c->dKeys.p, c->dNbrJ.p, c->dGeo.p, s),
-                 "nl fill");
-    hipCheck(launch_nl_partner(n, c->dOff.p, c->dKeys.p, c->dCellAtoms.p,
-                               c->dPartner.p, s), "nl partner");
+                                     c->dCount.p, c->dStats.p, c->dOff.p,
+                                     c->slab, c->dKeys.p, c->dNbrJ.p,
+                                     c->dGeo.p, s), "nl fill");
+        readStats();
+        if (hstat[2] > 0) cellKernels = false;    // staging overflow
+        else if (hstat[3] == 0) built = true;
+        clearStats();
+    }
+    int maxc = hstat[0];
+    if (!built)
+    {
+        for (int attempt = 0; attempt < 2; ++attempt)
+        {
+            if (cellKernels)
+                hipCheck(launch_nl_count_cell(p, ncell, c->dR.p,
+                                              c->dCellStart.p, c->dCellCount.p,
+                                              c->dCellAtoms.p, c->dCount.p,
+                                              c->dStats.p, s), "nl count");
+            else
+                hipCheck(launch_nl_count(p, c->dR.p, c->dElemAtom.p,
+                                         c->dCellOfAtom.p, c->dCellStart.p,
+                                         c->dCellCount.p, c->dCellAtoms.p,
+                                         c->dCount.p, c->dStats.p, s),
+                         "nl count");
+            hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s),
+                     "count max");
+            readStats();
+            if (!(cellKernels && hstat[2] > 0)) break;
+            cellKernels = false;     // a cell overflowed the staging: redo
+            clearStats();
+        }
+        maxc = hstat[0];
+        clearStats();
+        if (cellKernels)
+        {
+            c->slab = slabFor(maxc);
+            c->slabAtoms = n;
+            if ((int64_t)n * c->slab > (int64_t)2000000000)
+                throw Error(HDNNP_E_CAPACITY,
+                            "ERROR: Neighbour list too large.\n");
+            ensureEntries((int64_t)n * c->slab);
+            hipCheck(launch_slab_starts(n, c->slab, c->dOff.p, s),
+                     "slab starts");
+            hipCheck(launch_nl_fill_cell(p, ncell, c->dR.p, c->dCellStart.p,
+                                         c->dCellCount.p, c->dCellAtoms.p,
+                                         c->dCount.p, c->dStats.p, c->dOff.p,
+                                         c->slab, c->dKeys.p, c->dNbrJ.p,
+                                         c->dGeo.p, s), "nl fill");
+        }
+        else
+        {
+            hipCheck(scan_exclusive(c->dCount.p, c->dOff.p, n,
+                                    c->dBlockSums.p, c->dTotal.p, s),
+                     "count scan");
+            int total = 0;
+            hipCheck(hipMemcpyAsync(&total, c->dTotal.p, sizeof(int),
+                                    hipMemcpyDeviceToHost, s), "total");
+            hipCheck(hipStreamSynchronize(s), "sync");
+            int keyCap = 64;
+            while (keyCap < maxc) keyCap <<= 1;
+            if (keyCap > 8192)
+                throw Error(HDNNP_E_CAPACITY,
+                            "ERROR: More than 8192 neighbours for one "
+                            "atom.\n");
+            ensureEntries(std::max(total, 1));
+            hipCheck(launch_nl_fill(p, c->dR.p, c->dElemAtom.p,
+                                    c->dCellOfAtom.p, c->dCellStart.p,
+                                    c->dCellCount.p, c->dCellAtoms.p,
+                                    c->dStats.p, c->dOff.p, keyCap, c->dKeys.p,
+                                    c->dNbrJ.p, c->dGeo.p, s), "nl fill");
+            c->M = total;
+        }
+    }
+    c->slabLayout = cellKernels;
+    c->Mvalid = !cellKernels;
+    hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
+                               c->dCellAtoms.p, c->dPartner.p, s),
+             "nl partner");
     c->end(K_NEIGHBOR, ev0);
-    c->M = M;
-    c->maxNeigh = maxc;
+    // Slot capacity of the SF and force kernels: the largest count (the
+    // slab bounds it; a rebuild above raised the slab if it was exceeded).
+    c->maxNeigh = cellKernels ? std::min(maxc, c->slab) : maxc;
     c->haveNL = true;
     c->haveSF = c->haveNN = c->haveNNDeriv = false;
+}
+
+// Total neighbour-list entries (summed from the per-atom counts once per
+// build when the slab layout is used).
+int64_t totalEntries(hdnnp_ctx* c)
+{
+    if (!c->Mvalid)
+    {
+        std::vector<int> cnt((size_t)c->n);
+        if (c->n > 0)
+            hipCheck(hipMemcpy(cnt.data(), c->dCount.p, c->n * sizeof(int),
+                               hipMemcpyDeviceToHost), "counts");
+        int64_t t = 0;
+        for (int v : cnt) t += v;
+        c->M = t;
+        c->Mvalid = true;
+    }
+    return c->M;
 }
 
 void computeSF(hdnnp_ctx* c, bool raw)
@@ -511,6 +600,7 @@ void computeSF(hdnnp_ctx* c, bool raw)
     a.T = c->T;
     a.n = (int)c->n;
     a.off = c->dOff.p;
+    a.cnt = c->dCount.p;
     a.geo = c->dGeo.p;
     a.nbrJ = c->dNbrJ.p;
     a.elem = c->dElemAtom.p;
@@ -613,13 +703,14 @@ void computeForces(hdnnp_ctx* c, int mode)
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the force "
                     "kernel's LDS staging.\n");
-    c->dCslot.ensure(3 * (size_t)std::max<int64_t>(c->M, 1));
+    c->dCslot.ensure(3 * (size_t)std::max<int64_t>(c->entryCap, 1));
     c->dCself.ensure(3 * (size_t)c->n);
     c->dF.ensure(3 * (size_t)c->n);
     ForceArgs a{};
     a.T = c->T;
     a.n = (int)c->n;
     a.off = c->dOff.p;
+    a.cnt = c->dCount.p;
     a.geo = c->dGeo.p;
     a.nbrJ = c->dNbrJ.p;
     a.elem = c->dElemAtom.p;
@@ -643,10 +734,10 @@ void computeForces(hdnnp_ctx* c, int mode)
     else hipCheck(launch_force_backward(a, c->stream), "force backward");
     c->end(K_FORCE, ev);
     c->begin(K_GATHER, ev2);
-    hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dNbrJ.p,
-                                 c->dPartner.p, c->dCslot.p, c->dCself.p,
-                                 c->dCellAtoms.p, mode, c->dF.p, c->stream),
-             "force gather");
+    hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dCount.p,
+                                 c->dNbrJ.p, c->dPartner.p, c->dCslot.p,
+                                 c->dCself.p, c->dCellAtoms.p, mode, c->dF.p,
+                                 c->stream), "force gather");
     c->end(K_GATHER, ev2);
     c->sync();
 }
@@ -662,6 +753,48 @@ void copyOut(hdnnp_ctx* c, T* dst, const T* src, size_t count)
 }  // namespace
 
 // =========================================================== C ABI
+namespace {
+
+// Host copies of the per-atom first entries and counts, and the compact
+// CSR offsets (n + 1) of the list in atom order.
+struct EntryMap
+{
+    std::vector<int> off, cnt;
+    std::vector<int64_t> csr;
+};
+
+EntryMap entryMap(hdnnp_ctx* c)
+{
+    int const n = (int)c->n;
+    EntryMap m;
+    m.off.resize(n + 1);
+    m.cnt.resize(n);
+    copyOut(c, m.off.data(), c->dOff.p, n + 1);
+    copyOut(c, m.cnt.data(), c->dCount.p, n);
+    c->sync();
+    m.csr.resize(n + 1);
+    m.csr[0] = 0;
+    for (int i = 0; i < n; ++i) m.csr[i + 1] = m.csr[i] + m.cnt[i];
+    return m;
+}
+
+// Copies a per-entry device array (w values per entry) into compact atom
+// order.
+template <class T>
+void compactEntries(hdnnp_ctx* c, EntryMap const& m, T const* dev, int w,
+                    T* out)
+{
+    std::vector<T> all((size_t)c->entryCap * w);
+    copyOut(c, all.data(), dev, all.size());
+    c->sync();
+    for (size_t i = 0; i < m.cnt.size(); ++i)
+        std::copy(all.begin() + (size_t)m.off[i] * w,
+                  all.begin() + ((size_t)m.off[i] + m.cnt[i]) * w,
+                  out + (size_t)m.csr[i] * w);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* hdnnp_last_error(void) { return g_err.c_str(); }
@@ -953,7 +1086,14 @@ int hdnnp_evaluate(hdnnp_ctx* c, int force_mode, double* energy)
 int64_t hdnnp_num_atoms(const hdnnp_ctx* c) { return c->n; }
 int64_t hdnnp_num_neighbor_entries(const hdnnp_ctx* c)
 {
-    return c->haveNL ? c->M : 0;
+    if (!c || !c->haveNL) return 0;
+    int64_t m = 0;
+    int const rc = guarded([&]() {
+        hdnnp_ctx* cc = const_cast<hdnnp_ctx*>(c);
+        cc->setDevice();
+        m = totalEntries(cc);
+    });
+    return rc == HDNNP_OK ? m : rc;
 }
 
 int hdnnp_get_pbc(hdnnp_ctx* c, int32_t* pbc)
@@ -964,33 +1104,22 @@ int hdnnp_get_pbc(hdnnp_ctx* c, int32_t* pbc)
     });
 }
 
+
 int hdnnp_get_neighbors(hdnnp_ctx* c, int64_t* offsets, int32_t* index,
                         int32_t* image, double* d, double* dr)
 {
     return guarded([&]() {
         requireNL(c);
         c->setDevice();
-        size_t const M = (size_t)c->M;
-        int const n = (int)c->n;
-        std::vector<int> off(n + 1);
-        copyOut(c, off.data(), c->dOff.p, n + 1);
-        copyOut(c, index, c->dNbrJ.p, M);
-        std::vector<uint64_t> keys;
-        if (image)
-        {
-            keys.resize(M);
-            copyOut(c, keys.data(), c->dKeys.p, M);
-        }
-        std::vector<double4> geo;
-        if (d || dr)
-        {
-            geo.resize(M);
-            copyOut(c, geo.data(), c->dGeo.p, M);
-        }
-        c->sync();
+        EntryMap const m = entryMap(c);
+        size_t const M = (size_t)m.csr.back();
         if (offsets)
-            for (int i = 0; i <= n; ++i) offsets[i] = off[i];
+            for (size_t i = 0; i < m.csr.size(); ++i) offsets[i] = m.csr[i];
+        if (index) compactEntries(c, m, c->dNbrJ.p, 1, index);
         if (image)
+        {
+            std::vector<uint64_t> keys(M);
+            compactEntries(c, m, c->dKeys.p, 1, keys.data());
             for (size_t q = 0; q < M; ++q)
             {
                 uint64_t const k = keys[q];
@@ -998,14 +1127,20 @@ int hdnnp_get_neighbors(hdnnp_ctx* c, int64_t* offsets, int32_t* index,
                 image[3 * q + 1] = (int)((k >> 7) & 127) - 64;
                 image[3 * q + 2] = (int)(k & 127) - 64;
             }
-        for (size_t q = 0; q < geo.size(); ++q)
+        }
+        if (d || dr)
         {
-            if (d) d[q] = geo[q].w;
-            if (dr)
+            std::vector<double4> geo(M);
+            compactEntries(c, m, c->dGeo.p, 1, geo.data());
+            for (size_t q = 0; q < M; ++q)
             {
-                dr[3 * q] = geo[q].x;
-                dr[3 * q + 1] = geo[q].y;
-                dr[3 * q + 2] = geo[q].z;
+                if (d) d[q] = geo[q].w;
+                if (dr)
+                {
+                    dr[3 * q] = geo[q].x;
+                    dr[3 * q + 1] = geo[q].y;
+                    dr[3 * q + 2] = geo[q].z;
+                }
             }
         }
     });
@@ -1079,16 +1214,21 @@ int hdnnp_get_pair_forces(hdnnp_ctx* c, double* self_forces,
             throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
         c->setDevice();
         copyOut(c, self_forces, c->dCself.p, 3 * (size_t)c->n);
-        copyOut(c, pair_forces, c->dCslot.p, 3 * (size_t)c->M);
         c->sync();
+        size_t M = 0;
+        if (pair_forces)
+        {
+            EntryMap const m = entryMap(c);
+            M = (size_t)m.csr.back();
+            compactEntries(c, m, c->dCslot.p, 3, pair_forces);
+        }
         // Stored as +sum dE/dG dG/dr; the reference's self and pair forces
         // carry the minus sign.
         if (self_forces)
             for (size_t k = 0; k < 3 * (size_t)c->n; ++k)
                 self_forces[k] = -self_forces[k];
         if (pair_forces)
-            for (size_t k = 0; k < 3 * (size_t)c->M; ++k)
-                pair_forces[k] = -pair_forces[k];
+            for (size_t k = 0; k < 3 * M; ++k) pair_forces[k] = -pair_forces[k];
     });
 }
 
@@ -1196,6 +1336,7 @@ int hdnnp_work_counts(hdnnp_ctx* c, int64_t* counts)
         a.T = c->T;
         a.n = (int)c->n;
         a.off = c->dOff.p;
+    a.cnt = c->dCount.p;
         a.geo = c->dGeo.p;
         a.nbrJ = c->dNbrJ.p;
         a.elem = c->dElemAtom.p;

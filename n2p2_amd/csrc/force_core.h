@@ -346,7 +346,7 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     int const ec = a.elem[i];
     ElemDesc const& ed = T.elem[ec];
     int const s0 = a.off[i];
-    int const n = a.off[i + 1] - s0;
+    int const n = a.cnt[i];
     double const* __restrict__ dEdGi = a.dEdG + (size_t)i * a.ldG;
 
     stage(a, S, s0, n, sStart, sCnt);

@@ -13,7 +13,8 @@ struct SFArgs
 {
     ModelTables T;
     int n;                       // atoms
-    const int* off;              // CSR offsets (n+1)
+    const int* off;              // first entry of each atom
+    const int* cnt;              // entries of each atom
     const double4* geo;          // dx, dy, dz, d per entry
     const int* nbrJ;             // neighbour atom per entry
     const int* elem;             // element per atom
@@ -28,13 +29,14 @@ struct ForceArgs
 {
     ModelTables T;
     int n;
-    const int* off;
+    const int* off;              // first entry of each atom
+    const int* cnt;              // entries of each atom
     const double4* geo;
     const int* nbrJ;
     const int* elem;
     const double* dEdG;          // n x ldG
     int ldG;
-    double* cslot;               // 3 x M (x, y, z per entry)
+    double* cslot;               // 3 per entry (x, y, z), entry-indexed
     double* cself;               // 3 x n
     int cap;
     int fp32;                    // mixed-precision contraction

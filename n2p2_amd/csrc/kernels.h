@@ -49,14 +49,21 @@ hipError_t launch_nl_count_cell(NLParams const& p, int ncell, const double* r,
                                 const int* cellStart, const int* cellCount,
                                 const int* cellAtoms, int* count, int* stats,
                                 hipStream_t s);
+// Writes atom i's entries from off[i] on, at most limit of them; count[i]
+// receives the true count, stats[0] the largest, stats[3] = 1 if a count
+// exceeds limit.
 hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
                                const int* cellStart, const int* cellCount,
-                               const int* cellAtoms, int* stats,
-                               const int* off, uint64_t* keys, int* nbrJ,
-                               double4* geo, hipStream_t s);
-// order: the atoms sorted by cell (processing order only).
-hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
-                             const int* order, int* partner, hipStream_t s);
+                               const int* cellAtoms, int* count, int* stats,
+                               const int* off, int limit, uint64_t* keys,
+                               int* nbrJ, double4* geo, hipStream_t s);
+// off[i] = i * slab for i <= n.
+hipError_t launch_slab_starts(int n, int slab, int* off, hipStream_t s);
+// Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted
+// by cell (processing order only).
+hipError_t launch_nl_partner(int n, const int* off, const int* cnt,
+                             const uint64_t* keys, const int* order,
+                             int* partner, hipStream_t s);
 
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
 size_t sf_forward_lds_bytes(int cap, int fp32);
@@ -67,11 +74,13 @@ hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
 hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
 size_t force_backward_lds_bytes(int cap, int fp32);
 
-// order: the atoms sorted by cell (processing order only).
-hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
-                               const int* partner, const double* cslot,
-                               const double* cself, const int* order,
-                               int mode, double* F, hipStream_t s);
+// Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted
+// by cell (processing order only).
+hipError_t launch_force_gather(int n, const int* off, const int* cnt,
+                               const int* nbrJ, const int* partner,
+                               const double* cslot, const double* cself,
+                               const int* order, int mode, double* F,
+                               hipStream_t s);
 
 // Atomic networks for the atoms of one element.  mode -1: per-atom FP64
 // kernel (any width); 0: FP64 MFMA tiles; 1: FP32 MFMA tiles; 2: FP16 MFMA

@@ -25,7 +25,8 @@ void k_force_backward(ForceArgs a)
 // order with the XCD-aware mapping, so the partner entries read by
 // neighbouring atoms share an L2.
 __global__ __launch_bounds__(256) void k_gather(
-    int n, const int* __restrict__ off, const int* __restrict__ nbrJ,
+    int n, const int* __restrict__ off, const int* __restrict__ cnt,
+    const int* __restrict__ nbrJ,
     const int* __restrict__ partner, const double* __restrict__ cslot,
     const double* __restrict__ cself, const int* __restrict__ order,
     int mode, double* __restrict__ F)
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(256) void k_gather(
     int const lane = threadIdx.x & 63;
     if (t >= n) return;
     int const i = order[t];
-    int const s0 = off[i], s1 = off[i + 1];
+    int const s0 = off[i], s1 = s0 + cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int q = s0 + lane; q < s1; q += 64)
     {
@@ -101,14 +102,15 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_force_gather(int n, const int* off, const int* nbrJ,
-                               const int* partner, const double* cslot,
-                               const double* cself, const int* order,
-                               int mode, double* F, hipStream_t s)
+hipError_t launch_force_gather(int n, const int* off, const int* cnt,
+                               const int* nbrJ, const int* partner,
+                               const double* cslot, const double* cself,
+                               const int* order, int mode, double* F,
+                               hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    k_gather<<<(n + 3) / 4, 256, 0, s>>>(n, off, nbrJ, partner, cslot, cself,
-                                         order, mode, F);
+    k_gather<<<(n + 3) / 4, 256, 0, s>>>(n, off, cnt, nbrJ, partner, cslot,
+                                         cself, order, mode, F);
     return hipGetLastError();
 }
 

@@ -273,7 +273,10 @@ __global__ __launch_bounds__(64) void k_nl(
 // an atom's hits is already in list order and they are written directly.
 // The host uses these kernels when at most 64 neighbour cells are searched
 // and falls back to k_nl when a cell has more than kCellCand candidates
-// (stats[2]).
+// (stats[2]).  MODE 1 writes atom i's entries from off[i] on, at most
+// limit of them (a fixed slot range per atom), stores the true count in
+// count[i], raises stats[0] to the largest count and sets stats[3] when a
+// count exceeds limit (the host then rebuilds with a larger range).
 constexpr int kCellCand = 1024;
 constexpr int kCellWaves = 4;
 
@@ -282,11 +285,12 @@ __global__ __launch_bounds__(256) void k_nl_cell(
     NLParams p, const double* __restrict__ r,
     const int* __restrict__ cellStart, const int* __restrict__ cellCount,
     const int* __restrict__ cellAtoms, int* __restrict__ count,
-    int* __restrict__ stats, const int* __restrict__ off,
+    int* __restrict__ stats, const int* __restrict__ off, int limit,
     uint64_t* __restrict__ keysOut, int* __restrict__ nbrJ,
     double4* __restrict__ geo)
 {
     __shared__ double cX[kCellCand], cY[kCellCand], cZ[kCellCand];
+    __shared__ int sMax;
     // MODE 0: (j << 21 | image).  MODE 1: (j << 31 | image << 10 | t), t the
     // staging slot, sorted.
     __shared__ uint64_t cKey[kCellCand];
@@ -403,6 +407,8 @@ __global__ __launch_bounds__(256) void k_nl_cell(
             }
         }
     }
+    if (MODE == 1 && tid == 0) sMax = 0;
+    __syncthreads();
     int const a0 = cellStart[cell];
     for (int a = w; a < nAt; a += kCellWaves)
     {
@@ -443,9 +449,10 @@ __global__ __launch_bounds__(256) void k_nl_cell(
                 }
             }
             uint64_t const mask = __ballot(hit);
-            if (MODE == 1 && hit)
+            int const h = hits + lanes_below(mask);
+            if (MODE == 1 && hit && h < limit)
             {
-                int const q = o + hits + lanes_below(mask);
+                int const q = o + h;
                 keysOut[q] = ((uint64_t)(uint32_t)j << 21) | (uint64_t)img;
                 nbrJ[q] = j;
                 geo[q] = make_double4(dr[0], dr[1], dr[2],
@@ -453,8 +460,28 @@ __global__ __launch_bounds__(256) void k_nl_cell(
             }
             hits += __popcll(mask);
         }
-        if (MODE == 0 && lane == 0) count[i] = hits;
+        if (lane == 0)
+        {
+            count[i] = hits;
+            if (MODE == 1)
+            {
+                atomicMax(&sMax, hits);
+                if (hits > limit) stats[3] = 1;
+            }
+        }
     }
+    if (MODE == 1)
+    {
+        __syncthreads();
+        if (tid == 0) atomicMax(&stats[0], sMax);
+    }
+}
+
+// off[i] = i * slab: atom i's entries start at a fixed slot range.
+__global__ void k_slab_starts(int n, int slab, int* __restrict__ off)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) off[i] = i * slab;
 }
 
 // Partner slot: for entry q of atom i pointing at (j, bc) the entry of j
@@ -463,19 +490,20 @@ __global__ __launch_bounds__(256) void k_nl_cell(
 // XCD-aware mapping: the lists searched by neighbouring atoms overlap and
 // stay in one L2.
 __global__ __launch_bounds__(64) void k_partner(
-    int n, const int* __restrict__ off, const uint64_t* __restrict__ keys,
-    const int* __restrict__ order, int* __restrict__ partner)
+    int n, const int* __restrict__ off, const int* __restrict__ cnt,
+    const uint64_t* __restrict__ keys, const int* __restrict__ order,
+    int* __restrict__ partner)
 {
     int const w = xcd_work_index(blockIdx.x, gridDim.x);
     if (w >= n) return;
     int const i = order[w];
-    int const s0 = off[i], s1 = off[i + 1];
+    int const s0 = off[i], s1 = s0 + cnt[i];
     for (int q = s0 + (int)threadIdx.x; q < s1; q += 64)
     {
         int j, b0, b1, b2;
         split_key(keys[q], j, b0, b1, b2);
         uint64_t const want = make_key(i, -b0, -b1, -b2);
-        int lo = off[j], hi = off[j + 1] - 1;
+        int lo = off[j], hi = off[j] + cnt[j] - 1;
         int found = -1;
         while (lo <= hi)
         {
@@ -624,29 +652,38 @@ hipError_t launch_nl_count_cell(NLParams const& p, int ncell, const double* r,
 {
     if (ncell <= 0) return hipSuccess;
     k_nl_cell<0><<<ncell, 256, 0, s>>>(p, r, cellStart, cellCount, cellAtoms,
-                                       count, stats, nullptr, nullptr, nullptr,
-                                       nullptr);
+                                       count, stats, nullptr, 0, nullptr,
+                                       nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
                                const int* cellStart, const int* cellCount,
-                               const int* cellAtoms, int* stats,
-                               const int* off, uint64_t* keys, int* nbrJ,
-                               double4* geo, hipStream_t s)
+                               const int* cellAtoms, int* count, int* stats,
+                               const int* off, int limit, uint64_t* keys,
+                               int* nbrJ, double4* geo, hipStream_t s)
 {
     if (ncell <= 0) return hipSuccess;
     k_nl_cell<1><<<ncell, 256, 0, s>>>(p, r, cellStart, cellCount, cellAtoms,
-                                       nullptr, stats, off, keys, nbrJ, geo);
+                                       count, stats, off, limit, keys, nbrJ,
+                                       geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_slab_starts(int n, int slab, int* off, hipStream_t s)
+{
+    if (n < 0) return hipSuccess;
+    k_slab_starts<<<(n + 256) / 256, 256, 0, s>>>(n, slab, off);
     return hipGetLastError();
 }
 
 
-hipError_t launch_nl_partner(int n, const int* off, const uint64_t* keys,
-                             const int* order, int* partner, hipStream_t s)
+hipError_t launch_nl_partner(int n, const int* off, const int* cnt,
+                             const uint64_t* keys, const int* order,
+                             int* partner, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    k_partner<<<n, 64, 0, s>>>(n, off, keys, order, partner);
+    k_partner<<<n, 64, 0, s>>>(n, off, cnt, keys, order, partner);
     return hipGetLastError();
 }
 

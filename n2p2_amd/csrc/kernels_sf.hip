@@ -39,7 +39,7 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     int const ec = a.elem[i];
     ElemDesc const& ed = T.elem[ec];
     int const s0 = a.off[i];
-    int const n = a.off[i + 1] - s0;
+    int const n = a.cnt[i];
     stage_by_element(a, S, s0, n, sStart, sCnt);
     unsigned long long c[10];
 #pragma unroll

@@ -227,7 +227,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
     int const ec = a.elem[i];
     ElemDesc const& ed = T.elem[ec];
     int const s0 = a.off[i];
-    int const n = a.off[i + 1] - s0;
+    int const n = a.cnt[i];
 
     stage_by_element(a, S, s0, n, sStart, sCnt);
 
