@@ -200,6 +200,33 @@ int hdnnp_get_forces(hdnnp_ctx* c, double* f);
  * (InterfaceLammps::getForces, InterfaceLammps.cpp:546-581). */
 int hdnnp_get_pair_forces(hdnnp_ctx* c, double* self_forces,
                           double* pair_forces);
+/* ------------------------------------------- caller-given neighbours -- */
+
+/* The InterfaceLammps path (src/libnnpif/LAMMPS/InterfaceLammps.cpp:267-621,
+ * pair_hdnnp.cpp:81-114): the caller hands over, instead of positions and
+ * a cell, ncentre centre atoms with their neighbour lists, as
+ * InterfaceLammps::setLocalAtoms + addNeighbor do.  centre_index (ncentre,
+ * NULL = 0..ncentre-1) and nbr_index (M) are the caller's atom indices
+ * (local or ghost), used only to scatter forces; centre_elem (ncentre) and
+ * nbr_elem (M) element indices; offsets (ncentre + 1, offsets[0] = 0,
+ * M = offsets[ncentre]) delimit each centre's entries; dr (3M) holds
+ * r_centre - r_neighbour in the model's internal units.  Any order of the
+ * entries is accepted (the SF sums are then formed in that order).  The
+ * list replaces the structure: hdnnp_symmetry_functions,
+ * hdnnp_atomic_networks, hdnnp_energy, hdnnp_forces and hdnnp_evaluate
+ * work on the centres; hdnnp_neighbor_list, hdnnp_set_positions and
+ * hdnnp_get_forces are errors until hdnnp_set_structure is called. */
+int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
+                            const int32_t* centre_index,
+                            const int32_t* centre_elem,
+                            const int64_t* offsets, const int32_t* nbr_index,
+                            const int32_t* nbr_elem, const double* dr);
+/* InterfaceLammps::getForces (InterfaceLammps.cpp:546-581) after
+ * hdnnp_forces on a caller-given list: adds every centre's self force to
+ * f[centre_index] and every entry's pair force to f[nbr_index] (f has nout
+ * rows of 3, internal units), centres in order, entries in list order. */
+int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f);
+
 /* Device pointer of the n x 3 force array (valid until the next
  * hdnnp_set_structure); for callers that keep MD state on the GPU. */
 const double* hdnnp_forces_device(hdnnp_ctx* c);

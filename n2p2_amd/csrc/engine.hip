@@ -146,6 +146,11 @@ struct hdnnp_ctx
     bool Mvalid = false;
     int maxNeigh = 0;
     bool haveNL = false;
+    // Neighbour list given by the caller (hdnnp_set_neighbor_list): the
+    // caller's index of every centre and of every entry's neighbour, used
+    // to scatter the forces.
+    bool externalNL = false;
+    std::vector<int32_t> extCentre, extNbr;
     DBuf<int> dCellOfAtom, dRank, dCellCount, dCellStart, dCellAtoms, dCount,
         dOff, dStats, dBlockSums, dTotal;
     DBuf<uint64_t> dKeys;
@@ -733,12 +738,15 @@ void computeForces(hdnnp_ctx* c, int mode)
     }
     else hipCheck(launch_force_backward(a, c->stream), "force backward");
     c->end(K_FORCE, ev);
-    c->begin(K_GATHER, ev2);
-    hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dCount.p,
-                                 c->dNbrJ.p, c->dPartner.p, c->dCslot.p,
-                                 c->dCself.p, c->dCellAtoms.p, mode, c->dF.p,
-                                 c->stream), "force gather");
-    c->end(K_GATHER, ev2);
+    if (!c->externalNL)
+    {
+        c->begin(K_GATHER, ev2);
+        hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dCount.p,
+                                     c->dNbrJ.p, c->dPartner.p, c->dCslot.p,
+                                     c->dCself.p, c->dCellAtoms.p, mode, c->dF.p,
+                                     c->stream), "force gather");
+        c->end(K_GATHER, ev2);
+    }
     c->sync();
 }
 
@@ -983,6 +991,7 @@ int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
                             "ERROR: Element index out of range.\n");
             c->hAtomsOfElem[elem[i]].push_back((int)i);
         }
+        c->externalNL = false;
         c->periodic = box != nullptr;
         if (box)
         {
@@ -1016,6 +1025,9 @@ int hdnnp_set_positions(hdnnp_ctx* c, const double* r)
 {
     return guarded([&]() {
         requireStructure(c);
+        if (c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: No positions: the neighbour "
+                                         "list was given by the caller.\n");
         c->setDevice();
         hipCheck(hipMemcpyAsync(c->dR.p, r, 3 * c->n * sizeof(double),
                                 hipMemcpyHostToDevice, c->stream), "positions");
@@ -1029,6 +1041,9 @@ int hdnnp_set_positions_device(hdnnp_ctx* c, const double* r_device)
 {
     return guarded([&]() {
         requireStructure(c);
+        if (c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: No positions: the neighbour "
+                                         "list was given by the caller.\n");
         c->setDevice();
         hipCheck(hipMemcpyAsync(c->dR.p, r_device, 3 * c->n * sizeof(double),
                                 hipMemcpyDeviceToDevice, c->stream),
@@ -1043,6 +1058,9 @@ int hdnnp_neighbor_list(hdnnp_ctx* c, double cutoff)
     return guarded([&]() {
         if (!(cutoff > 0.0))
             throw Error(HDNNP_E_INVALID, "ERROR: Cutoff must be positive.\n");
+        if (c && c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: The neighbour list was given "
+                                         "by the caller.\n");
         buildNeighborList(c, cutoff);
     });
 }
@@ -1074,7 +1092,7 @@ int hdnnp_forces(hdnnp_ctx* c, int mode)
 int hdnnp_evaluate(hdnnp_ctx* c, int force_mode, double* energy)
 {
     return guarded([&]() {
-        buildNeighborList(c, c->T.maxCutoff);
+        if (!c->externalNL) buildNeighborList(c, c->T.maxCutoff);
         computeSF(c, false);
         computeNN(c, force_mode >= 0);
         double const e = computeEnergy(c);
@@ -1197,6 +1215,10 @@ int hdnnp_get_atomic_energies(hdnnp_ctx* c, double* e)
 int hdnnp_get_forces(hdnnp_ctx* c, double* f)
 {
     return guarded([&]() {
+        if (c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: Forces of a caller-given "
+                                         "neighbour list are scattered with "
+                                         "hdnnp_get_scattered_forces.\n");
         if (!c->dF.p) throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
         c->setDevice();
         copyOut(c, f, c->dF.p, 3 * (size_t)c->n);
@@ -1205,6 +1227,125 @@ int hdnnp_get_forces(hdnnp_ctx* c, double* f)
 }
 
 const double* hdnnp_forces_device(hdnnp_ctx* c) { return c->dF.p; }
+
+int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
+                            const int32_t* centre_index,
+                            const int32_t* centre_elem,
+                            const int64_t* offsets, const int32_t* nbr_index,
+                            const int32_t* nbr_elem, const double* dr)
+{
+    return guarded([&]() {
+        if (!c || ncentre <= 0 || !centre_elem || !offsets)
+            throw Error(HDNNP_E_INVALID, "ERROR: Empty neighbour list.\n");
+        int64_t const M = offsets[ncentre];
+        if (M < 0 || M > (int64_t)2000000000 || ncentre > 200000000)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: Neighbour list too large.\n");
+        if (M > 0 && (!nbr_index || !nbr_elem || !dr))
+            throw Error(HDNNP_E_INVALID, "ERROR: Missing neighbour data.\n");
+        c->setDevice();
+        int const ne = c->T.ne;
+        int const n = (int)ncentre;
+        // Centres 0..n-1; neighbour entry q is the virtual atom n + q.
+        std::vector<int32_t> elem((size_t)n + (size_t)M);
+        std::vector<int> off(n + 1), cnt(n), nbrJ((size_t)M);
+        std::vector<double4> geo((size_t)M);
+        c->hAtomsOfElem.assign(ne, {});
+        int maxc = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            if (centre_elem[i] < 0 || centre_elem[i] >= ne)
+                throw Error(HDNNP_E_INVALID,
+                            "ERROR: Element index out of range.\n");
+            elem[i] = centre_elem[i];
+            c->hAtomsOfElem[centre_elem[i]].push_back(i);
+            if (offsets[i + 1] < offsets[i])
+                throw Error(HDNNP_E_INVALID, "ERROR: Offsets not ascending.\n");
+            off[i] = (int)offsets[i];
+            cnt[i] = (int)(offsets[i + 1] - offsets[i]);
+            maxc = std::max(maxc, cnt[i]);
+        }
+        off[n] = (int)M;
+        for (int64_t q = 0; q < M; ++q)
+        {
+            if (nbr_elem[q] < 0 || nbr_elem[q] >= ne)
+                throw Error(HDNNP_E_INVALID,
+                            "ERROR: Element index out of range.\n");
+            elem[n + q] = nbr_elem[q];
+            nbrJ[q] = n + (int)q;
+            double const x = dr[3 * q], y = dr[3 * q + 1], z = dr[3 * q + 2];
+            geo[q] = make_double4(x, y, z, std::sqrt(x * x + y * y + z * z));
+        }
+        c->n = n;
+        c->hElem.assign(centre_elem, centre_elem + n);
+        c->periodic = false;
+        c->pbc[0] = c->pbc[1] = c->pbc[2] = 0;
+        auto up = [&](auto& buf, auto const& v) {
+            buf.ensure(std::max<size_t>(v.size(), 1));
+            if (!v.empty())
+                hipCheck(hipMemcpy(buf.p, v.data(), v.size() * sizeof(v[0]),
+                                   hipMemcpyHostToDevice), "neighbour list");
+        };
+        up(c->dElemAtom, elem);
+        up(c->dOff, off);
+        up(c->dCount, cnt);
+        up(c->dNbrJ, nbrJ);
+        up(c->dGeo, geo);
+        c->dAtomsOfElem.resize(ne);
+        for (int e = 0; e < ne; ++e) up(c->dAtomsOfElem[e], c->hAtomsOfElem[e]);
+        c->entryCap = std::max<int64_t>(M, 1);
+        c->M = M;
+        c->Mvalid = true;
+        c->slabLayout = false;
+        c->maxNeigh = maxc;
+        c->extCentre.resize(n);
+        for (int i = 0; i < n; ++i)
+            c->extCentre[i] = centre_index ? centre_index[i] : i;
+        c->extNbr.assign(nbr_index, nbr_index + M);
+        c->externalNL = true;
+        c->hostPosValid = false;
+        c->haveNL = true;
+        c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f)
+{
+    return guarded([&]() {
+        if (!c || !c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: No caller-given neighbour "
+                                         "list.\n");
+        if (!c->dCself.p || !c->dCslot.p)
+            throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
+        c->setDevice();
+        size_t const n = (size_t)c->n, M = (size_t)c->M;
+        std::vector<double> self(3 * n), pair(3 * M);
+        copyOut(c, self.data(), c->dCself.p, 3 * n);
+        if (M) copyOut(c, pair.data(), c->dCslot.p, 3 * M);
+        c->sync();
+        // InterfaceLammps::getForces (InterfaceLammps.cpp:546-581): each
+        // centre's self force, then its pair forces onto the neighbours,
+        // in list order.  Stored sums carry +dE/dG dG/dr.
+        std::vector<int> off(n + 1);
+        copyOut(c, off.data(), c->dOff.p, n + 1);
+        c->sync();
+        for (size_t i = 0; i < n; ++i)
+        {
+            int64_t const a = c->extCentre[i];
+            if (a < 0 || a >= nout)
+                throw Error(HDNNP_E_INVALID, "ERROR: Centre index out of "
+                                             "range.\n");
+            for (int k = 0; k < 3; ++k) f[3 * a + k] -= self[3 * i + k];
+            for (int q = off[i]; q < off[i + 1]; ++q)
+            {
+                int64_t const b = c->extNbr[q];
+                if (b < 0 || b >= nout)
+                    throw Error(HDNNP_E_INVALID, "ERROR: Neighbour index out "
+                                                 "of range.\n");
+                for (int k = 0; k < 3; ++k) f[3 * b + k] -= pair[3 * q + k];
+            }
+        }
+    });
+}
 
 int hdnnp_get_pair_forces(hdnnp_ctx* c, double* self_forces,
                           double* pair_forces)

@@ -60,6 +60,11 @@ _SIGNATURES = [
     ("hdnnp_ctx_set_specialized", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hdnnp_ctx_specialized_active", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    ("hdnnp_set_neighbor_list", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_int32_p, _c_int32_p, _c_int64_p,
+      _c_int32_p, _c_int32_p, _c_double_p]),
+    ("hdnnp_get_scattered_forces", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_double_p]),
     ("hdnnp_ctx_update_model", ctypes.c_int, [ctypes.c_void_p]),
     ("hdnnp_set_structure", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_int32_p, _c_double_p]),
@@ -238,6 +243,32 @@ class Context:
         _check(_lib.hdnnp_set_structure(self._h, r.shape[0], _ptr(r, ctypes.c_double),
                                         _ptr(elem, ctypes.c_int32), bp))
         self.n = r.shape[0]
+
+    def set_neighbor_list(self, centre_elem, offsets, nbr_index, nbr_elem,
+                          dr, centre_index=None):
+        """Caller-given neighbour lists (InterfaceLammps::addNeighbor):
+        centres with element centre_elem, entries offsets[i]:offsets[i+1]
+        with the caller's neighbour index, its element and r_i - r_j."""
+        ce = np.ascontiguousarray(centre_elem, dtype=np.int32)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ni = np.ascontiguousarray(nbr_index, dtype=np.int32)
+        ne = np.ascontiguousarray(nbr_elem, dtype=np.int32)
+        d = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1, 3)
+        ci = (None if centre_index is None
+              else np.ascontiguousarray(centre_index, dtype=np.int32))
+        i32, i64, f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        _check(_lib.hdnnp_set_neighbor_list(
+            self._h, len(ce), _ptr(ci, i32) if ci is not None else None,
+            _ptr(ce, i32), _ptr(off, i64), _ptr(ni, i32), _ptr(ne, i32),
+            _ptr(d, f64)))
+        self.n = len(ce)
+
+    def scattered_forces(self, nout):
+        """Forces on the caller's nout atoms from a caller-given list."""
+        f = np.zeros((nout, 3))
+        _check(_lib.hdnnp_get_scattered_forces(self._h, nout,
+                                               _ptr(f, ctypes.c_double)))
+        return f
 
     def set_positions(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)

@@ -136,9 +136,12 @@ protected:
     std::vector<double> energyOffset;      // per element (atom_energy)
     std::vector<std::size_t> numSF;        // per element
 
+protected:
+    // Creates the engine context on first use (after the weights are set).
+    void ensureEngine() const;
+
 private:
     void upload(Structure& structure) const;
-    void ensureEngine() const;
     void buildModel();
 
     std::string settingsFile;
