@@ -139,6 +139,32 @@ int  hdnnp_ctx_update_model(hdnnp_ctx* c);
  * reads (Structure.h:67-118). */
 int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
                         const int32_t* elem, const double* box);
+/* Loads a batch of nstruct independent structures, evaluated together by
+ * every following call.  Replaces the per-structure loop
+ *   for (Structure& s : structures) mode.evaluateNNP(s, useForces, true);
+ * of nnp-dataset.cpp:206-211 (Mode::evaluateNNP, Mode.cpp:1967-1990) with
+ * one launch per kernel over all atoms.  Structure s holds the atoms
+ * [atom_offsets[s], atom_offsets[s+1]) of the flat arrays r (n x 3) and
+ * elem (n), n = atom_offsets[nstruct]; atom_offsets[0] = 0 and every
+ * structure has at least one atom.  boxes (9 per structure, lattice vectors
+ * as rows) may be NULL for a batch of open structures; periodic (nstruct,
+ * may be NULL = all periodic when boxes is given) marks which structures
+ * use their box.  Each structure gets its own neighbour list with its own
+ * image counts, exactly as Structure::calculateNeighborList would build it.
+ * hdnnp_energy / hdnnp_evaluate return the batch total; per-structure
+ * energies come from hdnnp_get_structure_energies; forces, G, dE/dG and
+ * the neighbour list are over the flat atom index. */
+int hdnnp_set_structures(hdnnp_ctx* c, int64_t nstruct,
+                         const int64_t* atom_offsets, const double* r,
+                         const int32_t* elem, const double* boxes,
+                         const int32_t* periodic);
+/* Number of structures loaded (1 after hdnnp_set_structure). */
+int64_t hdnnp_num_structures(const hdnnp_ctx* c);
+/* Structure::energy of every structure (nstruct doubles) after the
+ * networks; each is the sum of its atoms' energies in a fixed order. */
+int hdnnp_get_structure_energies(hdnnp_ctx* c, double* energies);
+/* Structure::pbc of every structure (3 x nstruct) of the last list. */
+int hdnnp_get_structure_pbc(hdnnp_ctx* c, int32_t* pbc);
 /* New positions for the same atoms (host array, n x 3). */
 int hdnnp_set_positions(hdnnp_ctx* c, const double* r);
 /* New positions already on the device (n x 3 doubles); no copy to host. */

@@ -129,6 +129,17 @@ struct hdnnp_ctx
     std::vector<DBuf<int>> dAtomsOfElem;
     DBuf<double> dR;
     DBuf<int> dElemAtom;
+    // Batch of independent structures (hdnnp_set_structures): atoms of
+    // structure s are [structOff[s], structOff[s+1]) of the flat arrays.
+    bool batched = false;
+    int64_t nstruct = 0;
+    std::vector<int> hStructOff;
+    std::vector<double> hBoxes;     // 9 per structure (zero if open)
+    std::vector<char> hPeriodic;
+    std::vector<int> hPbcS;         // 3 per structure
+    DBuf<int> dStructOf, dStructOff;
+    DBuf<BatchCell> dCells;
+    DBuf<double> dEnergySeg;
 
     // neighbour list
     NLParams nl{};
@@ -360,10 +371,125 @@ void cellWidths(double const b[3][3], double rc, double proj[3], int pbc[3])
     }
 }
 
+// Neighbour lists of a batch of structures, one launch per pass.  Each
+// structure's image counts follow its own cell (cellWidths, as
+// Structure.cpp:287-300); open structures use none.  The list uses fixed
+// per-atom slot ranges like the cell path: a rebuild of the same batch is
+// one fill pass; the first build (or an overflow) counts first.
+void buildBatchNeighborList(hdnnp_ctx* c, double rc)
+{
+    int const n = (int)c->n;
+    int64_t const ns = c->nstruct;
+    hipStream_t s = c->stream;
+    c->hPbcS.assign(3 * ns, 0);
+    std::vector<BatchCell> cells((size_t)ns);
+    for (int64_t st = 0; st < ns; ++st)
+    {
+        BatchCell& C = cells[st];
+        C = BatchCell{};
+        if (!c->hPeriodic[st]) continue;
+        C.periodic = 1;
+        for (int k = 0; k < 3; ++k)
+            for (int q = 0; q < 3; ++q) C.box[k][q] = c->hBoxes[9 * st + 3 * k + q];
+        inverseBox(C.box, C.invbox);
+        cellWidths(C.box, rc, C.proj, C.pbc);
+        int64_t nimg = 1;
+        for (int k = 0; k < 3; ++k)
+        {
+            if (C.pbc[k] > 60)
+                throw Error(HDNNP_E_CAPACITY,
+                            "ERROR: Cell much smaller than the cutoff "
+                            "(more than 60 periodic images per "
+                            "direction).\n");
+            nimg *= 2 * C.pbc[k] + 1;
+            c->hPbcS[3 * st + k] = C.pbc[k];
+        }
+        int64_t const na = c->hStructOff[st + 1] - c->hStructOff[st];
+        if (na * nimg > ((int64_t)1 << 24))
+            throw Error(HDNNP_E_CAPACITY,
+                        "ERROR: Too many atoms times periodic images in one "
+                        "structure of the batch (limit 2^24).\n");
+    }
+    hipEvent_t ev0 = nullptr;
+    c->begin(K_NEIGHBOR, ev0);
+    c->dCells.ensure(ns);
+    hipCheck(hipMemcpyAsync(c->dCells.p, cells.data(), ns * sizeof(BatchCell),
+                            hipMemcpyHostToDevice, s), "cells");
+    hipCheck(launch_batch_extent((int)ns, c->dR.p, c->dStructOff.p,
+                                 c->dCells.p, s), "batch extent");
+    c->dCount.ensure(n);
+    c->dOff.ensure(n + 1);
+    c->dStats.ensure(4);
+    c->dBlockSums.ensure((n + 1023) / 1024 + 1);
+    c->dTotal.ensure(1);
+    hipCheck(hipMemsetAsync(c->dStats.p, 0, 4 * sizeof(int), s), "memset");
+    int hstat[4] = {0, 0, 0, 0};
+    auto readStats = [&]() {
+        hipCheck(hipMemcpyAsync(hstat, c->dStats.p, 4 * sizeof(int),
+                                hipMemcpyDeviceToHost, s), "stats");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        hipCheck(hipMemsetAsync(c->dStats.p, 0, 4 * sizeof(int), s),
+                 "memset");
+    };
+    auto fill = [&]() {
+        if ((int64_t)n * c->slab > (int64_t)2000000000)
+            throw Error(HDNNP_E_CAPACITY,
+                        "ERROR: Neighbour list too large.\n");
+        int64_t const E = std::max<int64_t>((int64_t)n * c->slab, 1);
+        c->dKeys.ensure(E);
+        c->dNbrJ.ensure(E);
+        c->dGeo.ensure(E);
+        c->dPartner.ensure(E);
+        c->entryCap = E;
+        hipCheck(launch_slab_starts(n, c->slab, c->dOff.p, s), "slab starts");
+        hipCheck(launch_nl_batch(1, n, rc, c->dR.p, c->dStructOf.p,
+                                 c->dStructOff.p, c->dCells.p, c->dCount.p,
+                                 c->dStats.p, c->dOff.p, c->slab, c->dKeys.p,
+                                 c->dNbrJ.p, c->dGeo.p, s), "batch nl fill");
+        readStats();
+    };
+    bool built = false;
+    if (c->slab > 0 && c->slabAtoms == n)
+    {
+        fill();
+        built = hstat[3] == 0;
+    }
+    if (!built)
+    {
+        hipCheck(launch_nl_batch(0, n, rc, c->dR.p, c->dStructOf.p,
+                                 c->dStructOff.p, c->dCells.p, c->dCount.p,
+                                 nullptr, nullptr, 0, nullptr, nullptr,
+                                 nullptr, s), "batch nl count");
+        hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "count max");
+        readStats();
+        int const maxc = hstat[0];
+        c->slab = ((maxc + maxc / 8 + 16) + 15) / 16 * 16;
+        c->slabAtoms = n;
+        fill();
+        if (hstat[3] != 0)
+            throw Error(HDNNP_E_DEVICE,
+                        "ERROR: Neighbour count changed between passes.\n");
+    }
+    hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
+                               c->dCellAtoms.p, c->dPartner.p, s),
+             "nl partner");
+    c->end(K_NEIGHBOR, ev0);
+    c->slabLayout = true;
+    c->Mvalid = false;
+    c->maxNeigh = std::min(hstat[0], c->slab);
+    c->haveNL = true;
+    c->haveSF = c->haveNN = c->haveNNDeriv = false;
+}
+
 void buildNeighborList(hdnnp_ctx* c, double rc)
 {
     requireStructure(c);
     c->setDevice();
+    if (c->batched)
+    {
+        buildBatchNeighborList(c, rc);
+        return;
+    }
     NLParams& p = c->nl;
     p = NLParams{};
     p.n = (int)c->n;
@@ -971,27 +1097,54 @@ int hdnnp_ctx_update_model(hdnnp_ctx* c)
     });
 }
 
+// Atoms, elements and per-element atom lists of a structure or a batch.
+static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* elem)
+{
+    if (n <= 0 || !r || !elem)
+        throw Error(HDNNP_E_INVALID, "ERROR: Empty structure.\n");
+    if (n > 200000000)
+        throw Error(HDNNP_E_CAPACITY, "ERROR: Too many atoms.\n");
+    c->setDevice();
+    int const ne = c->T.ne;
+    c->n = n;
+    c->hElem.assign(elem, elem + n);
+    c->hAtomsOfElem.assign(ne, {});
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (elem[i] < 0 || elem[i] >= ne)
+            throw Error(HDNNP_E_INVALID,
+                        "ERROR: Element index out of range.\n");
+        c->hAtomsOfElem[elem[i]].push_back((int)i);
+    }
+    c->externalNL = false;
+    c->dR.ensure(3 * n);
+    c->dElemAtom.ensure(n);
+    hipCheck(hipMemcpy(c->dR.p, r, 3 * n * sizeof(double),
+                       hipMemcpyHostToDevice), "positions");
+    hipCheck(hipMemcpy(c->dElemAtom.p, elem, n * sizeof(int32_t),
+                       hipMemcpyHostToDevice), "elements");
+    c->hR.assign(r, r + 3 * n);
+    c->hostPosValid = true;
+    c->dAtomsOfElem.resize(ne);
+    for (int e = 0; e < ne; ++e)
+    {
+        c->dAtomsOfElem[e].ensure(c->hAtomsOfElem[e].size());
+        if (!c->hAtomsOfElem[e].empty())
+            hipCheck(hipMemcpy(c->dAtomsOfElem[e].p,
+                               c->hAtomsOfElem[e].data(),
+                               c->hAtomsOfElem[e].size() * sizeof(int),
+                               hipMemcpyHostToDevice), "element lists");
+    }
+    c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+}
+
 int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
                         const int32_t* elem, const double* box)
 {
     return guarded([&]() {
-        if (n <= 0 || !r || !elem)
-            throw Error(HDNNP_E_INVALID, "ERROR: Empty structure.\n");
-        if (n > 200000000)
-            throw Error(HDNNP_E_CAPACITY, "ERROR: Too many atoms.\n");
-        c->setDevice();
-        int const ne = c->T.ne;
-        c->n = n;
-        c->hElem.assign(elem, elem + n);
-        c->hAtomsOfElem.assign(ne, {});
-        for (int64_t i = 0; i < n; ++i)
-        {
-            if (elem[i] < 0 || elem[i] >= ne)
-                throw Error(HDNNP_E_INVALID,
-                            "ERROR: Element index out of range.\n");
-            c->hAtomsOfElem[elem[i]].push_back((int)i);
-        }
-        c->externalNL = false;
+        loadAtoms(c, n, r, elem);
+        c->batched = false;
+        c->nstruct = 1;
         c->periodic = box != nullptr;
         if (box)
         {
@@ -999,25 +1152,92 @@ int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
                 for (int j = 0; j < 3; ++j) c->box[i][j] = box[3 * i + j];
             inverseBox(c->box, c->invbox);
         }
-        c->dR.ensure(3 * n);
-        c->dElemAtom.ensure(n);
-        hipCheck(hipMemcpy(c->dR.p, r, 3 * n * sizeof(double),
-                           hipMemcpyHostToDevice), "positions");
-        hipCheck(hipMemcpy(c->dElemAtom.p, elem, n * sizeof(int32_t),
-                           hipMemcpyHostToDevice), "elements");
-        c->hR.assign(r, r + 3 * n);
-        c->hostPosValid = true;
-        c->dAtomsOfElem.resize(ne);
-        for (int e = 0; e < ne; ++e)
+    });
+}
+
+int hdnnp_set_structures(hdnnp_ctx* c, int64_t nstruct,
+                         const int64_t* atom_offsets, const double* r,
+                         const int32_t* elem, const double* boxes,
+                         const int32_t* periodic)
+{
+    return guarded([&]() {
+        if (nstruct <= 0 || !atom_offsets)
+            throw Error(HDNNP_E_INVALID, "ERROR: Empty batch.\n");
+        if (atom_offsets[0] != 0)
+            throw Error(HDNNP_E_INVALID,
+                        "ERROR: atom_offsets[0] must be 0.\n");
+        for (int64_t s = 0; s < nstruct; ++s)
+            if (atom_offsets[s + 1] <= atom_offsets[s])
+                throw Error(HDNNP_E_INVALID,
+                            "ERROR: Every structure of a batch needs at "
+                            "least one atom (offsets must increase).\n");
+        int64_t const n = atom_offsets[nstruct];
+        loadAtoms(c, n, r, elem);
+        c->batched = true;
+        c->nstruct = nstruct;
+        c->periodic = false;
+        c->hStructOff.assign(atom_offsets, atom_offsets + nstruct + 1);
+        c->hBoxes.assign(9 * nstruct, 0.0);
+        c->hPeriodic.assign(nstruct, 0);
+        for (int64_t s = 0; s < nstruct; ++s)
         {
-            c->dAtomsOfElem[e].ensure(c->hAtomsOfElem[e].size());
-            if (!c->hAtomsOfElem[e].empty())
-                hipCheck(hipMemcpy(c->dAtomsOfElem[e].p,
-                                   c->hAtomsOfElem[e].data(),
-                                   c->hAtomsOfElem[e].size() * sizeof(int),
-                                   hipMemcpyHostToDevice), "element lists");
+            bool const per = boxes && (!periodic || periodic[s]);
+            c->hPeriodic[s] = per ? 1 : 0;
+            if (per)
+                for (int q = 0; q < 9; ++q)
+                    c->hBoxes[9 * s + q] = boxes[9 * s + q];
         }
-        c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+        std::vector<int> structOf((size_t)n), iota((size_t)n);
+        for (int64_t s = 0; s < nstruct; ++s)
+            for (int64_t i = atom_offsets[s]; i < atom_offsets[s + 1]; ++i)
+                structOf[i] = (int)s;
+        for (int64_t i = 0; i < n; ++i) iota[i] = (int)i;
+        c->dStructOf.ensure(n);
+        c->dStructOff.ensure(nstruct + 1);
+        c->dCellAtoms.ensure(n);
+        hipCheck(hipMemcpy(c->dStructOf.p, structOf.data(), n * sizeof(int),
+                           hipMemcpyHostToDevice), "structure index");
+        hipCheck(hipMemcpy(c->dStructOff.p, c->hStructOff.data(),
+                           (nstruct + 1) * sizeof(int),
+                           hipMemcpyHostToDevice), "structure offsets");
+        // Atoms are gathered in their own order.
+        hipCheck(hipMemcpy(c->dCellAtoms.p, iota.data(), n * sizeof(int),
+                           hipMemcpyHostToDevice), "atom order");
+    });
+}
+
+int64_t hdnnp_num_structures(const hdnnp_ctx* c) { return c ? c->nstruct : 0; }
+
+int hdnnp_get_structure_energies(hdnnp_ctx* c, double* energies)
+{
+    return guarded([&]() {
+        if (!c->haveNN)
+            throw Error(HDNNP_E_INVALID, "ERROR: Networks not evaluated.\n");
+        c->setDevice();
+        if (!c->batched)
+        {
+            energies[0] = computeEnergy(c);
+            return;
+        }
+        c->dEnergySeg.ensure(c->nstruct);
+        hipCheck(launch_energy_segments(c->dE.p, c->dStructOff.p,
+                                        (int)c->nstruct, c->dEnergySeg.p,
+                                        c->stream), "structure energies");
+        copyOut(c, energies, c->dEnergySeg.p, (size_t)c->nstruct);
+        c->sync();
+    });
+}
+
+int hdnnp_get_structure_pbc(hdnnp_ctx* c, int32_t* pbc)
+{
+    return guarded([&]() {
+        requireNL(c);
+        if (!c->batched)
+        {
+            for (int k = 0; k < 3; ++k) pbc[k] = c->pbc[k];
+            return;
+        }
+        for (int64_t q = 0; q < 3 * c->nstruct; ++q) pbc[q] = c->hPbcS[q];
     });
 }
 
@@ -1118,6 +1338,10 @@ int hdnnp_get_pbc(hdnnp_ctx* c, int32_t* pbc)
 {
     return guarded([&]() {
         requireNL(c);
+        if (c->batched)
+            throw Error(HDNNP_E_INVALID, "ERROR: A batch has image counts "
+                                         "per structure "
+                                         "(hdnnp_get_structure_pbc).\n");
         for (int k = 0; k < 3; ++k) pbc[k] = c->pbc[k];
     });
 }
@@ -1302,6 +1526,8 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
             c->extCentre[i] = centre_index ? centre_index[i] : i;
         c->extNbr.assign(nbr_index, nbr_index + M);
         c->externalNL = true;
+        c->batched = false;
+        c->nstruct = 1;
         c->hostPosValid = false;
         c->haveNL = true;
         c->haveSF = c->haveNN = c->haveNNDeriv = false;

@@ -57,6 +57,32 @@ hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
                                const int* cellAtoms, int* count, int* stats,
                                const int* off, int limit, uint64_t* keys,
                                int* nbrJ, double4* geo, hipStream_t s);
+// Cell of one structure of a batch.  The host sets everything but the
+// atoms' fractional extent fmin/fmax, which launch_batch_extent fills.
+struct BatchCell
+{
+    double box[3][3];      // lattice vectors as rows (zero if open)
+    double invbox[3][3];   // its inverse: fractional f = r invbox
+    double proj[3];        // distance between opposite faces
+    double fmin[3], fmax[3];
+    int    pbc[3];         // images searched per direction
+    int    periodic;
+};
+hipError_t launch_batch_extent(int nstruct, const double* r,
+                               const int* structOff, BatchCell* cells,
+                               hipStream_t s);
+// Batched structures: per-atom lists within each atom's structure
+// (structOf, structOff) over its cell's images.  mode 0 counts into count;
+// mode 1 writes at most `limit` entries per atom from off, stores count,
+// raises stats[0] to the largest count and sets stats[3] on overflow.
+hipError_t launch_nl_batch(int mode, int n, double rc, const double* r,
+                           const int* structOf, const int* structOff,
+                           const BatchCell* cells, int* count, int* stats,
+                           const int* off, int limit, uint64_t* keys,
+                           int* nbrJ, double4* geo, hipStream_t s);
+// Per-structure energy sums (one block per structure, fixed order).
+hipError_t launch_energy_segments(const double* E, const int* structOff,
+                                  int nstruct, double* out, hipStream_t s);
 // off[i] = i * slab for i <= n.
 hipError_t launch_slab_starts(int n, int slab, int* off, hipStream_t s);
 // Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted

@@ -477,6 +477,170 @@ __global__ __launch_bounds__(256) void k_nl_cell(
     }
 }
 
+// Fractional extent of each periodic structure's atoms along its three
+// lattice directions (f = r invbox): the bounds the image pruning of
+// k_nl_batch uses.  One wave per structure.
+__global__ __launch_bounds__(64) void k_batch_extent(
+    int nstruct, const double* __restrict__ r,
+    const int* __restrict__ structOff, BatchCell* __restrict__ cells)
+{
+    int const st = blockIdx.x;
+    if (st >= nstruct) return;
+    BatchCell& C = cells[st];
+    if (!C.periodic) return;
+    int const lane = threadIdx.x;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = structOff[st] + lane; i < structOff[st + 1]; i += 64)
+    {
+        double const x = r[3 * (size_t)i], y = r[3 * (size_t)i + 1],
+                     z = r[3 * (size_t)i + 2];
+        for (int k = 0; k < 3; ++k)
+        {
+            double const f = x * C.invbox[0][k] + y * C.invbox[1][k]
+                           + z * C.invbox[2][k];
+            lo[k] = fmin(lo[k], f);
+            hi[k] = fmax(hi[k], f);
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+    {
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], o));
+        }
+        if (lane == 0)
+        {
+            C.fmin[k] = lo[k];
+            C.fmax[k] = hi[k];
+        }
+    }
+}
+
+// Image range [lo, hi] along one lattice direction that can hold a
+// neighbour of an atom at fractional coordinate f.  The image b holds the
+// neighbours' copies at fractional [fmin - b, fmax - b]; its distance from
+// the atom along the plane normal, (gap in f) * proj, bounds |dr| from
+// below.  An image is kept unless that bound exceeds rc by a relative
+// margin far above rounding, so the pruned search finds every entry.
+__device__ __forceinline__ void image_range(double f, double fmin,
+                                            double fmax, double proj,
+                                            double rcm, int p, int& lo,
+                                            int& hi)
+{
+    lo = p + 1;
+    hi = -p - 1;
+    for (int b = -p; b <= p; ++b)
+    {
+        double gap = 0.0;
+        if (f < fmin - b) gap = (fmin - b - f) * proj;
+        else if (f > fmax - b) gap = (f - fmax + b) * proj;
+        if (gap <= rcm)
+        {
+            lo = min(lo, b);
+            hi = max(hi, b);
+        }
+    }
+}
+
+// Batched structures: atom i of structure s = structOf[i] searches the
+// atoms [structOff[s], structOff[s+1]) of its own structure over the
+// structure's periodic images |b_k| <= pbc_k (Structure.cpp:287-399),
+// candidates in the reference's loop order (j, then b0, b1, b2), 64 per
+// step; the ballot compaction therefore yields the list in order.  Only
+// the images image_range keeps are enumerated, which drops no entry and
+// keeps the order.  MODE 0 counts into count[i].  MODE 1 writes at most
+// `limit` entries from off[i], stores the count, raises stats[0] to it and
+// sets stats[3] when it exceeds the limit.  One wave per atom.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_nl_batch(
+    int n, double rc2, double rcm, const double* __restrict__ r,
+    const int* __restrict__ structOf, const int* __restrict__ structOff,
+    const BatchCell* __restrict__ cells, int* __restrict__ count,
+    int* __restrict__ stats, const int* __restrict__ off, int limit,
+    uint64_t* __restrict__ keysOut, int* __restrict__ nbrJ,
+    double4* __restrict__ geo)
+{
+    int const i = blockIdx.x;
+    if (i >= n) return;
+    int const lane = threadIdx.x;
+    int const st = structOf[i];
+    BatchCell const& C = cells[st];
+    int const a0 = structOff[st], na = structOff[st + 1] - a0;
+    NLParams p{};
+    for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c) p.box[k][c] = C.box[k][c];
+    double const ri[3] = {r[3 * (size_t)i], r[3 * (size_t)i + 1],
+                          r[3 * (size_t)i + 2]};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (C.periodic)
+        for (int k = 0; k < 3; ++k)
+        {
+            double const f = ri[0] * C.invbox[0][k] + ri[1] * C.invbox[1][k]
+                           + ri[2] * C.invbox[2][k];
+            image_range(f, C.fmin[k], C.fmax[k], C.proj[k], rcm, C.pbc[k],
+                        lo[k], hi[k]);
+        }
+    // The atom's own home image is always within range (gap 0), so every
+    // range is non-empty.
+    int const w1 = hi[1] - lo[1] + 1, w2 = hi[2] - lo[2] + 1;
+    int const w12 = w1 * w2;
+    int const nimg = (hi[0] - lo[0] + 1) * w12;
+    float const invImg = 1.0f / (float)nimg, invW12 = 1.0f / (float)w12,
+                invW2 = 1.0f / (float)w2;
+    int const total = na * nimg;
+    int const o = MODE == 1 ? off[i] : 0;
+    int hits = 0;
+    for (int base = 0; base < total; base += 64)
+    {
+        int const t = base + lane;
+        bool hit = false;
+        int j = 0, b0 = 0, b1 = 0, b2 = 0;
+        double dr[3] = {0.0, 0.0, 0.0};
+        if (t < total)
+        {
+            int jj, bi, q0, r0, q1, r2;
+            divmod_small(t, nimg, invImg, jj, bi);
+            divmod_small(bi, w12, invW12, q0, r0);
+            divmod_small(r0, w2, invW2, q1, r2);
+            j = a0 + jj;
+            b0 = lo[0] + q0;
+            b1 = lo[1] + q1;
+            b2 = lo[2] + r2;
+            if (!(j == i && b0 == 0 && b1 == 0 && b2 == 0))
+            {
+                double const rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                                      r[3 * (size_t)j + 2]};
+                image_dr(p, ri, rj, b0, b1, b2, dr);
+                hit = norm2_pinned(dr) <= rc2;
+            }
+        }
+        uint64_t const mask = __ballot(hit);
+        if (MODE == 1 && hit)
+        {
+            int const h = hits + lanes_below(mask);
+            if (h < limit)
+            {
+                int const q = o + h;
+                keysOut[q] = make_key(j, b0, b1, b2);
+                nbrJ[q] = j;
+                geo[q] = make_double4(dr[0], dr[1], dr[2],
+                                      sqrt(norm2_pinned(dr)));
+            }
+        }
+        hits += __popcll(mask);
+    }
+    if (lane == 0)
+    {
+        count[i] = hits;
+        if (MODE == 1)
+        {
+            atomicMax(&stats[0], hits);
+            if (hits > limit) stats[3] = 1;
+        }
+    }
+}
+
 // off[i] = i * slab: atom i's entries start at a fixed slot range.
 __global__ void k_slab_starts(int n, int slab, int* __restrict__ off)
 {
@@ -667,6 +831,34 @@ hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
     k_nl_cell<1><<<ncell, 256, 0, s>>>(p, r, cellStart, cellCount, cellAtoms,
                                        count, stats, off, limit, keys, nbrJ,
                                        geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_extent(int nstruct, const double* r,
+                               const int* structOff, BatchCell* cells,
+                               hipStream_t s)
+{
+    if (nstruct <= 0) return hipSuccess;
+    k_batch_extent<<<nstruct, 64, 0, s>>>(nstruct, r, structOff, cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_nl_batch(int mode, int n, double rc, const double* r,
+                           const int* structOf, const int* structOff,
+                           const BatchCell* cells, int* count, int* stats,
+                           const int* off, int limit, uint64_t* keys,
+                           int* nbrJ, double4* geo, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    double const rc2 = rc * rc, rcm = rc * (1.0 + 1e-6);
+    if (mode == 0)
+        k_nl_batch<0><<<n, 64, 0, s>>>(n, rc2, rcm, r, structOf, structOff,
+                                       cells, count, nullptr, nullptr, 0,
+                                       nullptr, nullptr, nullptr);
+    else
+        k_nl_batch<1><<<n, 64, 0, s>>>(n, rc2, rcm, r, structOf, structOff,
+                                       cells, count, stats, off, limit, keys,
+                                       nbrJ, geo);
     return hipGetLastError();
 }
 

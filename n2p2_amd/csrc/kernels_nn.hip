@@ -348,7 +348,37 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ E,
     if (t == 0) *out = s[0];
 }
 
+// Energy of each structure of a batch: one block per structure, the same
+// fixed-order tree as k_energy over the structure's atoms.
+__global__ __launch_bounds__(256) void k_energy_seg(
+    const double* __restrict__ E, const int* __restrict__ structOff,
+    double* __restrict__ out)
+{
+    __shared__ double s[256];
+    int const st = blockIdx.x;
+    int const t = threadIdx.x;
+    int const a0 = structOff[st], a1 = structOff[st + 1];
+    double acc = 0.0;
+    for (int i = a0 + t; i < a1; i += 256) acc += E[i];
+    s[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (t < o) s[t] += s[t + o];
+        __syncthreads();
+    }
+    if (t == 0) out[st] = s[0];
+}
+
 }  // namespace
+
+hipError_t launch_energy_segments(const double* E, const int* structOff,
+                                  int nstruct, double* out, hipStream_t s)
+{
+    if (nstruct <= 0) return hipSuccess;
+    k_energy_seg<<<nstruct, 256, 0, s>>>(E, structOff, out);
+    return hipGetLastError();
+}
 
 size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode)
 {

@@ -68,6 +68,12 @@ _SIGNATURES = [
     ("hdnnp_ctx_update_model", ctypes.c_int, [ctypes.c_void_p]),
     ("hdnnp_set_structure", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_int32_p, _c_double_p]),
+    ("hdnnp_set_structures", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_int64_p, _c_double_p, _c_int32_p,
+      _c_double_p, _c_int32_p]),
+    ("hdnnp_num_structures", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_get_structure_energies", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_get_structure_pbc", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_set_positions", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("hdnnp_set_positions_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("hdnnp_neighbor_list", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
@@ -243,6 +249,46 @@ class Context:
         _check(_lib.hdnnp_set_structure(self._h, r.shape[0], _ptr(r, ctypes.c_double),
                                         _ptr(elem, ctypes.c_int32), bp))
         self.n = r.shape[0]
+
+    def set_structures(self, structures):
+        """A batch of structures [(r, elem, box or None), ...] evaluated
+        together (the nnp-dataset loop over Mode::evaluateNNP).  Results
+        are over the concatenated atoms; structure_energies() splits the
+        energy."""
+        rs, es, boxes, per, off = [], [], [], [], [0]
+        for r, e, box in structures:
+            r = np.asarray(r, dtype=np.float64).reshape(-1, 3)
+            rs.append(r)
+            es.append(np.asarray(e, dtype=np.int32))
+            boxes.append(np.zeros((3, 3)) if box is None
+                         else np.asarray(box, dtype=np.float64).reshape(3, 3))
+            per.append(0 if box is None else 1)
+            off.append(off[-1] + r.shape[0])
+        r = np.ascontiguousarray(np.concatenate(rs))
+        elem = np.ascontiguousarray(np.concatenate(es))
+        bx = np.ascontiguousarray(np.stack(boxes))
+        per = np.ascontiguousarray(per, dtype=np.int32)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        self._r, self._e, self._box = r, elem, bx
+        _check(_lib.hdnnp_set_structures(
+            self._h, len(rs), _ptr(off, ctypes.c_int64),
+            _ptr(r, ctypes.c_double), _ptr(elem, ctypes.c_int32),
+            _ptr(bx, ctypes.c_double), _ptr(per, ctypes.c_int32)))
+        self.n = r.shape[0]
+        self.struct_offsets = off
+
+    def num_structures(self):
+        return _lib.hdnnp_num_structures(self._h)
+
+    def structure_energies(self):
+        e = np.zeros(self.num_structures())
+        _check(_lib.hdnnp_get_structure_energies(self._h, _ptr(e, ctypes.c_double)))
+        return e
+
+    def structure_pbc(self):
+        p = np.zeros((self.num_structures(), 3), dtype=np.int32)
+        _check(_lib.hdnnp_get_structure_pbc(self._h, _ptr(p, ctypes.c_int32)))
+        return p
 
     def set_neighbor_list(self, centre_elem, offsets, nbr_index, nbr_elem,
                           dr, centre_index=None):

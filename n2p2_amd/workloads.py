@@ -210,6 +210,24 @@ def random_structure(natoms, density=0.08, dmin=1.8, fractions=(0.4, 0.4, 0.2),
     return remap(pts, box), elem, box
 
 
+def batch_structures(nstruct=512, lo=16, hi=512, seed=2024):
+    """Batch of independent three-element periodic structures with atom
+    counts drawn uniformly from [lo, hi] (random_structure each: density
+    0.08 atoms/A^3, fractions 0.4/0.4/0.2).  Returns [(r, elem, box), ...]."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, nstruct)
+    return [random_structure(int(n), seed=seed * 1000 + k)
+            for k, n in enumerate(sizes)]
+
+
+def sheared(r, box, shear):
+    """The structure under the linear map x -> x @ shear (rows of box are
+    lattice vectors): fractional coordinates, hence the remap into the
+    cell, are unchanged."""
+    S = np.asarray(shear, dtype=np.float64)
+    return r @ S, box @ S
+
+
 WATER_CONFIGS = {
     "water192": (64, 12.42),
     "water24576": (8192, 62.1),
