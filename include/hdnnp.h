@@ -256,6 +256,37 @@ int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f);
 /* Device pointer of the n x 3 force array (valid until the next
  * hdnnp_set_structure); for callers that keep MD state on the GPU. */
 const double* hdnnp_forces_device(hdnnp_ctx* c);
+/* ---------------------------------------------------------------- MD -- */
+
+/* Velocity-Verlet NVE on the device.  No counterpart in libnnp: replaces
+ * the loop LAMMPS runs around pair_style hdnnp (fix nve with
+ * pair_hdnnp.cpp:81-114 computing forces each step) and its neighbour
+ * policy "neighbor <skin> bin; neigh_modify every 1 delay 0 check yes".
+ * The list is built with rc + skin and rebuilt (atoms wrapped into the
+ * cell) when an atom has moved more than skin / 2 since the last build;
+ * every step filters it to the entries within rc, in the reference's
+ * order, so each force evaluation sees the list a fresh build would give.
+ * Each step is two HIP graphs (kick + drift + filter; forces + kick +
+ * energy record) with a 2-int flag read in between; use_graph = 0 issues
+ * the same kernels one by one.
+ *
+ * hdnnp_md_init: the loaded periodic structure, masses per element,
+ * time step dt, skin, ftoa = acceleration per (force / mass) in the
+ * caller's units (eV/A/amu with dt in fs: 9.648533212e-3), initial
+ * velocities v (n x 3, NULL = zero), force mode (HDNNP_FORCES_*).  It
+ * computes the forces at the initial positions.  hdnnp_set_structure,
+ * hdnnp_set_positions*, hdnnp_neighbor_list, hdnnp_evaluate and
+ * hdnnp_set_neighbor_list end the run. */
+int hdnnp_md_init(hdnnp_ctx* c, const double* masses, double dt, double skin,
+                  double ftoa, const double* v, int force_mode, int use_graph);
+/* nsteps steps.  epot[k] (the model's energy) and ekin[k] = sum m v^2 / 2
+ * / ftoa after step k, each may be NULL. */
+int hdnnp_md_run(hdnnp_ctx* c, int64_t nsteps, double* epot, double* ekin);
+/* Positions (unwrapped since the last rebuild) and velocities, n x 3. */
+int hdnnp_md_get_state(hdnnp_ctx* c, double* r, double* v);
+/* Neighbour-list builds since hdnnp_md_init (the first one included). */
+int64_t hdnnp_md_rebuilds(const hdnnp_ctx* c);
+
 /* Number of extrapolation warnings of the last SF evaluation: G outside
  * [Gmin, Gmax] +- 1000 eps before scaling (Element.cpp:462-541).
  * per_atom (n, may be NULL) receives the count per atom. */

@@ -190,6 +190,29 @@ struct hdnnp_ctx
         return plan[fp32];
     }
 
+    // MD with a Verlet-skin list (hdnnp_md_*).  The skin list (rc + skin)
+    // lives in skinKeys / skinPartner / skinCnt; every step filters it into
+    // the ordinary list buffers.  Graph A: kick, drift, filter, flags to
+    // the host.  Graph B: the force evaluation, kick, energy record.
+    struct MDState
+    {
+        bool on = false;
+        double dt = 0.0, skin = 0.0, rc = 0.0, ftoa = 0.0;
+        int forceMode = HDNNP_FORCES_CONSISTENT;
+        bool useGraph = true;
+        int cap = 0;
+        int64_t rebuilds = 0, steps = 0;
+        MDCell cell{};
+        DBuf<double> v, r0, accel, mass, part, epot, ekin;
+        DBuf<int> flags, counter, skinPartner, skinCnt, src, newIdx;
+        DBuf<uint64_t> skinKeys;
+        int* hFlags = nullptr;
+        int recCap = 0;
+        hipGraph_t gA = nullptr, gB = nullptr;
+        hipGraphExec_t eA = nullptr, eB = nullptr;
+    } md;
+    bool capturing = false;
+
     // profiling
     bool prof = false;
     struct Pending { int cls; hipEvent_t a, b; };
@@ -213,13 +236,13 @@ struct hdnnp_ctx
     void begin(int cls, hipEvent_t& a)
     {
         (void)cls;
-        if (!prof) return;
+        if (!prof || capturing) return;
         a = getEvent();
         hipCheck(hipEventRecord(a, stream), "hipEventRecord");
     }
     void end(int cls, hipEvent_t a)
     {
-        if (!prof) return;
+        if (!prof || capturing) return;
         hipEvent_t b = getEvent();
         hipCheck(hipEventRecord(b, stream), "hipEventRecord");
         pending.push_back(Pending{cls, a, b});
@@ -802,14 +825,19 @@ void computeNN(hdnnp_ctx* c, bool deriv)
     c->haveNNDeriv = deriv;
 }
 
+void launchEnergy(hdnnp_ctx* c)
+{
+    c->dEnergy.ensure(1);
+    hipCheck(launch_energy_sum(c->dE.p, (int)c->n, c->dEnergy.p, c->stream),
+             "energy");
+}
+
 double computeEnergy(hdnnp_ctx* c)
 {
     if (!c->haveNN)
         throw Error(HDNNP_E_INVALID, "ERROR: Networks not evaluated.\n");
     c->setDevice();
-    c->dEnergy.ensure(1);
-    hipCheck(launch_energy_sum(c->dE.p, (int)c->n, c->dEnergy.p, c->stream),
-             "energy");
+    launchEnergy(c);
     double e = 0.0;
     hipCheck(hipMemcpyAsync(&e, c->dEnergy.p, sizeof(double),
                             hipMemcpyDeviceToHost, c->stream), "energy copy");
@@ -818,7 +846,7 @@ double computeEnergy(hdnnp_ctx* c)
     return e;
 }
 
-void computeForces(hdnnp_ctx* c, int mode)
+void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
 {
     if (!c->haveNNDeriv)
         throw Error(HDNNP_E_INVALID,
@@ -873,7 +901,164 @@ void computeForces(hdnnp_ctx* c, int mode)
                                      c->stream), "force gather");
         c->end(K_GATHER, ev2);
     }
-    c->sync();
+    if (wait) c->sync();
+}
+
+// ------------------------------------------------------------------ MD --
+
+void mdDropGraphs(hdnnp_ctx* c);
+
+// Calls that replace the structure, the positions or the list end an MD
+// run (hdnnp_md_init starts a new one).
+void mdEnd(hdnnp_ctx* c)
+{
+    if (!c->md.on) return;
+    mdDropGraphs(c);
+    c->md.on = false;
+}
+
+void mdDropGraphs(hdnnp_ctx* c)
+{
+    auto& m = c->md;
+    if (m.eA) (void)hipGraphExecDestroy(m.eA);
+    if (m.eB) (void)hipGraphExecDestroy(m.eB);
+    if (m.gA) (void)hipGraphDestroy(m.gA);
+    if (m.gB) (void)hipGraphDestroy(m.gB);
+    m.eA = m.eB = nullptr;
+    m.gA = m.gB = nullptr;
+}
+
+// Kick, drift and the filter of the skin list; the two flags (moved more
+// than half the skin, filtered list over capacity) to the host.
+void mdStepA(hdnnp_ctx* c)
+{
+    auto& m = c->md;
+    int const n = (int)c->n;
+    hipStream_t s = c->stream;
+    hipCheck(launch_md_clear_flags(m.flags.p, s), "md flags");
+    hipCheck(launch_md_kick(n, m.v.p, c->dF.p, c->dElemAtom.p, m.accel.p,
+                            0.5 * m.dt, s), "md kick");
+    double const half = 0.5 * m.skin;
+    hipCheck(launch_md_drift(n, c->dR.p, m.v.p, m.r0.p, m.dt, half * half,
+                             m.flags.p, s), "md drift");
+    hipCheck(launch_md_filter(n, m.cell, m.rc * m.rc, c->dR.p, c->dOff.p,
+                              m.skinCnt.p, m.skinKeys.p, c->dCount.p,
+                              c->dKeys.p, c->dNbrJ.p, c->dGeo.p, m.src.p,
+                              m.newIdx.p, m.cap, m.flags.p, s), "md filter");
+    hipCheck(hipMemcpyAsync(m.hFlags, m.flags.p, 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, s), "md flags");
+}
+
+// Partner slots of the filtered list, then SF, networks, energy, forces.
+void mdForces(hdnnp_ctx* c, bool wait)
+{
+    auto& m = c->md;
+    hipCheck(launch_md_partner((int)c->n, c->dOff.p, c->dCount.p, m.src.p,
+                               m.skinPartner.p, m.newIdx.p, c->dPartner.p,
+                               c->stream), "md partner");
+    c->haveNL = true;
+    computeSF(c, false);
+    computeNN(c, true);
+    launchEnergy(c);
+    computeForces(c, m.forceMode, wait);
+}
+
+// Forces, second kick, energy record.
+void mdStepB(hdnnp_ctx* c)
+{
+    auto& m = c->md;
+    mdForces(c, false);
+    hipCheck(launch_md_kick((int)c->n, m.v.p, c->dF.p, c->dElemAtom.p,
+                            m.accel.p, 0.5 * m.dt, c->stream), "md kick");
+    hipCheck(launch_md_record((int)c->n, m.v.p, c->dElemAtom.p, m.mass.p,
+                              m.part.p, c->dEnergy.p, 0.5 / m.ftoa,
+                              m.counter.p, m.recCap, m.epot.p, m.ekin.p,
+                              c->stream), "md record");
+}
+
+// Wrap the atoms into the cell, build the skin list, filter it at the
+// current positions and size the capacity of the filtered list.
+void mdRebuild(hdnnp_ctx* c)
+{
+    auto& m = c->md;
+    int const n = (int)c->n;
+    hipStream_t s = c->stream;
+    mdDropGraphs(c);
+    hipCheck(launch_md_wrap(n, c->dR.p, m.cell, s), "md wrap");
+    c->hostPosValid = false;
+    buildNeighborList(c, m.rc + m.skin);
+    std::swap(c->dKeys, m.skinKeys);
+    std::swap(c->dPartner, m.skinPartner);
+    std::swap(c->dCount, m.skinCnt);
+    size_t const E = (size_t)std::max<int64_t>(c->entryCap, 1);
+    c->dKeys.ensure(E);
+    c->dPartner.ensure(E);
+    c->dCount.ensure(n);
+    m.src.ensure(E);
+    m.newIdx.ensure(E);
+    hipCheck(hipMemcpyAsync(m.r0.p, c->dR.p, 3 * (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToDevice, s), "md r0");
+    hipCheck(launch_md_clear_flags(m.flags.p, s), "md flags");
+    hipCheck(launch_md_filter(n, m.cell, m.rc * m.rc, c->dR.p, c->dOff.p,
+                              m.skinCnt.p, m.skinKeys.p, c->dCount.p,
+                              c->dKeys.p, c->dNbrJ.p, c->dGeo.p, m.src.p,
+                              m.newIdx.p, INT32_MAX, m.flags.p, s),
+             "md filter");
+    int maxc = 0;
+    hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "md max");
+    hipCheck(hipMemcpyAsync(&maxc, c->dStats.p, sizeof(int),
+                            hipMemcpyDeviceToHost, s), "md max");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    m.cap = std::min(c->maxNeigh, maxc + maxc / 8 + 8);
+    c->maxNeigh = m.cap;
+    c->dCslot.ensure(3 * E);
+    c->Mvalid = false;
+    m.rebuilds += 1;
+}
+
+template <class F>
+void mdReplay(hdnnp_ctx* c, hipGraph_t& g, hipGraphExec_t& e, F&& body)
+{
+    if (!c->md.useGraph)
+    {
+        body();
+        return;
+    }
+    if (!e)
+    {
+        c->capturing = true;
+        hipError_t const rc = hipStreamBeginCapture(c->stream,
+                                                    hipStreamCaptureModeThreadLocal);
+        if (rc != hipSuccess)
+        {
+            c->capturing = false;
+            hipCheck(rc, "hipStreamBeginCapture");
+        }
+        try { body(); }
+        catch (...)
+        {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(c->stream, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            c->capturing = false;
+            throw;
+        }
+        c->capturing = false;
+        hipCheck(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+        hipCheck(hipGraphInstantiate(&e, g, nullptr, nullptr, 0),
+                 "hipGraphInstantiate");
+    }
+    hipCheck(hipGraphLaunch(e, c->stream), "hipGraphLaunch");
+}
+
+void mdStep(hdnnp_ctx* c)
+{
+    auto& m = c->md;
+    mdReplay(c, m.gA, m.eA, [&]() { mdStepA(c); });
+    hipCheck(hipStreamSynchronize(c->stream), "sync");
+    if (m.hFlags[0] || m.hFlags[1]) mdRebuild(c);
+    mdReplay(c, m.gB, m.eB, [&]() { mdStepB(c); });
+    m.steps += 1;
 }
 
 template <class T>
@@ -1050,6 +1235,19 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
     c->dG.release(); c->dE.release(); c->dEdG.release(); c->dScratch.release();
     c->dCslot.release(); c->dCself.release(); c->dF.release();
     c->dEnergy.release(); c->dStatOut.release(); c->dEw.release();
+    c->dStructOf.release(); c->dStructOff.release(); c->dCells.release();
+    c->dEnergySeg.release();
+    mdDropGraphs(c);
+    {
+        auto& m = c->md;
+        m.v.release(); m.r0.release(); m.accel.release(); m.mass.release();
+        m.part.release(); m.epot.release(); m.ekin.release();
+        m.flags.release(); m.counter.release(); m.skinPartner.release();
+        m.skinCnt.release(); m.src.release(); m.newIdx.release();
+        m.skinKeys.release();
+        if (m.hFlags) (void)hipHostFree(m.hFlags);
+        m.hFlags = nullptr;
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1100,6 +1298,7 @@ int hdnnp_ctx_update_model(hdnnp_ctx* c)
 // Atoms, elements and per-element atom lists of a structure or a batch.
 static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* elem)
 {
+    mdEnd(c);
     if (n <= 0 || !r || !elem)
         throw Error(HDNNP_E_INVALID, "ERROR: Empty structure.\n");
     if (n > 200000000)
@@ -1244,6 +1443,7 @@ int hdnnp_get_structure_pbc(hdnnp_ctx* c, int32_t* pbc)
 int hdnnp_set_positions(hdnnp_ctx* c, const double* r)
 {
     return guarded([&]() {
+        mdEnd(c);
         requireStructure(c);
         if (c->externalNL)
             throw Error(HDNNP_E_INVALID, "ERROR: No positions: the neighbour "
@@ -1260,6 +1460,7 @@ int hdnnp_set_positions(hdnnp_ctx* c, const double* r)
 int hdnnp_set_positions_device(hdnnp_ctx* c, const double* r_device)
 {
     return guarded([&]() {
+        mdEnd(c);
         requireStructure(c);
         if (c->externalNL)
             throw Error(HDNNP_E_INVALID, "ERROR: No positions: the neighbour "
@@ -1281,6 +1482,7 @@ int hdnnp_neighbor_list(hdnnp_ctx* c, double cutoff)
         if (c && c->externalNL)
             throw Error(HDNNP_E_INVALID, "ERROR: The neighbour list was given "
                                          "by the caller.\n");
+        mdEnd(c);
         buildNeighborList(c, cutoff);
     });
 }
@@ -1312,6 +1514,7 @@ int hdnnp_forces(hdnnp_ctx* c, int mode)
 int hdnnp_evaluate(hdnnp_ctx* c, int force_mode, double* energy)
 {
     return guarded([&]() {
+        mdEnd(c);
         if (!c->externalNL) buildNeighborList(c, c->T.maxCutoff);
         computeSF(c, false);
         computeNN(c, force_mode >= 0);
@@ -1452,6 +1655,118 @@ int hdnnp_get_forces(hdnnp_ctx* c, double* f)
 
 const double* hdnnp_forces_device(hdnnp_ctx* c) { return c->dF.p; }
 
+int hdnnp_md_init(hdnnp_ctx* c, const double* masses, double dt, double skin,
+                  double ftoa, const double* v, int force_mode, int use_graph)
+{
+    return guarded([&]() {
+        requireStructure(c);
+        if (c->externalNL || c->batched || !c->periodic)
+            throw Error(HDNNP_E_INVALID, "ERROR: MD needs one periodic "
+                                         "structure (hdnnp_set_structure "
+                                         "with a cell).\n");
+        if (!masses || !(dt > 0.0) || !(skin >= 0.0) || !(ftoa > 0.0))
+            throw Error(HDNNP_E_INVALID, "ERROR: MD needs masses, dt > 0, "
+                                         "skin >= 0 and ftoa > 0.\n");
+        if (force_mode != HDNNP_FORCES_REFERENCE
+            && force_mode != HDNNP_FORCES_CONSISTENT)
+            throw Error(HDNNP_E_INVALID, "ERROR: Unknown force mode.\n");
+        c->setDevice();
+        mdEnd(c);
+        auto& m = c->md;
+        int const n = (int)c->n, ne = c->T.ne;
+        std::vector<double> acc(ne), mass(ne);
+        for (int e = 0; e < ne; ++e)
+        {
+            if (!(masses[e] > 0.0))
+                throw Error(HDNNP_E_INVALID, "ERROR: Masses must be "
+                                             "positive.\n");
+            acc[e] = ftoa / masses[e];
+            mass[e] = masses[e];
+        }
+        m.dt = dt;
+        m.skin = skin;
+        m.ftoa = ftoa;
+        m.rc = c->T.maxCutoff;
+        m.forceMode = force_mode;
+        m.useGraph = use_graph != 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                m.cell.box[i][j] = c->box[i][j];
+                m.cell.invbox[i][j] = c->invbox[i][j];
+            }
+        m.v.ensure(3 * (size_t)n);
+        m.r0.ensure(3 * (size_t)n);
+        m.accel.ensure(ne);
+        m.mass.ensure(ne);
+        m.part.ensure(md_mv2_blocks(n));
+        m.flags.ensure(2);
+        m.counter.ensure(1);
+        if (!m.hFlags)
+            hipCheck(hipHostMalloc((void**)&m.hFlags, 2 * sizeof(int),
+                                   hipHostMallocDefault), "hipHostMalloc");
+        m.hFlags[0] = m.hFlags[1] = 0;
+        hipStream_t s = c->stream;
+        if (v)
+            hipCheck(hipMemcpyAsync(m.v.p, v, 3 * (size_t)n * sizeof(double),
+                                    hipMemcpyHostToDevice, s), "velocities");
+        else
+            hipCheck(hipMemsetAsync(m.v.p, 0, 3 * (size_t)n * sizeof(double),
+                                    s), "velocities");
+        hipCheck(hipMemcpyAsync(m.accel.p, acc.data(), ne * sizeof(double),
+                                hipMemcpyHostToDevice, s), "accelerations");
+        hipCheck(hipMemcpyAsync(m.mass.p, mass.data(), ne * sizeof(double),
+                                hipMemcpyHostToDevice, s), "masses");
+        m.steps = 0;
+        m.rebuilds = 0;
+        m.on = true;
+        mdRebuild(c);
+        c->dF.ensure(3 * (size_t)n);
+        mdForces(c, true);
+    });
+}
+
+int hdnnp_md_run(hdnnp_ctx* c, int64_t nsteps, double* epot, double* ekin)
+{
+    return guarded([&]() {
+        if (!c->md.on)
+            throw Error(HDNNP_E_INVALID, "ERROR: No MD run (hdnnp_md_init; a "
+                                         "later structure, position or list "
+                                         "call ends the run).\n");
+        if (nsteps < 0 || nsteps > 100000000)
+            throw Error(HDNNP_E_INVALID, "ERROR: Bad step count.\n");
+        c->setDevice();
+        auto& m = c->md;
+        int const need = (int)std::max<int64_t>(nsteps, 1);
+        if (need > m.recCap)
+        {
+            mdDropGraphs(c);
+            m.epot.ensure(need);
+            m.ekin.ensure(need);
+            m.recCap = need;
+        }
+        hipCheck(hipMemsetAsync(m.counter.p, 0, sizeof(int), c->stream),
+                 "md counter");
+        for (int64_t k = 0; k < nsteps; ++k) mdStep(c);
+        copyOut(c, epot, m.epot.p, (size_t)nsteps);
+        copyOut(c, ekin, m.ekin.p, (size_t)nsteps);
+        c->sync();
+    });
+}
+
+int hdnnp_md_get_state(hdnnp_ctx* c, double* r, double* v)
+{
+    return guarded([&]() {
+        if (!c->md.on) throw Error(HDNNP_E_INVALID, "ERROR: No MD run.\n");
+        c->setDevice();
+        copyOut(c, r, c->dR.p, 3 * (size_t)c->n);
+        copyOut(c, v, c->md.v.p, 3 * (size_t)c->n);
+        c->sync();
+    });
+}
+
+int64_t hdnnp_md_rebuilds(const hdnnp_ctx* c) { return c ? c->md.rebuilds : 0; }
+
 int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
                             const int32_t* centre_index,
                             const int32_t* centre_elem,
@@ -1525,6 +1840,7 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
         for (int i = 0; i < n; ++i)
             c->extCentre[i] = centre_index ? centre_index[i] : i;
         c->extNbr.assign(nbr_index, nbr_index + M);
+        mdEnd(c);
         c->externalNL = true;
         c->batched = false;
         c->nstruct = 1;

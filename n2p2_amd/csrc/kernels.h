@@ -80,6 +80,32 @@ hipError_t launch_nl_batch(int mode, int n, double rc, const double* r,
                            const BatchCell* cells, int* count, int* stats,
                            const int* off, int limit, uint64_t* keys,
                            int* nbrJ, double4* geo, hipStream_t s);
+// --- MD with a Verlet-skin list (kernels_md.hip)
+struct MDCell
+{
+    double box[3][3];      // lattice vectors as rows
+    double invbox[3][3];
+};
+hipError_t launch_md_kick(int n, double* v, const double* F, const int* elem,
+                          const double* accel, double h, hipStream_t s);
+hipError_t launch_md_drift(int n, double* r, const double* v, const double* r0,
+                           double dt, double half2, int* flags, hipStream_t s);
+hipError_t launch_md_clear_flags(int* flags, hipStream_t s);
+hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s);
+hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
+                            const int* off, const int* skinCnt,
+                            const uint64_t* skinKeys, int* cnt, uint64_t* keys,
+                            int* nbrJ, double4* geo, int* src, int* newIdx,
+                            int cap, int* flags, hipStream_t s);
+hipError_t launch_md_partner(int n, const int* off, const int* cnt,
+                             const int* src, const int* skinPartner,
+                             const int* newIdx, int* partner, hipStream_t s);
+int md_mv2_blocks(int n);
+hipError_t launch_md_record(int n, const double* v, const int* elem,
+                            const double* mass, double* part,
+                            const double* energy, double ekinScale,
+                            int* counter, int capRec, double* epot,
+                            double* ekin, hipStream_t s);
 // Per-structure energy sums (one block per structure, fixed order).
 hipError_t launch_energy_segments(const double* E, const int* structOff,
                                   int nstruct, double* out, hipStream_t s);

@@ -1,0 +1,310 @@
+// Velocity-Verlet MD on the device with a Verlet-skin neighbour list.
+//
+// The list is built with rc + skin by the ordinary neighbour builder (same
+// order: j, then the image vector).  Every step filters it down to the
+// entries within rc at the current positions, in the same order, so the SF
+// and force kernels see exactly the list a fresh build would give.  The
+// partner slot of every kept entry is the kept image of its skin partner:
+// |dr| is the same for (i, j, b) and (j, i, -b), so both are kept or both
+// dropped.
+//
+// No counterpart in libnnp: this is the loop LAMMPS runs around
+// pair_style hdnnp (fix nve, pair_hdnnp.cpp:81-114) with
+// "neigh_modify every 1 delay 0 check yes" (rebuild when an atom has moved
+// more than half the skin since the last build).
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace hdnnp {
+
+namespace {
+
+constexpr int kBias = 64;   // image components biased into 7 bits
+
+// Left-to-right dr of Structure.cpp:319-322 (the neighbour builder's form).
+__device__ __forceinline__ void md_image_dr(MDCell const& C, const double* ri,
+                                            const double* rj, int b0, int b1,
+                                            int b2, double* dr)
+{
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c)
+    {
+        double t = ri[c] - rj[c];
+        t = t + (double)b0 * C.box[0][c];
+        t = t + (double)b1 * C.box[1][c];
+        t = t + (double)b2 * C.box[2][c];
+        dr[c] = t;
+    }
+}
+
+// Vec3D::norm2 as the neighbour builder evaluates it.
+__device__ __forceinline__ double md_norm2(const double* r)
+{
+    double t = r[1] * r[1];
+    t = fma(r[0], r[0], t);
+    return fma(r[2], r[2], t);
+}
+
+// v += h * a(F): a = ftoa F / m, per element.
+__global__ void k_md_kick(int n, double* __restrict__ v,
+                          const double* __restrict__ F,
+                          const int* __restrict__ elem,
+                          const double* __restrict__ accel, double h)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double const a = h * accel[elem[i]];
+    for (int k = 0; k < 3; ++k) v[3 * i + k] += a * F[3 * i + k];
+}
+
+// r += dt v; flags[0] = 1 if an atom has moved more than half the skin
+// since the list was built (r0).
+__global__ void k_md_drift(int n, double* __restrict__ r,
+                           const double* __restrict__ v,
+                           const double* __restrict__ r0, double dt,
+                           double half2, int* __restrict__ flags)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k)
+    {
+        double const x = r[3 * i + k] + dt * v[3 * i + k];
+        r[3 * i + k] = x;
+        double const d = x - r0[3 * i + k];
+        d2 += d * d;
+    }
+    if (d2 > half2) flags[0] = 1;
+}
+
+__global__ void k_md_clear_flags(int* __restrict__ flags)
+{
+    if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+}
+
+// Atoms that left the cell are shifted back by whole lattice vectors,
+// r -= floor(f) box with f = r invbox; atoms inside keep their bits.
+__global__ void k_md_wrap(int n, double* __restrict__ r, MDCell C)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sh[3];
+    bool out = false;
+    for (int k = 0; k < 3; ++k)
+    {
+        double const f = r[3 * i] * C.invbox[0][k]
+                       + r[3 * i + 1] * C.invbox[1][k]
+                       + r[3 * i + 2] * C.invbox[2][k];
+        sh[k] = floor(f);
+        out = out || sh[k] != 0.0;
+    }
+    if (!out) return;
+    for (int k = 0; k < 3; ++k)
+        r[3 * i + k] -= sh[0] * C.box[0][k] + sh[1] * C.box[1][k]
+                      + sh[2] * C.box[2][k];
+}
+
+// Entries of the skin list within rc at the current positions, in order,
+// into the slot range of the same atom (the skin list's range bounds the
+// count).  src[q] = skin slot of kept entry q, newIdx[s] = kept slot of
+// skin slot s (-1 if dropped).  flags[1] = 1 if an atom has more than
+// `cap` kept entries.  One wave per atom.
+__global__ __launch_bounds__(64) void k_md_filter(
+    int n, MDCell C, double rc2, const double* __restrict__ r,
+    const int* __restrict__ off, const int* __restrict__ skinCnt,
+    const uint64_t* __restrict__ skinKeys, int* __restrict__ cnt,
+    uint64_t* __restrict__ keys, int* __restrict__ nbrJ,
+    double4* __restrict__ geo, int* __restrict__ src,
+    int* __restrict__ newIdx, int cap, int* __restrict__ flags)
+{
+    int const i = blockIdx.x;
+    if (i >= n) return;
+    int const lane = threadIdx.x;
+    int const o = off[i], m = skinCnt[i];
+    double const ri[3] = {r[3 * (size_t)i], r[3 * (size_t)i + 1],
+                          r[3 * (size_t)i + 2]};
+    int kept = 0;
+    for (int base = 0; base < m; base += 64)
+    {
+        int const t = base + lane;
+        bool keep = false;
+        uint64_t key = 0;
+        int j = 0;
+        double dr[3] = {0.0, 0.0, 0.0};
+        if (t < m)
+        {
+            key = skinKeys[o + t];
+            j = (int)(key >> 21);
+            int const b0 = (int)((key >> 14) & 127) - kBias;
+            int const b1 = (int)((key >> 7) & 127) - kBias;
+            int const b2 = (int)(key & 127) - kBias;
+            double const rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                                  r[3 * (size_t)j + 2]};
+            md_image_dr(C, ri, rj, b0, b1, b2, dr);
+            keep = md_norm2(dr) <= rc2;
+        }
+        uint64_t const mask = __ballot(keep);
+        if (t < m)
+        {
+            int const q = o + kept + lanes_below(mask);
+            newIdx[o + t] = keep ? q : -1;
+            if (keep)
+            {
+                keys[q] = key;
+                nbrJ[q] = j;
+                geo[q] = make_double4(dr[0], dr[1], dr[2], sqrt(md_norm2(dr)));
+                src[q] = o + t;
+            }
+        }
+        kept += __popcll(mask);
+    }
+    if (lane == 0)
+    {
+        cnt[i] = kept;
+        if (kept > cap) flags[1] = 1;
+    }
+}
+
+// partner[q] = kept slot of the skin partner of kept entry q.
+__global__ void k_md_partner(int n, const int* __restrict__ off,
+                             const int* __restrict__ cnt,
+                             const int* __restrict__ src,
+                             const int* __restrict__ skinPartner,
+                             const int* __restrict__ newIdx,
+                             int* __restrict__ partner)
+{
+    int const i = blockIdx.x;
+    if (i >= n) return;
+    int const o = off[i], m = cnt[i];
+    for (int t = threadIdx.x; t < m; t += blockDim.x)
+        partner[o + t] = newIdx[skinPartner[src[o + t]]];
+}
+
+// Per-block sums of m v^2 in a fixed order (kinetic energy, twice, in
+// mass * velocity^2 units).
+__global__ __launch_bounds__(256) void k_md_mv2(
+    int n, const double* __restrict__ v, const int* __restrict__ elem,
+    const double* __restrict__ mass, double* __restrict__ part)
+{
+    __shared__ double s[256];
+    int const t = threadIdx.x;
+    int const i = blockIdx.x * 256 + t;
+    double acc = 0.0;
+    if (i < n)
+    {
+        double const* vi = v + 3 * (size_t)i;
+        acc = mass[elem[i]] * (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+    }
+    s[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (t < o) s[t] += s[t + o];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = s[0];
+}
+
+// Step record: ekin[k] = ekinScale * sum(part), epot[k] = *energy for
+// k = counter[0] (if below cap), then counter[0] += 1.  One block.
+__global__ __launch_bounds__(256) void k_md_record(
+    int nb, const double* __restrict__ part, const double* __restrict__ energy,
+    double ekinScale, int* __restrict__ counter, int capRec,
+    double* __restrict__ epot, double* __restrict__ ekin)
+{
+    __shared__ double s[256];
+    int const t = threadIdx.x;
+    double acc = 0.0;
+    for (int b = t; b < nb; b += 256) acc += part[b];
+    s[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (t < o) s[t] += s[t + o];
+        __syncthreads();
+    }
+    if (t == 0)
+    {
+        int const k = counter[0];
+        if (k < capRec)
+        {
+            ekin[k] = ekinScale * s[0];
+            epot[k] = energy[0];
+        }
+        counter[0] = k + 1;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_md_kick(int n, double* v, const double* F, const int* elem,
+                          const double* accel, double h, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_kick<<<(n + 255) / 256, 256, 0, s>>>(n, v, F, elem, accel, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_drift(int n, double* r, const double* v, const double* r0,
+                           double dt, double half2, int* flags, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_drift<<<(n + 255) / 256, 256, 0, s>>>(n, r, v, r0, dt, half2, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_clear_flags(int* flags, hipStream_t s)
+{
+    k_md_clear_flags<<<1, 64, 0, s>>>(flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_wrap<<<(n + 255) / 256, 256, 0, s>>>(n, r, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
+                            const int* off, const int* skinCnt,
+                            const uint64_t* skinKeys, int* cnt, uint64_t* keys,
+                            int* nbrJ, double4* geo, int* src, int* newIdx,
+                            int cap, int* flags, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_filter<<<n, 64, 0, s>>>(n, C, rc2, r, off, skinCnt, skinKeys, cnt,
+                                 keys, nbrJ, geo, src, newIdx, cap, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_partner(int n, const int* off, const int* cnt,
+                             const int* src, const int* skinPartner,
+                             const int* newIdx, int* partner, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_partner<<<n, 64, 0, s>>>(n, off, cnt, src, skinPartner, newIdx,
+                                  partner);
+    return hipGetLastError();
+}
+
+int md_mv2_blocks(int n) { return (n + 255) / 256; }
+
+hipError_t launch_md_record(int n, const double* v, const int* elem,
+                            const double* mass, double* part,
+                            const double* energy, double ekinScale,
+                            int* counter, int capRec, double* epot,
+                            double* ekin, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int const nb = md_mv2_blocks(n);
+    k_md_mv2<<<nb, 256, 0, s>>>(n, v, elem, mass, part);
+    k_md_record<<<1, 256, 0, s>>>(nb, part, energy, ekinScale, counter,
+                                  capRec, epot, ekin);
+    return hipGetLastError();
+}
+
+}  // namespace hdnnp

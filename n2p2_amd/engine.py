@@ -93,6 +93,12 @@ _SIGNATURES = [
     ("hdnnp_get_forces", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
     ("hdnnp_forces_device", ctypes.c_void_p, [ctypes.c_void_p]),
     ("hdnnp_get_pair_forces", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    ("hdnnp_md_init", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      _c_double_p, ctypes.c_int, ctypes.c_int]),
+    ("hdnnp_md_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_double_p]),
+    ("hdnnp_md_get_state", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    ("hdnnp_md_rebuilds", ctypes.c_int64, [ctypes.c_void_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -289,6 +295,37 @@ class Context:
         p = np.zeros((self.num_structures(), 3), dtype=np.int32)
         _check(_lib.hdnnp_get_structure_pbc(self._h, _ptr(p, ctypes.c_int32)))
         return p
+
+    # --- MD (velocity Verlet, Verlet-skin list, HIP graphs)
+    FTOA_EV_A_AMU_FS = 9.648533212e-3
+
+    def md_init(self, masses, dt, skin=1.0, v=None, ftoa=FTOA_EV_A_AMU_FS,
+                force_mode=FORCES_CONSISTENT, graph=True):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        vp = None
+        if v is not None:
+            self._v0 = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+            vp = _ptr(self._v0, ctypes.c_double)
+        _check(_lib.hdnnp_md_init(self._h, _ptr(m, ctypes.c_double), dt, skin,
+                                  ftoa, vp, force_mode, int(graph)))
+
+    def md_run(self, nsteps):
+        """nsteps steps; returns (epot, ekin) per step."""
+        ep = np.zeros(nsteps)
+        ek = np.zeros(nsteps)
+        _check(_lib.hdnnp_md_run(self._h, nsteps, _ptr(ep, ctypes.c_double),
+                                 _ptr(ek, ctypes.c_double)))
+        return ep, ek
+
+    def md_state(self):
+        r = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        _check(_lib.hdnnp_md_get_state(self._h, _ptr(r, ctypes.c_double),
+                                       _ptr(v, ctypes.c_double)))
+        return r, v
+
+    def md_rebuilds(self):
+        return _lib.hdnnp_md_rebuilds(self._h)
 
     def set_neighbor_list(self, centre_elem, offsets, nbr_index, nbr_elem,
                           dr, centre_index=None):

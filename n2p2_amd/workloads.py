@@ -228,6 +228,22 @@ def sheared(r, box, shear):
     return r @ S, box @ S
 
 
+KB_EV = 8.617333262e-5               # Boltzmann constant, eV/K
+FTOA = 9.648533212e-3                # (eV/A)/amu in A/fs^2
+WATER_MASSES = (1.008, 15.999)       # amu, element order of WATER_ELEMENTS
+
+
+def maxwell_boltzmann(masses, T, seed=0):
+    """Velocities (A/fs) at temperature T for atoms of the given masses
+    (amu): normal components with variance kB T / m, total momentum
+    removed."""
+    m = np.asarray(masses, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(len(m), 3)) * np.sqrt(KB_EV * T * FTOA / m)[:, None]
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    return v
+
+
 WATER_CONFIGS = {
     "water192": (64, 12.42),
     "water24576": (8192, 62.1),
