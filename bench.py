@@ -120,6 +120,31 @@ def run_path(ctx, precision, steps, warmup):
     return dt, energy, forces, kt, spec
 
 
+def run_md(model, r, elem, box, precision, steps, warmup, skin=1.0,
+           dt_fs=0.25, temp=300.0):
+    """NVE MD of the same box (velocity Verlet, Verlet-skin list, HIP-graph
+    steps): `warmup` untimed steps, then `steps` timed ones.  md_run
+    returns after its stream has drained, so the timed region is
+    synchronous on both sides."""
+    ctx = engine.Context(model)
+    ctx.set_precision(precision)
+    ctx.set_structure(r, elem, box)
+    masses = np.array(W.WATER_MASSES)
+    v = W.maxwell_boltzmann(masses[elem], temp, seed=20261202)
+    ctx.md_init(masses, dt_fs, skin=skin, v=v)
+    ctx.md_run(warmup)
+    reb0 = ctx.md_rebuilds()
+    t0 = time.perf_counter()
+    ep, ek = ctx.md_run(steps)
+    dt = (time.perf_counter() - t0) / steps
+    et = ep + ek
+    out = {"atom_steps_per_s": len(r) / dt, "ms_per_step": dt * 1e3,
+           "steps": steps, "rebuilds_in_timed_steps": ctx.md_rebuilds() - reb0,
+           "energy_spread_eV_per_atom": float((et.max() - et.min()) / len(r))}
+    ctx.close()
+    return out
+
+
 def roofline(kt, counts, nnf, n, fp32, spec):
     """Roofline object for the dominant kernel of one path; spec: the
     model-specialised SF and force kernels ran (their names differ)."""
@@ -229,6 +254,20 @@ def main():
         fwd = int(costmodel.forward_flops(counts))
         bwd = int(costmodel.backward_flops(counts))
 
+        md_steps = 4 * args.steps
+        md = {}
+        for name, prec in (("fp64", engine.PRECISION_FP64),
+                           ("mixed", engine.PRECISION_MIXED)):
+            md[name] = run_md(model, r, elem, box, prec, md_steps,
+                              args.warmup)
+            log("MD %s: %s" % (name, md[name]))
+        md["mixed_vs_plain_fp64"] = (md["mixed"]["atom_steps_per_s"]
+                                     / md["fp64"]["atom_steps_per_s"])
+        md["what"] = ("NVE velocity Verlet on the same box, dt 0.25 fs, "
+                      "300 K Maxwell-Boltzmann start, 1.0 A Verlet skin "
+                      "(rebuild when an atom moved > 0.5 A), list filtered "
+                      "to rc every step, each step two HIP graphs")
+
         cpu = None
         if not args.no_cpu_baseline:
             log("cpu baseline (oracle, 24,576-atom sample)")
@@ -271,6 +310,7 @@ def main():
                                            "MFMA network layers (FP32 "
                                            "accumulation)"},
                        "mixed_vs_plain_fp64": dt64 / dtm,
+                       "md": md,
                        "mixed_error_vs_fp64": err,
                        "flops_per_atom_step": (fwd + bwd + nnf) / n,
                        "work_counts": counts,
