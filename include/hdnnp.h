@@ -286,6 +286,58 @@ int hdnnp_md_run(hdnnp_ctx* c, int64_t nsteps, double* epot, double* ekin);
 int hdnnp_md_get_state(hdnnp_ctx* c, double* r, double* v);
 /* Neighbour-list builds since hdnnp_md_init (the first one included). */
 int64_t hdnnp_md_rebuilds(const hdnnp_ctx* c);
+/* Filtered entries whose skin-list entry had no partner since
+ * hdnnp_md_init: 0 for a consistent list (such entries are left out of
+ * the force gather). */
+int64_t hdnnp_md_partner_misses(hdnnp_ctx* c);
+
+/* Multiple-time-step NVE (r-RESPA, Tuckerman, Berne & Martyna 1992) with
+ * a cheap surrogate model `sur` as the fast force and the full model
+ * `full` as the reference: each outer step (dt_outer) is a half kick with
+ * F_full - F_sur, n_inner velocity-Verlet substeps of dt_outer / n_inner on
+ * the surrogate, a full evaluation and a second half kick.  After every
+ * outer step the force RMSE of the surrogate against the full model is
+ * judged; when it exceeds rmse_gate the next outer step runs the full model
+ * for all n_inner substeps instead.  Both contexts hold the same periodic
+ * structure; masses, skin, ftoa, v and use_graph as for hdnnp_md_init.
+ * The state lives in `sur` (hdnnp_md_get_state(sur, ...)).  No counterpart
+ * in libnnp.  Currently switched off: hdnnp_mts_init returns
+ * HDNNP_E_UNSUPPORTED (device runs of it ended in a memory fault whose
+ * cause has not been found). */
+int hdnnp_mts_init(hdnnp_ctx* full, hdnnp_ctx* sur, const double* masses,
+                   double dt_outer, int n_inner, double skin, double ftoa,
+                   const double* v, double rmse_gate, int use_graph);
+/* nouter outer steps; per step (each array may be NULL): the full model's
+ * energy and the kinetic energy at the end of the step, the judged force
+ * RMSE and the mode of the step (0 surrogate substeps, 1 full substeps). */
+int hdnnp_mts_run(hdnnp_ctx* sur, int64_t nouter, double* epot, double* ekin,
+                  double* rmse, int32_t* mode);
+
+/* ---------------------------------------------------------- training -- */
+
+/* Training of a radial-only model on energies and forces (the distilled
+ * surrogate of the multiple-time-step integrator).  No counterpart in the
+ * 2G prediction path; the reference trains with libnnptrain
+ * (Training.cpp), which this does not mirror.  Weights are the flat array
+ * of every element's network in element order, per layer W[k][j]
+ * row-major then b[k] (the order of the weight files is per layer j-major,
+ * NeuralNetwork.cpp:147-169). */
+int64_t hdnnp_num_weights(const hdnnp_ctx* c);
+int hdnnp_get_weights(hdnnp_ctx* c, double* w);
+int hdnnp_set_weights(hdnnp_ctx* c, const double* w);
+/* Evaluates the loaded structure (FP64, consistent forces) and writes the
+ * gradient of L = w_e ((E - e_ref) / N)^2 + w_f / (3N) sum |F - f_ref|^2
+ * with respect to every weight (grad: hdnnp_num_weights doubles, may be
+ * NULL; it is kept for hdnnp_train_adam).  loss (2 doubles, may be NULL)
+ * = the two terms.  Needs radial SFs only and identity / tanh / logistic /
+ * softplus layers with one output (HDNNP_E_UNSUPPORTED otherwise). */
+int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
+                         double w_e, double w_f, double* grad, double* loss);
+/* One Adam step (Kingma & Ba 2015) with the last gradient. */
+int hdnnp_train_adam(hdnnp_ctx* c, double lr, double beta1, double beta2,
+                     double eps);
+/* Restart Adam (zero moments at the next step). */
+int hdnnp_train_reset(hdnnp_ctx* c);
 
 /* Number of extrapolation warnings of the last SF evaluation: G outside
  * [Gmin, Gmax] +- 1000 eps before scaling (Element.cpp:462-541).

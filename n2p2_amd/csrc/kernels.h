@@ -80,6 +80,37 @@ hipError_t launch_nl_batch(int mode, int n, double rc, const double* r,
                            const BatchCell* cells, int* count, int* stats,
                            const int* off, int limit, uint64_t* keys,
                            int* nbrJ, double4* geo, hipStream_t s);
+// --- Training of a radial-only model (kernels_train.hip)
+struct TrainArgs
+{
+    ModelTables T;
+    int n;
+    const int* off;
+    const int* cnt;
+    const int* nbrJ;
+    const double4* geo;
+    const int* elem;
+    const double* G;        // scaled SFs of the last evaluation
+    int ldG;
+    const double* F;        // forces of the last evaluation
+    const double* Fref;     // reference forces
+    double* U;              // n x ldG: u_as
+    double* d2;             // n: |F - Fref|^2
+    const double* energy;   // device: E of the last evaluation
+    double Eref, wE, cF;    // cF = -2 wF / (3N)
+};
+int train_atoms_per_block();
+size_t train_lds_per_atom(ElemDesc const& ed);
+hipError_t launch_train_u(TrainArgs const& A, hipStream_t s);
+hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,
+                             const int* atoms, int na, double* part, int nw,
+                             double* grad, hipStream_t s);
+hipError_t launch_adam(int n, double* w, float* w32, const double* g,
+                       double* m, double* v, double lr, double b1, double b2,
+                       double eps, int64_t step, hipStream_t s);
+hipError_t launch_weights_to_f32(int n, const double* w, float* w32,
+                                 hipStream_t s);
+
 // --- MD with a Verlet-skin list (kernels_md.hip)
 struct MDCell
 {
@@ -91,6 +122,11 @@ hipError_t launch_md_kick(int n, double* v, const double* F, const int* elem,
 hipError_t launch_md_drift(int n, double* r, const double* v, const double* r0,
                            double dt, double half2, int* flags, hipStream_t s);
 hipError_t launch_md_clear_flags(int* flags, hipStream_t s);
+hipError_t launch_md_kick_diff(int n, double* v, const double* Ff,
+                               const double* Fs, const int* elem,
+                               const double* accel, double h, hipStream_t s);
+hipError_t launch_md_fdiff2(int n, const double* F1, const double* F2,
+                            double* d2, hipStream_t s);
 hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s);
 hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
                             const int* off, const int* skinCnt,
@@ -99,7 +135,8 @@ hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
                             int cap, int* flags, hipStream_t s);
 hipError_t launch_md_partner(int n, const int* off, const int* cnt,
                              const int* src, const int* skinPartner,
-                             const int* newIdx, int* partner, hipStream_t s);
+                             const int* newIdx, int* partner, int* misses,
+                             hipStream_t s);
 int md_mv2_blocks(int n);
 hipError_t launch_md_record(int n, const double* v, const int* elem,
                             const double* mass, double* part,
@@ -131,8 +168,8 @@ size_t force_backward_lds_bytes(int cap, int fp32);
 hipError_t launch_force_gather(int n, const int* off, const int* cnt,
                                const int* nbrJ, const int* partner,
                                const double* cslot, const double* cself,
-                               const int* order, int mode, double* F,
-                               hipStream_t s);
+                               const int* order, int mode, int64_t nslots,
+                               double* F, hipStream_t s);
 
 // Atomic networks for the atoms of one element.  mode -1: per-atom FP64
 // kernel (any width); 0: FP64 MFMA tiles; 1: FP32 MFMA tiles; 2: FP16 MFMA

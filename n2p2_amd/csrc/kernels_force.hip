@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void k_gather(
     const int* __restrict__ nbrJ,
     const int* __restrict__ partner, const double* __restrict__ cslot,
     const double* __restrict__ cself, const int* __restrict__ order,
-    int mode, double* __restrict__ F)
+    int mode, int64_t nslots, double* __restrict__ F)
 {
     int const t = xcd_work_index(blockIdx.x, gridDim.x) * 4
                 + (int)(threadIdx.x >> 6);
@@ -41,9 +41,10 @@ __global__ __launch_bounds__(256) void k_gather(
     for (int q = s0 + lane; q < s1; q += 64)
     {
         int const pq = partner[q];
-        // The list is symmetric, so every entry has a partner; pq < 0 would
-        // mean an inconsistent list and is skipped rather than read.
-        if ((mode == 1 || nbrJ[q] != i) && pq >= 0)
+        // The list is symmetric, so every entry has a partner; a partner
+        // outside [0, nslots) would mean an inconsistent list and is
+        // skipped rather than read.
+        if ((mode == 1 || nbrJ[q] != i) && pq >= 0 && pq < nslots)
         {
             size_t const p = (size_t)pq;
             fx += cslot[3 * p];
@@ -105,12 +106,12 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s)
 hipError_t launch_force_gather(int n, const int* off, const int* cnt,
                                const int* nbrJ, const int* partner,
                                const double* cslot, const double* cself,
-                               const int* order, int mode, double* F,
-                               hipStream_t s)
+                               const int* order, int mode, int64_t nslots,
+                               double* F, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_gather<<<(n + 3) / 4, 256, 0, s>>>(n, off, cnt, nbrJ, partner, cslot,
-                                         cself, order, mode, F);
+                                         cself, order, mode, nslots, F);
     return hipGetLastError();
 }
 

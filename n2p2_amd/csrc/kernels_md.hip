@@ -61,7 +61,8 @@ __global__ void k_md_kick(int n, double* __restrict__ v,
 }
 
 // r += dt v; flags[0] = 1 if an atom has moved more than half the skin
-// since the list was built (r0).
+// since the list was built (r0); flags[2] = 1 if a coordinate is not
+// finite or beyond 1e7 (a run that has become unstable).
 __global__ void k_md_drift(int n, double* __restrict__ r,
                            const double* __restrict__ v,
                            const double* __restrict__ r0, double dt,
@@ -70,19 +71,22 @@ __global__ void k_md_drift(int n, double* __restrict__ r,
     int const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d2 = 0.0;
+    bool bad = false;
     for (int k = 0; k < 3; ++k)
     {
         double const x = r[3 * i + k] + dt * v[3 * i + k];
         r[3 * i + k] = x;
         double const d = x - r0[3 * i + k];
         d2 += d * d;
+        bad = bad || !(fabs(x) <= 1e7);
     }
     if (d2 > half2) flags[0] = 1;
+    if (bad) flags[2] = 1;
 }
 
 __global__ void k_md_clear_flags(int* __restrict__ flags)
 {
-    if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+    if (threadIdx.x < 3) flags[threadIdx.x] = 0;
 }
 
 // Atoms that left the cell are shifted back by whole lattice vectors,
@@ -174,13 +178,21 @@ __global__ void k_md_partner(int n, const int* __restrict__ off,
                              const int* __restrict__ src,
                              const int* __restrict__ skinPartner,
                              const int* __restrict__ newIdx,
-                             int* __restrict__ partner)
+                             int* __restrict__ partner,
+                             int* __restrict__ misses)
 {
     int const i = blockIdx.x;
     if (i >= n) return;
     int const o = off[i], m = cnt[i];
     for (int t = threadIdx.x; t < m; t += blockDim.x)
-        partner[o + t] = newIdx[skinPartner[src[o + t]]];
+    {
+        // A skin entry without a partner (-1 from the partner search) can
+        // only come from an inconsistent list; it stays -1, which the
+        // gather skips.
+        int const sp = skinPartner[src[o + t]];
+        partner[o + t] = sp >= 0 ? newIdx[sp] : -1;
+        if (sp < 0) atomicAdd(misses, 1);
+    }
 }
 
 // Per-block sums of m v^2 in a fixed order (kinetic energy, twice, in
@@ -238,7 +250,55 @@ __global__ __launch_bounds__(256) void k_md_record(
     }
 }
 
+// v += h a(Ffull - Fsur): the slow-force kick of r-RESPA.
+__global__ void k_md_kick_diff(int n, double* __restrict__ v,
+                               const double* __restrict__ Ff,
+                               const double* __restrict__ Fs,
+                               const int* __restrict__ elem,
+                               const double* __restrict__ accel, double h)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double const a = h * accel[elem[i]];
+    for (int k = 0; k < 3; ++k)
+        v[3 * i + k] += a * (Ff[3 * i + k] - Fs[3 * i + k]);
+}
+
+// d2[i] = |F1_i - F2_i|^2.
+__global__ void k_md_fdiff2(int n, const double* __restrict__ F1,
+                            const double* __restrict__ F2,
+                            double* __restrict__ d2)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = 0; k < 3; ++k)
+    {
+        double const d = F1[3 * i + k] - F2[3 * i + k];
+        acc += d * d;
+    }
+    d2[i] = acc;
+}
+
 }  // namespace
+
+hipError_t launch_md_kick_diff(int n, double* v, const double* Ff,
+                               const double* Fs, const int* elem,
+                               const double* accel, double h, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_kick_diff<<<(n + 255) / 256, 256, 0, s>>>(n, v, Ff, Fs, elem, accel,
+                                                   h);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_fdiff2(int n, const double* F1, const double* F2,
+                            double* d2, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_md_fdiff2<<<(n + 255) / 256, 256, 0, s>>>(n, F1, F2, d2);
+    return hipGetLastError();
+}
 
 hipError_t launch_md_kick(int n, double* v, const double* F, const int* elem,
                           const double* accel, double h, hipStream_t s)
@@ -283,11 +343,12 @@ hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
 
 hipError_t launch_md_partner(int n, const int* off, const int* cnt,
                              const int* src, const int* skinPartner,
-                             const int* newIdx, int* partner, hipStream_t s)
+                             const int* newIdx, int* partner, int* misses,
+                             hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_md_partner<<<n, 64, 0, s>>>(n, off, cnt, src, skinPartner, newIdx,
-                                  partner);
+                                  partner, misses);
     return hipGetLastError();
 }
 

@@ -58,8 +58,12 @@ __device__ __forceinline__ void cell_coords(NLParams const& p,
         }
         else
             f = (r[c] - p.origin[c]) * p.invbox[c][c];
-        int k = (int)floor(f * (double)p.nc[c]);
-        ci[c] = k < 0 ? 0 : (k >= p.nc[c] ? p.nc[c] - 1 : k);
+        // Clamped in floating point before the conversion, so that a
+        // non-finite coordinate still lands in a valid cell (fmax/fmin
+        // return the number when the other operand is NaN).
+        double const x = fmin(fmax(floor(f * (double)p.nc[c]), 0.0),
+                              (double)(p.nc[c] - 1));
+        ci[c] = (int)x;
     }
 }
 

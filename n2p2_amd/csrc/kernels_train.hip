@@ -1,0 +1,371 @@
+// Training kernels for a radial-only HDNNP (the distilled surrogate of the
+// multiple-time-step integrator): the gradient of
+//
+//   L = wE ((E - Eref) / N)^2 + wF / (3N) sum_i |F_i - Fref_i|^2
+//
+// with respect to every network weight, and an Adam step.
+//
+// With F_i = -sum_a sum_s (dE_a/dG_as)(dG_as/dr_i) the force term's
+// gradient is sum_a d/dw [ sum_s u_as dE_a/dG_as ], where
+//
+//   u_as = -(2 wF / 3N) sum_i (F_i - Fref_i) . dG_as/dr_i,
+//
+// the derivative of E_a along the direction u_a: a forward-mode tangent
+// through the atom's network, then reverse mode through that tangent.
+// For a radial G2 member g(r) = exp(-eta (r - rs)^2) fc(r) of neighbour
+// element e1 (SymGrpExpRad.cpp:144-239):
+//
+//   sum_i d_i . dG_as/dr_i = sfac_s sum_j g'(r_aj) (dr_aj . (d_a - d_j)) / r_aj
+//
+// with dr_aj = r_a - r_j and sfac_s the SF's scaling slope.  The energy
+// term is ordinary backpropagation of kappa = 2 wE (E - Eref) / N^2.
+// Everything is FP64 and summed in a fixed order.
+//
+// No counterpart in the reference's 2G prediction path; the reference
+// trains with libnnptrain (Training.cpp, NeuralNetwork::calculateDEdc /
+// calculateDFdc, NeuralNetwork.cpp:444-719), which this does not mirror.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace hdnnp {
+
+namespace {
+
+// Value, first and second derivative of the activations training supports.
+__device__ __forceinline__ void activate2(int a, double x, double& v,
+                                          double& d, double& dd)
+{
+    switch (a)
+    {
+    case AF_IDENTITY: v = x; d = 1.0; dd = 0.0; return;
+    case AF_TANH:
+    {
+        double const t = tanh(x);
+        v = t; d = 1.0 - t * t; dd = -2.0 * t * d;
+        return;
+    }
+    case AF_LOGISTIC:
+    case AF_SOFTPLUS:
+    {
+        double s1;
+        activate(a, x, v, s1);
+        if (x > 35.0 || x < -35.0) { d = s1; dd = 0.0; return; }
+        double const s = 1.0 / (1.0 + exp(-x));
+        d = s1;
+        dd = a == AF_SOFTPLUS ? s * (1.0 - s) : s1 * (1.0 - 2.0 * s);
+        return;
+    }
+    default: v = 0.0; d = 0.0; dd = 0.0; return;
+    }
+}
+
+__device__ __forceinline__ double scale_slope(ScaleDesc const& s)
+{
+    return (s.type == ST_SCALE || s.type == ST_SCALECENTER
+            || s.type == ST_SCALESIGMA) ? s.sfac : 1.0;
+}
+
+// u_as for every SF of atom a, and |F_a - Fref_a|^2.  One wave per atom.
+__global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
+{
+    int const a = blockIdx.x;
+    if (a >= A.n) return;
+    int const lane = threadIdx.x;
+    ModelTables const& T = A.T;
+    int const e = A.elem[a];
+    ElemDesc const& ed = T.elem[e];
+    double da[3];
+    for (int k = 0; k < 3; ++k)
+        da[k] = A.F[3 * (size_t)a + k] - A.Fref[3 * (size_t)a + k];
+    if (lane == 0) A.d2[a] = da[0] * da[0] + da[1] * da[1] + da[2] * da[2];
+    double* __restrict__ U = A.U + (size_t)a * A.ldG;
+    int const o = A.off[a], m = A.cnt[a];
+    for (int gi = ed.g0; gi < ed.g0 + ed.nrad; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        double acc[kMaxMembers];
+#pragma unroll
+        for (int q = 0; q < kMaxMembers; ++q) acc[q] = 0.0;
+        for (int t = lane; t < m; t += 64)
+        {
+            int const j = A.nbrJ[o + t];
+            if (A.elem[j] != g.e1) continue;
+            double4 const gq = A.geo[o + t];
+            double const r = gq.w;
+            if (!(r < g.rc)) continue;
+            double fc, dfc;
+            cutoff_fdf<double>(cc, r, fc, dfc);
+            double proj = 0.0;
+            proj += gq.x * (da[0] - (A.F[3 * (size_t)j] - A.Fref[3 * (size_t)j]));
+            proj += gq.y * (da[1] - (A.F[3 * (size_t)j + 1]
+                                     - A.Fref[3 * (size_t)j + 1]));
+            proj += gq.z * (da[2] - (A.F[3 * (size_t)j + 2]
+                                     - A.Fref[3 * (size_t)j + 2]));
+            proj /= r;
+#pragma unroll
+            for (int q = 0; q < kMaxMembers; ++q)
+                if (q < g.nmem)
+                {
+                    double const eta = mem[q].eta, x = r - mem[q].rs;
+                    double const ex = exp(-eta * x * x);
+                    acc[q] += (dfc - 2.0 * eta * x * fc) * ex * proj;
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxMembers; ++q)
+            if (q < g.nmem)
+            {
+                double const s = wave_sum(acc[q]);
+                int const sf = mem[q].sf;
+                if (lane == 0)
+                    U[sf] = A.cF * scale_slope(T.scale[ed.sf0 + sf]) * s;
+            }
+    }
+}
+
+// Per-block gradient of sum over the block's atoms of
+// kappa E_a + (derivative of E_a along u_a), atoms of element e, kAtoms per
+// block.  LDS per atom: the inputs x, tangents t of every layer and the
+// adjoints of z and s of every layer, plus two adjoint scratch vectors.
+constexpr int kAtoms = 32;
+
+__global__ __launch_bounds__(64) void k_train_grad(TrainArgs A, int e,
+                                                   const int* __restrict__ atoms,
+                                                   int na, int perAtom,
+                                                   double* __restrict__ part,
+                                                   int nw)
+{
+    extern __shared__ double lds[];
+    ModelTables const& T = A.T;
+    ElemDesc const& ed = T.elem[e];
+    int const L = ed.nlayers - 1;
+    int const t = threadIdx.x;
+    int const b = blockIdx.x;
+    int const nat = min(kAtoms, na - b * kAtoms);
+    const double* __restrict__ W = T.weights;
+    // LDS layout per atom: for l = 1..L: X[l-1] (w_{l-1}), Tg[l-1]
+    // (w_{l-1}), Z[l] (w_l), S[l] (w_l); then Xb, Tb (maxwidth each).
+    int lo[kMaxLayers];   // offset of layer l's block within an atom
+    {
+        int p = 0;
+        for (int l = 1; l <= L; ++l)
+        {
+            lo[l] = p;
+            p += 2 * ed.width[l - 1] + 2 * ed.width[l];
+        }
+        lo[0] = p;        // adjoint scratch
+    }
+    double const kappa = 2.0 * A.wE * (A.energy[0] - A.Eref)
+                       / ((double)A.n * (double)A.n);
+    if (t < nat)
+    {
+        int const a = atoms[b * kAtoms + t];
+        double* __restrict__ P = lds + (size_t)t * perAtom;
+        // inputs: scaled G and its tangent u
+        {
+            double* X0 = P + lo[1];
+            double* T0 = X0 + ed.width[0];
+            for (int j = 0; j < ed.width[0]; ++j)
+            {
+                X0[j] = A.G[(size_t)a * A.ldG + j];
+                T0[j] = A.U[(size_t)a * A.ldG + j];
+            }
+        }
+        // forward with tangent
+        for (int l = 1; l <= L; ++l)
+        {
+            int const nin = ed.width[l - 1], nout = ed.width[l];
+            double* X = P + lo[l];
+            double* Tg = X + nin;
+            double* Z = Tg + nin;
+            double* S = Z + nout;
+            double* Xn = l < L ? P + lo[l + 1] : nullptr;
+            double* Tn = l < L ? Xn + nout : nullptr;
+            const double* Wl = W + ed.woff[l];
+            const double* bl = W + ed.boff[l];
+            for (int k = 0; k < nout; ++k)
+            {
+                double z = bl[k], s = 0.0;
+                for (int j = 0; j < nin; ++j)
+                {
+                    z += Wl[(size_t)k * nin + j] * X[j];
+                    s += Wl[(size_t)k * nin + j] * Tg[j];
+                }
+                Z[k] = z;
+                S[k] = s;
+                if (l < L)
+                {
+                    double v, d, dd;
+                    activate2(ed.act[l], z, v, d, dd);
+                    Xn[k] = v;
+                    Tn[k] = d * s;
+                }
+            }
+        }
+        // reverse through value and tangent
+        double* Xb = P + lo[0];
+        double* Tb = Xb + ed.maxwidth;
+        Xb[0] = kappa;
+        Tb[0] = 1.0;
+        for (int l = L; l >= 1; --l)
+        {
+            int const nin = ed.width[l - 1], nout = ed.width[l];
+            double* X = P + lo[l];
+            double* Z = X + 2 * nin;
+            double* S = Z + nout;
+            const double* Wl = W + ed.woff[l];
+            for (int k = 0; k < nout; ++k)
+            {
+                double v, d, dd;
+                activate2(ed.act[l], Z[k], v, d, dd);
+                double const zb = Xb[k] * d + Tb[k] * S[k] * dd;
+                double const sb = Tb[k] * d;
+                Z[k] = zb;       // adjoint of z replaces z
+                S[k] = sb;       // adjoint of s replaces s
+            }
+            if (l > 1)
+                for (int j = 0; j < nin; ++j)
+                {
+                    double xb = 0.0, tb = 0.0;
+                    for (int k = 0; k < nout; ++k)
+                    {
+                        xb += Wl[(size_t)k * nin + j] * Z[k];
+                        tb += Wl[(size_t)k * nin + j] * S[k];
+                    }
+                    Xb[j] = xb;
+                    Tb[j] = tb;
+                }
+        }
+    }
+    __syncthreads();
+    // dW_l[k][j] = sum_atoms zb_k x_j + sb_k t_j;  db_l[k] = sum_atoms zb_k
+    int const base = ed.woff[1];
+    for (int idx = t; idx < nw; idx += 64)
+    {
+        int const gidx = base + idx;
+        int l = 1;
+        while (l < L && gidx >= ed.woff[l + 1]) ++l;
+        int const nin = ed.width[l - 1], nout = ed.width[l];
+        double acc = 0.0;
+        if (gidx < ed.boff[l])
+        {
+            int const r = gidx - ed.woff[l];
+            int const k = r / nin, j = r - k * nin;
+            for (int q = 0; q < nat; ++q)
+            {
+                double const* X = lds + (size_t)q * perAtom + lo[l];
+                double const* Z = X + 2 * nin;
+                acc += Z[k] * X[j] + Z[nout + k] * X[nin + j];
+            }
+        }
+        else
+        {
+            int const k = gidx - ed.boff[l];
+            for (int q = 0; q < nat; ++q)
+                acc += (lds + (size_t)q * perAtom + lo[l])[2 * nin + k];
+        }
+        part[(size_t)b * nw + idx] = acc;
+    }
+}
+
+// grad[base + idx] = sum over blocks of part (fixed order).
+__global__ void k_train_reduce(const double* __restrict__ part, int nb,
+                               int nw, double* __restrict__ grad)
+{
+    int const idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nw) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += part[(size_t)b * nw + idx];
+    grad[idx] = acc;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) on w with gradient g at step t >= 1.
+__global__ void k_adam(int n, double* __restrict__ w,
+                       float* __restrict__ w32,
+                       const double* __restrict__ g, double* __restrict__ m,
+                       double* __restrict__ v, double lr, double b1,
+                       double b2, double eps, double c1, double c2)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double const gi = g[i];
+    double const mi = b1 * m[i] + (1.0 - b1) * gi;
+    double const vi = b2 * v[i] + (1.0 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    double const wi = w[i] - lr * (mi / c1) / (sqrt(vi / c2) + eps);
+    w[i] = wi;
+    w32[i] = (float)wi;
+}
+
+__global__ void k_to_f32(int n, const double* __restrict__ w,
+                         float* __restrict__ w32)
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w32[i] = (float)w[i];
+}
+
+}  // namespace
+
+int train_atoms_per_block() { return kAtoms; }
+
+size_t train_lds_per_atom(ElemDesc const& ed)
+{
+    size_t p = 0;
+    for (int l = 1; l < ed.nlayers; ++l)
+        p += 2 * ed.width[l - 1] + 2 * ed.width[l];
+    return p + 2 * ed.maxwidth;
+}
+
+hipError_t launch_train_u(TrainArgs const& A, hipStream_t s)
+{
+    if (A.n <= 0) return hipSuccess;
+    k_train_u<<<A.n, 64, 0, s>>>(A);
+    return hipGetLastError();
+}
+
+hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,
+                             const int* atoms, int na, double* part, int nw,
+                             double* grad, hipStream_t s)
+{
+    if (na <= 0)
+    {
+        k_train_reduce<<<(nw + 255) / 256, 256, 0, s>>>(part, 0, nw, grad);
+        return hipGetLastError();
+    }
+    int const per = (int)train_lds_per_atom(ed);
+    int const nb = (na + kAtoms - 1) / kAtoms;
+    size_t const shm = (size_t)per * kAtoms * sizeof(double);
+    k_train_grad<<<nb, 64, shm, s>>>(A, e, atoms, na, per, part, nw);
+    k_train_reduce<<<(nw + 255) / 256, 256, 0, s>>>(part, nb, nw, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(int n, double* w, float* w32, const double* g,
+                       double* m, double* v, double lr, double b1, double b2,
+                       double eps, int64_t step, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    double const c1 = 1.0 - std::pow(b1, (double)step);
+    double const c2 = 1.0 - std::pow(b2, (double)step);
+    k_adam<<<(n + 255) / 256, 256, 0, s>>>(n, w, w32, g, m, v, lr, b1, b2,
+                                           eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_weights_to_f32(int n, const double* w, float* w32,
+                                 hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_to_f32<<<(n + 255) / 256, 256, 0, s>>>(n, w, w32);
+    return hipGetLastError();
+}
+
+}  // namespace hdnnp

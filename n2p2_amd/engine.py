@@ -99,6 +99,21 @@ _SIGNATURES = [
     ("hdnnp_md_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     ("hdnnp_md_get_state", ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     ("hdnnp_md_rebuilds", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_md_partner_misses", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_mts_init", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_double, ctypes.c_int,
+      ctypes.c_double, ctypes.c_double, _c_double_p, ctypes.c_double, ctypes.c_int]),
+    ("hdnnp_mts_run", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_double_p, _c_double_p, _c_double_p, _c_int32_p]),
+    ("hdnnp_num_weights", ctypes.c_int64, [ctypes.c_void_p]),
+    ("hdnnp_get_weights", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_set_weights", ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    ("hdnnp_train_gradient", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      _c_double_p, _c_double_p]),
+    ("hdnnp_train_adam", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    ("hdnnp_train_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -326,6 +341,63 @@ class Context:
 
     def md_rebuilds(self):
         return _lib.hdnnp_md_rebuilds(self._h)
+
+    def md_partner_misses(self):
+        return _lib.hdnnp_md_partner_misses(self._h)
+
+    # --- multiple-time-step MD (this context = the fast surrogate)
+    def mts_init(self, full, masses, dt_outer=1.0, n_inner=4, skin=1.0,
+                 v=None, rmse_gate=5e-3, ftoa=FTOA_EV_A_AMU_FS, graph=True):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        vp = None
+        if v is not None:
+            self._v0 = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+            vp = _ptr(self._v0, ctypes.c_double)
+        _check(_lib.hdnnp_mts_init(full._h, self._h, _ptr(m, ctypes.c_double),
+                                   dt_outer, n_inner, skin, ftoa, vp,
+                                   rmse_gate, int(graph)))
+
+    def mts_run(self, nouter):
+        """nouter outer steps; returns (epot, ekin, rmse, mode) per step."""
+        ep, ek, rm = np.zeros(nouter), np.zeros(nouter), np.zeros(nouter)
+        md = np.zeros(nouter, dtype=np.int32)
+        _check(_lib.hdnnp_mts_run(self._h, nouter, _ptr(ep, ctypes.c_double),
+                                  _ptr(ek, ctypes.c_double),
+                                  _ptr(rm, ctypes.c_double),
+                                  _ptr(md, ctypes.c_int32)))
+        return ep, ek, rm, md
+
+    # --- training of a radial-only model
+    def num_weights(self):
+        return _lib.hdnnp_num_weights(self._h)
+
+    def get_weights(self):
+        w = np.zeros(self.num_weights())
+        _check(_lib.hdnnp_get_weights(self._h, _ptr(w, ctypes.c_double)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.num_weights()
+        _check(_lib.hdnnp_set_weights(self._h, _ptr(w, ctypes.c_double)))
+
+    def train_gradient(self, f_ref, e_ref, w_e=1.0, w_f=1.0, want_grad=True):
+        """Gradient of w_e ((E - e_ref)/N)^2 + w_f/(3N) sum |F - f_ref|^2 at
+        the loaded structure; returns (grad or None, (loss_e, loss_f))."""
+        f = np.ascontiguousarray(f_ref, dtype=np.float64).reshape(-1, 3)
+        g = np.zeros(self.num_weights()) if want_grad else None
+        loss = np.zeros(2)
+        _check(_lib.hdnnp_train_gradient(
+            self._h, _ptr(f, ctypes.c_double), e_ref, w_e, w_f,
+            _ptr(g, ctypes.c_double) if g is not None else None,
+            _ptr(loss, ctypes.c_double)))
+        return g, (float(loss[0]), float(loss[1]))
+
+    def train_adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(_lib.hdnnp_train_adam(self._h, lr, beta1, beta2, eps))
+
+    def train_reset(self):
+        _check(_lib.hdnnp_train_reset(self._h))
 
     def set_neighbor_list(self, centre_elem, offsets, nbr_index, nbr_elem,
                           dr, centre_index=None):

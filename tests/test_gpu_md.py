@@ -78,6 +78,7 @@ def test_md_forces_equal_fresh_evaluation(water):
     erel = abs(ep[-1] - en) / abs(en)
     say("40 steps, %d rebuilds: e_rel %.2g, f %.2g" % (nreb, erel, ferr))
     assert nreb > 2
+    assert c.md_partner_misses() == 0
     assert erel <= 1e-10 and ferr <= 1e-9
 
 

@@ -1,0 +1,90 @@
+"""Multiple-time-step MD (hdnnp_mts_*): r-RESPA with a surrogate as the fast
+force and the full model as the reference, with a force-RMSE gate that
+hands whole outer steps to the full model.
+
+- A surrogate identical to the full model makes the slow force exactly
+  zero: the run equals plain velocity Verlet at the inner step (positions
+  within 1e-9 A after 80 substeps, energies within 1e-10 relative).
+- A gate below zero sends every outer step to the full model: again plain
+  velocity Verlet at the inner step.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from n2p2_amd import engine
+from n2p2_amd import workloads as W
+from tests import cases
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="the multiple-time-step integrator is switched "
+                            "off: two device runs of it ended in a memory "
+                            "fault whose cause has not been found"),
+]
+
+DT_OUTER, N_INNER = 1.0, 4
+
+
+def say(*a):
+    print("[gpu-mts]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def water(tmp_path_factory):
+    d, r, e, box = cases.water_case(tmp_path_factory.mktemp("mts"))
+    masses = np.array(W.WATER_MASSES)
+    v = W.maxwell_boltzmann(masses[e], 300.0, seed=9)
+    return d, r, e, box, masses, v
+
+
+def _plain(d, r, e, box, masses, v, nsteps):
+    c = engine.Context(engine.Model.load_dir(d))
+    c.set_structure(r, e, box)
+    c.md_init(masses, DT_OUTER / N_INNER, skin=1.0, v=v)
+    ep, ek = c.md_run(nsteps)
+    x, u = c.md_state()
+    return ep, ek, x, u
+
+
+def _mts(d_full, d_sur, r, e, box, masses, v, nouter, gate):
+    full = engine.Context(engine.Model.load_dir(d_full))
+    sur = engine.Context(engine.Model.load_dir(d_sur))
+    for c in (full, sur):
+        c.set_structure(r, e, box)
+    sur.mts_init(full, masses, DT_OUTER, N_INNER, skin=1.0, v=v,
+                 rmse_gate=gate)
+    ep, ek, rm, md = sur.mts_run(nouter)
+    x, u = sur.md_state()
+    return ep, ek, rm, md, x, u
+
+
+def _same_mod_lattice(x, y, box):
+    d = W.remap(x, box) - W.remap(y, box)
+    L = np.diag(box)
+    d -= L * np.round(d / L)
+    return float(np.abs(d).max())
+
+
+@pytest.mark.parametrize("gate", [1.0, -1.0])
+def test_mts_reduces_to_plain_verlet(water, gate):
+    d, r, e, box, masses, v = water
+    nouter = 20
+    ep, ek, rm, md, x, u = _mts(d, d, r, e, box, masses, v, nouter, gate)
+    pep, pek, px, pu = _plain(d, r, e, box, masses, v, nouter * N_INNER)
+    dx = _same_mod_lattice(x, px, box)
+    du = float(np.abs(u - pu).max())
+    de = float(np.max(np.abs(ep - pep[N_INNER - 1::N_INNER])
+                      / np.abs(pep[N_INNER - 1::N_INNER])))
+    dk = float(np.max(np.abs(ek - pek[N_INNER - 1::N_INNER])
+                      / np.abs(pek[N_INNER - 1::N_INNER])))
+    say("gate %g: modes %s, max rmse %.3g, dx %.2g A, dv %.2g, de %.2g, dk %.2g"
+        % (gate, np.bincount(md, minlength=2).tolist(), rm.max(), dx, du, de,
+           dk))
+    if gate > 0:
+        assert np.all(md == 0) and rm.max() == 0.0
+    else:
+        assert np.all(md == 1)
+    assert dx <= 1e-9 and du <= 1e-9 and de <= 1e-10 and dk <= 1e-10
