@@ -349,6 +349,12 @@ int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom);
  * scaling, per SF of element e: min, max, mean, sigma (4 doubles each).
  * Needs hdnnp_neighbor_list first.  Returns the number of atoms of e. */
 int64_t hdnnp_sf_statistics(hdnnp_ctx* c, int e, double* out);
+/* The accumulators behind it (SymFncStatistics::Container,
+ * SymFncStatistics.h:28-147): per SF min, max, sum, sum of squares, for
+ * combining several loads (batches of structures) into one statistics set,
+ * as nnp-scaling combines structures and MPI ranks
+ * (Dataset::collectSymmetryFunctionStatistics).  Returns the atom count. */
+int64_t hdnnp_sf_statistics_raw(hdnnp_ctx* c, int e, double* out);
 
 /* ---------------------------------------------------------- profiling -- */
 

@@ -2382,6 +2382,26 @@ int64_t hdnnp_sf_statistics(hdnnp_ctx* c, int e, double* out)
     return rc == HDNNP_OK ? na : rc;
 }
 
+int64_t hdnnp_sf_statistics_raw(hdnnp_ctx* c, int e, double* out)
+{
+    int64_t na = -1;
+    int rc = guarded([&]() {
+        requireNL(c);
+        if (e < 0 || e >= c->T.ne)
+            throw Error(HDNNP_E_INVALID, "ERROR: Element out of range.\n");
+        computeSF(c, true);
+        int const nsf = c->flat.elem[e].nsf;
+        na = (int64_t)c->hAtomsOfElem[e].size();
+        c->dStatOut.ensure(4 * (size_t)nsf);
+        hipCheck(launch_sf_stats(c->dG.p, c->ld, c->dAtomsOfElem[e].p, (int)na,
+                                 nsf, c->dStatOut.p, c->stream), "sf stats");
+        copyOut(c, out, c->dStatOut.p, 4 * (size_t)nsf);
+        c->sync();
+        c->haveSF = false;
+    });
+    return rc == HDNNP_OK ? na : rc;
+}
+
 int hdnnp_profile_enable(hdnnp_ctx* c, int enable)
 {
     c->prof = enable != 0;

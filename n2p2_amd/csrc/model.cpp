@@ -404,6 +404,8 @@ void Model::load(std::string const& settingsFile,
 
     // Mode.cpp:2270-2289 weights, NeuralNetwork.cpp:147-169 file order:
     // per layer, for each input j, for each neuron k: W[k][j]; then biases.
+    // An empty weightsFmt leaves the weights zero (the symmetry-function
+    // tools, e.g. nnp-scaling, set up no networks).
     for (auto& e : el)
     {
         for (size_t j = 0; j < e.width.size(); ++j)
@@ -411,21 +413,31 @@ void Model::load(std::string const& settingsFile,
                 throw std::runtime_error(
                     "ERROR: NN \"short\", element " + e.symbol
                     + ": layer topology or activation unset.\n");
-        char buf[4096];
-        std::snprintf(buf, sizeof buf, weightsFmt.c_str(), (size_t)e.Z);
-        std::ifstream f(buf);
-        if (!f.is_open())
-            throw std::runtime_error(std::string("ERROR: Could not open "
-                                     "file: \"") + buf + "\".\n");
+        char buf[4096] = "";
         std::vector<double> file;
-        std::string line;
-        while (std::getline(f, line))
+        if (!weightsFmt.empty())
         {
-            if (line.empty() || line[0] == '#') continue;
-            auto w = words(line);
-            if (!w.empty()) file.push_back(std::atof(w[0].c_str()));
+            std::snprintf(buf, sizeof buf, weightsFmt.c_str(), (size_t)e.Z);
+            std::ifstream f(buf);
+            if (!f.is_open())
+                throw std::runtime_error(std::string("ERROR: Could not open "
+                                         "file: \"") + buf + "\".\n");
+            std::string line;
+            while (std::getline(f, line))
+            {
+                if (line.empty() || line[0] == '#') continue;
+                auto w = words(line);
+                if (!w.empty()) file.push_back(std::atof(w[0].c_str()));
+            }
         }
         int const L = (int)e.width.size();
+        if (weightsFmt.empty())
+        {
+            size_t nw = 0;
+            for (int l = 1; l < L; ++l)
+                nw += (size_t)e.width[l] * e.width[l - 1] + e.width[l];
+            file.assign(nw, 0.0);
+        }
         e.weights.clear();
         e.woff.assign(L, 0);
         e.boff.assign(L, 0);

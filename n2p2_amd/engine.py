@@ -116,6 +116,7 @@ _SIGNATURES = [
     ("hdnnp_train_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
+    ("hdnnp_sf_statistics_raw", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hdnnp_profile_avg_ms", ctypes.c_double, [ctypes.c_void_p, ctypes.c_char_p, _c_int64_p]),
     ("hdnnp_profile_reset", ctypes.c_int, [ctypes.c_void_p]),
@@ -519,6 +520,14 @@ class Context:
     def sf_statistics(self, e):
         out = np.zeros((self.model.num_sf(e), 4))
         na = _lib.hdnnp_sf_statistics(self._h, e, _ptr(out, ctypes.c_double))
+        if na < 0:
+            _check(int(na))
+        return na, out
+
+    def sf_statistics_raw(self, e):
+        """(count, [nsf x 4]: min, max, sum, sum of squares) of element e."""
+        out = np.zeros((self.model.num_sf(e), 4))
+        na = _lib.hdnnp_sf_statistics_raw(self._h, e, _ptr(out, ctypes.c_double))
         if na < 0:
             _check(int(na))
         return na, out

@@ -89,6 +89,53 @@ def read_structure(path, element_index):
     return r, elem, b, energy
 
 
+def read_structures(path, element_index):
+    """Every structure of an input.data file (Structure::readFromFile over
+    the whole file, Dataset::readFromFile): a list of (r, elem, box or None,
+    energy_ref), positions remapped into the cell."""
+    out, block = [], []
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if not w:
+                continue
+            block.append(line)
+            if w[0] == "end":
+                out.append(_parse_block(block, element_index))
+                block = []
+    return out
+
+
+def _parse_block(lines, element_index):
+    r, elem, box = [], [], []
+    energy = 0.0
+    for line in lines:
+        w = line.split()
+        if w[0] in ("begin", "comment", "charge", "end"):
+            continue
+        if w[0] == "lattice":
+            box.append([float(x) for x in w[1:4]])
+        elif w[0] == "atom":
+            r.append([float(x) for x in w[1:4]])
+            e = element_index(w[4])
+            if e < 0:
+                raise RuntimeError("ERROR: Unknown element \"%s\".\n" % w[4])
+            elem.append(e)
+        elif w[0] == "energy":
+            energy = float(w[1])
+        else:
+            raise RuntimeError("ERROR: Unexpected file content, unknown "
+                               "keyword \"%s\".\n" % w[0])
+    if len(box) not in (0, 3):
+        raise RuntimeError("ERROR: Strange number of box vectors.\n")
+    r = np.array(r, dtype=np.float64).reshape(-1, 3)
+    elem = np.array(elem, dtype=np.int32)
+    b = np.array(box, dtype=np.float64) if box else None
+    if b is not None:
+        r = remap(r, b)
+    return r, elem, b, energy
+
+
 def to_normalized(r, box, conv_length):
     """Structure::toNormalizedUnits for positions and box."""
     r = r * conv_length

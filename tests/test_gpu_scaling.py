@@ -1,0 +1,88 @@
+"""nnp-scaling on the GPU (n2p2_amd/scaling.py) against the scaling.data
+the reference wrote for its own example data set (tests/golden/
+nnp_scaling_H2O: examples/nnp-train/H2O_RPBE-D3, 20 structures, G2 + G3,
+tanh-type cutoff, normalised units).
+
+- One batch of all 20 structures: min, max and mean within 1e-13
+  relative, sigma within 1e-10 (the SF sums and the statistics sums run in
+  another order than the reference's sequential loops).
+- Several batches (a small atom limit per batch) combine to the same
+  statistics: min and max identical, mean and sigma within 1e-13.
+- The command-line tool writes a scaling.data that reads back to the same
+  values.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from n2p2_amd import engine
+from n2p2_amd import io as nio
+from n2p2_amd import scaling as SC
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "nnp_scaling_H2O")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print("[gpu-scaling]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _golden():
+    rows = np.array([[float(x) for x in l.split()]
+                     for l in open(os.path.join(GOLD, "scaling.data"))
+                     if l.strip() and not l.startswith("#")])
+    return [rows[rows[:, 0] == e][:, 2:] for e in (1, 2)]
+
+
+def _structures():
+    m = engine.Model.load(os.path.join(GOLD, "input.nn"), None, "")
+    return [(r, e, b) for r, e, b, _ in
+            nio.read_structures(os.path.join(GOLD, "input.data"),
+                                m.element_index)]
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def test_gpu_scaling_matches_reference_file():
+    st = SC.statistics(os.path.join(GOLD, "input.nn"), _structures())
+    for k, (ours, ref) in enumerate(zip(st, _golden())):
+        errs = [_rel(ours[:, c], ref[:, c]) for c in range(4)]
+        say("element %d: min %.2g max %.2g mean %.2g sigma %.2g"
+            % ((k,) + tuple(errs)))
+        assert max(errs[:3]) <= 1e-13 and errs[3] <= 1e-10
+
+
+def test_batches_combine():
+    sts = _structures()
+    one = SC.statistics(os.path.join(GOLD, "input.nn"), sts)
+    many = SC.statistics(os.path.join(GOLD, "input.nn"), sts, max_atoms=700)
+    for a, b in zip(one, many):
+        assert np.array_equal(a[:, :2], b[:, :2])
+        assert _rel(b[:, 2], a[:, 2]) <= 1e-13
+        assert _rel(b[:, 3], a[:, 3]) <= 1e-13
+
+
+def test_command_line_tool(tmp_path):
+    for f in ("input.nn", "input.data"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path)
+    env = dict(os.environ, PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, "-m", "n2p2_amd.scaling", "500"],
+                       cwd=str(tmp_path), env=env, capture_output=True,
+                       text=True, timeout=120)
+    say(r.stdout.strip(), r.stderr.strip()[-300:])
+    assert r.returncode == 0
+    rows = np.array([[float(x) for x in l.split()]
+                     for l in open(tmp_path / "scaling.data")
+                     if l.strip() and not l.startswith("#")])
+    ref = np.concatenate(_golden())
+    assert rows.shape == (len(ref), 6)
+    assert _rel(rows[:, 2:5], ref[:, :3]) <= 1e-13
