@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--nmol", type=int, default=65536)
     ap.add_argument("--box", type=float, default=124.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--path", choices=("both", "mixed", "fp64"),
+    ap.add_argument("--path", choices=("both", "mixed", "fp64", "md"),
                     default="both",
                     help="time one path only (for profiling runs; the "
                          "reported line needs both)")
@@ -247,6 +247,11 @@ def main():
                           [[model.num_sf(e)] + list(hidden) + [1]])
                       for e in range(model.num_elements)))
 
+        if args.path == "md":
+            res = run_md(model, r, elem, box, engine.PRECISION_MIXED,
+                         args.steps, args.warmup)
+            log("MD mixed only: %s (no JSON line)" % res)
+            return
         if args.path != "mixed":
             dt64, e64, f64, kt64, sp64 = run_path(
                 ctx, engine.PRECISION_FP64, args.steps, args.warmup)

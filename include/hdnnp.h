@@ -338,6 +338,16 @@ int hdnnp_train_adam(hdnnp_ctx* c, double lr, double beta1, double beta2,
                      double eps);
 /* Restart Adam (zero moments at the next step). */
 int hdnnp_train_reset(hdnnp_ctx* c);
+/* Derivative of every structure's energy with respect to every network
+ * weight, for any model (only the SFs enter): J[s * nw + w], nw =
+ * hdnnp_num_weights, s over the loaded structures (one, or the batch of
+ * hdnnp_set_structures).  Replaces, per structure, the sum over its atoms
+ * of NeuralNetwork::calculateDEdc (NeuralNetwork.cpp:444-490) as training
+ * forms it (Training.cpp, energy update).  file_order != 0: weights in the
+ * reference's connection order (NeuralNetwork.cpp:147-169, element after
+ * element); 0: the engine's flat order of hdnnp_get_weights.  Needs the
+ * neighbour list; normalize_nodes is not supported. */
+int hdnnp_energy_weight_jacobian(hdnnp_ctx* c, double* J, int file_order);
 
 /* Number of extrapolation warnings of the last SF evaluation: G outside
  * [Gmin, Gmax] +- 1000 eps before scaling (Element.cpp:462-541).

@@ -227,6 +227,10 @@ struct hdnnp_ctx
         int recCap = 0;
     } mts;
 
+    // Energy weight Jacobian (hdnnp_energy_weight_jacobian).
+    DBuf<int> jAtoms, jB0, jBN, jBE, jGrp0, jBase, jNwe;
+    DBuf<double> jPart, jOut;
+
     // Training of a radial-only model (hdnnp_train_*).
     DBuf<double> tU, tD2, tPart, tGrad, tM, tV, tFref, tLoss;
     int64_t adamStep = 0;
@@ -1280,6 +1284,9 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
         c->tU.release(); c->tD2.release(); c->tPart.release();
         c->tGrad.release(); c->tM.release(); c->tV.release();
         c->tFref.release(); c->tLoss.release();
+        c->jAtoms.release(); c->jB0.release(); c->jBN.release();
+        c->jBE.release(); c->jGrp0.release(); c->jBase.release();
+        c->jNwe.release(); c->jPart.release(); c->jOut.release();
         c->mts.d2.release(); c->mts.sum.release(); c->mts.epot.release();
         c->mts.ekin.release(); c->mts.counter.release();
         c->mts.full = nullptr;
@@ -2167,6 +2174,120 @@ int hdnnp_train_adam(hdnnp_ctx* c, double lr, double beta1, double beta2,
                              eps, c->adamStep, c->stream), "adam");
         c->sync();
         c->haveNN = c->haveNNDeriv = false;
+    });
+}
+
+int hdnnp_energy_weight_jacobian(hdnnp_ctx* c, double* J, int file_order)
+{
+    return guarded([&]() {
+        requireNL(c);
+        int const ne = c->T.ne, n = (int)c->n;
+        int const per = train_atoms_per_block();
+        int perAtom = 0, maxP = 0;
+        std::vector<int> base(ne), nwe(ne);
+        for (int e = 0; e < ne; ++e)
+        {
+            ElemDesc const& ed = c->flat.elem[e];
+            if (ed.normalize)
+                throw Error(HDNNP_E_UNSUPPORTED, "ERROR: normalize_nodes is "
+                                                 "not supported here.\n");
+            int pa = ed.maxwidth, nw = 0;
+            for (int l = 1; l < ed.nlayers; ++l)
+            {
+                pa += ed.width[l - 1] + ed.width[l];
+                nw += ed.width[l] * ed.width[l - 1] + ed.width[l];
+            }
+            perAtom = std::max(perAtom, pa);
+            maxP = std::max(maxP, nw);
+            base[e] = ed.woff[1];
+            nwe[e] = nw;
+        }
+        if ((size_t)perAtom * per * sizeof(double) > 65536)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: Networks too wide for the "
+                                          "weight-gradient kernel.\n");
+        std::vector<int> soff;
+        if (c->batched) soff = c->hStructOff;
+        else soff = {0, n};
+        int const ns = (int)soff.size() - 1;
+        std::vector<int> atoms, b0, bn, be, grp0;
+        for (int s = 0; s < ns; ++s)
+            for (int e = 0; e < ne; ++e)
+            {
+                grp0.push_back((int)b0.size());
+                int cnt = 0;
+                for (int i = soff[s]; i < soff[s + 1]; ++i)
+                {
+                    if (c->hElem[i] != e) continue;
+                    if (cnt % per == 0)
+                    {
+                        b0.push_back((int)atoms.size());
+                        bn.push_back(0);
+                        be.push_back(e);
+                    }
+                    atoms.push_back(i);
+                    bn.back() += 1;
+                    ++cnt;
+                }
+            }
+        grp0.push_back((int)b0.size());
+        int const nblk = (int)b0.size();
+        c->setDevice();
+        hipStream_t st = c->stream;
+        auto up = [&](DBuf<int>& d, std::vector<int> const& v) {
+            d.ensure(std::max<size_t>(v.size(), 1));
+            if (!v.empty())
+                hipCheck(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(int),
+                                        hipMemcpyHostToDevice, st),
+                         "jacobian tables");
+        };
+        up(c->jAtoms, atoms);
+        up(c->jB0, b0);
+        up(c->jBN, bn);
+        up(c->jBE, be);
+        up(c->jGrp0, grp0);
+        up(c->jBase, base);
+        up(c->jNwe, nwe);
+        size_t const nwTot = c->flat.weights.size();
+        c->jPart.ensure((size_t)std::max(nblk, 1) * maxP);
+        c->jOut.ensure((size_t)ns * nwTot);
+        hipCheck(hipMemsetAsync(c->jOut.p, 0, (size_t)ns * nwTot * sizeof(double),
+                                st), "jacobian");
+        computeSF(c, false);
+        hipCheck(launch_energy_wgrad(c->T, c->dG.p, c->ld, c->jAtoms.p,
+                                     c->jB0.p, c->jBN.p, c->jBE.p, nblk,
+                                     perAtom, c->jPart.p, maxP, c->jGrp0.p,
+                                     c->jBase.p, c->jNwe.p, ne, ns,
+                                     (int)nwTot, c->jOut.p, st),
+                 "energy weight gradient");
+        std::vector<double> h((size_t)ns * nwTot);
+        copyOut(c, h.data(), c->jOut.p, h.size());
+        c->sync();
+        if (!file_order)
+        {
+            std::copy(h.begin(), h.end(), J);
+            return;
+        }
+        // NeuralNetwork::setConnections order (NeuralNetwork.cpp:147-169):
+        // per layer, for each input j, for each neuron k: W[k][j]; then b.
+        for (int s = 0; s < ns; ++s)
+        {
+            double const* src = h.data() + (size_t)s * nwTot;
+            double* dst = J + (size_t)s * nwTot;
+            size_t q = 0;
+            for (int e = 0; e < ne; ++e)
+            {
+                ElemDesc const& ed = c->flat.elem[e];
+                for (int l = 1; l < ed.nlayers; ++l)
+                {
+                    int const nin = ed.width[l - 1], nout = ed.width[l];
+                    for (int j = 0; j < nin; ++j)
+                        for (int k = 0; k < nout; ++k)
+                            dst[q++] = src[ed.woff[l] + (size_t)k * nin + j];
+                    for (int k = 0; k < nout; ++k)
+                        dst[q++] = src[ed.boff[l] + k];
+                }
+            }
+        }
     });
 }
 

@@ -110,6 +110,15 @@ hipError_t launch_adam(int n, double* w, float* w32, const double* g,
                        double eps, int64_t step, hipStream_t s);
 hipError_t launch_weights_to_f32(int n, const double* w, float* w32,
                                  hipStream_t s);
+// dE_s/dw per structure (blocks of at most train_atoms_per_block() atoms of
+// one element and one structure; LDS perAtom doubles per atom).
+hipError_t launch_energy_wgrad(ModelTables const& T, const double* G, int ldG,
+                               const int* blkAtoms, const int* blk0,
+                               const int* blkN, const int* blkElem, int nblk,
+                               int perAtom, double* part, int maxP,
+                               const int* grp0, const int* base,
+                               const int* nwe, int ne, int nstruct,
+                               int nwTotal, double* J, hipStream_t s);
 
 // --- MD with a Verlet-skin list (kernels_md.hip)
 struct MDCell

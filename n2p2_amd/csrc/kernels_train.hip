@@ -312,7 +312,158 @@ __global__ void k_to_f32(int n, const double* __restrict__ w,
     if (i < n) w32[i] = (float)w[i];
 }
 
+// dE_s/dw for every structure s: per block (atoms of one element and one
+// structure, at most kAtoms) the forward pass and backpropagation of each
+// atom's network (NeuralNetwork::calculateDEdc, NeuralNetwork.cpp:444-490),
+// then per weight the sum over the block's atoms in atom order.  LDS per
+// atom: for every layer its input x and the adjoint of z, plus one adjoint
+// scratch vector.
+__global__ __launch_bounds__(64) void k_energy_wgrad(
+    ModelTables T, const double* __restrict__ G, int ldG,
+    const int* __restrict__ blkAtoms, const int* __restrict__ blk0,
+    const int* __restrict__ blkN, const int* __restrict__ blkElem,
+    int perAtom, double* __restrict__ part, int maxP)
+{
+    extern __shared__ double lds[];
+    int const b = blockIdx.x;
+    int const t = threadIdx.x;
+    int const e = blkElem[b];
+    ElemDesc const& ed = T.elem[e];
+    int const L = ed.nlayers - 1;
+    int const nat = blkN[b];
+    const double* __restrict__ W = T.weights;
+    int lo[kMaxLayers];
+    {
+        int p = 0;
+        for (int l = 1; l <= L; ++l)
+        {
+            lo[l] = p;
+            p += ed.width[l - 1] + ed.width[l];
+        }
+        lo[0] = p;
+    }
+    if (t < nat)
+    {
+        int const a = blkAtoms[blk0[b] + t];
+        double* __restrict__ P = lds + (size_t)t * perAtom;
+        for (int j = 0; j < ed.width[0]; ++j)
+            P[lo[1] + j] = G[(size_t)a * ldG + j];
+        for (int l = 1; l <= L; ++l)
+        {
+            int const nin = ed.width[l - 1], nout = ed.width[l];
+            double* X = P + lo[l];
+            double* Z = X + nin;
+            double* Xn = l < L ? P + lo[l + 1] : nullptr;
+            const double* Wl = W + ed.woff[l];
+            const double* bl = W + ed.boff[l];
+            for (int k = 0; k < nout; ++k)
+            {
+                double z = bl[k];
+                for (int j = 0; j < nin; ++j)
+                    z += Wl[(size_t)k * nin + j] * X[j];
+                Z[k] = z;
+                if (l < L)
+                {
+                    double v, d;
+                    activate(ed.act[l], z, v, d);
+                    Xn[k] = v;
+                }
+            }
+        }
+        double* Xb = P + lo[0];
+        for (int k = 0; k < ed.width[L]; ++k) Xb[k] = 1.0;
+        for (int l = L; l >= 1; --l)
+        {
+            int const nin = ed.width[l - 1], nout = ed.width[l];
+            double* X = P + lo[l];
+            double* Z = X + nin;
+            const double* Wl = W + ed.woff[l];
+            for (int k = 0; k < nout; ++k)
+            {
+                double v, d;
+                activate(ed.act[l], Z[k], v, d);
+                Z[k] = Xb[k] * d;        // adjoint of z replaces z
+            }
+            if (l > 1)
+                for (int j = 0; j < nin; ++j)
+                {
+                    double xb = 0.0;
+                    for (int k = 0; k < nout; ++k)
+                        xb += Wl[(size_t)k * nin + j] * Z[k];
+                    Xb[j] = xb;
+                }
+        }
+    }
+    __syncthreads();
+    int const base = ed.woff[1];
+    int nw = 0;
+    for (int l = 1; l <= L; ++l) nw += ed.width[l] * ed.width[l - 1] + ed.width[l];
+    for (int idx = t; idx < nw; idx += 64)
+    {
+        int const gidx = base + idx;
+        int l = 1;
+        while (l < L && gidx >= ed.woff[l + 1]) ++l;
+        int const nin = ed.width[l - 1];
+        double acc = 0.0;
+        if (gidx < ed.boff[l])
+        {
+            int const r = gidx - ed.woff[l];
+            int const k = r / nin, j = r - k * nin;
+            for (int q = 0; q < nat; ++q)
+            {
+                double const* X = lds + (size_t)q * perAtom + lo[l];
+                acc += X[nin + k] * X[j];
+            }
+        }
+        else
+        {
+            int const k = gidx - ed.boff[l];
+            for (int q = 0; q < nat; ++q)
+                acc += (lds + (size_t)q * perAtom + lo[l])[nin + k];
+        }
+        part[(size_t)b * maxP + idx] = acc;
+    }
+}
+
+// J[s][base_e + idx] = sum over the blocks of (structure s, element e), in
+// block order.  One workgroup per (s, e); blocks of (s, e) are
+// [grp0[s*ne+e], grp0[s*ne+e+1]).
+__global__ void k_energy_wgrad_reduce(const double* __restrict__ part,
+                                      int maxP, const int* __restrict__ grp0,
+                                      const int* __restrict__ base,
+                                      const int* __restrict__ nwe, int ne,
+                                      int nwTotal, double* __restrict__ J)
+{
+    int const se = blockIdx.x;
+    int const s = se / ne, e = se - s * ne;
+    int const b0 = grp0[se], b1 = grp0[se + 1];
+    for (int idx = threadIdx.x; idx < nwe[e]; idx += blockDim.x)
+    {
+        double acc = 0.0;
+        for (int b = b0; b < b1; ++b) acc += part[(size_t)b * maxP + idx];
+        J[(size_t)s * nwTotal + base[e] + idx] = acc;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_energy_wgrad(ModelTables const& T, const double* G, int ldG,
+                               const int* blkAtoms, const int* blk0,
+                               const int* blkN, const int* blkElem, int nblk,
+                               int perAtom, double* part, int maxP,
+                               const int* grp0, const int* base,
+                               const int* nwe, int ne, int nstruct,
+                               int nwTotal, double* J, hipStream_t s)
+{
+    if (nblk > 0)
+        k_energy_wgrad<<<nblk, 64, (size_t)perAtom * kAtoms * sizeof(double),
+                         s>>>(T, G, ldG, blkAtoms, blk0, blkN, blkElem,
+                              perAtom, part, maxP);
+    if (nstruct * ne > 0)
+        k_energy_wgrad_reduce<<<nstruct * ne, 256, 0, s>>>(
+            part, maxP, grp0, base, nwe, ne, nwTotal, J);
+    return hipGetLastError();
+}
 
 int train_atoms_per_block() { return kAtoms; }
 

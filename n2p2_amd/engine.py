@@ -114,6 +114,8 @@ _SIGNATURES = [
     ("hdnnp_train_adam", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("hdnnp_train_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("hdnnp_energy_weight_jacobian", ctypes.c_int,
+     [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_sf_statistics_raw", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
@@ -399,6 +401,14 @@ class Context:
 
     def train_reset(self):
         _check(_lib.hdnnp_train_reset(self._h))
+
+    def energy_weight_jacobian(self, file_order=True):
+        """dE_s/dw for every loaded structure s: (nstruct, nweights)."""
+        ns = max(self.num_structures(), 1)
+        J = np.zeros((ns, self.num_weights()))
+        _check(_lib.hdnnp_energy_weight_jacobian(
+            self._h, _ptr(J, ctypes.c_double), int(file_order)))
+        return J
 
     def set_neighbor_list(self, centre_elem, offsets, nbr_index, nbr_elem,
                           dr, centre_index=None):
