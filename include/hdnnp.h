@@ -315,8 +315,8 @@ int hdnnp_mts_run(hdnnp_ctx* sur, int64_t nouter, double* epot, double* ekin,
 
 /* ---------------------------------------------------------- training -- */
 
-/* Training of a radial-only model on energies and forces (the distilled
- * surrogate of the multiple-time-step integrator).  No counterpart in the
+/* Training on energies and forces (the distilled surrogate of the
+ * multiple-time-step integrator, or any 2G model of the supported SFs).  No counterpart in the
  * 2G prediction path; the reference trains with libnnptrain
  * (Training.cpp), which this does not mirror.  Weights are the flat array
  * of every element's network in element order, per layer W[k][j]
@@ -329,8 +329,9 @@ int hdnnp_set_weights(hdnnp_ctx* c, const double* w);
  * gradient of L = w_e ((E - e_ref) / N)^2 + w_f / (3N) sum |F - f_ref|^2
  * with respect to every weight (grad: hdnnp_num_weights doubles, may be
  * NULL; it is kept for hdnnp_train_adam).  loss (2 doubles, may be NULL)
- * = the two terms.  Needs radial SFs only and identity / tanh / logistic /
- * softplus layers with one output (HDNNP_E_UNSUPPORTED otherwise). */
+ * = the two terms.  Radial and angular SFs (types 2, 3, 9); identity /
+ * tanh / logistic / softplus layers with one output (HDNNP_E_UNSUPPORTED
+ * otherwise). */
 int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
                          double w_e, double w_f, double* grad, double* loss);
 /* One Adam step (Kingma & Ba 2015) with the last gradient. */

@@ -2063,18 +2063,17 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
         for (size_t e = 0; e < c->flat.elem.size(); ++e)
         {
             ElemDesc const& ed = c->flat.elem[e];
-            bool ok = ed.ng == ed.nrad && !ed.normalize
+            bool ok = !ed.normalize
                    && ed.width[ed.nlayers - 1] == 1
-                   && train_lds_per_atom(ed) * train_atoms_per_block()
-                          * sizeof(double) <= 65536;
+                   && train_atoms_per_grad_block(ed) >= 1;
             for (int l = 1; l < ed.nlayers; ++l)
                 ok = ok && (ed.act[l] == AF_IDENTITY || ed.act[l] == AF_TANH
                             || ed.act[l] == AF_LOGISTIC
                             || ed.act[l] == AF_SOFTPLUS);
             if (!ok)
                 throw Error(HDNNP_E_UNSUPPORTED,
-                            "ERROR: Training supports radial symmetry "
-                            "functions only, networks of identity, tanh, "
+                            "ERROR: Training supports networks of "
+                            "identity, tanh, "
                             "logistic or softplus layers with one output and "
                             "no normalize_nodes, of at most a few thousand "
                             "weights.\n");
@@ -2115,10 +2114,10 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
         A.wE = w_e;
         A.cF = -2.0 * w_f / (3.0 * (double)n);
         hipCheck(launch_train_u(A, s), "train u");
-        int const per = train_atoms_per_block();
         for (size_t e = 0; e < c->flat.elem.size(); ++e)
         {
             ElemDesc const& ed = c->flat.elem[e];
+            int const per = train_atoms_per_grad_block(ed);
             int nwe = 0;
             for (int l = 1; l < ed.nlayers; ++l)
                 nwe += ed.width[l] * ed.width[l - 1] + ed.width[l];

@@ -100,6 +100,7 @@ struct TrainArgs
     double Eref, wE, cF;    // cF = -2 wF / (3N)
 };
 int train_atoms_per_block();
+int train_atoms_per_grad_block(ElemDesc const& ed);
 size_t train_lds_per_atom(ElemDesc const& ed);
 hipError_t launch_train_u(TrainArgs const& A, hipStream_t s);
 hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,

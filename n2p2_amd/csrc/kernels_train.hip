@@ -1,5 +1,5 @@
-// Training kernels for a radial-only HDNNP (the distilled surrogate of the
-// multiple-time-step integrator): the gradient of
+// Training kernels (the distilled surrogate of the multiple-time-step
+// integrator, and 2G models with radial and angular SFs): the gradient of
 //
 //   L = wE ((E - Eref) / N)^2 + wF / (3N) sum_i |F_i - Fref_i|^2
 //
@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "device_common.h"
@@ -128,10 +129,110 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
                     U[sf] = A.cF * scale_slope(T.scale[ed.sf0 + sf]) * s;
             }
     }
+    // Angular groups, G3 (SymFncExpAngn.cpp:79-225) and G9
+    // (SymFncExpAngw.cpp:79-208): for the pair (j, k), j < k in list order,
+    // the reference's p1, p2, p3 give
+    //   sum_i d_i . dG/dr_i = p1 dr_ij.(d_a - d_j) + p2 dr_ik.(d_a - d_k)
+    //                       + p3 dr_jk.(d_k - d_j),   dr_jk = dr_ik - dr_ij.
+    for (int gi = ed.g0 + ed.nrad; gi < ed.g0 + ed.ng; ++gi)
+    {
+        GroupDesc const g = T.group[gi];
+        CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
+        MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        bool const wide = g.type == 9;
+        double acc[kMaxMembers];
+#pragma unroll
+        for (int q = 0; q < kMaxMembers; ++q) acc[q] = 0.0;
+        for (int j = 0; j + 1 < m; ++j)
+        {
+            int const aj = A.nbrJ[o + j];
+            int const ej = A.elem[aj];
+            double4 const gj = A.geo[o + j];
+            double const rij = gj.w;
+            if (!((ej == g.e1 || ej == g.e2) && rij < g.rc)) continue;
+            double fcij, dfcij;
+            cutoff_fdf<double>(cc, rij, fcij, dfcij);
+            double dj[3];
+            dj[0] = A.F[3 * (size_t)aj] - A.Fref[3 * (size_t)aj];
+            dj[1] = A.F[3 * (size_t)aj + 1] - A.Fref[3 * (size_t)aj + 1];
+            dj[2] = A.F[3 * (size_t)aj + 2] - A.Fref[3 * (size_t)aj + 2];
+            for (int k = j + 1 + lane; k < m; k += 64)
+            {
+                int const ak = A.nbrJ[o + k];
+                int const ek = A.elem[ak];
+                if (!((g.e1 == ej && g.e2 == ek) || (g.e2 == ej && g.e1 == ek)))
+                    continue;
+                double4 const gk = A.geo[o + k];
+                double const rik = gk.w;
+                if (!(rik < g.rc)) continue;
+                double const jx = gk.x - gj.x, jy = gk.y - gj.y,
+                             jz = gk.z - gj.z;
+                double rjk = jx * jx + jy * jy + jz * jz;
+                if (!wide && !(rjk < g.rc2)) continue;
+                rjk = sqrt(rjk);
+                double fcik, dfcik, fcjk = 1.0, dfcjk = 0.0;
+                cutoff_fdf<double>(cc, rik, fcik, dfcik);
+                if (!wide) cutoff_fdf<double>(cc, rjk, fcjk, dfcjk);
+                double const dk0 = A.F[3 * (size_t)ak] - A.Fref[3 * (size_t)ak];
+                double const dk1 = A.F[3 * (size_t)ak + 1]
+                                 - A.Fref[3 * (size_t)ak + 1];
+                double const dk2 = A.F[3 * (size_t)ak + 2]
+                                 - A.Fref[3 * (size_t)ak + 2];
+                double const tij = (gj.x * (da[0] - dj[0]) + gj.y * (da[1] - dj[1])
+                                    + gj.z * (da[2] - dj[2]));
+                double const tik = (gk.x * (da[0] - dk0) + gk.y * (da[1] - dk1)
+                                    + gk.z * (da[2] - dk2));
+                double const tjk = (jx * (dk0 - dj[0]) + jy * (dk1 - dj[1])
+                                    + jz * (dk2 - dj[2]));
+                double const rinv = 1.0 / rij / rik;
+                double const cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z)
+                                  * rinv;
+                double const pfc = fcij * fcik * fcjk;
+#pragma unroll
+                for (int q = 0; q < kMaxMembers; ++q)
+                    if (q < g.nmem)
+                    {
+                        MemberDesc const& md = mem[q];
+                        double const rijs = rij - md.rs, riks = rik - md.rs,
+                                     rjks = rjk - md.rs;
+                        double const ex = wide
+                            ? exp(-md.eta * (rijs * rijs + riks * riks))
+                            : exp(-md.eta * (rijs * rijs + riks * riks
+                                             + rjks * rjks));
+                        double const pl = 1.0 + md.lambda * cost;
+                        if (pl <= 0.0) continue;
+                        double fg = ex * (md.useint ? pow_int(pl, md.zint - 1)
+                                                    : pow(pl, md.zeta - 1.0));
+                        fg *= pow(2.0, 1.0 - md.zeta);
+                        double const pzl = md.zeta * md.lambda;
+                        double const ri = rinv * pzl, ct = cost * pzl;
+                        double const p2e = 2.0 * md.eta * pl;
+                        double const p1 = fg * (pfc * (ri - ct / (rij * rij)
+                                                - p2e * rijs / rij)
+                                                + fcik * fcjk * dfcij * pl / rij);
+                        double const p2 = fg * (pfc * (ri - ct / (rik * rik)
+                                                - p2e * riks / rik)
+                                                + fcij * fcjk * dfcik * pl / rik);
+                        double const p3 = wide
+                            ? fg * pfc * ri
+                            : fg * (pfc * (ri + p2e * rjks / rjk)
+                                    - fcij * fcik * dfcjk * pl / rjk);
+                        acc[q] += md.sfac * (p1 * tij + p2 * tik + p3 * tjk);
+                    }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxMembers; ++q)
+            if (q < g.nmem)
+            {
+                double const s = wave_sum(acc[q]);
+                if (lane == 0) U[mem[q].sf] = A.cF * s;
+            }
+    }
 }
 
 // Per-block gradient of sum over the block's atoms of
-// kappa E_a + (derivative of E_a along u_a), atoms of element e, kAtoms per
+// kappa E_a + (derivative of E_a along u_a), atoms of element e, apb per
 // block.  LDS per atom: the inputs x, tangents t of every layer and the
 // adjoints of z and s of every layer, plus two adjoint scratch vectors.
 constexpr int kAtoms = 32;
@@ -139,6 +240,7 @@ constexpr int kAtoms = 32;
 __global__ __launch_bounds__(64) void k_train_grad(TrainArgs A, int e,
                                                    const int* __restrict__ atoms,
                                                    int na, int perAtom,
+                                                   int apb,
                                                    double* __restrict__ part,
                                                    int nw)
 {
@@ -148,7 +250,7 @@ __global__ __launch_bounds__(64) void k_train_grad(TrainArgs A, int e,
     int const L = ed.nlayers - 1;
     int const t = threadIdx.x;
     int const b = blockIdx.x;
-    int const nat = min(kAtoms, na - b * kAtoms);
+    int const nat = min(apb, na - b * apb);
     const double* __restrict__ W = T.weights;
     // LDS layout per atom: for l = 1..L: X[l-1] (w_{l-1}), Tg[l-1]
     // (w_{l-1}), Z[l] (w_l), S[l] (w_l); then Xb, Tb (maxwidth each).
@@ -166,7 +268,7 @@ __global__ __launch_bounds__(64) void k_train_grad(TrainArgs A, int e,
                        / ((double)A.n * (double)A.n);
     if (t < nat)
     {
-        int const a = atoms[b * kAtoms + t];
+        int const a = atoms[b * apb + t];
         double* __restrict__ P = lds + (size_t)t * perAtom;
         // inputs: scaled G and its tangent u
         {
@@ -467,6 +569,15 @@ hipError_t launch_energy_wgrad(ModelTables const& T, const double* G, int ldG,
 
 int train_atoms_per_block() { return kAtoms; }
 
+// Atoms per block of the loss-gradient kernel: up to kAtoms, fewer for wide
+// networks so that their LDS stays within 64 KiB (0 if one atom does not
+// fit).
+int train_atoms_per_grad_block(ElemDesc const& ed)
+{
+    size_t const per = train_lds_per_atom(ed) * sizeof(double);
+    return (int)std::min<size_t>(kAtoms, 65536 / per);
+}
+
 size_t train_lds_per_atom(ElemDesc const& ed)
 {
     size_t p = 0;
@@ -492,9 +603,10 @@ hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,
         return hipGetLastError();
     }
     int const per = (int)train_lds_per_atom(ed);
-    int const nb = (na + kAtoms - 1) / kAtoms;
-    size_t const shm = (size_t)per * kAtoms * sizeof(double);
-    k_train_grad<<<nb, 64, shm, s>>>(A, e, atoms, na, per, part, nw);
+    int const apb = train_atoms_per_grad_block(ed);
+    int const nb = (na + apb - 1) / apb;
+    size_t const shm = (size_t)per * apb * sizeof(double);
+    k_train_grad<<<nb, 64, shm, s>>>(A, e, atoms, na, per, apb, part, nw);
     k_train_reduce<<<(nw + 255) / 256, 256, 0, s>>>(part, nb, nw, grad);
     return hipGetLastError();
 }

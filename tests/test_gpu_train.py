@@ -95,6 +95,57 @@ def test_gradient_vs_oracle_finite_differences(sur, tmp_path):
         assert rel <= 1e-6
 
 
+@pytest.mark.parametrize("which", ["water_g2_g3_tanh", "mixed_g2_g9_softplus"])
+def test_gradient_full_models_vs_oracle_fd(which, tmp_path):
+    """The same check on the full models with angular SFs: water (G2 + G3,
+    tanh) and the three-element set (G2 + G9, softplus)."""
+    if which.startswith("water"):
+        d, r, e, box = cases.water_case(tmp_path / "w")
+        z, hidden = (1, 8), (25, 25)
+    else:
+        d, r, e, box = cases.mixed_case(tmp_path / "m", natoms=40, seed=3)
+        z, hidden = (5, 6, 7), (30, 30)
+    sd = str(tmp_path / "fd")
+    os.makedirs(sd)
+    for f in ("input.nn", "scaling.data"):
+        with open(os.path.join(d, f)) as a, open(os.path.join(sd, f), "w") as b:
+            b.write(a.read())
+    m = engine.Model.load_dir(d)
+    ctx = engine.Context(m)
+    ctx.set_structure(r, e, box)
+    rng = np.random.default_rng(8)
+    w0 = ctx.get_weights() + rng.normal(scale=0.02, size=ctx.num_weights())
+    ctx.set_weights(w0)
+    fref = rng.normal(scale=0.3, size=(len(e), 3))
+    eref, w_e, w_f = -3.0, 0.5, 1.5
+    g, loss = ctx.train_gradient(fref, eref, w_e, w_f)
+    nsf = [m.num_sf(k) for k in range(m.num_elements)]
+
+    def oloss(w):
+        S.write_weights(sd, w, nsf, z, hidden)
+        s = OStruct(OModel.from_dir(sd), r, e, box)
+        en = s.evaluate(cells=True, force_mode=1)
+        df = s.forces() - fref
+        n = len(e)
+        return w_e * ((en - eref) / n) ** 2 + w_f * np.sum(df ** 2) / (3 * n)
+
+    lo = oloss(w0)
+    say("%s: loss gpu %.12g oracle %.12g" % (which, loss[0] + loss[1], lo))
+    assert abs(loss[0] + loss[1] - lo) <= 1e-10 * abs(lo)
+    h = 1e-4
+    gs = np.abs(g).max()
+    picks = list(rng.choice(len(w0), 8, replace=False)) + [0, len(w0) - 1]
+    for k in picks:
+        wp, wm = w0.copy(), w0.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (oloss(wp) - oloss(wm)) / (2 * h)
+        rel = abs(g[k] - fd) / max(abs(fd), 1e-3 * gs)
+        say("  %s weight %d: gpu %.10g fd %.10g rel %.2g" % (which, k, g[k],
+                                                             fd, rel))
+        assert rel <= 1e-6
+
+
 def test_adam_step_matches_update_rule(sur):
     d, sd, r, e, box = sur
     m = engine.Model.load_dir(sd)
