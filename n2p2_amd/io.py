@@ -89,10 +89,12 @@ def read_structure(path, element_index):
     return r, elem, b, energy
 
 
-def read_structures(path, element_index):
+def read_structures(path, element_index, with_forces=False):
     """Every structure of an input.data file (Structure::readFromFile over
     the whole file, Dataset::readFromFile): a list of (r, elem, box or None,
-    energy_ref), positions remapped into the cell."""
+    energy_ref), positions remapped into the cell; with_forces appends the
+    reference forces (atom columns 7-9, Structure.cpp:214-232) to each
+    tuple."""
     out, block = [], []
     with open(path) as f:
         for line in f:
@@ -101,13 +103,14 @@ def read_structures(path, element_index):
                 continue
             block.append(line)
             if w[0] == "end":
-                out.append(_parse_block(block, element_index))
+                s = _parse_block(block, element_index)
+                out.append(s if with_forces else s[:4])
                 block = []
     return out
 
 
 def _parse_block(lines, element_index):
-    r, elem, box = [], [], []
+    r, elem, box, force = [], [], [], []
     energy = 0.0
     for line in lines:
         w = line.split()
@@ -121,6 +124,8 @@ def _parse_block(lines, element_index):
             if e < 0:
                 raise RuntimeError("ERROR: Unknown element \"%s\".\n" % w[4])
             elem.append(e)
+            force.append([float(x) for x in w[7:10]] if len(w) >= 10
+                         else [0.0, 0.0, 0.0])
         elif w[0] == "energy":
             energy = float(w[1])
         else:
@@ -133,7 +138,7 @@ def _parse_block(lines, element_index):
     b = np.array(box, dtype=np.float64) if box else None
     if b is not None:
         r = remap(r, b)
-    return r, elem, b, energy
+    return r, elem, b, energy, np.array(force, dtype=np.float64).reshape(-1, 3)
 
 
 def to_normalized(r, box, conv_length):
