@@ -121,6 +121,52 @@ def run_path(ctx, precision, steps, warmup):
     return dt, energy, forces, kt, spec
 
 
+def run_batch(tmp, steps, warmup, log):
+    """The 512-structure batch (three elements B/C/N at 0.4/0.4/0.2, 16-512
+    atoms each, 0.08 atoms/A^3, 10 G2 per element and 12 G9 per element
+    pair, polynomial cutoff 6 A, 102-30-30-1 softplus): one E+F evaluation
+    of all structures per step, neighbour lists rebuilt, FP64 and mixed;
+    the mixed path's force error against FP64 over all atoms."""
+    sts = W.batch_structures(512, 16, 512, seed=2024)
+    d = os.path.join(tmp, "batch_model")
+    r0, e0, b0 = sts[0]
+    S.write_model_gpu_scaled(d, W.mixed_settings(), W.MIXED_ELEMENTS,
+                             (5, 6, 7), (30, 30), r0, e0, b0, seed=11)
+    model = engine.Model.load_dir(d)
+    ctx = engine.Context(model)
+    ctx.set_structures(sts)
+    out = {"structures": len(sts), "atoms": int(ctx.n)}
+    res = {}
+    for name, prec in (("fp64", engine.PRECISION_FP64),
+                       ("mixed", engine.PRECISION_MIXED)):
+        ctx.set_precision(prec)
+        for _ in range(warmup):
+            ctx.evaluate(engine.FORCES_REFERENCE)
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.evaluate(engine.FORCES_REFERENCE)
+        dt = (time.perf_counter() - t0) / steps
+        kt = {k: ctx.profile_avg_ms(k)[0]
+              for k in ("neighbor", "symfunc", "network", "force", "gather")}
+        ctx.profile(False)
+        res[name] = (ctx.forces(), ctx.structure_energies())
+        out[name] = {"atom_steps_per_s": ctx.n / dt, "ms_per_step": dt * 1e3,
+                     "kernel_ms": kt}
+    df = res["mixed"][0] - res["fp64"][0]
+    de = np.abs(res["mixed"][1] - res["fp64"][1]) / np.abs(res["fp64"][1])
+    out["mixed_error_vs_fp64"] = {
+        "force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
+        "force_max_eV_A": float(np.abs(df).max()),
+        "structure_energy_rel_max": float(de.max())}
+    out["mixed_vs_plain_fp64"] = (out["mixed"]["atom_steps_per_s"]
+                                  / out["fp64"]["atom_steps_per_s"])
+    ctx.close()
+    log("batch of 512 structures: %s" % out)
+    return out
+
+
 def run_md(model, r, elem, box, precision, steps, warmup, skin=1.0,
            dt_fs=0.25, temp=300.0):
     """NVE MD of the same box (velocity Verlet, Verlet-skin list, HIP-graph
@@ -268,6 +314,14 @@ def main():
             return
         dt16, e16, f16, kt16, _ = run_path(ctx, engine.PRECISION_MIXED16,
                                            args.steps, args.warmup)
+        dtb, eb, fb, ktb, _ = run_path(ctx, engine.PRECISION_MIXED_BF16,
+                                       args.steps, args.warmup)
+        dbf = fb - f64
+        errbf = {"force_rmse_eV_A": float(np.sqrt(np.mean(dbf ** 2))),
+                 "force_max_eV_A": float(np.abs(dbf).max()),
+                 "energy_rel": float(abs(eb - e64) / abs(e64))}
+        log("mixed with BF16 networks: %.3f ms/step, errors %s"
+            % (dtb * 1e3, errbf))
         log("mixed with FP16 networks: %.3f ms/step, E=%.10f, kernel ms %s"
             % (dt16 * 1e3, e16, kt16))
         df = fm - f64
@@ -296,6 +350,7 @@ def main():
         md["mixed_vs_plain_fp64"] = (md["mixed"]["atom_steps_per_s"]
                                      / md["fp64"]["atom_steps_per_s"])
         sur_model, hist, ttrain = train_surrogate(tmp, log)
+        batch = run_batch(tmp, args.steps, args.warmup, log)
         md["surrogate"] = {
             "what": "radial-only HDNNP, 8 G2 per element (4 eta x 2 "
                     "neighbour elements), 8-16-16-1 softplus, trained on the "
@@ -353,8 +408,16 @@ def main():
                                    "what": "as the mixed path with FP16 "
                                            "MFMA network layers (FP32 "
                                            "accumulation)"},
+                       "mixed_bf16": {"atom_steps_per_s": n / dtb,
+                                      "ms_per_step": dtb * 1e3,
+                                      "kernel_ms": ktb,
+                                      "error_vs_fp64": errbf,
+                                      "what": "as the mixed path with BF16 "
+                                              "MFMA network layers (FP32 "
+                                              "accumulation)"},
                        "mixed_vs_plain_fp64": dt64 / dtm,
                        "md": md,
+                       "batch_512_structures": batch,
                        "mixed_error_vs_fp64": err,
                        "flops_per_atom_step": (fwd + bwd + nnf) / n,
                        "work_counts": counts,

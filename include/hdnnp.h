@@ -55,6 +55,8 @@ extern "C" {
                                      and force accumulation */
 #define HDNNP_PRECISION_MIXED16 2 /* as MIXED with FP16 MFMA network layers
                                      (FP32 accumulation) */
+#define HDNNP_PRECISION_MIXED_BF16 3 /* as MIXED with BF16 MFMA network
+                                        layers (FP32 accumulation) */
 
 typedef struct hdnnp_model hdnnp_model;
 typedef struct hdnnp_ctx   hdnnp_ctx;
@@ -112,7 +114,7 @@ int    hdnnp_model_set_scaling(hdnnp_model* m, int e, int scale_type,
  * structure and work buffers, one HIP stream.  device = HIP ordinal. */
 int  hdnnp_ctx_create(const hdnnp_model* m, int device, hdnnp_ctx** out);
 void hdnnp_ctx_destroy(hdnnp_ctx* c);
-/* HDNNP_PRECISION_FP64 (default), _MIXED or _MIXED16. */
+/* HDNNP_PRECISION_FP64 (default), _MIXED, _MIXED16 or _MIXED_BF16. */
 int  hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision);
 /* 1 (default unless the environment sets N2P2_AMD_RTC=0): the symmetry-
  * function and force kernels are compiled for this model at first use

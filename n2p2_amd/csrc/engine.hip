@@ -829,7 +829,8 @@ void computeNN(hdnnp_ctx* c, bool deriv)
         NNArgs a{};
         a.T = c->T;
         a.ed = c->flat.elem[e];
-        a.mode = c->precision == HDNNP_PRECISION_MIXED16 ? 2
+        a.mode = c->precision == HDNNP_PRECISION_MIXED_BF16 ? 3
+               : c->precision == HDNNP_PRECISION_MIXED16 ? 2
                : c->precision == HDNNP_PRECISION_MIXED ? 1 : 0;
         if (nn_mfma_lds_bytes(a.ed, a.mode) > 65536) a.mode = -1;
         a.e = (int)e;
@@ -1302,7 +1303,8 @@ int hdnnp_ctx_set_precision(hdnnp_ctx* c, int precision)
     return guarded([&]() {
         if (precision != HDNNP_PRECISION_FP64
             && precision != HDNNP_PRECISION_MIXED
-            && precision != HDNNP_PRECISION_MIXED16)
+            && precision != HDNNP_PRECISION_MIXED16
+            && precision != HDNNP_PRECISION_MIXED_BF16)
             throw Error(HDNNP_E_INVALID, "ERROR: Unknown precision.\n");
         if (precision != c->precision)
             c->haveSF = c->haveNN = c->haveNNDeriv = false;

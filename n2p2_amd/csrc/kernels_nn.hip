@@ -109,7 +109,9 @@ __global__ __launch_bounds__(256) void k_nn(NNArgs a)
 // layer's output is written back in the operand layout of the next GEMM.
 // MODE 0: v_mfma_f64_16x16x4f64 (plain FP64 path).  MODE 1:
 // v_mfma_f32_16x16x4f32 (mixed path).  MODE 2: v_mfma_f32_16x16x16f16,
-// FP16 operands with FP32 accumulation.  Biases, activations and
+// FP16 operands with FP32 accumulation.  MODE 3:
+// v_mfma_f32_16x16x16bf16_1k, BF16 operands (round to nearest even) in the
+// FP16 operand layout, FP32 accumulation.  Biases, activations and
 // normalisation in FP64 in every mode.
 // Operand layouts (16x16 tiles, lane l, v < 4): A[i = l % 16][k = K0 (l /
 // 16) + v'], B[k = K0 (l / 16) + v'][j = l % 16] with K0 = 1 (FP64, FP32;
@@ -119,6 +121,15 @@ __global__ __launch_bounds__(256) void k_nn(NNArgs a)
 typedef double nn_d4 __attribute__((ext_vector_type(4)));
 typedef float nn_f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 nn_h4 __attribute__((ext_vector_type(4)));
+typedef short nn_s4 __attribute__((ext_vector_type(4)));
+
+// BF16 bits of x, round to nearest even (finite inputs).
+__device__ __forceinline__ short to_bf16(float x)
+{
+    unsigned const u = __float_as_uint(x);
+    unsigned const r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (short)(r >> 16);
+}
 
 template <int MODE> struct NNMma;
 
@@ -209,6 +220,44 @@ template <> struct NNMma<2>
     __device__ static Acc mma(nn_h4 a, nn_h4 b, Acc c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <> struct NNMma<3>
+{
+    typedef float Act;
+    typedef nn_f4 Acc;
+    static constexpr int K = 16;
+    __device__ static nn_s4 a(Act const* X, int ld, int nk, int k0, int lane)
+    {
+        int const m = lane & 15, k = k0 + 4 * (lane >> 4);
+        nn_s4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = to_bf16(k + q < nk ? X[m * ld + k + q] : 0.0f);
+        return v;
+    }
+    __device__ static nn_s4 b(ModelTables const& T, int off, bool trans,
+                              int ldw, int nk, int nj, int k0, int j0,
+                              int lane)
+    {
+        int const j = j0 + (lane & 15), k = k0 + 4 * (lane >> 4);
+        nn_s4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            float w = 0.0f;
+            if (k + q < nk && j < nj)
+                w = T.weights32[off + (trans ? j * ldw + k + q
+                                             : (k + q) * ldw + j)];
+            v[q] = to_bf16(w);
+        }
+        return v;
+    }
+    __device__ static Acc mma(nn_s4 a, nn_s4 b, Acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
     }
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
@@ -397,7 +446,8 @@ hipError_t launch_nn(NNArgs const& a, hipStream_t s)
     {
         size_t const shm = nn_mfma_lds_bytes(a.ed, a.mode);
         int const nb = (a.na + 15) / 16;
-        if (a.mode == 2) k_nn_mfma<2><<<nb, 64, shm, s>>>(a);
+        if (a.mode == 3) k_nn_mfma<3><<<nb, 64, shm, s>>>(a);
+        else if (a.mode == 2) k_nn_mfma<2><<<nb, 64, shm, s>>>(a);
         else if (a.mode == 1) k_nn_mfma<1><<<nb, 64, shm, s>>>(a);
         else k_nn_mfma<0><<<nb, 64, shm, s>>>(a);
     }

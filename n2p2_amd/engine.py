@@ -31,6 +31,7 @@ FORCES_CONSISTENT = 1
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 PRECISION_MIXED16 = 2
+PRECISION_MIXED_BF16 = 3
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)

@@ -33,6 +33,8 @@ Mode::Mode()
     if (p && std::strcmp(p, "mixed") == 0) precision = HDNNP_PRECISION_MIXED;
     else if (p && std::strcmp(p, "mixed16") == 0)
         precision = HDNNP_PRECISION_MIXED16;
+    else if (p && std::strcmp(p, "mixedbf16") == 0)
+        precision = HDNNP_PRECISION_MIXED_BF16;
     char const* d = std::getenv("N2P2_AMD_DEVICE");
     if (d && *d) device = std::atoi(d);
 }

@@ -11,7 +11,11 @@ FP64.  For MIXED the tests assert ten times tighter RMSE (1e-4 eV/A), the
 and G within 1e-5 of each SF's scale.  For MIXED16 they assert the 1 meV/A
 RMSE bound and energy within 5e-3 relative: rounding the weights to FP16
 biases every atomic energy the same way, so the energy error adds up
-coherently.  Every test prints the errors it saw.
+coherently.  PRECISION_MIXED_BF16 (BF16 MFMA network layers, 8-bit
+mantissa) misses the 1 meV/A bound (observed force RMSE 2.5e-3 eV/A on the
+small boxes, 1.2e-3 on 196,608 atoms); the tests assert force RMSE 5e-3,
+largest error 3e-2 and energy 5e-3 relative, which a wrong operand layout
+would exceed by orders of magnitude.  Every test prints the errors it saw.
 """
 import sys
 
@@ -28,8 +32,11 @@ pytestmark = pytest.mark.gpu
 BOUNDS = {engine.PRECISION_MIXED: dict(e_rel=1e-6, f_rmse=1e-4, f_max=1e-3,
                                        g_rel=1e-5),
           engine.PRECISION_MIXED16: dict(e_rel=5e-3, f_rmse=1e-3,
-                                         f_max=1e-2, g_rel=1e-5)}
-PRECISIONS = [engine.PRECISION_MIXED, engine.PRECISION_MIXED16]
+                                         f_max=1e-2, g_rel=1e-5),
+          engine.PRECISION_MIXED_BF16: dict(e_rel=5e-3, f_rmse=5e-3,
+                                            f_max=3e-2, g_rel=1e-5)}
+PRECISIONS = [engine.PRECISION_MIXED, engine.PRECISION_MIXED16,
+              engine.PRECISION_MIXED_BF16]
 
 
 def say(*a):
