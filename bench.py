@@ -218,6 +218,36 @@ def train_surrogate(tmp, log):
     return engine.Model.load_dir(sd), hist, ttrain
 
 
+PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_MFMA_FP64_TFLOPS = 78.6
+
+
+def other_rooflines(ktm, kt64, counts, nnf, n):
+    """The neighbour build against HBM bandwidth (algorithmic bytes:
+    positions read, index, image key, dr, d and partner written per entry)
+    and the network layers against the MFMA peak of their operand type
+    (algorithmic flops: forward and backward GEMMs), both from the HIP-event
+    kernel times of the timed steps."""
+    nb = costmodel.neighbor_bytes(n, int(counts[0]))
+    out = {}
+    for name, kt in (("mixed", ktm), ("plain_fp64", kt64)):
+        ms_nl = kt["neighbor"]
+        ms_nn = kt["network"]
+        peak = PEAK_MFMA_FP32_TFLOPS if name == "mixed" \
+            else PEAK_MFMA_FP64_TFLOPS
+        a_nn = nnf / (ms_nn * 1e-3) / 1e12
+        a_nl = nb / (ms_nl * 1e-3) / 1e9
+        out[name] = {
+            "neighbor_build": {"bound": "hbm", "achieved": a_nl,
+                               "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": a_nl / PEAK_HBM_GBS,
+                               "algorithmic_bytes": nb, "ms": ms_nl},
+            "networks": {"bound": "mfma", "achieved": a_nn, "peak": peak,
+                         "unit": "TFLOP/s", "frac": a_nn / peak,
+                         "algorithmic_flops": nnf, "ms": ms_nn}}
+    return out
+
+
 def roofline(kt, counts, nnf, n, fp32, spec):
     """Roofline object for the dominant kernel of one path; spec: the
     model-specialised SF and force kernels ran (their names differ)."""
@@ -419,6 +449,8 @@ def main():
                        "md": md,
                        "batch_512_structures": batch,
                        "mixed_error_vs_fp64": err,
+                       "other_kernels": other_rooflines(ktm, kt64, counts,
+                                                        nnf, n),
                        "flops_per_atom_step": (fwd + bwd + nnf) / n,
                        "work_counts": counts,
                        "kernel_ms": ktm},

@@ -517,7 +517,8 @@ void buildBatchNeighborList(hdnnp_ctx* c, double rc)
                         "ERROR: Neighbour count changed between passes.\n");
     }
     hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
-                               c->dCellAtoms.p, c->dPartner.p, s),
+                               c->dCellAtoms.p, c->dPartner.p,
+                               c->entryCap, s),
              "nl partner");
     c->end(K_NEIGHBOR, ev0);
     c->slabLayout = true;
@@ -734,7 +735,8 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     c->slabLayout = cellKernels;
     c->Mvalid = !cellKernels;
     hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
-                               c->dCellAtoms.p, c->dPartner.p, s),
+                               c->dCellAtoms.p, c->dPartner.p,
+                               c->entryCap, s),
              "nl partner");
     c->end(K_NEIGHBOR, ev0);
     // Slot capacity of the SF and force kernels: the largest count (the

@@ -160,9 +160,11 @@ hipError_t launch_energy_segments(const double* E, const int* structOff,
 hipError_t launch_slab_starts(int n, int slab, int* off, hipStream_t s);
 // Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted
 // by cell (processing order only).
+// Partner slots of a symmetric list (nslots = slots allocated; all are
+// set to -1 first).
 hipError_t launch_nl_partner(int n, const int* off, const int* cnt,
                              const uint64_t* keys, const int* order,
-                             int* partner, hipStream_t s);
+                             int* partner, int64_t nslots, hipStream_t s);
 
 hipError_t launch_sf_forward(SFArgs const& a, hipStream_t s);
 size_t sf_forward_lds_bytes(int cap, int fp32);

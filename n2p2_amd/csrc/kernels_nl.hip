@@ -652,11 +652,16 @@ __global__ void k_slab_starts(int n, int slab, int* __restrict__ off)
     if (i <= n) off[i] = i * slab;
 }
 
-// Partner slot: for entry q of atom i pointing at (j, bc) the entry of j
-// pointing at (i, -bc); binary search in j's sorted keys.  One wave per
-// atom, atoms taken in cell order (order = atoms sorted by cell) with the
-// XCD-aware mapping: the lists searched by neighbouring atoms overlap and
-// stay in one L2.
+// Partner slots: for entry q of atom i pointing at (j, bc) the entry p of
+// j pointing at (i, -bc).  Each pair is searched once, from its canonical
+// side, the entry whose key exceeds its partner's (j > i, or j == i with a
+// positive image), by binary search in j's sorted keys; both slots are
+// written, partner[q] = p and partner[p] = q.  The canonical entries of
+// atom i are the tail of its list from the first key with j >= i, so the
+// lanes work on that range only.  Slots left unwritten (an inconsistent
+// list) keep the -1 the launcher fills in.  One wave per atom, atoms in
+// cell order (order = atoms sorted by cell) with the XCD-aware mapping: the
+// lists searched by neighbouring atoms overlap and stay in one L2.
 __global__ __launch_bounds__(64) void k_partner(
     int n, const int* __restrict__ off, const int* __restrict__ cnt,
     const uint64_t* __restrict__ keys, const int* __restrict__ order,
@@ -666,11 +671,20 @@ __global__ __launch_bounds__(64) void k_partner(
     if (w >= n) return;
     int const i = order[w];
     int const s0 = off[i], s1 = s0 + cnt[i];
-    for (int q = s0 + (int)threadIdx.x; q < s1; q += 64)
+    uint64_t const first = (uint64_t)(uint32_t)i << 21;
+    int a = s0, b = s1;
+    while (a < b)
     {
+        int const mid = (a + b) >> 1;
+        if (keys[mid] < first) a = mid + 1; else b = mid;
+    }
+    for (int q = a + (int)threadIdx.x; q < s1; q += 64)
+    {
+        uint64_t const key = keys[q];
         int j, b0, b1, b2;
-        split_key(keys[q], j, b0, b1, b2);
+        split_key(key, j, b0, b1, b2);
         uint64_t const want = make_key(i, -b0, -b1, -b2);
+        if (!(key > want)) continue;
         int lo = off[j], hi = off[j] + cnt[j] - 1;
         int found = -1;
         while (lo <= hi)
@@ -681,6 +695,7 @@ __global__ __launch_bounds__(64) void k_partner(
             if (v < want) lo = mid + 1; else hi = mid - 1;
         }
         partner[q] = found;
+        if (found >= 0) partner[found] = q;
     }
 }
 
@@ -876,9 +891,12 @@ hipError_t launch_slab_starts(int n, int slab, int* off, hipStream_t s)
 
 hipError_t launch_nl_partner(int n, const int* off, const int* cnt,
                              const uint64_t* keys, const int* order,
-                             int* partner, hipStream_t s)
+                             int* partner, int64_t nslots, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
+    hipError_t const e = hipMemsetAsync(partner, 0xff,
+                                        (size_t)nslots * sizeof(int), s);
+    if (e != hipSuccess) return e;
     k_partner<<<n, 64, 0, s>>>(n, off, cnt, keys, order, partner);
     return hipGetLastError();
 }
