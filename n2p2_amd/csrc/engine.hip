@@ -204,7 +204,7 @@ struct hdnnp_ctx
         int64_t rebuilds = 0, steps = 0;
         MDCell cell{};
         DBuf<double> v, r0, accel, mass, part, epot, ekin;
-        DBuf<int> flags, counter, skinPartner, skinCnt, src, newIdx;
+        DBuf<int> flags, counter, skinPartner, skinCnt, newIdx;
         DBuf<uint64_t> skinKeys;
         int* hFlags = nullptr;
         int recCap = 0;
@@ -918,7 +918,18 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
     }
     else hipCheck(launch_force_backward(a, c->stream), "force backward");
     c->end(K_FORCE, ev);
-    if (!c->externalNL)
+    if (!c->externalNL && c->md.on)
+    {
+        auto& m = c->md;
+        c->begin(K_GATHER, ev2);
+        hipCheck(launch_gather_md((int)c->n, c->dOff.p, m.skinCnt.p,
+                                  m.newIdx.p, m.skinPartner.p, m.skinKeys.p,
+                                  c->dCslot.p, c->dCself.p, c->dCellAtoms.p,
+                                  mode, c->entryCap, c->dF.p, m.flags.p + 3,
+                                  c->stream), "force gather (MD)");
+        c->end(K_GATHER, ev2);
+    }
+    else if (!c->externalNL)
     {
         c->begin(K_GATHER, ev2);
         hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dCount.p,
@@ -970,19 +981,17 @@ void mdStepA(hdnnp_ctx* c)
                              m.flags.p, s), "md drift");
     hipCheck(launch_md_filter(n, m.cell, m.rc * m.rc, c->dR.p, c->dOff.p,
                               m.skinCnt.p, m.skinKeys.p, c->dCount.p,
-                              c->dKeys.p, c->dNbrJ.p, c->dGeo.p, m.src.p,
-                              m.newIdx.p, m.cap, m.flags.p, s), "md filter");
+                              c->dNbrJ.p, c->dGeo.p, m.newIdx.p, m.cap,
+                              m.flags.p, s), "md filter");
     hipCheck(hipMemcpyAsync(m.hFlags, m.flags.p, 4 * sizeof(int),
                             hipMemcpyDeviceToHost, s), "md flags");
 }
 
-// Partner slots of the filtered list, then SF, networks, energy, forces.
+// SF, networks, energy, forces on the filtered list (the gather follows
+// partners through the skin list).
 void mdForces(hdnnp_ctx* c, bool wait)
 {
     auto& m = c->md;
-    hipCheck(launch_md_partner((int)c->n, c->dOff.p, c->dCount.p, m.src.p,
-                               m.skinPartner.p, m.newIdx.p, c->dPartner.p,
-                               m.flags.p + 3, c->stream), "md partner");
     c->haveNL = true;
     computeSF(c, false);
     computeNN(c, true);
@@ -1021,16 +1030,14 @@ void mdRebuild(hdnnp_ctx* c)
     c->dKeys.ensure(E);
     c->dPartner.ensure(E);
     c->dCount.ensure(n);
-    m.src.ensure(E);
     m.newIdx.ensure(E);
     hipCheck(hipMemcpyAsync(m.r0.p, c->dR.p, 3 * (size_t)n * sizeof(double),
                             hipMemcpyDeviceToDevice, s), "md r0");
     hipCheck(launch_md_clear_flags(m.flags.p, s), "md flags");
     hipCheck(launch_md_filter(n, m.cell, m.rc * m.rc, c->dR.p, c->dOff.p,
                               m.skinCnt.p, m.skinKeys.p, c->dCount.p,
-                              c->dKeys.p, c->dNbrJ.p, c->dGeo.p, m.src.p,
-                              m.newIdx.p, INT32_MAX, m.flags.p, s),
-             "md filter");
+                              c->dNbrJ.p, c->dGeo.p, m.newIdx.p, INT32_MAX,
+                              m.flags.p, s), "md filter");
     int maxc = 0;
     hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "md max");
     hipCheck(hipMemcpyAsync(&maxc, c->dStats.p, sizeof(int),
@@ -1282,7 +1289,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
         m.v.release(); m.r0.release(); m.accel.release(); m.mass.release();
         m.part.release(); m.epot.release(); m.ekin.release();
         m.flags.release(); m.counter.release(); m.skinPartner.release();
-        m.skinCnt.release(); m.src.release(); m.newIdx.release();
+        m.skinCnt.release(); m.newIdx.release();
         m.skinKeys.release();
         c->tU.release(); c->tD2.release(); c->tPart.release();
         c->tGrad.release(); c->tM.release(); c->tV.release();
@@ -1613,6 +1620,10 @@ int hdnnp_get_neighbors(hdnnp_ctx* c, int64_t* offsets, int32_t* index,
         if (image)
         {
             std::vector<uint64_t> keys(M);
+            if (c->md.on)   // the MD filter leaves the kept keys unwritten
+                hipCheck(launch_md_keys((int)c->n, c->dOff.p, c->md.skinCnt.p,
+                                        c->md.skinKeys.p, c->md.newIdx.p,
+                                        c->dKeys.p, c->stream), "md keys");
             compactEntries(c, m, c->dKeys.p, 1, keys.data());
             for (size_t q = 0; q < M; ++q)
             {
@@ -1854,9 +1865,8 @@ static bool mdEvalAt(hdnnp_ctx* c, const double* r_src, bool forceRebuild)
                                  m.flags.p, s), "md check");
         hipCheck(launch_md_filter(n, m.cell, m.rc * m.rc, c->dR.p, c->dOff.p,
                                   m.skinCnt.p, m.skinKeys.p, c->dCount.p,
-                                  c->dKeys.p, c->dNbrJ.p, c->dGeo.p, m.src.p,
-                                  m.newIdx.p, m.cap, m.flags.p, s),
-                 "md filter");
+                                  c->dNbrJ.p, c->dGeo.p, m.newIdx.p, m.cap,
+                                  m.flags.p, s), "md filter");
         hipCheck(hipMemcpyAsync(m.hFlags, m.flags.p, 4 * sizeof(int),
                                 hipMemcpyDeviceToHost, s), "md flags");
         hipCheck(hipStreamSynchronize(s), "sync");

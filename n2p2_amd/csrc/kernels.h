@@ -140,13 +140,20 @@ hipError_t launch_md_fdiff2(int n, const double* F1, const double* F2,
 hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s);
 hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
                             const int* off, const int* skinCnt,
-                            const uint64_t* skinKeys, int* cnt, uint64_t* keys,
-                            int* nbrJ, double4* geo, int* src, int* newIdx,
-                            int cap, int* flags, hipStream_t s);
-hipError_t launch_md_partner(int n, const int* off, const int* cnt,
-                             const int* src, const int* skinPartner,
-                             const int* newIdx, int* partner, int* misses,
-                             hipStream_t s);
+                            const uint64_t* skinKeys, int* cnt, int* nbrJ,
+                            double4* geo, int* newIdx, int cap, int* flags,
+                            hipStream_t s);
+// keys[newIdx[s]] = skinKeys[s] for the kept skin slots.
+hipError_t launch_md_keys(int n, const int* off, const int* skinCnt,
+                          const uint64_t* skinKeys, const int* newIdx,
+                          uint64_t* keys, hipStream_t s);
+// Force gather over a filtered list through the skin list's partners.
+hipError_t launch_gather_md(int n, const int* off, const int* skinCnt,
+                            const int* newIdx, const int* skinPartner,
+                            const uint64_t* skinKeys, const double* cslot,
+                            const double* cself, const int* order, int mode,
+                            int64_t nslots, double* F, int* misses,
+                            hipStream_t s);
 int md_mv2_blocks(int n);
 hipError_t launch_md_record(int n, const double* v, const int* elem,
                             const double* mass, double* part,

@@ -4,9 +4,9 @@
 // order: j, then the image vector).  Every step filters it down to the
 // entries within rc at the current positions, in the same order, so the SF
 // and force kernels see exactly the list a fresh build would give.  The
-// partner slot of every kept entry is the kept image of its skin partner:
-// |dr| is the same for (i, j, b) and (j, i, -b), so both are kept or both
-// dropped.
+// partner of every kept entry is the kept image of its skin partner: |dr|
+// is the same for (i, j, b) and (j, i, -b), so both are kept or both
+// dropped; the force gather follows it through the skin list.
 //
 // No counterpart in libnnp: this is the loop LAMMPS runs around
 // pair_style hdnnp (fix nve, pair_hdnnp.cpp:81-114) with
@@ -113,15 +113,15 @@ __global__ void k_md_wrap(int n, double* __restrict__ r, MDCell C)
 
 // Entries of the skin list within rc at the current positions, in order,
 // into the slot range of the same atom (the skin list's range bounds the
-// count).  src[q] = skin slot of kept entry q, newIdx[s] = kept slot of
-// skin slot s (-1 if dropped).  flags[1] = 1 if an atom has more than
-// `cap` kept entries.  One wave per atom.
+// count): neighbour index and geometry.  newIdx[s] = kept slot of skin
+// slot s (-1 if dropped); the kept entries' keys are not written (the step
+// does not read them; k_md_keys fills them on request).  flags[1] = 1 if
+// an atom has more than `cap` kept entries.  One wave per atom.
 __global__ __launch_bounds__(64) void k_md_filter(
     int n, MDCell C, double rc2, const double* __restrict__ r,
     const int* __restrict__ off, const int* __restrict__ skinCnt,
     const uint64_t* __restrict__ skinKeys, int* __restrict__ cnt,
-    uint64_t* __restrict__ keys, int* __restrict__ nbrJ,
-    double4* __restrict__ geo, int* __restrict__ src,
+    int* __restrict__ nbrJ, double4* __restrict__ geo,
     int* __restrict__ newIdx, int cap, int* __restrict__ flags)
 {
     int const i = blockIdx.x;
@@ -135,12 +135,11 @@ __global__ __launch_bounds__(64) void k_md_filter(
     {
         int const t = base + lane;
         bool keep = false;
-        uint64_t key = 0;
         int j = 0;
         double dr[3] = {0.0, 0.0, 0.0};
         if (t < m)
         {
-            key = skinKeys[o + t];
+            uint64_t const key = skinKeys[o + t];
             j = (int)(key >> 21);
             int const b0 = (int)((key >> 14) & 127) - kBias;
             int const b1 = (int)((key >> 7) & 127) - kBias;
@@ -157,10 +156,8 @@ __global__ __launch_bounds__(64) void k_md_filter(
             newIdx[o + t] = keep ? q : -1;
             if (keep)
             {
-                keys[q] = key;
                 nbrJ[q] = j;
                 geo[q] = make_double4(dr[0], dr[1], dr[2], sqrt(md_norm2(dr)));
-                src[q] = o + t;
             }
         }
         kept += __popcll(mask);
@@ -172,26 +169,73 @@ __global__ __launch_bounds__(64) void k_md_filter(
     }
 }
 
-// partner[q] = kept slot of the skin partner of kept entry q.
-__global__ void k_md_partner(int n, const int* __restrict__ off,
-                             const int* __restrict__ cnt,
-                             const int* __restrict__ src,
-                             const int* __restrict__ skinPartner,
-                             const int* __restrict__ newIdx,
-                             int* __restrict__ partner,
-                             int* __restrict__ misses)
+// Keys of the kept entries: keys[newIdx[s]] = skinKeys[s].
+__global__ void k_md_keys(int n, const int* __restrict__ off,
+                          const int* __restrict__ skinCnt,
+                          const uint64_t* __restrict__ skinKeys,
+                          const int* __restrict__ newIdx,
+                          uint64_t* __restrict__ keys)
 {
     int const i = blockIdx.x;
     if (i >= n) return;
-    int const o = off[i], m = cnt[i];
+    int const o = off[i], m = skinCnt[i];
     for (int t = threadIdx.x; t < m; t += blockDim.x)
     {
-        // A skin entry without a partner (-1 from the partner search) can
-        // only come from an inconsistent list; it stays -1, which the
-        // gather skips.
-        int const sp = skinPartner[src[o + t]];
-        partner[o + t] = sp >= 0 ? newIdx[sp] : -1;
-        if (sp < 0) atomicAdd(misses, 1);
+        int const q = newIdx[o + t];
+        if (q >= 0) keys[q] = skinKeys[o + t];
+    }
+}
+
+// The force gather (k_gather) over a filtered list, with each kept entry's
+// partner found through the skin list: skin slot s of atom i is kept as
+// newIdx[s]; its skin partner skinPartner[s] is kept as newIdx[sp], whose
+// contribution cslot[newIdx[sp]] the atom receives.  Walks the skin slots
+// in order, so the kept entries are summed in list order.  A skin entry
+// without a partner (-1) is counted in misses[0] and skipped.  mode 0
+// skips the atom's own images (reference force mode); the neighbour index
+// then comes from the skin key.  Four waves per block, one atom each.
+__global__ __launch_bounds__(256) void k_gather_md(
+    int n, const int* __restrict__ off, const int* __restrict__ skinCnt,
+    const int* __restrict__ newIdx, const int* __restrict__ skinPartner,
+    const uint64_t* __restrict__ skinKeys, const double* __restrict__ cslot,
+    const double* __restrict__ cself, const int* __restrict__ order,
+    int mode, int64_t nslots, double* __restrict__ F,
+    int* __restrict__ misses)
+{
+    int const t = xcd_work_index(blockIdx.x, gridDim.x) * 4
+                + (int)(threadIdx.x >> 6);
+    int const lane = threadIdx.x & 63;
+    if (t >= n) return;
+    int const i = order[t];
+    int const s0 = off[i], s1 = s0 + skinCnt[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int s = s0 + lane; s < s1; s += 64)
+    {
+        if (newIdx[s] < 0) continue;
+        if (mode == 0 && (int)(skinKeys[s] >> 21) == i) continue;
+        int const sp = skinPartner[s];
+        if (sp < 0)
+        {
+            atomicAdd(misses, 1);
+            continue;
+        }
+        int const pq = newIdx[sp];
+        if (pq >= 0 && pq < nslots)
+        {
+            size_t const p = (size_t)pq;
+            fx += cslot[3 * p];
+            fy += cslot[3 * p + 1];
+            fz += cslot[3 * p + 2];
+        }
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0)
+    {
+        F[3 * (size_t)i] = -(cself[3 * (size_t)i] + fx);
+        F[3 * (size_t)i + 1] = -(cself[3 * (size_t)i + 1] + fy);
+        F[3 * (size_t)i + 2] = -(cself[3 * (size_t)i + 2] + fz);
     }
 }
 
@@ -331,24 +375,37 @@ hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s)
 
 hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
                             const int* off, const int* skinCnt,
-                            const uint64_t* skinKeys, int* cnt, uint64_t* keys,
-                            int* nbrJ, double4* geo, int* src, int* newIdx,
-                            int cap, int* flags, hipStream_t s)
+                            const uint64_t* skinKeys, int* cnt, int* nbrJ,
+                            double4* geo, int* newIdx, int cap, int* flags,
+                            hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     k_md_filter<<<n, 64, 0, s>>>(n, C, rc2, r, off, skinCnt, skinKeys, cnt,
-                                 keys, nbrJ, geo, src, newIdx, cap, flags);
+                                 nbrJ, geo, newIdx, cap, flags);
     return hipGetLastError();
 }
 
-hipError_t launch_md_partner(int n, const int* off, const int* cnt,
-                             const int* src, const int* skinPartner,
-                             const int* newIdx, int* partner, int* misses,
-                             hipStream_t s)
+hipError_t launch_md_keys(int n, const int* off, const int* skinCnt,
+                          const uint64_t* skinKeys, const int* newIdx,
+                          uint64_t* keys, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    k_md_partner<<<n, 64, 0, s>>>(n, off, cnt, src, skinPartner, newIdx,
-                                  partner, misses);
+    k_md_keys<<<n, 64, 0, s>>>(n, off, skinCnt, skinKeys, newIdx, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_md(int n, const int* off, const int* skinCnt,
+                            const int* newIdx, const int* skinPartner,
+                            const uint64_t* skinKeys, const double* cslot,
+                            const double* cself, const int* order, int mode,
+                            int64_t nslots, double* F, int* misses,
+                            hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_gather_md<<<(n + 3) / 4, 256, 0, s>>>(n, off, skinCnt, newIdx,
+                                            skinPartner, skinKeys, cslot,
+                                            cself, order, mode, nslots, F,
+                                            misses);
     return hipGetLastError();
 }
 
