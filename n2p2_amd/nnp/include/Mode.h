@@ -79,6 +79,13 @@ public:
     void calculateForces(Structure& structure) const;
     void evaluateNNP(Structure& structure, bool useForces = true,
                      bool useDEdG = true);
+    // evaluateNNP over every structure of a data set, as nnp-dataset loops
+    // it (nnp-dataset.cpp:206-211), evaluated as one batch on the device
+    // (one launch per kernel for all structures).  Writes back what
+    // evaluateNNP writes: per structure energy and pbc, per atom G,
+    // energy, dEdG and (useForces) f.  No counterpart in the reference.
+    void evaluateNNPBatch(std::vector<Structure>& structures,
+                          bool useForces = true);
     void addEnergyOffset(Structure& structure, bool ref = true);
     void removeEnergyOffset(Structure& structure, bool ref = true);
     double getEnergyOffset(Structure const& structure) const;

@@ -429,6 +429,76 @@ void Mode::evaluateNNP(Structure& structure, bool useForces, bool useDEdG)
     if (useForces) calculateForces(structure);
 }
 
+void Mode::evaluateNNPBatch(std::vector<Structure>& structures,
+                            bool useForces)
+{
+    ensureEngine();
+    upValid = false;
+    std::size_t const ns = structures.size();
+    if (ns == 0) return;
+    std::vector<int64_t> off(ns + 1, 0);
+    std::vector<double> r, box(9 * ns, 0.0);
+    std::vector<int32_t> el, per(ns, 0);
+    for (std::size_t s = 0; s < ns; ++s)
+    {
+        Structure const& st = structures[s];
+        for (Atom const& a : st.atoms)
+        {
+            for (int k = 0; k < 3; ++k) r.push_back(a.r[k]);
+            el.push_back((int32_t)a.element);
+        }
+        off[s + 1] = off[s] + (int64_t)st.atoms.size();
+        per[s] = st.isPeriodic ? 1 : 0;
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) box[9 * s + 3 * k + c] = st.box[k][c];
+    }
+    check(hdnnp_set_structures(ctx, (int64_t)ns, off.data(), r.data(),
+                               el.data(), box.data(), per.data()));
+    double etot = 0.0;
+    check(hdnnp_evaluate(ctx, useForces ? HDNNP_FORCES_REFERENCE : -1,
+                         &etot));
+    std::size_t const n = (std::size_t)off[ns];
+    std::size_t ld = 1;
+    for (auto v : numSF) ld = std::max(ld, v);
+    std::vector<double> es(ns), G(n * ld), e(n), d, f;
+    std::vector<int32_t> pbc(3 * ns);
+    check(hdnnp_get_structure_energies(ctx, es.data()));
+    check(hdnnp_get_structure_pbc(ctx, pbc.data()));
+    check(hdnnp_get_G(ctx, G.data(), (int)ld));
+    check(hdnnp_get_atomic_energies(ctx, e.data()));
+    if (useForces)
+    {
+        d.resize(n * ld);
+        f.resize(3 * n);
+        check(hdnnp_get_dEdG(ctx, d.data(), (int)ld));
+        check(hdnnp_get_forces(ctx, f.data()));
+    }
+    for (std::size_t s = 0; s < ns; ++s)
+    {
+        Structure& st = structures[s];
+        st.energy = es[s];
+        for (int k = 0; k < 3; ++k) st.pbc[k] = pbc[3 * s + k];
+        for (std::size_t q = 0; q < st.atoms.size(); ++q)
+        {
+            std::size_t const i = (std::size_t)off[s] + q;
+            Atom& a = st.atoms[q];
+            std::size_t const nsf = numSF.at(a.element);
+            a.numSymmetryFunctions = nsf;
+            a.G.assign(G.begin() + i * ld, G.begin() + i * ld + nsf);
+            a.energy = e[i];
+            a.hasSymmetryFunctions = true;
+            a.hasSymmetryFunctionDerivatives = useForces;
+            if (useForces)
+            {
+                a.dEdG.assign(d.begin() + i * ld, d.begin() + i * ld + nsf);
+                for (int k = 0; k < 3; ++k) a.f[k] = f[3 * i + k];
+            }
+        }
+        st.hasSymmetryFunctions = true;
+        st.hasSymmetryFunctionDerivatives = useForces;
+    }
+}
+
 void Mode::fetchNeighborList(Structure& structure) const
 {
     ensureEngine();

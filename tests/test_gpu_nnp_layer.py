@@ -74,3 +74,27 @@ def test_reference_nnp_predict_on_layer(tmp_path, name):
     f = np.array([[float(x) for x in r[5:8]] for r in rows])
     assert np.abs(f - np.array(g["forces"])).max() <= F_ABS
     assert os.path.exists(tmp_path / "output.data")
+
+
+def test_dataset_batch_through_layer(tmp_path):
+    """Mode::evaluateNNPBatch on the 20 water structures of the reference's
+    example data set with the example H2O_RPBE-D3 potential: every
+    structure's energy and forces equal Mode::evaluateNNP of the structure
+    alone (1e-12 relative energy, 1e-12 force units)."""
+    src = os.path.join(cases.EXAMPLES, "H2O_RPBE-D3")
+    for f in ("input.nn", "scaling.data", "weights.001.data",
+              "weights.008.data"):
+        shutil.copy(os.path.join(src, f), tmp_path)
+    shutil.copy(os.path.join(cases.GOLDEN, "nnp_scaling_H2O", "input.data"),
+                tmp_path)
+    out = subprocess.run([os.path.join(BIN, "dataset_batch"), str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+    assert out.returncode == 0
+    rows = [l.split() for l in out.stdout.splitlines()
+            if l.startswith("structure")]
+    assert len(rows) == 20
+    for w in rows:
+        eb, es, df = float(w[3]), float(w[4]), float(w[5])
+        assert abs(eb - es) <= 1e-12 * abs(es), w
+        assert df <= 1e-12, w
