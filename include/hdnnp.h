@@ -341,6 +341,18 @@ int hdnnp_train_adam(hdnnp_ctx* c, double lr, double beta1, double beta2,
                      double eps);
 /* Restart Adam (zero moments at the next step). */
 int hdnnp_train_reset(hdnnp_ctx* c);
+/* Kalman filter on the weights (KalmanFilter.cpp, standard type): the
+ * error covariance P = I / epsilon (hdnnp_num_weights squared doubles on
+ * the device), process noise q0 annealed by exp(-qtau) per update down to
+ * qmin, learning rate eta0 raised by exp(etatau) per update up to etamax
+ * (KalmanFilter::setParametersStandard, KalmanFilter.cpp:199-220). */
+int hdnnp_kalman_init(hdnnp_ctx* c, double epsilon, double q0, double qtau,
+                      double qmin, double eta0, double etatau, double etamax);
+/* One update with a scalar observation (KalmanFilter::update,
+ * KalmanFilter.cpp:126-196): Jacobian h (hdnnp_num_weights doubles, the
+ * flat weight order, host memory) and innovation xi = observed - predicted.
+ * X = P h, K = X / (h.X + 1/eta), P -= K X^T, P += q I, w += K xi. */
+int hdnnp_kalman_update(hdnnp_ctx* c, const double* h, double xi);
 /* Derivative of every structure's energy with respect to every network
  * weight, for any model (only the SFs enter): J[s * nw + w], nw =
  * hdnnp_num_weights, s over the loaded structures (one, or the batch of

@@ -227,6 +227,12 @@ struct hdnnp_ctx
         int recCap = 0;
     } mts;
 
+    // Kalman filter on the weights (hdnnp_kalman_*).
+    DBuf<double> kfP, kfX, kfA, kfH;
+    double kfEta = 0.0, kfEtaTau = 0.0, kfEtaMax = 0.0, kfQ = 0.0,
+           kfQTau = 0.0, kfQMin = 0.0;
+    int64_t kfUpdates = -1;
+
     // Energy weight Jacobian (hdnnp_energy_weight_jacobian).
     DBuf<int> jAtoms, jB0, jBN, jBE, jGrp0, jBase, jNwe;
     DBuf<double> jPart, jOut;
@@ -1297,6 +1303,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
         c->jAtoms.release(); c->jB0.release(); c->jBN.release();
         c->jBE.release(); c->jGrp0.release(); c->jBase.release();
         c->jNwe.release(); c->jPart.release(); c->jOut.release();
+        c->kfP.release(); c->kfX.release(); c->kfA.release(); c->kfH.release();
         c->mts.d2.release(); c->mts.sum.release(); c->mts.epot.release();
         c->mts.ekin.release(); c->mts.counter.release();
         c->mts.full = nullptr;
@@ -2301,6 +2308,54 @@ int hdnnp_energy_weight_jacobian(hdnnp_ctx* c, double* J, int file_order)
                 }
             }
         }
+    });
+}
+
+int hdnnp_kalman_init(hdnnp_ctx* c, double epsilon, double q0, double qtau,
+                      double qmin, double eta0, double etatau, double etamax)
+{
+    return guarded([&]() {
+        if (!(epsilon > 0.0) || !(eta0 > 0.0))
+            throw Error(HDNNP_E_INVALID, "ERROR: Kalman filter needs epsilon "
+                                         "> 0 and eta > 0.\n");
+        c->setDevice();
+        size_t const nw = c->flat.weights.size();
+        c->kfP.ensure(nw * nw);
+        c->kfX.ensure(nw);
+        c->kfA.ensure(1);
+        c->kfH.ensure(nw);
+        hipCheck(launch_kf_init((int)nw, c->kfP.p, 1.0 / epsilon, c->stream),
+                 "kalman init");
+        c->kfEta = eta0;
+        c->kfEtaTau = etatau;
+        c->kfEtaMax = etamax;
+        c->kfQ = q0;
+        c->kfQTau = qtau;
+        c->kfQMin = qmin;
+        c->kfUpdates = 0;
+        c->sync();
+    });
+}
+
+int hdnnp_kalman_update(hdnnp_ctx* c, const double* h, double xi)
+{
+    return guarded([&]() {
+        if (c->kfUpdates < 0)
+            throw Error(HDNNP_E_INVALID, "ERROR: No Kalman filter "
+                                         "(hdnnp_kalman_init).\n");
+        c->setDevice();
+        size_t const nw = c->flat.weights.size();
+        hipCheck(hipMemcpyAsync(c->kfH.p, h, nw * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream), "jacobian");
+        if (c->kfEta < c->kfEtaMax) c->kfEta *= std::exp(c->kfEtaTau);
+        hipCheck(launch_kf_update((int)nw, c->kfP.p, c->kfH.p, c->kfX.p,
+                                  c->kfA.p, 1.0 / c->kfEta, c->kfQ, xi,
+                                  c->dWeights.p, c->dWeights32.p, c->stream),
+                 "kalman update");
+        if (c->kfQ > c->kfQMin) c->kfQ *= std::exp(-c->kfQTau);
+        c->kfUpdates += 1;
+        c->sync();
+        c->haveNN = c->haveNNDeriv = false;
     });
 }
 

@@ -111,6 +111,13 @@ hipError_t launch_adam(int n, double* w, float* w32, const double* g,
                        double eps, int64_t step, hipStream_t s);
 hipError_t launch_weights_to_f32(int n, const double* w, float* w32,
                                  hipStream_t s);
+// Kalman filter on the flat weights, one scalar observation: P (nw x nw)
+// = I diag; update with Jacobian h (device), innovation xi, measurement
+// noise rinv = 1/eta and process noise q (KalmanFilter.cpp:126-196).
+hipError_t launch_kf_init(int nw, double* P, double diag, hipStream_t s);
+hipError_t launch_kf_update(int nw, double* P, const double* h, double* X,
+                            double* a, double rinv, double q, double xi,
+                            double* w, float* w32, hipStream_t s);
 // dE_s/dw per structure (blocks of at most train_atoms_per_block() atoms of
 // one element and one structure; LDS perAtom doubles per atom).
 hipError_t launch_energy_wgrad(ModelTables const& T, const double* G, int ldG,

@@ -547,7 +547,99 @@ __global__ void k_energy_wgrad_reduce(const double* __restrict__ part,
     }
 }
 
+// Kalman filter, one scalar observation (KalmanFilter.cpp:126-196 with
+// sizeObservation 1): X = P h, one wave per row of the symmetric P.
+__global__ __launch_bounds__(64) void k_kf_gemv(int nw,
+                                                const double* __restrict__ P,
+                                                const double* __restrict__ h,
+                                                double* __restrict__ X)
+{
+    int const i = blockIdx.x;
+    if (i >= nw) return;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nw; j += 64)
+        acc += P[(size_t)i * nw + j] * h[j];
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) X[i] = acc;
+}
+
+// a = h . X (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void k_kf_dot(int nw,
+                                                const double* __restrict__ h,
+                                                const double* __restrict__ X,
+                                                double* __restrict__ a)
+{
+    __shared__ double s[256];
+    int const t = threadIdx.x;
+    double acc = 0.0;
+    for (int j = t; j < nw; j += 256) acc += h[j] * X[j];
+    s[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if (t < o) s[t] += s[t + o];
+        __syncthreads();
+    }
+    if (t == 0) a[0] = s[0];
+}
+
+// K = X / (a + 1/eta); P -= K X^T; P += q I; w += K xi.  One workgroup
+// per row of P, threads over its columns.
+__global__ __launch_bounds__(256) void k_kf_update(
+    int nw, double* __restrict__ P, const double* __restrict__ X,
+    const double* __restrict__ a, double rinv, double q, double xi,
+    double* __restrict__ w, float* __restrict__ w32)
+{
+    int const i = blockIdx.x;
+    if (i >= nw) return;
+    double const den = a[0] + rinv;
+    double const Ki = X[i] / den;
+    double const Xi = X[i];
+    double* __restrict__ Pi = P + (size_t)i * nw;
+    // (X_i X_j) / den: the same operands for (i, j) and (j, i), so P stays
+    // exactly symmetric.
+    for (int j = threadIdx.x; j < nw; j += 256)
+    {
+        double v = Pi[j] - (Xi * X[j]) / den;
+        if (j == i) v += q;
+        Pi[j] = v;
+    }
+    if (threadIdx.x == 0)
+    {
+        double const wi = w[i] + Ki * xi;
+        w[i] = wi;
+        w32[i] = (float)wi;
+    }
+}
+
+__global__ void k_kf_init(int nw, double* __restrict__ P, double diag)
+{
+    size_t const k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)nw * nw) return;
+    int const i = (int)(k / nw), j = (int)(k - (size_t)i * nw);
+    P[k] = i == j ? diag : 0.0;
+}
+
 }  // namespace
+
+hipError_t launch_kf_init(int nw, double* P, double diag, hipStream_t s)
+{
+    size_t const tot = (size_t)nw * nw;
+    if (tot == 0) return hipSuccess;
+    k_kf_init<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(nw, P, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_kf_update(int nw, double* P, const double* h, double* X,
+                            double* a, double rinv, double q, double xi,
+                            double* w, float* w32, hipStream_t s)
+{
+    if (nw <= 0) return hipSuccess;
+    k_kf_gemv<<<nw, 64, 0, s>>>(nw, P, h, X);
+    k_kf_dot<<<1, 256, 0, s>>>(nw, h, X, a);
+    k_kf_update<<<nw, 256, 0, s>>>(nw, P, X, a, rinv, q, xi, w, w32);
+    return hipGetLastError();
+}
 
 hipError_t launch_energy_wgrad(ModelTables const& T, const double* G, int ldG,
                                const int* blkAtoms, const int* blk0,

@@ -115,6 +115,10 @@ _SIGNATURES = [
     ("hdnnp_train_adam", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("hdnnp_train_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("hdnnp_kalman_init", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    ("hdnnp_kalman_update", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_double]),
     ("hdnnp_energy_weight_jacobian", ctypes.c_int,
      [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
@@ -402,6 +406,18 @@ class Context:
 
     def train_reset(self):
         _check(_lib.hdnnp_train_reset(self._h))
+
+    def kalman_init(self, epsilon=1e-2, q0=0.01, qtau=2.302, qmin=1e-6,
+                    eta0=0.01, etatau=2.302, etamax=1.0):
+        _check(_lib.hdnnp_kalman_init(self._h, epsilon, q0, qtau, qmin, eta0,
+                                      etatau, etamax))
+
+    def kalman_update(self, h, xi):
+        """One scalar-observation Kalman update of the weights: Jacobian h
+        (flat weight order), innovation xi."""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        _check(_lib.hdnnp_kalman_update(self._h, _ptr(h, ctypes.c_double),
+                                        float(xi)))
 
     def energy_weight_jacobian(self, file_order=True):
         """dE_s/dw for every loaded structure s: (nstruct, nweights)."""

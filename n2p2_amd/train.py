@@ -1,13 +1,14 @@
-"""Training of a 2G-HDNNP on the GPU with the engine's gradient and Adam
-kernels: the gradient-descent (Adam) updater of nnp-train on energies and
-forces (nnp-train.cpp, Training.cpp, GradientDescent.cpp).
+"""Training of a 2G-HDNNP on the GPU with the engine's Jacobian, gradient,
+Adam and Kalman-filter kernels: the two updaters of nnp-train on energies
+and forces (nnp-train.cpp, Training.cpp, GradientDescent.cpp,
+KalmanFilter.cpp).
 
 What it does, in the reference's terms:
 - settings from input.nn: epochs, random_seed, test_fraction,
-  use_short_forces, force_weight, weights_min / weights_max and
-  gradient_adam_eta / beta1 / beta2 / epsilon (updater_type and
-  gradient_type are read as Adam whatever they say: the Kalman filter is
-  not built);
+  use_short_forces, force_weight, short_force_fraction, weights_min /
+  weights_max, updater_type (0 gradient descent with Adam,
+  gradient_adam_eta / beta1 / beta2 / epsilon; 1 the standard Kalman
+  filter, kalman_epsilon / q0 / qtau / qmin / eta / etatau / etamax);
 - data from input.data (energies and forces), converted to the model's
   internal units as Dataset::toNormalizedUnits does (energy offsets of
   atom_energy removed, mean_energy per atom subtracted, conv_energy and
@@ -16,9 +17,15 @@ What it does, in the reference's terms:
   n2p2_amd.scaling writes it);
 - initial weights uniform in [weights_min, weights_max] (own generator,
   seeded with random_seed) unless weights.%03zu.data files are present;
-- per epoch one Adam step per training structure on
+- Adam: per epoch one step per training structure on
   ((E - Eref)/N)^2 + force_weight / (3N) sum |F - Fref|^2, in random
   order;
+- Kalman filter: per epoch one energy update per training structure
+  (observation E/N, innovation (Eref - E)/N, Jacobian dE/dw / N) and
+  short_force_fraction of all force components as force updates
+  (innovation Fref - F, Jacobian dF/dw), in random order; the reference's
+  update-candidate selection modes and force weighting are not
+  reproduced;
 - learning-curve.out: per epoch the energy RMSE per atom and the force
   RMSE of the training and test sets in physical units, and the weights of
   every epoch as weights.%03zu.%06zu.out (the reference's names).
@@ -62,6 +69,14 @@ class Training:
         self.test_fraction = g("test_fraction", 0.1)
         self.use_forces = "use_short_forces" in kw
         self.force_weight = g("force_weight", 1.0)
+        self.updater = g("updater_type", 0)
+        self.force_fraction = g("short_force_fraction", 0.1)
+        self.kalman = dict(epsilon=g("kalman_epsilon", 1e-2),
+                           q0=g("kalman_q0", 0.01), qtau=g("kalman_qtau", 2.302),
+                           qmin=g("kalman_qmin", 1e-6),
+                           eta0=g("kalman_eta", 0.01),
+                           etatau=g("kalman_etatau", 2.302),
+                           etamax=g("kalman_etamax", 1.0))
         self.eta = g("gradient_adam_eta", 1e-3)
         self.beta1 = g("gradient_adam_beta1", 0.9)
         self.beta2 = g("gradient_adam_beta2", 0.999)
@@ -143,6 +158,33 @@ class Training:
         hidden = [int(x) for x in kw["global_nodes_short"]]
         return [self.model.num_sf(e)] + hidden + [1]
 
+    def _kalman_epoch(self):
+        """One epoch of Kalman updates (energies, then a fraction of the
+        force components), in random order."""
+        ups = [("e", k) for k in self.train_set]
+        if self.use_forces:
+            comps = [(k, i, c) for k in self.train_set
+                     for i in range(len(self.data[k][1])) for c in range(3)]
+            nf = int(round(self.force_fraction * len(comps)))
+            for q in self.rng.choice(len(comps), nf, replace=False):
+                ups.append(("f",) + comps[q])
+        for q in self.rng.permutation(len(ups)):
+            u = ups[q]
+            r, elem, box, e_int, f_int = self.data[u[1]]
+            n = len(elem)
+            self.ctx.set_structure(r, elem, box)
+            if u[0] == "e":
+                self.ctx.calculate_neighbor_list(self.model.max_cutoff)
+                h = self.ctx.energy_weight_jacobian(file_order=False)[0] / n
+                e = self.ctx.evaluate(-1)
+                self.ctx.kalman_update(h, (e_int - e) / n)
+            else:
+                _, _, i, c = u
+                self.ctx.evaluate(engine.FORCES_CONSISTENT)
+                f = self.ctx.forces()
+                h = _force_jacobian(self.ctx, f, i, c)
+                self.ctx.kalman_update(h, f_int[i, c] - f[i, c])
+
     def run(self, epochs=None, log=print):
         epochs = self.epochs if epochs is None else epochs
         curve = []
@@ -161,15 +203,26 @@ class Training:
 
         record(0)
         self.ctx.train_reset()
+        if self.updater == 1:
+            k = self.kalman
+            # Per-update annealing from per-epoch settings, as the
+            # reference's tau parameters count epochs.
+            nupd = max(len(self.train_set), 1)
+            self.ctx.kalman_init(k["epsilon"], k["q0"], k["qtau"] / nupd,
+                                 k["qmin"], k["eta0"], k["etatau"] / nupd,
+                                 k["etamax"])
         wf = self.force_weight if self.use_forces else 0.0
         for ep in range(1, epochs + 1):
-            for k in self.rng.permutation(self.train_set):
-                r, elem, box, e_int, f_int = self.data[k]
-                self.ctx.set_structure(r, elem, box)
-                self.ctx.train_gradient(f_int, e_int, 1.0, wf,
-                                        want_grad=False)
-                self.ctx.train_adam(self.eta, self.beta1, self.beta2,
-                                    self.eps)
+            if self.updater == 1:
+                self._kalman_epoch()
+            else:
+                for k in self.rng.permutation(self.train_set):
+                    r, elem, box, e_int, f_int = self.data[k]
+                    self.ctx.set_structure(r, elem, box)
+                    self.ctx.train_gradient(f_int, e_int, 1.0, wf,
+                                            want_grad=False)
+                    self.ctx.train_adam(self.eta, self.beta1, self.beta2,
+                                        self.eps)
             record(ep)
         with open(self._p("learning-curve.out"), "w") as f:
             f.write("# Learning curve (GPU Adam training): epoch, RMSE of "
@@ -177,6 +230,18 @@ class Training:
                     "(train, test), physical units\n")
             f.writelines(rows)
         return curve
+
+
+def _force_jacobian(ctx, f, i, c):
+    """dF_ic/dw from the loss gradient: with Fref = F - e_ic (3N / 2) the
+    gradient of 1/(3N) sum |F - Fref|^2 is exactly dF_ic/dw."""
+    n = len(f)
+    fref = f.copy()
+    fref[i, c] -= 1.5 * n
+    g, _ = ctx.train_gradient(fref, 0.0, 0.0, 1.0)
+    return g
+
+
 
 
 def main():

@@ -1,6 +1,7 @@
-"""The GPU trainer (n2p2_amd/train.py, the Adam updater of nnp-train) on
-the reference's example data set (tests/golden/nnp_scaling_H2O: 20 water
-structures with energies and forces, its input.nn and scaling.data).
+"""The GPU trainer (n2p2_amd/train.py) on the reference's example data set
+(tests/golden/nnp_scaling_H2O: 20 water structures with energies and
+forces, its input.nn and scaling.data; the settings ask for the Kalman
+filter, updater_type 1).
 
 - Three epochs from random weights lower the training RMSE of energies and
   forces (the reference's own test only checks that nnp-train runs).
