@@ -297,7 +297,8 @@ def main():
     ap.add_argument("--nmol", type=int, default=65536)
     ap.add_argument("--box", type=float, default=124.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--path", choices=("both", "mixed", "fp64", "md"),
+    ap.add_argument("--path", choices=("both", "mixed", "fp64", "md",
+                                       "batch"),
                     default="both",
                     help="time one path only (for profiling runs; the "
                          "reported line needs both)")
@@ -323,6 +324,10 @@ def main():
                           [[model.num_sf(e)] + list(hidden) + [1]])
                       for e in range(model.num_elements)))
 
+        if args.path == "batch":
+            run_batch(tmp, args.steps, args.warmup, log)
+            log("batch only: no JSON line")
+            return
         if args.path == "md":
             res = run_md(model, r, elem, box, engine.PRECISION_MIXED,
                          args.steps, args.warmup)
