@@ -667,6 +667,21 @@ __device__ __forceinline__ int lanes_below(uint64_t mask)
     return __popcll(mask & ((1ull << lane_id()) - 1ull));
 }
 
+// Weight of staged neighbour t for the weighted SF types: the atomic number
+// of the element block (neighbours staged partitioned by element) that
+// holds it.
+__device__ __forceinline__ double staged_weight(ModelTables const& T,
+                                                int const* sStart,
+                                                int const* sCnt, int t)
+{
+    double w = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMaxElements; ++q)
+        if (q < T.ne && t >= sStart[q] && t < sStart[q] + sCnt[q])
+            w = T.zw[q];
+    return w;
+}
+
 }  // namespace hdnnp
 
 #endif

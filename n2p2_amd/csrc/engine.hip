@@ -325,13 +325,15 @@ struct hdnnp_ctx
         T.ne = (int)flat.elem.size();
         T.maxnsf = model->m.maxNumSF();
         T.maxCutoff = model->m.maxCutoffRadius;
+        for (int k = 0; k < kMaxElements; ++k)
+            T.zw[k] = k < T.ne ? (double)model->m.el[k].Z : 0.0;
         T.cutType = model->m.cutoffType;
         T.genpow = 0;
         T.maxGroupMembers = 0;
         for (auto const& g : flat.group)
         {
             T.maxGroupMembers = std::max(T.maxGroupMembers, g.nmem);
-            if (g.type != 2)
+            if (!sf_radial(g.type))
                 for (int m = g.mem0; m < g.mem0 + g.nmem; ++m)
                     if (!flat.member[m].useint) T.genpow = 1;
         }
@@ -1225,7 +1227,7 @@ int hdnnp_model_sf_table(const hdnnp_model* m, int e, double* out)
         for (size_t i = 0; i < sf.size(); ++i)
         {
             double* o = out + 12 * i;
-            o[0] = sf[i].type; o[1] = sf[i].e1; o[2] = sf[i].type == 2 ? -1 : sf[i].e2;
+            o[0] = sf[i].type; o[1] = sf[i].e1; o[2] = sf_radial(sf[i].type) ? -1 : sf[i].e2;
             o[3] = sf[i].eta; o[4] = sf[i].rs; o[5] = sf[i].lambda;
             o[6] = sf[i].zeta; o[7] = sf[i].rc; o[8] = sf[i].cutoffType;
             o[9] = sf[i].cutoffAlpha; o[10] = sf[i].scalingFactor;

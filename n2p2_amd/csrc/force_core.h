@@ -355,19 +355,32 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
         GroupDesc const g = T.group[gi];
+        bool const weighted = sf_weighted(g.type);
+        // Cache key: the cutoff configuration and whether the values carry
+        // the neighbour's atomic number.
         int const cfg = ed.cfg0 + g.cfg;
+        int const key = 2 * cfg + (weighted ? 1 : 0);
         CutCfg const cc = T.cfg[cfg];
-        if (cfg != cfgCached)
+        if (key != cfgCached)
         {
-            // f_c and f_c'/r of every staged neighbour for this cutoff.
+            // f_c and f_c'/r of every staged neighbour for this cutoff;
+            // weighted types: both times the neighbour's atomic number, so
+            // every radial term carries Z_j and every triplet term Z_j Z_k
+            // (SymGrpExpAngnWeighted.cpp:263-270).
             for (int t = lane; t < n; t += 64)
             {
                 R f, df;
                 cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
+                if (weighted)
+                {
+                    R const z = (R)staged_weight(T, sStart, sCnt, t);
+                    f *= z;
+                    df *= z;
+                }
                 S.Q[t].x = f;
                 S.Q[t].z = df * S.Q[t].y;
             }
-            cfgCached = cfg;
+            cfgCached = key;
         }
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         int const nmem = g.nmem;
@@ -376,17 +389,18 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         {
             MemberDesc const& md = mem[lane];
             double const w = dEdGi[md.sf]
-                           * (g.type == 2 ? md.sfac : md.fnorm);
+                           * (sf_radial(g.type) ? md.sfac : md.fnorm);
             MP[lane] = make_mparam<R>(md, w);
         }
         __syncthreads();
         R const grc = (R)g.rc;
 
-        if (g.type == 2)
+        if (sf_radial(g.type))
         {
             // SymGrpExpRad.cpp:194-233: p1 = sf (f' - 2 eta (r-rs) f) e / r;
             // slot j receives -p1 dr_ij.  One lane per slot.
-            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            int const a0 = weighted ? 0 : sStart[g.e1];
+            int const na = weighted ? n : sCnt[g.e1];
             for (int t = lane; t < na; t += 64)
             {
                 int const j = a0 + t;
@@ -412,8 +426,11 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         }
 
         bool const wide = (g.type == 9);
-        DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
-                           sStart[g.e2], sCnt[g.e2]);
+        // Weighted type: all unordered pairs of the whole list.
+        DiagPairs const dp = weighted
+            ? DiagPairs(true, 0, n, 0, n)
+            : DiagPairs(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                        sStart[g.e2], sCnt[g.e2]);
         if (dp.npairs == 0)
         {
             __syncthreads();

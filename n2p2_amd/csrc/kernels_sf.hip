@@ -47,9 +47,11 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
         GroupDesc const g = T.group[gi];
-        if (g.type == 2)
+        bool const weighted = sf_weighted(g.type);
+        if (sf_radial(g.type))
         {
-            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            int const a0 = weighted ? 0 : sStart[g.e1];
+            int const na = weighted ? n : sCnt[g.e1];
             for (int t = lane; t < na; t += 64)
                 if (S.P[a0 + t].w < g.rc) c[1] += g.nmem;
             continue;
@@ -64,8 +66,10 @@ __global__ __launch_bounds__(64) void k_sf_count(SFArgs a,
             do { bits += (e & 1); e >>= 1; ++it; } while (e);
             powm += md.useint ? it + bits : 0;
         }
-        DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
-                           sStart[g.e2], sCnt[g.e2]);
+        DiagPairs const dp = weighted
+            ? DiagPairs(true, 0, n, 0, n)
+            : DiagPairs(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                        sStart[g.e2], sCnt[g.e2]);
         int pc = 0, pr = 0;
         dp.start(lane, pc, pr);
         for (int base = 0; base < dp.npairs; base += 64)

@@ -91,18 +91,25 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
         GroupDesc const g = T.group[gi];
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
+        bool const weighted = sf_weighted(g.type);
         double acc[kMaxMembers];
 #pragma unroll
         for (int q = 0; q < kMaxMembers; ++q) acc[q] = 0.0;
         for (int t = lane; t < m; t += 64)
         {
             int const j = A.nbrJ[o + t];
-            if (A.elem[j] != g.e1) continue;
+            if (!weighted && A.elem[j] != g.e1) continue;
             double4 const gq = A.geo[o + t];
             double const r = gq.w;
             if (!(r < g.rc)) continue;
             double fc, dfc;
             cutoff_fdf<double>(cc, r, fc, dfc);
+            if (weighted)
+            {
+                // SymGrpExpRadWeighted: every term times Z_j.
+                fc *= T.zw[A.elem[j]];
+                dfc *= T.zw[A.elem[j]];
+            }
             double proj = 0.0;
             proj += gq.x * (da[0] - (A.F[3 * (size_t)j] - A.Fref[3 * (size_t)j]));
             proj += gq.y * (da[1] - (A.F[3 * (size_t)j + 1]
@@ -129,8 +136,10 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
                     U[sf] = A.cF * scale_slope(T.scale[ed.sf0 + sf]) * s;
             }
     }
-    // Angular groups, G3 (SymFncExpAngn.cpp:79-225) and G9
-    // (SymFncExpAngw.cpp:79-208): for the pair (j, k), j < k in list order,
+    // Angular groups, G3 (SymFncExpAngn.cpp:79-225), G9
+    // (SymFncExpAngw.cpp:79-208) and the weighted G3 (type 13,
+    // SymFncExpAngnWeighted.cpp:79-228, every pair of the list):
+    // for the pair (j, k), j < k in list order,
     // the reference's p1, p2, p3 give
     //   sum_i d_i . dG/dr_i = p1 dr_ij.(d_a - d_j) + p2 dr_ik.(d_a - d_k)
     //                       + p3 dr_jk.(d_k - d_j),   dr_jk = dr_ik - dr_ij.
@@ -140,6 +149,7 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         bool const wide = g.type == 9;
+        bool const weighted = sf_weighted(g.type);
         double acc[kMaxMembers];
 #pragma unroll
         for (int q = 0; q < kMaxMembers; ++q) acc[q] = 0.0;
@@ -149,7 +159,8 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
             int const ej = A.elem[aj];
             double4 const gj = A.geo[o + j];
             double const rij = gj.w;
-            if (!((ej == g.e1 || ej == g.e2) && rij < g.rc)) continue;
+            if (!((weighted || ej == g.e1 || ej == g.e2) && rij < g.rc))
+                continue;
             double fcij, dfcij;
             cutoff_fdf<double>(cc, rij, fcij, dfcij);
             double dj[3];
@@ -160,7 +171,8 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
             {
                 int const ak = A.nbrJ[o + k];
                 int const ek = A.elem[ak];
-                if (!((g.e1 == ej && g.e2 == ek) || (g.e2 == ej && g.e1 == ek)))
+                if (!weighted
+                    && !((g.e1 == ej && g.e2 == ek) || (g.e2 == ej && g.e1 == ek)))
                     continue;
                 double4 const gk = A.geo[o + k];
                 double const rik = gk.w;
@@ -187,7 +199,10 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
                 double const rinv = 1.0 / rij / rik;
                 double const cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z)
                                   * rinv;
-                double const pfc = fcij * fcik * fcjk;
+                // Weighted type 13: the cutoff products times Z_j Z_k
+                // (SymGrpExpAngnWeighted.cpp:263-270).
+                double const z = weighted ? T.zw[ej] * T.zw[ek] : 1.0;
+                double const pfc = z * fcij * fcik * fcjk;
 #pragma unroll
                 for (int q = 0; q < kMaxMembers; ++q)
                     if (q < g.nmem)
@@ -209,14 +224,16 @@ __global__ __launch_bounds__(64) void k_train_u(TrainArgs A)
                         double const p2e = 2.0 * md.eta * pl;
                         double const p1 = fg * (pfc * (ri - ct / (rij * rij)
                                                 - p2e * rijs / rij)
-                                                + fcik * fcjk * dfcij * pl / rij);
+                                                + z * fcik * fcjk * dfcij * pl
+                                                / rij);
                         double const p2 = fg * (pfc * (ri - ct / (rik * rik)
                                                 - p2e * riks / rik)
-                                                + fcij * fcjk * dfcik * pl / rik);
+                                                + z * fcij * fcjk * dfcik * pl
+                                                / rik);
                         double const p3 = wide
                             ? fg * pfc * ri
                             : fg * (pfc * (ri + p2e * rjks / rjk)
-                                    - fcij * fcik * dfcjk * pl / rjk);
+                                    - z * fcij * fcik * dfcjk * pl / rjk);
                         acc[q] += md.sfac * (p1 * tij + p2 * tik + p3 * tjk);
                     }
             }

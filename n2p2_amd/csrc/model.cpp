@@ -65,7 +65,9 @@ int activationCode(std::string const& c)
 
 // Ordering of SymFncExpRad.cpp:55-75 / SymFncExpAngn.cpp:51-77 /
 // SymFncExpAngw.cpp (centre element, type, cutoff, rc, eta, rs, [zeta,
-// lambda], e1, [e2]).
+// lambda], e1, [e2]); the weighted types (SymFncExpRadWeighted.cpp:39-58,
+// SymFncExpAngnWeighted.cpp:41-64) have no neighbour elements (e1 = e2 =
+// -1 here).
 bool lessSF(SymFn const& a, SymFn const& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
@@ -75,7 +77,7 @@ bool lessSF(SymFn const& a, SymFn const& b)
     if (a.rc != b.rc) return a.rc < b.rc;
     if (a.eta != b.eta) return a.eta < b.eta;
     if (a.rs != b.rs) return a.rs < b.rs;
-    if (a.type != 2)
+    if (!sf_radial(a.type))
     {
         if (a.zeta != b.zeta) return a.zeta < b.zeta;
         if (a.lambda != b.lambda) return a.lambda < b.lambda;
@@ -87,7 +89,9 @@ bool lessSF(SymFn const& a, SymFn const& b)
 
 bool sameGroup(SymFn const& a, SymFn const& b)
 {
-    // SymGrpExpRad.cpp:72-109 / SymGrpExpAngn.cpp:68-105 addMember checks.
+    // SymGrpExpRad.cpp:72-109 / SymGrpExpAngn.cpp:68-105 addMember checks
+    // (the weighted groups, SymGrpExpRadWeighted.cpp:68-99, have no
+    // neighbour elements to compare).
     return a.type == b.type && a.cutoffType == b.cutoffType
         && a.cutoffAlpha == b.cutoffAlpha && a.ec == b.ec && a.rc == b.rc
         && a.e1 == b.e1 && (a.type == 2 || a.e2 == b.e2);
@@ -259,10 +263,36 @@ void Model::load(std::string const& settingsFile,
             s.useIntegerPow = std::fabs(s.zeta - s.zetaInt)
                            <= std::numeric_limits<double>::min();
         }
+        else if (s.type == 12)   // SymFncExpRadWeighted.cpp:71-90
+        {
+            s.e1 = s.e2 = -1;
+            s.eta = std::atof(a.at(2).c_str());
+            s.rs = std::atof(a.at(3).c_str());
+            s.rc = std::atof(a.at(4).c_str());
+        }
+        else if (s.type == 13)   // SymFncExpAngnWeighted.cpp:92-123
+        {
+            s.e1 = s.e2 = -1;
+            s.eta = std::atof(a.at(2).c_str());
+            s.rs = std::atof(a.at(3).c_str());
+            s.lambda = std::atof(a.at(4).c_str());
+            s.zeta = std::atof(a.at(5).c_str());
+            s.rc = std::atof(a.at(6).c_str());
+            s.zetaInt = (int)std::round(s.zeta);
+            s.useIntegerPow = std::fabs(s.zeta - s.zetaInt)
+                           <= std::numeric_limits<double>::min();
+            // The kernels shift the exponent by rs when rs > 0 (the narrow
+            // type's rule); type 13 always shifts, which is the same for
+            // rs >= 0.
+            if (s.rs < 0.0)
+                throw std::runtime_error("ERROR: Type 13 symmetry functions "
+                                         "with rs < 0 are not supported.\n");
+        }
         else
             throw std::runtime_error(
                 "ERROR: Symmetry function type " + a.at(1)
-                + " is outside this engine (types 2, 3, 9 are supported).\n");
+                + " is outside this engine (types 2, 3, 9, 12, 13 are "
+                "supported).\n");
         s.indexPerElement.assign(ne, -1);
         el[s.ec].sf.push_back(s);
     }
@@ -292,7 +322,8 @@ void Model::load(std::string const& settingsFile,
         e.table.assign(ne, {});
         for (auto& s : e.sf)
             for (int i = 0; i < ne; ++i)
-                if (i == s.e1 || (s.type != 2 && i == s.e2))
+                if (sf_weighted(s.type) || i == s.e1
+                    || (s.type != 2 && i == s.e2))
                 {
                     s.indexPerElement[i] = (int)e.table[i].size();
                     e.table[i].push_back(s.index);
@@ -314,7 +345,7 @@ void Model::load(std::string const& settingsFile,
                 GroupHost g;
                 g.type = s.type;
                 g.e1 = s.e1;
-                g.e2 = s.type == 2 ? -1 : s.e2;
+                g.e2 = sf_radial(s.type) ? -1 : s.e2;
                 g.rc = s.rc;
                 g.members.push_back(s.index);
                 e.groups.push_back(g);
@@ -590,7 +621,7 @@ Model::Flat Model::flatten() const
         for (int pass = 0; pass < 2; ++pass)
             for (auto const& g : e.groups)
             {
-                if ((pass == 0) != (g.type == 2)) continue;
+                if ((pass == 0) != sf_radial(g.type)) continue;
                 // Members in SF order (SymGrp*::sortMembers), then, for the
                 // angular types, stably ordered by (eta, rs, lambda, zeta):
                 // an (eta, rs) run shares one exponential, an (eta, rs,
@@ -598,7 +629,7 @@ Model::Flat Model::flatten() const
                 // keeps its own SF slot, so the order is free.
                 std::vector<int> mem = g.members;
                 std::sort(mem.begin(), mem.end());
-                if (g.type != 2)
+                if (!sf_radial(g.type))
                     std::stable_sort(mem.begin(), mem.end(),
                                      [&](int a, int b) {
                         SymFn const& x = e.sf[a];
@@ -633,7 +664,7 @@ Model::Flat Model::flatten() const
                     // Within each (eta, rs, lambda) run: chain members by
                     // level, then generic members.
                     std::vector<int> cm(mem.begin() + c0, mem.begin() + c1);
-                    if (g.type != 2)
+                    if (!sf_radial(g.type))
                     {
                         size_t r0 = 0;
                         while (r0 < cm.size())
@@ -662,7 +693,7 @@ Model::Flat Model::flatten() const
                         md.zeta = s.zeta;
                         md.zl = s.zeta * s.lambda;
                         md.sfac = s.scalingFactor;
-                        if (g.type != 2)
+                        if (!sf_radial(g.type))
                         {
                             // SymGrpExpAngn.cpp:107-148 then
                             // SymGrpBaseExpAng.cpp:45-57.
@@ -690,7 +721,7 @@ Model::Flat Model::flatten() const
                         md.newlam = md.calcexp
                                  || e.sf[cm[q - 1]].lambda != s.lambda;
                         md.level = levelOf(s);
-                        if (g.type != 2)
+                        if (!sf_radial(g.type))
                         {
                             if (md.newlam)
                             {
@@ -726,6 +757,7 @@ Model::Flat Model::flatten() const
         for (int gi = d.g0 + d.nrad; gi < d.g0 + d.ng; ++gi)
         {
             GroupDesc const& gd = F.group[gi];
+            if (sf_weighted(gd.type)) continue;   // not tied to an element pair
             for (int o = 0; o < 2; ++o)
             {
                 int const a = o ? gd.e2 : gd.e1, b = o ? gd.e1 : gd.e2;

@@ -8,8 +8,11 @@
 // summed in FP64, scaling in FP64).
 //
 // Restates SymGrpExpRad::calculate (SymGrpExpRad.cpp:144-239),
-// SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403) and
-// SymGrpExpAngw::calculate (SymGrpExpAngw.cpp:156-376) without derivatives
+// SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403),
+// SymGrpExpAngw::calculate (SymGrpExpAngw.cpp:156-376) and the weighted
+// SymGrpExpRadWeighted / SymGrpExpAngnWeighted::calculate
+// (SymGrpExpRadWeighted.cpp:144-252, SymGrpExpAngnWeighted.cpp:166-385)
+// without derivatives
 // (the force kernel recomputes those).  One wavefront per atom:
 //   1. the atom's neighbours are staged in LDS, partitioned by element
 //      (stable ballot compaction), so every group reads exactly the
@@ -232,6 +235,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
     stage_by_element(a, S, s0, n, sStart, sCnt);
 
     double rcCached = -1.0;
+    bool wCached = false;
     double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
@@ -239,25 +243,33 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         int const nmem = g.nmem;
+        bool const weighted = sf_weighted(g.type);
         if (lane < nmem) MP[lane] = make_mparam<R>(mem[lane], 0.0);
-        if (g.rc != rcCached)
+        if (g.rc != rcCached || weighted != wCached)
         {
+            // Weighted types: f_c times the neighbour's atomic number, so
+            // radial terms carry Z_j and triplet products Z_j Z_k
+            // (SymGrpExpRadWeighted.cpp:200-207,
+            // SymGrpExpAngnWeighted.cpp:263-270).
             for (int t = lane; t < n; t += 64)
             {
                 R f, df;
                 cutoff_fdf_t<CT>(cc, S.P[t].w, f, df);
+                if (weighted) f *= (R)staged_weight(T, sStart, sCnt, t);
                 S.Q[t].x = f;
             }
             rcCached = g.rc;
+            wCached = weighted;
         }
         __syncthreads();
         R acc[MAXM];
 #pragma unroll
         for (int m = 0; m < MAXM; ++m) acc[m] = R(0);
 
-        if (g.type == 2)
+        if (sf_radial(g.type))
         {
-            int const a0 = sStart[g.e1], na = sCnt[g.e1];
+            int const a0 = weighted ? 0 : sStart[g.e1];
+            int const na = weighted ? n : sCnt[g.e1];
             R const rc = (R)g.rc;
             for (int t = lane; t < na; t += 64)
             {
@@ -278,8 +290,11 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
         else
         {
             bool const wide = (g.type == 9);
-            DiagPairs const dp(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
-                               sStart[g.e2], sCnt[g.e2]);
+            // Weighted type: all unordered pairs of the whole list.
+            DiagPairs const dp = weighted
+                ? DiagPairs(true, 0, n, 0, n)
+                : DiagPairs(g.e1 == g.e2, sStart[g.e1], sCnt[g.e1],
+                            sStart[g.e2], sCnt[g.e2]);
             R const grc = (R)g.rc, grc2 = (R)g.rc2;
             FwdMembers<R, MAXM> M;
 #pragma unroll
@@ -333,7 +348,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
         if ((lane & ((1 << kSh) - 1)) == 0 && m < nmem)
         {
             MemberDesc const& md = mem[m];
-            double const raw = (g.type == 2) ? v : v * md.gfac;
+            double const raw = sf_radial(g.type) ? v : v * md.gfac;
             Gi[md.sf] = a.raw ? raw : scale_sf(T.scale[ed.sf0 + md.sf], raw);
         }
         __syncthreads();

@@ -305,7 +305,7 @@ std::string plan_source(Model::Flat const& F)
     for (size_t gi = 0; gi < F.group.size(); ++gi)
     {
         GroupDesc const& g = F.group[gi];
-        if (g.type == 2) continue;
+        if (sf_radial(g.type)) continue;
         emit_backward_case(b, (int)gi, g, F);
         emit_forward_case(f, (int)gi, g, F);
     }

@@ -40,6 +40,12 @@ enum ActType { AF_UNSET = 0, AF_IDENTITY, AF_TANH, AF_LOGISTIC, AF_SOFTPLUS,
                AF_RELU, AF_GAUSSIAN, AF_COS, AF_REVLOGISTIC, AF_EXP,
                AF_HARMONIC };
 
+// Symmetry-function types: 2 and 12 are radial, 3, 9 and 13 angular; 12
+// and 13 are the element-weighted variants (every neighbour counts, times
+// its atomic number; SymFncExpRadWeighted, SymFncExpAngnWeighted).
+constexpr bool sf_radial(int type) { return type == 2 || type == 12; }
+constexpr bool sf_weighted(int type) { return type == 12 || type == 13; }
+
 struct CutCfg
 {
     int    type;
@@ -50,8 +56,9 @@ struct CutCfg
 
 struct GroupDesc
 {
-    int    type;      // 2, 3 or 9
-    int    e1, e2;    // neighbour elements (e2 = -1 for radial)
+    int    type;      // 2, 3, 9, 12 or 13
+    int    e1, e2;    // neighbour elements (e2 = -1 for radial; both -1 for
+                      // the weighted types)
     int    nmem;      // members in this chunk (<= kMaxMembers)
     int    mem0;      // first member in the member table
     int    cfg;       // index into the element's CutCfg range (local)
@@ -129,6 +136,8 @@ struct ModelTables
     int genpow;           // some angular member has a non-integer zeta
     int maxGroupMembers;  // largest group chunk
     double maxCutoff;
+    double zw[kMaxElements];      // atomic number per element (weights of
+                                  // the weighted types)
     const ElemDesc*   elem;
     const CutCfg*     cfg;
     const GroupDesc*  group;

@@ -128,6 +128,40 @@ def mixed_settings(rc=6.0, cutoff_type=6):
     return "\n".join(lines) + "\n"
 
 
+def weighted_settings(rc=6.0, cutoff_type=6):
+    """B/C/N settings that mix the element-resolved types (2, 3) with the
+    element-weighted types 12 and 13 (SymFncExpRadWeighted,
+    SymFncExpAngnWeighted): two cutoff radii, shifted and unshifted
+    Gaussians, integer and non-integer zeta."""
+    el = MIXED_ELEMENTS
+    lines = ["number_of_elements 3", "elements B C N",
+             "cutoff_type %d 0.0" % cutoff_type,
+             "scale_symmetry_functions", "scale_min_short 0.0",
+             "scale_max_short 1.0",
+             "global_hidden_layers_short 2", "global_nodes_short 20 20",
+             "global_activation_short t t l"]
+    for e in el:
+        for e1 in el:
+            for eta in (0.01, 0.1, 0.5):
+                lines.append("symfunction_short %s 2 %s %.4f 0.0 %.4f"
+                             % (e, e1, eta, rc))
+        for e1, e2 in ((el[0], el[1]), (el[1], el[2])):
+            for lam in (1.0, -1.0):
+                for zeta in (1.0, 4.0):
+                    lines.append("symfunction_short %s 3 %s %s 0.01 %.1f %.1f "
+                                 "%.4f" % (e, e1, e2, lam, zeta, rc))
+        for rcw in (4.5, rc):
+            for eta, rs in ((0.02, 0.0), (0.3, 0.0), (0.5, 1.5), (1.0, 2.5)):
+                lines.append("symfunction_short %s 12 %.4f %.4f %.4f"
+                             % (e, eta, rs, rcw))
+        for eta, rs in ((0.005, 0.0), (0.05, 1.2)):
+            for lam in (1.0, -1.0):
+                for zeta in (1.0, 2.0, 3.5, 16.0):
+                    lines.append("symfunction_short %s 13 %.4f %.4f %.1f %.1f "
+                                 "%.4f" % (e, eta, rs, lam, zeta, rc))
+    return "\n".join(lines) + "\n"
+
+
 def count_sf(settings_text, symbol):
     return sum(1 for l in settings_text.splitlines()
                if l.startswith("symfunction_short %s " % symbol))

@@ -204,6 +204,12 @@ struct Cutoff
 enum ScalingType { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER,
                    ST_SCALESIGMA };
 
+// Types 2 and 12 are radial; 12 and 13 are the element-weighted variants
+// (SymFncExpRadWeighted, SymFncExpAngnWeighted): every neighbour counts,
+// weighted by its atomic number.
+inline bool isRadial(int type) { return type == 2 || type == 12; }
+inline bool isWeighted(int type) { return type == 12 || type == 13; }
+
 struct SF
 {
     int    type = 0;
@@ -221,8 +227,11 @@ struct SF
     double Gmin = 0, Gmax = 0, Gmean = 0, Gsigma = 0, Smin = 0, Smax = 0;
     double scalingFactor = 1.0;
 
-    bool relevant(size_t e) const  // SymFncExpRad.cpp:230, SymFncBaseExpAng.cpp:179
+    // SymFncExpRad.cpp:230, SymFncBaseExpAng.cpp:179,
+    // SymFncExpRadWeighted.cpp:213-216, SymFncExpAngnWeighted.cpp:338-341
+    bool relevant(size_t e) const
     {
+        if (isWeighted(type)) return true;
         if (type == 2) return (int)e == e1;
         return (int)e == e1 || (int)e == e2;
     }
@@ -256,7 +265,9 @@ struct SF
 };
 
 // SymFncExpRad.cpp:55-75 and SymFncExpAngn.cpp:51-77 / SymFncExpAngw.cpp
-// (identical field order for types 3 and 9): strict weak ordering.
+// (identical field order for types 3 and 9); SymFncExpRadWeighted.cpp and
+// SymFncExpAngnWeighted.cpp order the same fields without the neighbour
+// elements (e1 = e2 = -1 here): strict weak ordering.
 bool sfLess(SF const& a, SF const& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
@@ -266,7 +277,7 @@ bool sfLess(SF const& a, SF const& b)
     if (a.rc != b.rc) return a.rc < b.rc;
     if (a.eta != b.eta) return a.eta < b.eta;
     if (a.rs != b.rs) return a.rs < b.rs;
-    if (a.type != 2)
+    if (!isRadial(a.type))
     {
         if (a.zeta != b.zeta) return a.zeta < b.zeta;
         if (a.lambda != b.lambda) return a.lambda < b.lambda;
@@ -288,6 +299,8 @@ struct Group
     Cutoff fc;
     vector<int> members;             // indices into element's SF list
     vector<double> factorNorm, factorDeriv, scalingFactors;
+    vector<double> weight;           // weighted types: atomic number per
+                                     // neighbour element
     vector<char> calculateExp;
 };
 
@@ -675,16 +688,19 @@ double pow_int(double x, int n)
     return result;
 }
 
-// SymGrpExpRad.cpp:144-239
+// SymGrpExpRad.cpp:144-239; the weighted type 12 as
+// SymGrpExpRadWeighted.cpp:144-252 (every neighbour, pexp times its atomic
+// number).
 void calcRadial(Group const& g, vector<SF> const& sfs, Atom& atom,
                 bool derivatives)
 {
     size_t const nm = g.members.size();
+    bool const weighted = isWeighted(g.type);
     vector<double> result(nm, 0.0);
     for (size_t j = 0; j < atom.neighbors.size(); ++j)
     {
         Neighbor& n = atom.neighbors[j];
-        if (g.e1 == (int)n.element && n.d < g.rc)
+        if ((weighted || g.e1 == (int)n.element) && n.d < g.rc)
         {
             double const rij = n.d;
             double pfc, pdfc;
@@ -695,6 +711,7 @@ void calcRadial(Group const& g, vector<SF> const& sfs, Atom& atom,
                 SF const& sf = sfs[g.members[k]];
                 double const eta = sf.eta, rs = sf.rs;
                 double pexp = std::exp(-eta * (rij - rs) * (rij - rs));
+                if (weighted) pexp = g.weight[n.element] * pexp;
                 result[k] += pexp * pfc;
                 if (!derivatives) continue;
                 double const p1 = g.scalingFactors[k] * (pdfc - 2.0 * eta
@@ -705,7 +722,7 @@ void calcRadial(Group const& g, vector<SF> const& sfs, Atom& atom,
                 size_t ki = sf.index;
                 double* dGdr = &atom.dGdr[3 * ki];
                 dGdr[0] += p1x; dGdr[1] += p1y; dGdr[2] += p1z;
-                ki = sf.indexPerElement[g.e1];
+                ki = sf.indexPerElement[n.element];
                 dGdr = &n.dGdr[3 * ki];
                 dGdr[0] -= p1x; dGdr[1] -= p1y; dGdr[2] -= p1z;
             }
@@ -718,12 +735,15 @@ void calcRadial(Group const& g, vector<SF> const& sfs, Atom& atom,
     }
 }
 
-// SymGrpExpAngn.cpp:156-403 (narrow, type 3) and SymGrpExpAngw.cpp:156-376
-// (wide, type 9).
+// SymGrpExpAngn.cpp:156-403 (narrow, type 3), SymGrpExpAngw.cpp:156-376
+// (wide, type 9) and SymGrpExpAngnWeighted.cpp:166-385 (narrow over all
+// neighbour pairs, type 13: the cutoff products times Z_j Z_k, the shift rs
+// always applied).
 void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
                  bool derivatives)
 {
     bool const wide = (g.type == 9);
+    bool const weighted = isWeighted(g.type);
     size_t const nm = g.members.size();
     vector<double> result(nm, 0.0);
     double const rc = g.rc;
@@ -736,7 +756,7 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
         Neighbor& nj = atom.neighbors[j];
         int const nej = (int)nj.element;
         double const rij = nj.d;
-        if ((e1 == nej || e2 == nej) && rij < rc)
+        if ((weighted || e1 == nej || e2 == nej) && rij < rc)
         {
             double const r2ij = rij * rij;
             double pfcij, pdfcij;
@@ -746,7 +766,8 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
             {
                 Neighbor& nk = atom.neighbors[k];
                 int const nek = (int)nk.element;
-                if (!((e1 == nej && e2 == nek) || (e2 == nej && e1 == nek)))
+                if (!weighted
+                    && !((e1 == nej && e2 == nek) || (e2 == nej && e1 == nek)))
                     continue;
                 double const rik = nk.d;
                 if (!(rik < rc)) continue;
@@ -781,6 +802,14 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
                     pr1 = pfcik * pfcjk * pdfcij / rij;
                     pr2 = pfcij * pfcjk * pdfcik / rik;
                     pr3 = pfcij * pfcik * pdfcjk / rjk;
+                    if (weighted)
+                    {
+                        double const z = g.weight[nej] * g.weight[nek];
+                        pfc = z * pfcij * pfcik * pfcjk;
+                        pr1 = z * pfcik * pfcjk * pdfcij / rij;
+                        pr2 = z * pfcij * pfcjk * pdfcik / rik;
+                        pr3 = z * pfcij * pfcik * pdfcjk / rjk;
+                    }
                 }
                 else
                 {
@@ -793,9 +822,10 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
                 for (size_t l = 0; l < nm; ++l)
                 {
                     SF const& sf = sfs[g.members[l]];
+                    bool const shifted = sf.rs > 0.0 || weighted;
                     if (g.calculateExp[l])
                     {
-                        if (sf.rs > 0.0)
+                        if (shifted)
                         {
                             rijs = rij - sf.rs;
                             riks = rik - sf.rs;
@@ -830,7 +860,7 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
                     double p1, p2, p3;
                     if (!wide)
                     {
-                        if (sf.rs > 0.0)
+                        if (shifted)
                         {
                             p1 = fg * (pfczl * (rinvijik - costijk / r2ij
                                - p2etapl * rijs / rij) + pr1 * plambda);
@@ -926,7 +956,7 @@ void symmetryFunctions(orc_struct& s, orc_model const& m, bool derivatives)
         }
         for (auto const& g : e.groups)
         {
-            if (g.type == 2) calcRadial(g, e.sfs, a, derivatives);
+            if (isRadial(g.type)) calcRadial(g, e.sfs, a, derivatives);
             else calcAngular(g, e.sfs, a, derivatives);
         }
     }
@@ -1251,9 +1281,29 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
             sf.useIntegerPow = std::fabs(sf.zeta - sf.zetaInt)
                              <= std::numeric_limits<double>::min();
         }
+        else if (sf.type == 12)   // SymFncExpRadWeighted.cpp:71-90
+        {
+            sf.e1 = sf.e2 = -1;
+            sf.eta = std::atof(a.at(2).c_str());
+            sf.rs = std::atof(a.at(3).c_str());
+            sf.rc = std::atof(a.at(4).c_str());
+        }
+        else if (sf.type == 13)   // SymFncExpAngnWeighted.cpp:92-123
+        {
+            sf.e1 = sf.e2 = -1;
+            sf.eta = std::atof(a.at(2).c_str());
+            sf.rs = std::atof(a.at(3).c_str());
+            sf.lambda = std::atof(a.at(4).c_str());
+            sf.zeta = std::atof(a.at(5).c_str());
+            sf.rc = std::atof(a.at(6).c_str());
+            sf.zetaInt = (int)std::round(sf.zeta);
+            sf.useIntegerPow = std::fabs(sf.zeta - sf.zetaInt)
+                             <= std::numeric_limits<double>::min();
+        }
         else
             throw std::runtime_error("ERROR: Unknown symmetry function type"
-                                     " (this engine covers types 2, 3, 9).\n");
+                                     " (this engine covers types 2, 3, 9, 12,"
+                                     " 13).\n");
         sf.indexPerElement.assign(ne, std::numeric_limits<size_t>::max());
         m->el[sf.ec].sfs.push_back(sf);
     }
@@ -1319,6 +1369,9 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
                 g.cutoffType = sf.cutoffType;
                 g.cutoffAlpha = sf.cutoffAlpha;
                 g.fc.set(sf.cutoffType, sf.rc, sf.cutoffAlpha);
+                if (isWeighted(sf.type))
+                    for (auto const& x : m->el)
+                        g.weight.push_back((double)x.Z);
                 g.members.push_back((int)s);
                 e.groups.push_back(g);
             }
@@ -1332,7 +1385,7 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
             for (size_t i = 0; i < g.members.size(); ++i)
             {
                 SF const& sf = e.sfs[g.members[i]];
-                if (g.type != 2)
+                if (!isRadial(g.type))
                 {
                     g.factorNorm.push_back(std::pow(2.0, 1.0 - sf.zeta));
                     g.factorDeriv.push_back(2.0 * sf.eta / sf.zeta / sf.lambda);
@@ -1407,7 +1460,7 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
             {
                 double const sfac = e.sfs[g.members[i]].scalingFactor;
                 g.scalingFactors.push_back(sfac);
-                if (g.type != 2) g.factorNorm[i] *= sfac;
+                if (!isRadial(g.type)) g.factorNorm[i] *= sfac;
             }
         }
     // Mode.cpp:1158-1443 setupNeuralNetwork (2G, "short" suffix)
@@ -1941,7 +1994,7 @@ int orc_sf_statistics(orc_struct* s, orc_model* m, int e, double* out)
             sfSaved.push_back(g.scalingFactors);
             for (size_t i = 0; i < g.members.size(); ++i)
             {
-                if (g.type != 2) g.factorNorm[i] /= g.scalingFactors[i];
+                if (!isRadial(g.type)) g.factorNorm[i] /= g.scalingFactors[i];
                 g.scalingFactors[i] = 1.0;
             }
         }

@@ -61,6 +61,18 @@ def mixed_case(tmpdir, natoms=64, seed=3):
     return d, r, elem, box
 
 
+def weighted_case(tmpdir, natoms=64, seed=5):
+    """B/C/N structure with the element-weighted SF types 12 and 13 beside
+    types 2 and 3."""
+    d = os.path.join(str(tmpdir), "weighted_%d_%d" % (natoms, seed))
+    r, elem, box = W.random_structure(natoms, seed=seed)
+    if not os.path.exists(os.path.join(d, "scaling.data")):
+        W.write_model(d, W.weighted_settings(), W.MIXED_ELEMENTS, (5, 6, 7),
+                      (20, 20), seed=13)
+        scale_from_oracle(d, r, elem, box)
+    return d, r, elem, box
+
+
 def single_sf_model(tmpdir, name, elements, lines, cutoff_type=2,
                     cutoff_alpha=None, act="l l", all_sf=False):
     """Model with the given SF lines and a 1-neuron network; with the linear

@@ -14,8 +14,9 @@ Writes (all data, no code):
       f and df of all 9 cutoff types at r = 2.4, rc = 3.0, alpha = 0.2 from
       test/cpp/test_CutoffFunction.cpp (tested there at 100 eps);
   tests/golden/symfunc_fixtures.json
-      the three-atom and four-atom fixtures of test/cpp/fixtures.h with the
-      setup lines and values of the type 2, 3 and 9 SFs of atom 0.
+      the three-atom (one and two elements) and four-atom fixtures of
+      test/cpp/fixtures.h with the setup lines and values of the type 2, 3,
+      9, 12 and 13 SFs of atom 0.
 """
 import json
 import os
@@ -76,6 +77,8 @@ def fixtures():
     structs = {
         "Mono": {"elements": "H", "r": [[0, 0, 0], [1, 2, 1], [0, 3, -1]],
                  "elem": [0, 0, 0]},
+        "Dual": {"elements": "H He", "r": [[0, 0, 0], [1, 2, 1], [0, 3, -1]],
+                 "elem": [0, 1, 1]},
         "Mixed": {"elements": "S Cu U",
                   "r": [[0, 0, 0], [1, 2, 1], [0, 3, -1], [-2, 1, -0.5]],
                   "elem": [0, 0, 1, 2]},
@@ -84,7 +87,7 @@ def fixtures():
         types = [int(x) for x in re.findall(r"\d+", vec(txt, "vector<size_t> const types" + key))]
         lines = re.findall(r'"([^"]*)"', vec(txt, "vector<string> const setupLines" + key))
         vals = [float(x) for x in re.findall(NUM, vec(txt, "vector<double> const values" + key))]
-        keep = [(t, l, v) for t, l, v in zip(types, lines, vals) if t in (2, 3, 9)]
+        keep = [(t, l, v) for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 12, 13)]
         st = dict(st)
         st["sf"] = [{"type": t, "line": l, "value": v} for t, l, v in keep]
         out["sets"][key] = st

@@ -43,6 +43,19 @@ def test_host_model_setup_matches_oracle(name):
         assert np.array_equal(a, b)
 
 
+def test_host_model_setup_weighted_types(tmp_path):
+    """Types 12 and 13 (no neighbour elements, relevant to every element)
+    sort, group and index like the oracle's."""
+    d, _, _, _ = cases.weighted_case(tmp_path)
+    m = engine.Model.load_dir(d)
+    o = OModel.from_dir(d)
+    for e in range(m.num_elements):
+        a, b = m.sf_table(e), o.sf_table(e)
+        assert a.shape == b.shape == (41, 12)
+        assert np.array_equal(a, b)
+        assert set(a[:, 0].astype(int)) == {2, 3, 12, 13}
+
+
 def test_model_errors_are_reported(tmp_path):
     with pytest.raises(RuntimeError, match="Could not open file"):
         engine.Model.load(str(tmp_path / "missing.nn"), None, "w.%03zu.data")

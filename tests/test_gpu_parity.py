@@ -23,6 +23,7 @@ pytestmark = pytest.mark.gpu
 
 E_REL = 1e-10
 F_ABS = 1e-9
+EPS = np.finfo(float).eps
 
 
 def say(*a):
@@ -218,10 +219,14 @@ def test_symfunc_fixture_values_gpu(workdir):
             m, ctx = _engine(d, scaling=False)
             ctx.set_structure(np.array(st["r"], dtype=float), st["elem"])
             ctx.evaluate(0)
-            assert abs(ctx.G()[0, 0] - sf["value"]) <= 1e-13
+            # The reference test's bound, 1000 eps absolute (the weighted
+            # values reach 78 with uranium neighbours).
+            assert abs(ctx.G()[0, 0] - sf["value"]) <= 1000 * EPS
             s = OStruct(OModel.from_dir(d, scaling=False), st["r"], st["elem"])
             s.evaluate()
-            assert np.abs(ctx.forces() - s.forces()).max() < 1e-12
+            fo = s.forces()
+            assert np.abs(ctx.forces() - fo).max() < 1e-12 * max(
+                1.0, np.abs(fo).max())
 
 
 def test_self_image_modes_gpu(workdir):

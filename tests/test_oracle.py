@@ -8,8 +8,10 @@ What pins what:
   - Cutoff goldens of test/cpp/test_CutoffFunction.cpp (all 9 types, f and
     df at r = 2.4, rc = 3, alpha = 0.2) through a two-atom structure whose
     energy is the cutoff value itself.
-  - SF goldens of test/cpp/fixtures.h (types 2, 3, 9 incl. rs > 0, lambda
-    -1, zeta 4) on the three- and four-atom fixtures.
+  - SF goldens of test/cpp/fixtures.h (types 2, 3, 9, 12, 13 incl. rs > 0,
+    lambda -1, zeta 4) on the three-atom (H; H + He) and four-atom (S, Cu,
+    U) fixtures; the weighted types 12 and 13 carry the neighbours' atomic
+    numbers, so the two- and three-element fixtures pin the weights.
   - Derivatives: forces = -dE/dr against central differences (the nnp-checkf
     procedure) for every SF type and for a periodic water box.
   - Neighbour list: an independent numpy enumeration (set) and the cell-list
@@ -94,7 +96,7 @@ def test_symfunc_fixture_values(workdir):
         G = s.G()
         assert abs(G[0, 0] - sf["value"]) <= 1000 * EPS, (key, k, sf)
         n += 1
-    assert n == 11
+    assert n == 23
 
 
 def test_symfunc_derivatives_vs_finite_differences(workdir):
