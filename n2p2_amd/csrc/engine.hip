@@ -112,6 +112,8 @@ struct hdnnp_ctx
     DBuf<MemberDesc> dMember;
     DBuf<LamRun> dRun;
     DBuf<ScaleDesc> dScale;
+    DBuf<CompGroup> dCompGroup;
+    DBuf<CompMember> dCompMember;
     DBuf<int> dPairGroups, dPairCount;
     DBuf<double> dWeights;
     DBuf<float> dWeights32;
@@ -315,6 +317,8 @@ struct hdnnp_ctx
         up(dMember, flat.member);
         up(dRun, flat.run);
         up(dScale, flat.scale);
+        up(dCompGroup, flat.cgroup);
+        up(dCompMember, flat.cmember);
         up(dPairGroups, flat.pairGroups);
         up(dPairCount, flat.pairCount);
         up(dWeights, flat.weights);
@@ -347,6 +351,9 @@ struct hdnnp_ctx
         T.pairCount = dPairCount.p;
         T.weights = dWeights.p;
         T.weights32 = dWeights32.p;
+        T.cgroup = dCompGroup.p;
+        T.cmember = dCompMember.p;
+        T.ncgroup = (int)flat.cgroup.size();
         ld = std::max(1, T.maxnsf);
         maxMembers = 0;
         maxCfg = 1;
@@ -778,7 +785,8 @@ void computeSF(hdnnp_ctx* c, bool raw)
     c->setDevice();
     int const cap = std::max(c->maxNeigh, 1);
     int const fp32 = (!raw && c->precision != HDNNP_PRECISION_FP64) ? 1 : 0;
-    if (sf_forward_lds_bytes(cap, fp32) > 65536)
+    if (sf_forward_lds_bytes(cap, fp32) > 65536
+        || (c->T.ncgroup > 0 && sf_compact_lds_bytes(cap) > 65536))
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the symmetry-"
                     "function kernel's LDS staging.\n");
@@ -809,6 +817,7 @@ void computeSF(hdnnp_ctx* c, bool raw)
                  "symmetry functions (specialised)");
     }
     else hipCheck(launch_sf_forward(a, c->stream), "symmetry functions");
+    hipCheck(launch_sf_compact(a, c->stream), "compact symmetry functions");
     c->end(K_SYMFUNC, ev);
     c->haveSF = !raw;
     c->haveNN = c->haveNNDeriv = false;
@@ -892,7 +901,8 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
     int const cap = std::max(c->maxNeigh, 1);
     int const fp32 = c->precision != HDNNP_PRECISION_FP64 ? 1 : 0;
     size_t const shm = force_backward_lds_bytes(cap, fp32);
-    if (shm > 65536)
+    if (shm > 65536
+        || (c->T.ncgroup > 0 && force_compact_lds_bytes(cap) > 65536))
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the force "
                     "kernel's LDS staging.\n");
@@ -925,6 +935,7 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
                  "force backward (specialised)");
     }
     else hipCheck(launch_force_backward(a, c->stream), "force backward");
+    hipCheck(launch_force_compact(a, c->stream), "compact force backward");
     c->end(K_FORCE, ev);
     if (!c->externalNL && c->md.on)
     {
@@ -1277,6 +1288,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
     for (auto e : c->eventPool) (void)hipEventDestroy(e);
     c->dElem.release(); c->dCfg.release(); c->dGroup.release();
     c->dMember.release(); c->dRun.release(); c->dScale.release(); c->dPairGroups.release();
+    c->dCompGroup.release(); c->dCompMember.release();
     c->dPairCount.release(); c->dWeights.release();
     c->dWeights32.release();
     for (auto& b : c->dAtomsOfElem) b.release();
@@ -2083,6 +2095,10 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
             throw Error(HDNNP_E_INVALID, "ERROR: Training needs one structure "
                                          "from hdnnp_set_structure and its "
                                          "reference forces.\n");
+        if (c->T.ncgroup > 0)
+            throw Error(HDNNP_E_UNSUPPORTED,
+                        "ERROR: Training gradients do not cover the compact "
+                        "symmetry functions (types 20-25).\n");
         for (size_t e = 0; e < c->flat.elem.size(); ++e)
         {
             ElemDesc const& ed = c->flat.elem[e];

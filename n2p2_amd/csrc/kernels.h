@@ -187,6 +187,14 @@ hipError_t launch_sf_count(SFArgs const& a, unsigned long long* counters,
                            hipStream_t s);
 
 hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
+
+// Compact symmetry functions (types 20-25), FP64: values after
+// launch_sf_forward, per-slot force contributions added to cslot / cself
+// after launch_force_backward.
+hipError_t launch_sf_compact(SFArgs const& a, hipStream_t s);
+hipError_t launch_force_compact(ForceArgs const& a, hipStream_t s);
+size_t sf_compact_lds_bytes(int cap);
+size_t force_compact_lds_bytes(int cap);
 size_t force_backward_lds_bytes(int cap, int fp32);
 
 // Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted

@@ -72,6 +72,26 @@ bool lessSF(SymFn const& a, SymFn const& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
     if (a.type != b.type) return a.type < b.type;
+    if (sf_compact(a.type))
+    {
+        // SymFncCompRad.cpp / SymFncCompAngn.cpp / SymFncCompAngw.cpp and
+        // the weighted variants: subtype, [e1, e2], rc, rl, [angles].
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (!sf_weighted(a.type))
+        {
+            if (a.e1 != b.e1) return a.e1 < b.e1;
+            if (a.type != 20 && a.e2 != b.e2) return a.e2 < b.e2;
+        }
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (!sf_radial(a.type))
+        {
+            if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+            if (a.angleRight != b.angleRight)
+                return a.angleRight < b.angleRight;
+        }
+        return false;
+    }
     if (a.cutoffType != b.cutoffType) return a.cutoffType < b.cutoffType;
     if (a.cutoffAlpha != b.cutoffAlpha) return a.cutoffAlpha < b.cutoffAlpha;
     if (a.rc != b.rc) return a.rc < b.rc;
@@ -92,6 +112,9 @@ bool sameGroup(SymFn const& a, SymFn const& b)
     // SymGrpExpRad.cpp:72-109 / SymGrpExpAngn.cpp:68-105 addMember checks
     // (the weighted groups, SymGrpExpRadWeighted.cpp:68-99, have no
     // neighbour elements to compare).
+    if (sf_compact(a.type))   // SymGrpCompRad.cpp:68-100 and alike
+        return a.type == b.type && a.ec == b.ec && a.e1 == b.e1
+            && a.e2 == b.e2;
     return a.type == b.type && a.cutoffType == b.cutoffType
         && a.cutoffAlpha == b.cutoffAlpha && a.ec == b.ec && a.rc == b.rc
         && a.e1 == b.e1 && (a.type == 2 || a.e2 == b.e2);
@@ -288,10 +311,73 @@ void Model::load(std::string const& settingsFile,
                 throw std::runtime_error("ERROR: Type 13 symmetry functions "
                                          "with rs < 0 are not supported.\n");
         }
+        else if (sf_compact(s.type))
+        {
+            // SymFncCompRad.cpp / SymFncBaseCompAng.cpp and the weighted
+            // variants' setParameters; SymFncBaseComp.cpp:48-100.
+            size_t p = 2;
+            s.e1 = s.e2 = -1;
+            if (!sf_weighted(s.type))
+            {
+                s.e1 = eidx(a.at(p++));
+                if (s.type != 20) s.e2 = eidx(a.at(p++));
+                if (s.type != 20 && s.e1 > s.e2) std::swap(s.e1, s.e2);
+            }
+            s.rl = std::atof(a.at(p++).c_str());
+            s.rc = std::atof(a.at(p++).c_str());
+            if (!sf_radial(s.type))
+            {
+                s.angleLeft = std::atof(a.at(p++).c_str());
+                s.angleRight = std::atof(a.at(p++).c_str());
+            }
+            s.subtype = a.at(p++);
+            if (s.rl > s.rc)
+                throw std::runtime_error("ERROR: Lower radial boundary >= "
+                                         "upper radial boundary.\n");
+            std::string const& st = s.subtype;
+            if (st.size() < 1 || st.size() > 3)
+                throw std::runtime_error("ERROR: Invalid compact function "
+                                         "type specification: \"" + st
+                                         + "\".\n");
+            if (st[0] == 'p' && st.size() >= 2 && st[1] >= '1' && st[1] <= '4')
+            {
+                s.core = CT_POLY1 + (st[1] - '1');
+                if (st.size() == 3)
+                {
+                    if (st[2] != 'a')
+                        throw std::runtime_error("ERROR: Invalid polynom "
+                                                 "specifier: \"" + st
+                                                 + "\".\n");
+                    s.asym = true;
+                }
+            }
+            else if (st[0] == 'e') s.core = CT_EXP;
+            else
+                throw std::runtime_error("ERROR: Unknown compact SF type: \""
+                                         + st + "\".\n");
+            if (!sf_radial(s.type))
+            {
+                double const al = s.angleLeft, ar = s.angleRight;
+                double const center = (al + ar) / 2.0;
+                if (al >= ar)
+                    throw std::runtime_error("ERROR: Left angle boundary right"
+                                             " of or equal to right angle "
+                                             "boundary.\n");
+                if ((al < 0.0 && center != 0.0)
+                    || (ar > 180.0 && center != 180.0))
+                    throw std::runtime_error("ERROR: Angle boundary out of "
+                                             "[0,180] and center of angular "
+                                             "function /= 0 or /= 180.\n");
+                if (ar - al > 360.0)
+                    throw std::runtime_error("ERROR: Periodic symmetry "
+                                             "function cannot spread over "
+                                             "domain > 360 degrees\n");
+            }
+        }
         else
             throw std::runtime_error(
                 "ERROR: Symmetry function type " + a.at(1)
-                + " is outside this engine (types 2, 3, 9, 12, 13 are "
+                + " is outside this engine (types 2, 3, 9, 12, 13, 20-25 are "
                 "supported).\n");
         s.indexPerElement.assign(ne, -1);
         el[s.ec].sf.push_back(s);
@@ -306,6 +392,7 @@ void Model::load(std::string const& settingsFile,
                 s.eta /= convLength * convLength;
                 s.rs *= convLength;
                 s.rc *= convLength;
+                s.rl *= convLength;   // SymFncCompRad::changeLengthUnit
             }
         std::sort(e.sf.begin(), e.sf.end(), lessSF);   // Element.cpp:154-166
         for (size_t i = 0; i < e.sf.size(); ++i)
@@ -323,7 +410,7 @@ void Model::load(std::string const& settingsFile,
         for (auto& s : e.sf)
             for (int i = 0; i < ne; ++i)
                 if (sf_weighted(s.type) || i == s.e1
-                    || (s.type != 2 && i == s.e2))
+                    || (s.type != 2 && s.type != 20 && i == s.e2))
                 {
                     s.indexPerElement[i] = (int)e.table[i].size();
                     e.table[i].push_back(s.index);
@@ -598,7 +685,8 @@ Model::Flat Model::flatten() const
         d.cfg0 = (int)F.cfg.size();
         std::vector<double> rcs;
         for (auto const& g : e.groups)
-            if (std::find(rcs.begin(), rcs.end(), g.rc) == rcs.end())
+            if (!sf_compact(g.type)
+                && std::find(rcs.begin(), rcs.end(), g.rc) == rcs.end())
                 rcs.push_back(g.rc);
         if ((int)rcs.size() > kMaxCutCfg)
             throw std::runtime_error("ERROR: More distinct cutoff radii per "
@@ -621,6 +709,7 @@ Model::Flat Model::flatten() const
         for (int pass = 0; pass < 2; ++pass)
             for (auto const& g : e.groups)
             {
+                if (sf_compact(g.type)) continue;   // own tables below
                 if ((pass == 0) != sf_radial(g.type)) continue;
                 // Members in SF order (SymGrp*::sortMembers), then, for the
                 // angular types, stably ordered by (eta, rs, lambda, zeta):
@@ -753,6 +842,49 @@ Model::Flat Model::flatten() const
                 }
             }
         d.ng = (int)F.group.size() - d.g0;
+        // Compact groups: members in SF order (SymGrpComp*::sortMembers).
+        d.cg0 = (int)F.cgroup.size();
+        for (auto const& g : e.groups)
+        {
+            if (!sf_compact(g.type)) continue;
+            if ((int)g.members.size() > kMaxCompMembers)
+                throw std::runtime_error("ERROR: More compact symmetry "
+                                         "functions in one group than the "
+                                         "engine supports (32).\n");
+            std::vector<int> mem = g.members;
+            std::sort(mem.begin(), mem.end());
+            CompGroup cg{};
+            cg.type = g.type;
+            cg.e1 = g.e1;
+            cg.e2 = sf_radial(g.type) ? -1 : g.e2;
+            cg.nmem = (int)mem.size();
+            cg.mem0 = (int)F.cmember.size();
+            cg.rmax = 0.0;
+            for (size_t q = 0; q < mem.size(); ++q)
+            {
+                SymFn const& s = e.sf[mem[q]];
+                CompMember cm{};
+                cm.rl = s.rl;
+                cm.rc = s.rc;
+                cm.rcenter = (s.rc + s.rl) / 2.0;       // setLeftRight
+                cm.rscale = 1.0 / ((s.rc - s.rl) / 2.0);
+                // angleLeft * M_PI / 180.0 as SymFncBaseCompAng.cpp:106-108
+                cm.al = s.angleLeft * M_PI / 180.0;
+                cm.ar = s.angleRight * M_PI / 180.0;
+                cm.acenter = (cm.ar + cm.al) / 2.0;
+                cm.ascale = 1.0 / ((cm.ar - cm.al) / 2.0);
+                cm.sfac = s.scalingFactor;
+                cm.core = s.core;
+                cm.asym = s.asym ? 1 : 0;
+                cm.sf = s.index;
+                cm.newrad = (q == 0 || e.sf[mem[q - 1]].rc != s.rc
+                             || e.sf[mem[q - 1]].rl != s.rl) ? 1 : 0;
+                cg.rmax = std::max(cg.rmax, s.rc);
+                F.cmember.push_back(cm);
+            }
+            F.cgroup.push_back(cg);
+        }
+        d.ncg = (int)F.cgroup.size() - d.cg0;
         // Angular chunks per neighbour-element pair (both orders).
         for (int gi = d.g0 + d.nrad; gi < d.g0 + d.ng; ++gi)
         {

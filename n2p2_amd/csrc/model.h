@@ -25,6 +25,12 @@ struct SymFn
     size_t line = 0;
     int    index = 0;                   // position after sorting
     std::vector<int> indexPerElement;   // Element.cpp:289-311
+    // compact types (20-25): radial interval, angle interval in degrees,
+    // subtype string, core function and asymmetry
+    double rl = 0.0, angleLeft = 0.0, angleRight = 0.0;
+    std::string subtype;
+    int    core = CT_POLY2;
+    bool   asym = false;
     int    scalingType = ST_NONE;
     double gmin = 0, gmax = 0, gmean = 0, gsigma = 0, smin = 0, smax = 0;
     double scalingFactor = 1.0;
@@ -88,6 +94,8 @@ public:
         std::vector<MemberDesc> member;
         std::vector<LamRun>     run;
         std::vector<ScaleDesc>  scale;
+        std::vector<CompGroup>  cgroup;
+        std::vector<CompMember> cmember;
         std::vector<int>        pairGroups, pairCount;
         std::vector<double>     weights;
     };

@@ -43,8 +43,22 @@ enum ActType { AF_UNSET = 0, AF_IDENTITY, AF_TANH, AF_LOGISTIC, AF_SOFTPLUS,
 // Symmetry-function types: 2 and 12 are radial, 3, 9 and 13 angular; 12
 // and 13 are the element-weighted variants (every neighbour counts, times
 // its atomic number; SymFncExpRadWeighted, SymFncExpAngnWeighted).
-constexpr bool sf_radial(int type) { return type == 2 || type == 12; }
-constexpr bool sf_weighted(int type) { return type == 12 || type == 13; }
+// Types 20-25 are the compact (polynomial) families, evaluated by their own
+// kernels: 20 radial, 21 narrow and 22 wide angular, 23-25 the weighted
+// variants.
+constexpr bool sf_compact(int type) { return type >= 20 && type <= 25; }
+constexpr bool sf_radial(int type)
+{
+    return type == 2 || type == 12 || type == 20 || type == 23;
+}
+constexpr bool sf_weighted(int type)
+{
+    return type == 12 || type == 13 || (type >= 23 && type <= 25);
+}
+
+// Members per compact group (SymGrpComp*: all compact SFs of one type and
+// centre / neighbour elements).
+constexpr int kMaxCompMembers = 32;
 
 struct CutCfg
 {
@@ -102,6 +116,31 @@ struct MemberDesc
     int    level;     // zeta = 2^level (integer power), else -1
 };
 
+// A compact group's member: open radial interval (rl, rc) with its compact
+// function (x - center) * scale, and for the angular types the open angle
+// interval (al, ar) in radians with its compact function
+// (CompactFunction.cpp:57-72, SymFncBaseCompAng.cpp:57-110).
+struct CompMember
+{
+    double rl, rc, rcenter, rscale;
+    double al, ar, acenter, ascale;
+    double sfac;      // scalingFactor
+    int    core;      // CT_POLY1..CT_POLY4 or CT_EXP (CoreFunction type)
+    int    asym;      // asymmetric radial core (subtype "p?a")
+    int    sf;        // SF index within the centre element
+    int    newrad;    // (rl, rc) differ from the previous member's
+                      // (calculateComp of SymGrpCompAngn.cpp:98-112)
+};
+
+struct CompGroup
+{
+    int    type;      // 20-25
+    int    e1, e2;    // neighbour elements (-1 where not used)
+    int    nmem, mem0;
+    int    pad;
+    double rmax;      // largest member rc
+};
+
 struct ScaleDesc
 {
     int    type;
@@ -116,6 +155,7 @@ struct ElemDesc
     int nrad;            // of which radial chunks: [g0, g0+nrad)
     int cfg0, ncfg;      // cutoff configurations
     int sf0;             // first ScaleDesc of this element
+    int cg0, ncg;        // compact groups [cg0, cg0+ncg)
     int nsfd[kMaxElements];   // SF derivatives per neighbour element
                               // (Element::getSymmetryFunctionNumTable)
     // network
@@ -149,6 +189,9 @@ struct ModelTables
     const int*        pairCount;    // [pb]
     const double*     weights;
     const float*      weights32;    // the same weights rounded to FP32
+    const CompGroup*  cgroup;       // compact groups (may be empty)
+    const CompMember* cmember;
+    int               ncgroup;      // compact groups over all elements
 };
 
 }  // namespace hdnnp

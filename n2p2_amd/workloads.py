@@ -162,6 +162,48 @@ def weighted_settings(rc=6.0, cutoff_type=6):
     return "\n".join(lines) + "\n"
 
 
+def compact_settings(rc=6.0):
+    """B/C/N settings with the compact (polynomial) families 20-25 beside a
+    few G2: every core (p1-p4, asymmetric p2a, e), negative rl, angle
+    intervals centred on 0 and 180 degrees, and within each angular group
+    two members with equal (rl, rc) and different cores next to each other
+    in sort order."""
+    el = MIXED_ELEMENTS
+    lines = ["number_of_elements 3", "elements B C N",
+             "cutoff_type 6 0.0",
+             "scale_symmetry_functions", "scale_min_short 0.0",
+             "scale_max_short 1.0",
+             "global_hidden_layers_short 2", "global_nodes_short 20 20",
+             "global_activation_short t t l"]
+    rad = ((-5.0, 5.0, "p2"), (0.0, 6.0, "p3"), (1.5, 6.0, "p2a"),
+           (0.5, 5.5, "e"))
+    ang = ((0.0, 4.5, 0.0, 180.0, "p2"), (0.0, 4.5, 30.0, 120.0, "p3"),
+           (1.0, 6.0, 60.0, 180.0, "p3"), (-6.0, 6.0, -90.0, 90.0, "p1"),
+           (0.0, 5.0, 90.0, 270.0, "p4"))
+    for e in el:
+        for e1 in el:
+            lines.append("symfunction_short %s 2 %s 0.1 0.0 %.4f" % (e, e1, rc))
+            for rl, rr, st in rad:
+                lines.append("symfunction_short %s 20 %s %.2f %.2f %s"
+                             % (e, e1, rl, rr, st))
+        for e1, e2 in ((el[0], el[1]), (el[1], el[2]), (el[2], el[2])):
+            for t in (21, 22):
+                for rl, rr, al, ar, st in ang:
+                    lines.append("symfunction_short %s %d %s %s %.2f %.2f "
+                                 "%.1f %.1f %s" % (e, t, e1, e2, rl, rr, al,
+                                                   ar, st))
+        for rl, rr, st in ((-5.0, 5.0, "p2"), (0.0, 6.0, "e"), (1.0, 6.0, "p4")):
+            lines.append("symfunction_short %s 23 %.2f %.2f %s"
+                         % (e, rl, rr, st))
+        for t in (24, 25):
+            for rl, rr, al, ar, st in ((0.0, 4.5, 0.0, 180.0, "p2"),
+                                       (0.5, 6.0, 45.0, 135.0, "p3"),
+                                       (-6.0, 6.0, -60.0, 60.0, "p2a")):
+                lines.append("symfunction_short %s %d %.2f %.2f %.1f %.1f %s"
+                             % (e, t, rl, rr, al, ar, st))
+    return "\n".join(lines) + "\n"
+
+
 def count_sf(settings_text, symbol):
     return sum(1 for l in settings_text.splitlines()
                if l.startswith("symfunction_short %s " % symbol))

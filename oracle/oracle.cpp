@@ -207,8 +207,49 @@ enum ScalingType { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER,
 // Types 2 and 12 are radial; 12 and 13 are the element-weighted variants
 // (SymFncExpRadWeighted, SymFncExpAngnWeighted): every neighbour counts,
 // weighted by its atomic number.
-inline bool isRadial(int type) { return type == 2 || type == 12; }
-inline bool isWeighted(int type) { return type == 12 || type == 13; }
+// Types 20-25 are the compact (polynomial) families: 20 radial, 21 narrow
+// and 22 wide angular, 23-25 their weighted variants (SymFncCompRad,
+// SymFncCompAngn, SymFncCompAngw, SymFncComp*Weighted).
+inline bool isCompact(int type) { return type >= 20 && type <= 25; }
+inline bool isRadial(int type)
+{
+    return type == 2 || type == 12 || type == 20 || type == 23;
+}
+inline bool isWeighted(int type)
+{
+    return type == 12 || type == 13 || (type >= 23 && type <= 25);
+}
+
+// CompactFunction (CompactFunction.h:139-160, CompactFunction.cpp:42-72)
+// with its CoreFunction (CoreFunction.h:193-318): a = (x - center) / width,
+// f = core(|a|), f' = core'(|a|) sign(a) / width; the asymmetric core maps
+// its argument to (2 - y) y first (CoreFunction.h:178-191).
+struct Compact
+{
+    double center = 0.0, width = 1.0, scale = 1.0;
+    int    core = CT_POLY2;
+    bool   asym = false;
+    void setLeftRight(double left, double right)
+    {
+        center = (right + left) / 2.0;
+        width = (right - left) / 2.0;
+        scale = 1.0 / width;
+    }
+    void fdf(double x, double& f, double& df) const;
+};
+
+void Compact::fdf(double x, double& f, double& df) const
+{
+    double const a = (x - center) * scale;
+    double y = std::fabs(a);
+    if (asym)
+    {
+        Cutoff::core(core, (2.0 - y) * y, f, df);
+        df *= 2.0 - 2.0 * y;
+    }
+    else Cutoff::core(core, y, f, df);
+    df *= std::copysign(scale, a);
+}
 
 struct SF
 {
@@ -222,6 +263,11 @@ struct SF
     int    index = 0;
     size_t line = 0;
     vector<size_t> indexPerElement;
+    // compact types: radial interval (rl, rc), angle interval in degrees,
+    // subtype string and the radial / angular compact functions
+    double rl = 0.0, angleLeft = 0.0, angleRight = 0.0;
+    string subtype;
+    Compact cr, ca;
     // SymFnc.cpp:52-98 scaling
     int    scalingType = ST_NONE;
     double Gmin = 0, Gmax = 0, Gmean = 0, Gsigma = 0, Smin = 0, Smax = 0;
@@ -232,7 +278,7 @@ struct SF
     bool relevant(size_t e) const
     {
         if (isWeighted(type)) return true;
-        if (type == 2) return (int)e == e1;
+        if (type == 2 || type == 20) return (int)e == e1;
         return (int)e == e1 || (int)e == e2;
     }
 
@@ -272,6 +318,26 @@ bool sfLess(SF const& a, SF const& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
     if (a.type != b.type) return a.type < b.type;
+    if (isCompact(a.type))
+    {
+        // SymFncCompRad.cpp / SymFncCompAngn.cpp / SymFncCompAngw.cpp and
+        // the weighted variants: subtype, [e1, e2], rc, rl, [angles].
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (!isWeighted(a.type))
+        {
+            if (a.e1 != b.e1) return a.e1 < b.e1;
+            if (a.type != 20 && a.e2 != b.e2) return a.e2 < b.e2;
+        }
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (!isRadial(a.type))
+        {
+            if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+            if (a.angleRight != b.angleRight)
+                return a.angleRight < b.angleRight;
+        }
+        return false;
+    }
     if (a.cutoffType != b.cutoffType) return a.cutoffType < b.cutoffType;
     if (a.cutoffAlpha != b.cutoffAlpha) return a.cutoffAlpha < b.cutoffAlpha;
     if (a.rc != b.rc) return a.rc < b.rc;
@@ -301,6 +367,8 @@ struct Group
     vector<double> factorNorm, factorDeriv, scalingFactors;
     vector<double> weight;           // weighted types: atomic number per
                                      // neighbour element
+    double rmin = 0.0, rmax = 0.0;   // compact types: radial range of the
+                                     // members (SymGrpCompRad.cpp:82-84)
     vector<char> calculateExp;
 };
 
@@ -930,6 +998,205 @@ void calcAngular(Group const& g, vector<SF> const& sfs, Atom& atom,
     }
 }
 
+// SymGrpCompRad.cpp:128-214 (type 20) and SymGrpCompRadWeighted.cpp
+// (type 23: every neighbour, times its atomic number).
+void calcCompRad(Group const& g, vector<SF> const& sfs, Atom& atom,
+                 bool derivatives)
+{
+    size_t const nm = g.members.size();
+    bool const weighted = isWeighted(g.type);
+    vector<double> result(nm, 0.0);
+    for (size_t j = 0; j < atom.neighbors.size(); ++j)
+    {
+        Neighbor& n = atom.neighbors[j];
+        double const rij = n.d;
+        if ((weighted || g.e1 == (int)n.element) && rij < g.rmax
+            && rij > g.rmin)
+        {
+            double const* const d1 = n.dr;
+            double const z = weighted ? g.weight[n.element] : 1.0;
+            for (size_t k = 0; k < nm; ++k)
+            {
+                SF const& sf = sfs[g.members[k]];
+                if (rij <= sf.rl || rij >= sf.rc) continue;
+                double rad, drad;
+                sf.cr.fdf(rij, rad, drad);
+                result[k] += weighted ? z * rad : rad;
+                if (!derivatives || drad == 0.0) continue;
+                drad *= weighted ? z * g.scalingFactors[k] / rij
+                                 : g.scalingFactors[k] / rij;
+                double const px = drad * d1[0], py = drad * d1[1],
+                             pz = drad * d1[2];
+                double* dGdr = &atom.dGdr[3 * (size_t)sf.index];
+                dGdr[0] += px; dGdr[1] += py; dGdr[2] += pz;
+                dGdr = &n.dGdr[3 * sf.indexPerElement[n.element]];
+                dGdr[0] -= px; dGdr[1] -= py; dGdr[2] -= pz;
+            }
+        }
+    }
+    for (size_t k = 0; k < nm; ++k)
+    {
+        SF const& sf = sfs[g.members[k]];
+        atom.G[sf.index] = sf.scale(result[k]);
+    }
+}
+
+// SymGrpCompAngn.cpp:148-388 (type 21), SymGrpCompAngw.cpp (type 22, no
+// r_jk factor or test) and the weighted SymGrpCompAngnWeighted.cpp /
+// SymGrpCompAngwWeighted.cpp (types 24, 25: every pair, the angular part
+// times Z_j Z_k).  The radial value at r_jk is recomputed only at a member
+// whose (rl, rc) differ from the previous member's or after a skipped
+// member, as the reference's group loop does (calculateComp).
+void calcCompAng(Group const& g, vector<SF> const& sfs, Atom& atom,
+                 bool derivatives)
+{
+    size_t const nm = g.members.size();
+    bool const weighted = isWeighted(g.type);
+    bool const wide = (g.type == 22 || g.type == 25);
+    vector<double> result(nm, 0.0), radij(nm, 0.0), dradij(nm, 0.0);
+    double const r2min = g.rmin * g.rmin, r2max = g.rmax * g.rmax;
+    double const PI = 4.0 * std::atan(1.0);
+    size_t numNeighbors = atom.neighbors.size();
+    if (numNeighbors == 0) numNeighbors = 1;
+    for (size_t j = 0; j < numNeighbors - 1; j++)
+    {
+        Neighbor& nj = atom.neighbors[j];
+        int const nej = (int)nj.element;
+        double const rij = nj.d;
+        if (!((weighted || g.e1 == nej || g.e2 == nej) && rij < g.rmax
+              && rij > g.rmin))
+            continue;
+        for (size_t l = 0; l < nm; ++l)
+        {
+            SF const& sf = sfs[g.members[l]];
+            if (rij > sf.rl && rij < sf.rc)
+                sf.cr.fdf(rij, radij[l], dradij[l]);
+            else radij[l] = 0.0;
+        }
+        double const* const dr1 = nj.dr;
+        for (size_t k = j + 1; k < numNeighbors; k++)
+        {
+            Neighbor& nk = atom.neighbors[k];
+            int const nek = (int)nk.element;
+            if (!weighted
+                && !((g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek)))
+                continue;
+            double const rik = nk.d;
+            if (!(rik < g.rmax && rik > g.rmin)) continue;
+            double const* const dr2 = nk.dr;
+            double const dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1],
+                                   dr2[2] - dr1[2]};
+            double rjk = 0.0, rinvjk = 0.0;
+            if (!wide)
+            {
+                rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
+                if (rjk >= r2max || rjk <= r2min) continue;
+                rjk = std::sqrt(rjk);
+                rinvjk = 1.0 / rjk;
+            }
+            double const rinvijik = 1.0 / (rij * rik);
+            double const costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1]
+                                  + dr1[2] * dr2[2]) * rinvijik;
+            if (costijk <= -1.0 || costijk >= 1.0) continue;
+            double const acostijk = std::acos(costijk);
+            double const rinvij = rinvijik * rik;
+            double const rinvik = rinvijik * rij;
+            double const phiijik0 = rinvij * (rinvik - rinvij * costijk);
+            double const phiikij0 = rinvik * (rinvij - rinvik * costijk);
+            double dacostijk = 0.0;
+            if (derivatives) dacostijk = -1.0 / std::sqrt(1.0 - costijk * costijk);
+            double const z = weighted ? g.weight[nej] * g.weight[nek] : 1.0;
+            bool skip = false;
+            double radjk = 1.0, dradjk = 0.0;
+            for (size_t l = 0; l < nm; ++l)
+            {
+                SF const& sf = sfs[g.members[l]];
+                double const mal = sf.angleLeft * PI / 180.0;
+                double const mar = sf.angleRight * PI / 180.0;
+                if (radij[l] == 0.0 || rik <= sf.rl || rik >= sf.rc
+                    || (!wide && (rjk <= sf.rl || rjk >= sf.rc))
+                    || acostijk <= mal || acostijk >= mar)
+                {
+                    skip = true;
+                    continue;
+                }
+                double radik, dradik;
+                sf.cr.fdf(rik, radik, dradik);
+                if (!wide && (g.calculateExp[l] || skip))
+                {
+                    sf.cr.fdf(rjk, radjk, dradjk);
+                    skip = false;
+                }
+                double ang, dang;
+                sf.ca.fdf(acostijk, ang, dang);
+                if (weighted) ang *= z;
+                double rad = wide ? radij[l] * radik
+                                  : radij[l] * radik * radjk;
+                result[l] += rad * ang;
+                if (!derivatives) continue;
+                dang *= dacostijk;
+                double const phiijik = phiijik0 * dang;
+                double const phiikij = phiikij0 * dang;
+                double const psiijik = rinvijik * dang;
+                double chiij, chiik, chijk;
+                if (!wide)
+                {
+                    chiij = rinvij * dradij[l] * radik * radjk;
+                    chiik = rinvik * radij[l] * dradik * radjk;
+                    chijk = -rinvjk * radij[l] * radik * dradjk;
+                }
+                else
+                {
+                    chiij = rinvij * radik * dradij[l];
+                    chiik = rinvik * radij[l] * dradik;
+                    chijk = 0.0;
+                }
+                double const sfac = g.scalingFactors[l];
+                double p1, p2, p3;
+                if (dang != 0.0 && rad != 0.0)
+                {
+                    rad *= weighted ? sfac * z : sfac;
+                    ang *= sfac;
+                    p1 = rad * phiijik + ang * chiij;
+                    p2 = rad * phiikij + ang * chiik;
+                    p3 = rad * psiijik + ang * chijk;
+                }
+                else if (ang != 0.0)
+                {
+                    ang *= sfac;
+                    p1 = ang * chiij;
+                    p2 = ang * chiik;
+                    p3 = ang * chijk;
+                }
+                else continue;
+                double const p1x = p1 * dr1[0], p1y = p1 * dr1[1],
+                             p1z = p1 * dr1[2];
+                double const p2x = p2 * dr2[0], p2y = p2 * dr2[1],
+                             p2z = p2 * dr2[2];
+                double const p3x = p3 * dr3[0], p3y = p3 * dr3[1],
+                             p3z = p3 * dr3[2];
+                double* dGdr = &atom.dGdr[3 * (size_t)sf.index];
+                dGdr[0] += p1x + p2x;
+                dGdr[1] += p1y + p2y;
+                dGdr[2] += p1z + p2z;
+                dGdr = &nj.dGdr[3 * sf.indexPerElement[nej]];
+                dGdr[0] -= p1x + p3x;
+                dGdr[1] -= p1y + p3y;
+                dGdr[2] -= p1z + p3z;
+                dGdr = &nk.dGdr[3 * sf.indexPerElement[nek]];
+                dGdr[0] -= p2x - p3x;
+                dGdr[1] -= p2y - p3y;
+                dGdr[2] -= p2z - p3z;
+            }
+        }
+    }
+    for (size_t l = 0; l < nm; ++l)
+    {
+        SF const& sf = sfs[g.members[l]];
+        atom.G[sf.index] = sf.scale(result[l]);
+    }
+}
+
 // Mode.cpp:1561-1640 (no SF cache; the cache only avoids recomputing the
 // same cutoff values) with Atom::allocate (Atom.cpp:168-240).
 void symmetryFunctions(orc_struct& s, orc_model const& m, bool derivatives)
@@ -956,7 +1223,11 @@ void symmetryFunctions(orc_struct& s, orc_model const& m, bool derivatives)
         }
         for (auto const& g : e.groups)
         {
-            if (isRadial(g.type)) calcRadial(g, e.sfs, a, derivatives);
+            if (g.type == 20 || g.type == 23)
+                calcCompRad(g, e.sfs, a, derivatives);
+            else if (isCompact(g.type))
+                calcCompAng(g, e.sfs, a, derivatives);
+            else if (isRadial(g.type)) calcRadial(g, e.sfs, a, derivatives);
             else calcAngular(g, e.sfs, a, derivatives);
         }
     }
@@ -1300,10 +1571,74 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
             sf.useIntegerPow = std::fabs(sf.zeta - sf.zetaInt)
                              <= std::numeric_limits<double>::min();
         }
+        else if (isCompact(sf.type))
+        {
+            // SymFncCompRad.cpp / SymFncBaseCompAng.cpp /
+            // SymFncCompRadWeighted.cpp / SymFncBaseCompAngWeighted.cpp
+            // setParameters, SymFncBaseComp.cpp:48-100 setCompactFunction.
+            size_t p = 2;
+            sf.e1 = sf.e2 = -1;
+            if (!isWeighted(sf.type))
+            {
+                sf.e1 = (int)eidx(a.at(p++));
+                if (sf.type != 20) sf.e2 = (int)eidx(a.at(p++));
+                if (sf.type != 20 && sf.e1 > sf.e2) std::swap(sf.e1, sf.e2);
+            }
+            sf.rl = std::atof(a.at(p++).c_str());
+            sf.rc = std::atof(a.at(p++).c_str());
+            if (!isRadial(sf.type))
+            {
+                sf.angleLeft = std::atof(a.at(p++).c_str());
+                sf.angleRight = std::atof(a.at(p++).c_str());
+            }
+            sf.subtype = a.at(p++);
+            if (sf.rl > sf.rc)
+                throw std::runtime_error("ERROR: Lower radial boundary >= "
+                                         "upper radial boundary.\n");
+            string const& st = sf.subtype;
+            if (st.size() < 1 || st.size() > 3)
+                throw std::runtime_error("ERROR: Invalid compact function "
+                                         "type specification.\n");
+            if (st[0] == 'p' && st.size() >= 2 && st[1] >= '1' && st[1] <= '4')
+            {
+                sf.cr.core = CT_POLY1 + (st[1] - '1');
+                if (st.size() == 3)
+                {
+                    if (st[2] != 'a')
+                        throw std::runtime_error("ERROR: Invalid polynom "
+                                                 "specifier.\n");
+                    sf.cr.asym = true;
+                }
+            }
+            else if (st[0] == 'e') sf.cr.core = CT_EXP;
+            else
+                throw std::runtime_error("ERROR: Unknown compact SF type.\n");
+            sf.cr.setLeftRight(sf.rl, sf.rc);
+            if (!isRadial(sf.type))
+            {
+                double const al = sf.angleLeft, ar = sf.angleRight;
+                double const center = (al + ar) / 2.0;
+                if (al >= ar)
+                    throw std::runtime_error("ERROR: Left angle boundary right"
+                                             " of or equal to right angle "
+                                             "boundary.\n");
+                if ((al < 0.0 && center != 0.0)
+                    || (ar > 180.0 && center != 180.0))
+                    throw std::runtime_error("ERROR: Angle boundary out of "
+                                             "[0,180] and center of angular "
+                                             "function /= 0 or /= 180.\n");
+                if (ar - al > 360.0)
+                    throw std::runtime_error("ERROR: Periodic symmetry "
+                                             "function cannot spread over "
+                                             "domain > 360 degrees\n");
+                sf.ca.core = sf.cr.core;
+                sf.ca.setLeftRight(al * M_PI / 180.0, ar * M_PI / 180.0);
+            }
+        }
         else
             throw std::runtime_error("ERROR: Unknown symmetry function type"
                                      " (this engine covers types 2, 3, 9, 12,"
-                                     " 13).\n");
+                                     " 13, 20-25).\n");
         sf.indexPerElement.assign(ne, std::numeric_limits<size_t>::max());
         m->el[sf.ec].sfs.push_back(sf);
     }
@@ -1316,6 +1651,8 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
                 sf.eta /= m->convLength * m->convLength;
                 sf.rs *= m->convLength;
                 sf.rc *= m->convLength;
+                sf.rl *= m->convLength;   // SymFncCompRad::changeLengthUnit
+                if (isCompact(sf.type)) sf.cr.setLeftRight(sf.rl, sf.rc);
             }
         std::sort(e.sfs.begin(), e.sfs.end(), sfLess);   // Element.cpp:154-166
         for (size_t i = 0; i < e.sfs.size(); ++i) e.sfs[i].index = (int)i;
@@ -1350,6 +1687,17 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
             for (auto& g : e.groups)
             {
                 SF const& f0 = e.sfs[g.members[0]];
+                if (isCompact(sf.type))
+                {
+                    // SymGrpCompRad.cpp:68-100: centre and neighbour
+                    // elements only (none for the weighted types).
+                    if (sf.type != f0.type || sf.ec != f0.ec
+                        || sf.e1 != f0.e1 || sf.e2 != f0.e2)
+                        continue;
+                    g.members.push_back((int)s);
+                    added = true;
+                    break;
+                }
                 if (sf.type != f0.type || sf.cutoffType != f0.cutoffType
                     || sf.cutoffAlpha != f0.cutoffAlpha || sf.ec != f0.ec
                     || sf.rc != f0.rc || sf.e1 != f0.e1
@@ -1399,7 +1747,21 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
                 else
                 {
                     SF const& p = e.sfs[g.members[i - 1]];
-                    g.calculateExp.push_back(p.eta != sf.eta || p.rs != sf.rs);
+                    if (isCompact(g.type))   // calculateComp
+                        g.calculateExp.push_back(p.rc != sf.rc
+                                                 || p.rl != sf.rl);
+                    else
+                        g.calculateExp.push_back(p.eta != sf.eta
+                                                 || p.rs != sf.rs);
+                }
+                if (isCompact(g.type))
+                {
+                    // SymGrpCompRad.cpp:82-84 with rmin starting at 0
+                    // (SymGrpBaseComp.cpp:26): rmin stays 0.
+                    if (i == 0) { g.rmin = 0.0; g.rmax = 0.0; }
+                    if (sf.rl <= 0.0) g.rmin = 0.0;
+                    else g.rmin = std::min(g.rmin, sf.rl);
+                    g.rmax = std::max(g.rmax, sf.rc);
                 }
             }
         }
@@ -1703,7 +2065,7 @@ int orc_model_sf_table(const orc_model* m, int e, double* out)
     {
         SF const& s = sfs[i];
         double* o = out + 12 * i;
-        o[0] = s.type; o[1] = s.e1; o[2] = (s.type == 2) ? -1 : s.e2;
+        o[0] = s.type; o[1] = s.e1; o[2] = isRadial(s.type) ? -1 : s.e2;
         o[3] = s.eta; o[4] = s.rs; o[5] = s.lambda; o[6] = s.zeta;
         o[7] = s.rc; o[8] = s.cutoffType; o[9] = s.cutoffAlpha;
         o[10] = s.scalingFactor; o[11] = (double)s.line;
@@ -1721,7 +2083,7 @@ int orc_model_groups(const orc_model* m, int e, int32_t* out, int cap)
         if (out && pos + need <= cap)
         {
             out[pos] = g.type; out[pos + 1] = g.e1;
-            out[pos + 2] = (g.type == 2) ? -1 : g.e2;
+            out[pos + 2] = isRadial(g.type) ? -1 : g.e2;
             out[pos + 3] = (int)g.members.size();
             for (size_t i = 0; i < g.members.size(); ++i)
                 out[pos + 4 + i] = g.members[i];

@@ -73,6 +73,17 @@ def weighted_case(tmpdir, natoms=64, seed=5):
     return d, r, elem, box
 
 
+def compact_case(tmpdir, natoms=64, seed=7):
+    """B/C/N structure with the compact SF families 20-25."""
+    d = os.path.join(str(tmpdir), "compact_%d_%d" % (natoms, seed))
+    r, elem, box = W.random_structure(natoms, seed=seed)
+    if not os.path.exists(os.path.join(d, "scaling.data")):
+        W.write_model(d, W.compact_settings(), W.MIXED_ELEMENTS, (5, 6, 7),
+                      (20, 20), seed=17)
+        scale_from_oracle(d, r, elem, box)
+    return d, r, elem, box
+
+
 def single_sf_model(tmpdir, name, elements, lines, cutoff_type=2,
                     cutoff_alpha=None, act="l l", all_sf=False):
     """Model with the given SF lines and a 1-neuron network; with the linear

@@ -15,8 +15,8 @@ Writes (all data, no code):
       test/cpp/test_CutoffFunction.cpp (tested there at 100 eps);
   tests/golden/symfunc_fixtures.json
       the three-atom (one and two elements) and four-atom fixtures of
-      test/cpp/fixtures.h with the setup lines and values of the type 2, 3,
-      9, 12 and 13 SFs of atom 0.
+      test/cpp/fixtures.h with the setup lines and values of the SFs of
+      atom 0 (types 2, 3, 9, 12, 13 and the compact types 20-25).
 """
 import json
 import os
@@ -87,7 +87,7 @@ def fixtures():
         types = [int(x) for x in re.findall(r"\d+", vec(txt, "vector<size_t> const types" + key))]
         lines = re.findall(r'"([^"]*)"', vec(txt, "vector<string> const setupLines" + key))
         vals = [float(x) for x in re.findall(NUM, vec(txt, "vector<double> const values" + key))]
-        keep = [(t, l, v) for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 12, 13)]
+        keep = [(t, l, v) for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 12, 13, 20, 21, 22, 23, 24, 25)]
         st = dict(st)
         st["sf"] = [{"type": t, "line": l, "value": v} for t, l, v in keep]
         out["sets"][key] = st

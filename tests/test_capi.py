@@ -56,12 +56,24 @@ def test_host_model_setup_weighted_types(tmp_path):
         assert set(a[:, 0].astype(int)) == {2, 3, 12, 13}
 
 
+def test_host_model_setup_compact_types(tmp_path):
+    """The compact types 20-25 (subtype, interval and angle ordering,
+    grouping by centre and neighbour elements) sort like the oracle's."""
+    d, _, _, _ = cases.compact_case(tmp_path)
+    m = engine.Model.load_dir(d)
+    o = OModel.from_dir(d)
+    for e in range(m.num_elements):
+        a, b = m.sf_table(e), o.sf_table(e)
+        assert a.shape == b.shape == (54, 12)
+        assert np.array_equal(a, b)
+
+
 def test_model_errors_are_reported(tmp_path):
     with pytest.raises(RuntimeError, match="Could not open file"):
         engine.Model.load(str(tmp_path / "missing.nn"), None, "w.%03zu.data")
     p = tmp_path / "input.nn"
     p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"
-                 "symfunction_short H 20 H 0.0 5.0 p2\n")
+                 "symfunction_short H 5 H 0.0 5.0\n")
     with pytest.raises(RuntimeError, match="outside this engine"):
         engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
 

@@ -12,6 +12,10 @@ What pins what:
     lambda -1, zeta 4) on the three-atom (H; H + He) and four-atom (S, Cu,
     U) fixtures; the weighted types 12 and 13 carry the neighbours' atomic
     numbers, so the two- and three-element fixtures pin the weights.
+  - The compact families 20-25: the fixture goldens above (p2 cores),
+    the cores shared with the cutoff goldens (p1-p4, e), the asymmetric
+    core through its defining identity, and finite differences of a model
+    with every family and core.
   - Derivatives: forces = -dE/dr against central differences (the nnp-checkf
     procedure) for every SF type and for a periodic water box.
   - Neighbour list: an independent numpy enumeration (set) and the cell-list
@@ -96,7 +100,7 @@ def test_symfunc_fixture_values(workdir):
         G = s.G()
         assert abs(G[0, 0] - sf["value"]) <= 1000 * EPS, (key, k, sf)
         n += 1
-    assert n == 23
+    assert n == 50
 
 
 def test_symfunc_derivatives_vs_finite_differences(workdir):
@@ -107,6 +111,46 @@ def test_symfunc_derivatives_vs_finite_differences(workdir):
         f = s.forces()
         fn = s.numeric_forces(np.arange(s.n), delta=1e-5)
         assert np.abs(f - fn).max() < 1e-8, (key, k, sf["line"])
+
+
+def test_compact_asymmetric_core_identity(workdir):
+    """CoreFunction.h:178-191: the asymmetric core at y = |r - center| /
+    width equals the symmetric core at (2 - y) y.  Two-atom structures
+    with one type 20 SF each (p2a and p2): G_asym(r) = G_sym(r') where r'
+    is placed so that |r' - center| / width = (2 - y) y; a dropped or
+    wrong mapping fails."""
+    rl, rc = 1.0, 5.0
+    center, width = 3.0, 2.0
+    da = cases.single_sf_model(workdir, "asym", "H",
+                               ["H 20 H %.1f %.1f p2a" % (rl, rc)])
+    ds = cases.single_sf_model(workdir, "sym", "H",
+                               ["H 20 H %.1f %.1f p2" % (rl, rc)])
+    ma = OModel.from_dir(da, scaling=False)
+    ms = OModel.from_dir(ds, scaling=False)
+    for r in (1.3, 2.2, 3.0, 3.7, 4.6):
+        y = abs(r - center) / width
+        r2 = center + (2.0 - y) * y * width
+        sa = OStruct(ma, [[0, 0, 0], [r, 0, 0]], [0, 0])
+        ss = OStruct(ms, [[0, 0, 0], [r2, 0, 0]], [0, 0])
+        sa.evaluate(cells=False)
+        ss.evaluate(cells=False)
+        assert abs(sa.G()[0, 0] - ss.G()[0, 0]) <= 1e-14
+    # at the centre the core is 1, at the interval ends 0 (both forms)
+    s = OStruct(ma, [[0, 0, 0], [center, 0, 0]], [0, 0])
+    s.evaluate(cells=False)
+    assert s.G()[0, 0] == 1.0
+
+
+def test_compact_model_forces_vs_finite_differences(workdir):
+    """All compact families (20-25, every core, negative rl, angle
+    intervals around 0 and 180 degrees) in one model: forces against
+    central differences of the oracle's energy."""
+    d, r, e, box = cases.compact_case(workdir, natoms=40, seed=9)
+    s = OStruct(OModel.from_dir(d), r, e, box)
+    s.evaluate(cells=True, force_mode=1)
+    f = s.forces()
+    fn = s.numeric_forces(np.arange(10), delta=1e-5)
+    assert np.abs(f[:10] - fn).max() < 5e-8
 
 
 def _numpy_neighbors(r, box, rc):
