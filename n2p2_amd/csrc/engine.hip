@@ -334,9 +334,11 @@ struct hdnnp_ctx
         T.cutType = model->m.cutoffType;
         T.genpow = 0;
         T.maxGroupMembers = 0;
+        T.weighted = 0;
         for (auto const& g : flat.group)
         {
             T.maxGroupMembers = std::max(T.maxGroupMembers, g.nmem);
+            if (sf_weighted(g.type)) T.weighted = 1;
             if (!sf_radial(g.type))
                 for (int m = g.mem0; m < g.mem0 + g.nmem; ++m)
                     if (!flat.member[m].useint) T.genpow = 1;

@@ -59,6 +59,12 @@
 #include "device_common.h"
 #include "kernel_args.h"
 
+// The model-specialised build defines HDNNP_WEIGHTED 0 for models without
+// weighted groups, which removes their code.
+#ifndef HDNNP_WEIGHTED
+#define HDNNP_WEIGHTED 1
+#endif
+
 namespace hdnnp {
 
 namespace {
@@ -355,7 +361,7 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
         GroupDesc const g = T.group[gi];
-        bool const weighted = sf_weighted(g.type);
+        bool const weighted = HDNNP_WEIGHTED && sf_weighted(g.type);
         // Cache key: the cutoff configuration and whether the values carry
         // the neighbour's atomic number.
         int const cfg = ed.cfg0 + g.cfg;

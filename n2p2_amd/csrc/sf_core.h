@@ -35,6 +35,12 @@
 #include "device_common.h"
 #include "kernel_args.h"
 
+// The model-specialised build defines HDNNP_WEIGHTED 0 for models without
+// weighted groups, which removes their code.
+#ifndef HDNNP_WEIGHTED
+#define HDNNP_WEIGHTED 1
+#endif
+
 namespace hdnnp {
 
 namespace {
@@ -243,7 +249,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
         CutCfg const cc = T.cfg[ed.cfg0 + g.cfg];
         MemberDesc const* __restrict__ mem = T.member + g.mem0;
         int const nmem = g.nmem;
-        bool const weighted = sf_weighted(g.type);
+        bool const weighted = HDNNP_WEIGHTED && sf_weighted(g.type);
         if (lane < nmem) MP[lane] = make_mparam<R>(mem[lane], 0.0);
         if (g.rc != rcCached || weighted != wCached)
         {

@@ -198,7 +198,9 @@ std::string full_source(std::string const& plan, ModelTables const& T,
     char const* R = fp32 ? "float" : "double";
     int const maxm = plan_maxm(T);
     int const ct = plan_cut(T);
+    // Models without weighted groups leave the weighted code out.
     o << "#define HDNNP_PLAN 1\n"
+      << "#define HDNNP_WEIGHTED " << (T.weighted ? 1 : 0) << "\n"
       << "#include \"device_common.h\"\n"
       << "#include \"kernel_args.h\"\n"
       << "namespace hdnnp {\nnamespace {\n" << plan << "}\n}\n"

@@ -175,6 +175,7 @@ struct ModelTables
     int cutType;          // cutoff_type of the settings (global in 2G)
     int genpow;           // some angular member has a non-integer zeta
     int maxGroupMembers;  // largest group chunk
+    int weighted;         // some group is of a weighted type (12, 13)
     double maxCutoff;
     double zw[kMaxElements];      // atomic number per element (weights of
                                   // the weighted types)
