@@ -73,9 +73,11 @@ const char* hdnnp_last_error(void);
  *   or ""); setupNeuralNetworkWeights
  * (Mode.cpp:55-236, :692-840, :1158-1478, :2270-2289; Prediction.cpp:34-46).
  * weights_fmt is a printf format with one size_t conversion for the atomic
- * number, e.g. "weights.%03zu.data".  Symmetry-function types 2, 3 and 9
- * and all cutoff, scaling and activation types of the reference are
- * supported; other SF types return HDNNP_E_UNSUPPORTED. */
+ * number, e.g. "weights.%03zu.data".  Every symmetry-function type of the
+ * reference (2, 3, 9; weighted 12, 13; compact 20-25) and all cutoff,
+ * scaling and activation types are supported; unknown SF types, more than
+ * 32 compact SFs in one group and type-13 SFs with rs < 0 return
+ * HDNNP_E_UNSUPPORTED. */
 int  hdnnp_model_load(const char* settings_file, const char* scaling_file,
                       const char* weights_fmt, hdnnp_model** out);
 void hdnnp_model_free(hdnnp_model* m);
@@ -331,7 +333,8 @@ int hdnnp_set_weights(hdnnp_ctx* c, const double* w);
  * gradient of L = w_e ((E - e_ref) / N)^2 + w_f / (3N) sum |F - f_ref|^2
  * with respect to every weight (grad: hdnnp_num_weights doubles, may be
  * NULL; it is kept for hdnnp_train_adam).  loss (2 doubles, may be NULL)
- * = the two terms.  Radial and angular SFs (types 2, 3, 9); identity /
+ * = the two terms.  SF types 2, 3, 9, 12 and 13 (the compact types 20-25
+ * return HDNNP_E_UNSUPPORTED); identity /
  * tanh / logistic / softplus layers with one output (HDNNP_E_UNSUPPORTED
  * otherwise). */
 int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,

@@ -309,7 +309,8 @@ void Model::load(std::string const& settingsFile,
             // rs >= 0.
             if (s.rs < 0.0)
                 throw std::runtime_error("ERROR: Type 13 symmetry functions "
-                                         "with rs < 0 are not supported.\n");
+                                         "with rs < 0 are outside this "
+                                         "engine.\n");
         }
         else if (sf_compact(s.type))
         {
@@ -439,6 +440,13 @@ void Model::load(std::string const& settingsFile,
             }
         }
     }
+
+    for (auto const& e : el)
+        for (auto const& g : e.groups)
+            if (sf_compact(g.type) && (int)g.members.size() > kMaxCompMembers)
+                throw std::runtime_error("ERROR: More than 32 compact "
+                                         "symmetry functions in one group "
+                                         "are outside this engine.\n");
 
     setupScaling(scalingFile);
 
@@ -847,10 +855,6 @@ Model::Flat Model::flatten() const
         for (auto const& g : e.groups)
         {
             if (!sf_compact(g.type)) continue;
-            if ((int)g.members.size() > kMaxCompMembers)
-                throw std::runtime_error("ERROR: More compact symmetry "
-                                         "functions in one group than the "
-                                         "engine supports (32).\n");
             std::vector<int> mem = g.members;
             std::sort(mem.begin(), mem.end());
             CompGroup cg{};

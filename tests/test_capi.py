@@ -73,6 +73,15 @@ def test_model_errors_are_reported(tmp_path):
         engine.Model.load(str(tmp_path / "missing.nn"), None, "w.%03zu.data")
     p = tmp_path / "input.nn"
     p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"
+                 "symfunction_short H 13 0.01 -1.0 1.0 1.0 5.0\n")
+    with pytest.raises(RuntimeError, match="outside this engine"):
+        engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
+    p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"
+                 + "".join("symfunction_short H 20 H 0.0 %.1f p2\n"
+                           % (3.0 + 0.1 * k) for k in range(33)))
+    with pytest.raises(RuntimeError, match="outside this engine"):
+        engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
+    p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"
                  "symfunction_short H 5 H 0.0 5.0\n")
     with pytest.raises(RuntimeError, match="outside this engine"):
         engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
