@@ -333,8 +333,7 @@ int hdnnp_set_weights(hdnnp_ctx* c, const double* w);
  * gradient of L = w_e ((E - e_ref) / N)^2 + w_f / (3N) sum |F - f_ref|^2
  * with respect to every weight (grad: hdnnp_num_weights doubles, may be
  * NULL; it is kept for hdnnp_train_adam).  loss (2 doubles, may be NULL)
- * = the two terms.  SF types 2, 3, 9, 12 and 13 (the compact types 20-25
- * return HDNNP_E_UNSUPPORTED); identity /
+ * = the two terms.  Every SF type; identity /
  * tanh / logistic / softplus layers with one output (HDNNP_E_UNSUPPORTED
  * otherwise). */
 int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,

@@ -2097,10 +2097,6 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
             throw Error(HDNNP_E_INVALID, "ERROR: Training needs one structure "
                                          "from hdnnp_set_structure and its "
                                          "reference forces.\n");
-        if (c->T.ncgroup > 0)
-            throw Error(HDNNP_E_UNSUPPORTED,
-                        "ERROR: Training gradients do not cover the compact "
-                        "symmetry functions (types 20-25).\n");
         for (size_t e = 0; e < c->flat.elem.size(); ++e)
         {
             ElemDesc const& ed = c->flat.elem[e];
@@ -2155,6 +2151,14 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
         A.wE = w_e;
         A.cF = -2.0 * w_f / (3.0 * (double)n);
         hipCheck(launch_train_u(A, s), "train u");
+        {
+            int const cap = std::max(c->maxNeigh, 1);
+            if (c->T.ncgroup > 0 && train_u_compact_lds_bytes(cap) > 65536)
+                throw Error(HDNNP_E_CAPACITY,
+                            "ERROR: Too many neighbours per atom for the "
+                            "compact training kernel's LDS staging.\n");
+            hipCheck(launch_train_u_compact(A, cap, s), "train u (compact)");
+        }
         for (size_t e = 0; e < c->flat.elem.size(); ++e)
         {
             ElemDesc const& ed = c->flat.elem[e];

@@ -103,6 +103,10 @@ int train_atoms_per_block();
 int train_atoms_per_grad_block(ElemDesc const& ed);
 size_t train_lds_per_atom(ElemDesc const& ed);
 hipError_t launch_train_u(TrainArgs const& A, hipStream_t s);
+// The compact SF types' part of u (after launch_train_u); cap = the largest
+// neighbour count.
+hipError_t launch_train_u_compact(TrainArgs const& A, int cap, hipStream_t s);
+size_t train_u_compact_lds_bytes(int cap);
 hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,
                              const int* atoms, int na, double* part, int nw,
                              double* grad, hipStream_t s);

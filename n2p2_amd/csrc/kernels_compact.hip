@@ -80,18 +80,32 @@ __device__ __forceinline__ bool pair_elements(CompGroup const& g, int ej,
         || (g.e2 == ej && g.e1 == ek);
 }
 
-// One pair {j, k} of an angular compact group, all members in order.
-// Values: acc[l] += rad * ang.  Derivatives (DERIV): with the member
-// weights w[l] (dE/dG), P1 += w p1, P2 += w p2, P3 += w p3 of
-// SymGrpCompAngn.cpp:296-343, so that slot j receives -(P1 dr_ij + P3
-// dr_jk) and slot k -(P2 dr_ik - P3 dr_jk).  z = Z_j Z_k for the weighted
-// types, else 1.
-template <bool DERIV>
+// What angular_pair accumulates.
+enum PairMode
+{
+    PM_VALUE = 0,     // acc[l] += rad * ang
+    PM_FORCE = 1,     // P1, P2, P3 += w[l] (p1, p2, p3)
+    PM_TANGENT = 2    // acc[l] += p1 t1 + p2 t2 + p3 t3
+};
+
+// One pair {j, k} of an angular compact group, all members in order, with
+// p1, p2, p3 of SymGrpCompAngn.cpp:296-343 (dG/dr_j = -(p1 dr_ij + p3
+// dr_jk), dG/dr_k = -(p2 dr_ik - p3 dr_jk), dG/dr_i = p1 dr_ij + p2
+// dr_ik):
+//  - PM_VALUE: the member values;
+//  - PM_FORCE: with the member weights w[l] (dE/dG), slot j receives
+//    -(P1 dr_ij + P3 dr_jk) and slot k -(P2 dr_ik - P3 dr_jk);
+//  - PM_TANGENT: sum_i d_i . dG/dr_i for the residual d, given
+//    t1 = dr_ij . (d_i - d_j), t2 = dr_ik . (d_i - d_k),
+//    t3 = dr_jk . (d_k - d_j).
+// z = Z_j Z_k for the weighted types, else 1.
+template <int MODE>
 __device__ __forceinline__ void angular_pair(
     CompGroup const& g, CompMember const* __restrict__ mem, double4 const pj,
     double4 const pk, double z, double* acc, double const* w, double& P1,
     double& P2, double& P3)
 {
+    constexpr bool DERIV = MODE != PM_VALUE;
     bool const wide = (g.type == 22 || g.type == 25);
     bool const weighted = sf_weighted(g.type);
     double const rij = pj.w, rik = pk.w;
@@ -185,9 +199,13 @@ __device__ __forceinline__ void angular_pair(
             p3 = ang * chijk;
         }
         else continue;
-        P1 += w[l] * p1;
-        P2 += w[l] * p2;
-        P3 += w[l] * p3;
+        if (MODE == PM_FORCE)
+        {
+            P1 += w[l] * p1;
+            P2 += w[l] * p2;
+            P3 += w[l] * p3;
+        }
+        else acc[l] += p1 * P1 + p2 * P2 + p3 * P3;   // P1..P3 hold t1..t3
     }
 }
 
@@ -253,8 +271,8 @@ __global__ __launch_bounds__(64) void k_sf_compact(SFArgs a)
                 if (base + lane >= dp.npairs) continue;
                 if (!pair_elements(g, E[j], E[k])) continue;
                 double const z = weighted ? T.zw[E[j]] * T.zw[E[k]] : 1.0;
-                angular_pair<false>(g, mem, P[j], P[k], z, row, nullptr, d1,
-                                    d2, d3);
+                angular_pair<PM_VALUE>(g, mem, P[j], P[k], z, row, nullptr,
+                                       d1, d2, d3);
             }
         }
         __syncthreads();
@@ -339,7 +357,8 @@ __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
                 double const z = weighted ? T.zw[E[j]] * T.zw[E[k]] : 1.0;
                 double4 const pj = P[j], pk = P[k];
                 double P1 = 0.0, P2 = 0.0, P3 = 0.0;
-                angular_pair<true>(g, mem, pj, pk, z, nullptr, W, P1, P2, P3);
+                angular_pair<PM_FORCE>(g, mem, pj, pk, z, nullptr, W, P1, P2,
+                                       P3);
                 if (P1 == 0.0 && P2 == 0.0 && P3 == 0.0) continue;
                 double const dx = pk.x - pj.x, dy = pk.y - pj.y,
                              dz = pk.z - pj.z;
@@ -376,7 +395,118 @@ __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
     }
 }
 
+// Training: u_as = cF sum_i d_i . dG_s/dr_i for the compact SFs s of atom
+// a (the part of the force-loss tangent k_train_u forms for the other
+// types), d = F - Fref.  LDS: per-lane member sums (64 x kCM), staged
+// neighbours, their residuals d_j and elements.
+__global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
+{
+    extern __shared__ __align__(16) double sm[];
+    ModelTables const& T = A.T;
+    int const a = blockIdx.x;
+    ElemDesc const& ed = T.elem[A.elem[a]];
+    if (ed.ncg == 0) return;
+    int const lane = threadIdx.x;
+    double* ACC = sm;
+    double4* P = (double4*)(ACC + 64 * kCM);
+    double* D = (double*)(P + cap);
+    int* E = (int*)(D + 3 * cap);
+    int const s0 = A.off[a], n = A.cnt[a];
+    for (int t = lane; t < n; t += 64)
+    {
+        int const j = A.nbrJ[s0 + t];
+        for (int q = 0; q < 3; ++q)
+            D[3 * t + q] = A.F[3 * (size_t)j + q] - A.Fref[3 * (size_t)j + q];
+    }
+    stage_list(A.geo, A.nbrJ, A.elem, s0, n, P, E);
+    double da[3];
+    for (int q = 0; q < 3; ++q)
+        da[q] = A.F[3 * (size_t)a + q] - A.Fref[3 * (size_t)a + q];
+    double* row = ACC + lane * kCM;
+    double* __restrict__ U = A.U + (size_t)a * A.ldG;
+    for (int cg = ed.cg0; cg < ed.cg0 + ed.ncg; ++cg)
+    {
+        CompGroup const g = T.cgroup[cg];
+        CompMember const* __restrict__ mem = T.cmember + g.mem0;
+        bool const weighted = sf_weighted(g.type);
+        for (int l = 0; l < g.nmem; ++l) row[l] = 0.0;
+        if (sf_radial(g.type))
+        {
+            for (int t = lane; t < n; t += 64)
+            {
+                if (!weighted && E[t] != g.e1) continue;
+                double4 const pj = P[t];
+                double const r = pj.w;
+                if (!(r < g.rmax && r > 0.0)) continue;
+                double const z = weighted ? T.zw[E[t]] : 1.0;
+                double const t1 = pj.x * (da[0] - D[3 * t])
+                                + pj.y * (da[1] - D[3 * t + 1])
+                                + pj.z * (da[2] - D[3 * t + 2]);
+                for (int l = 0; l < g.nmem; ++l)
+                {
+                    CompMember const& m = mem[l];
+                    if (r <= m.rl || r >= m.rc) continue;
+                    double rad, drad;
+                    compact_fdf(m.core, m.asym, m.rcenter, m.rscale, r, rad,
+                                drad);
+                    if (drad == 0.0) continue;
+                    drad *= weighted ? z * m.sfac / r : m.sfac / r;
+                    row[l] += drad * t1;
+                }
+            }
+        }
+        else
+        {
+            DiagPairs const dp(true, 0, n, 0, n);
+            int pc = 0, pr = 0;
+            dp.start(lane, pc, pr);
+            for (int base = 0; base < dp.npairs; base += 64)
+            {
+                int j, k;
+                dp.pair(pc, pr, j, k);
+                dp.advance(pc, pr);
+                if (base + lane >= dp.npairs) continue;
+                if (!pair_elements(g, E[j], E[k])) continue;
+                double const z = weighted ? T.zw[E[j]] * T.zw[E[k]] : 1.0;
+                double4 const pj = P[j], pk = P[k];
+                double const* dj = D + 3 * j;
+                double const* dk = D + 3 * k;
+                double t1 = pj.x * (da[0] - dj[0]) + pj.y * (da[1] - dj[1])
+                          + pj.z * (da[2] - dj[2]);
+                double t2 = pk.x * (da[0] - dk[0]) + pk.y * (da[1] - dk[1])
+                          + pk.z * (da[2] - dk[2]);
+                double t3 = (pk.x - pj.x) * (dk[0] - dj[0])
+                          + (pk.y - pj.y) * (dk[1] - dj[1])
+                          + (pk.z - pj.z) * (dk[2] - dj[2]);
+                angular_pair<PM_TANGENT>(g, mem, pj, pk, z, row, nullptr, t1,
+                                         t2, t3);
+            }
+        }
+        __syncthreads();
+        if (lane < g.nmem)
+        {
+            double v = 0.0;
+            for (int q = 0; q < 64; ++q) v += ACC[q * kCM + lane];
+            U[mem[lane].sf] = A.cF * v;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+size_t train_u_compact_lds_bytes(int cap)
+{
+    return (size_t)64 * kCM * sizeof(double)
+         + (size_t)cap * (sizeof(double4) + 3 * sizeof(double) + sizeof(int));
+}
+
+hipError_t launch_train_u_compact(TrainArgs const& A, int cap, hipStream_t s)
+{
+    if (A.n <= 0 || A.T.ncgroup == 0) return hipSuccess;
+    k_train_u_compact<<<A.n, 64, train_u_compact_lds_bytes(cap), s>>>(A, cap);
+    return hipGetLastError();
+}
 
 size_t sf_compact_lds_bytes(int cap)
 {

@@ -128,9 +128,3 @@ def test_compact_batch_and_statistics(tmp_path):
         assert np.allclose(st, ost, rtol=1e-11, atol=1e-14)
 
 
-def test_compact_training_reports_unsupported(ccase):
-    d, r, e, box = ccase
-    ctx = engine.Context(engine.Model.load_dir(d))
-    ctx.set_structure(r, e, box)
-    with pytest.raises(RuntimeError, match="compact"):
-        ctx.train_gradient(np.zeros((len(e), 3)), 0.0)

@@ -96,16 +96,21 @@ def test_gradient_vs_oracle_finite_differences(sur, tmp_path):
 
 
 @pytest.mark.parametrize("which", ["water_g2_g3_tanh", "mixed_g2_g9_softplus",
-                                   "weighted_g12_g13_tanh"])
+                                   "weighted_g12_g13_tanh",
+                                   "compact_g20_g25_tanh"])
 def test_gradient_full_models_vs_oracle_fd(which, tmp_path):
     """The same check on the full models with angular SFs: water (G2 + G3,
-    tanh), the three-element set (G2 + G9, softplus) and the set with the
-    element-weighted types 12 and 13."""
+    tanh), the three-element set (G2 + G9, softplus), the set with the
+    element-weighted types 12 and 13 and the set with the compact types
+    20-25."""
     if which.startswith("water"):
         d, r, e, box = cases.water_case(tmp_path / "w")
         z, hidden = (1, 8), (25, 25)
     elif which.startswith("weighted"):
         d, r, e, box = cases.weighted_case(tmp_path / "g", natoms=40, seed=3)
+        z, hidden = (5, 6, 7), (20, 20)
+    elif which.startswith("compact"):
+        d, r, e, box = cases.compact_case(tmp_path / "c", natoms=40, seed=3)
         z, hidden = (5, 6, 7), (20, 20)
     else:
         d, r, e, box = cases.mixed_case(tmp_path / "m", natoms=40, seed=3)
