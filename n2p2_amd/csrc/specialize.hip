@@ -214,6 +214,10 @@ std::string full_source(std::string const& plan, ModelTables const& T,
       << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
       << R << ">(a); }\n"
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
+      // FP32: at most 72 VGPRs, seven waves per SIMD (measured 3% faster
+      // than the compiler's choice on the 196,608-atom water box; eight
+      // waves measured 7% slower).
+      << (fp32 ? "__attribute__((amdgpu_waves_per_eu(7, 8)))\n" : "")
       << "void hdnnp_plan_sf_" << (fp32 ? "f32" : "f64")
       << "(hdnnp::SFArgs a)\n"
       << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
