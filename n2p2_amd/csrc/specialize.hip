@@ -214,10 +214,13 @@ std::string full_source(std::string const& plan, ModelTables const& T,
       << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
       << R << ">(a); }\n"
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
-      // FP32: at most 72 VGPRs, seven waves per SIMD (measured 3% faster
-      // than the compiler's choice on the 196,608-atom water box; eight
-      // waves measured 7% slower).
-      << (fp32 ? "__attribute__((amdgpu_waves_per_eu(7, 8)))\n" : "")
+      // FP32 with up to 8 members per group chunk: at most 72 VGPRs, seven
+      // waves per SIMD (measured 3% faster than the compiler's choice on
+      // the 196,608-atom water box; eight waves 7% slower).  Chunks of 16
+      // members need more registers (the bound doubled the SF time of a
+      // G2 + G9 model).
+      << (fp32 && maxm <= 8 ? "__attribute__((amdgpu_waves_per_eu(7, 8)))\n"
+                            : "")
       << "void hdnnp_plan_sf_" << (fp32 ? "f32" : "f64")
       << "(hdnnp::SFArgs a)\n"
       << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
