@@ -301,7 +301,7 @@ int64_t hdnnp_md_partner_misses(hdnnp_ctx* c);
  * F_full - F_sur, n_inner velocity-Verlet substeps of dt_outer / n_inner on
  * the surrogate, a full evaluation and a second half kick.  After every
  * outer step the force RMSE of the surrogate against the full model is
- * judged; when it exceeds rmse_gate the next outer step runs the full model
+ * measured; when it exceeds rmse_gate the next outer step runs the full model
  * for all n_inner substeps instead.  Both contexts hold the same periodic
  * structure; masses, skin, ftoa, v and use_graph as for hdnnp_md_init.
  * The state lives in `sur` (hdnnp_md_get_state(sur, ...)).  No counterpart
@@ -312,7 +312,7 @@ int hdnnp_mts_init(hdnnp_ctx* full, hdnnp_ctx* sur, const double* masses,
                    double dt_outer, int n_inner, double skin, double ftoa,
                    const double* v, double rmse_gate, int use_graph);
 /* nouter outer steps; per step (each array may be NULL): the full model's
- * energy and the kinetic energy at the end of the step, the judged force
+ * energy and the kinetic energy at the end of the step, the measured force
  * RMSE and the mode of the step (0 surrogate substeps, 1 full substeps). */
 int hdnnp_mts_run(hdnnp_ctx* sur, int64_t nouter, double* epot, double* ekin,
                   double* rmse, int32_t* mode);

@@ -1921,7 +1921,7 @@ static double mtsRmse(hdnnp_ctx* sur)
 // surrogate as the fast force and F_full - F_sur as the slow force:
 // half kick with the slow force, n_inner velocity-Verlet substeps on the
 // surrogate, full evaluation, half kick with the new slow force.  When
-// the last judged RMSE exceeded the gate, the step is instead n_inner
+// the last measured RMSE exceeded the gate, the step is instead n_inner
 // velocity-Verlet substeps on the full model.  The positions of both
 // contexts stay identical: whenever one rebuilds (and may wrap atoms into
 // the cell) the other takes its positions and rebuilds too.
