@@ -68,6 +68,18 @@ def test_external_list_g9_small_box(workdir):
     _compare(d, r, e, box, engine.PRECISION_FP64, 1e-10, 1e-9)
 
 
+@pytest.mark.parametrize("which", ["weighted", "compact"])
+def test_external_list_weighted_and_compact(workdir, which):
+    """The element-weighted types 12, 13 and the compact types 20-25 through
+    a LAMMPS-like domain: the compact pass adds its slot contributions
+    before the scatter to ghosts."""
+    say(which, "96 atoms as a LAMMPS-like domain")
+    case = cases.weighted_case if which == "weighted" else cases.compact_case
+    d, r, e, box = case(workdir, natoms=96, seed=5)
+    _compare(d, r, e, box, engine.PRECISION_FP64, 1e-10, 1e-9)
+    _compare(d, r, e, box, engine.PRECISION_MIXED, 1e-6, 1e-5)
+
+
 def test_external_list_errors(workdir):
     d, r, e, box = cases.water_case(workdir)
     m = engine.Model.load_dir(d)
