@@ -10,13 +10,21 @@ Here the structures are loaded as batches (hdnnp_set_structures), one
 batch evaluated per launch, and the accumulators of the batches are
 combined on the host.
 
+The SF histograms (Dataset::writeSymmetryFunctionHistograms,
+Dataset.cpp:1103-1240) come from a second pass over the batches:
+- the raw G of each batch are evaluated on the device;
+- they are binned on the host into <nbins> uniform bins over
+  [min, max + (max - min) / (nbins - 1)] (the reference's GSL histogram;
+  a value lands in bin i when edge_i <= G < edge_(i+1));
+- each element and SF gets a file sf.<Z>.<SF index>.histo, with the SF's
+  #SFINFO parameter lines (SymFnc*::parameterInfo, 12-wide names), its
+  statistics and the column header.
+
 Usage, in a directory with input.nn and input.data (as nnp-scaling):
 
     python -m n2p2_amd.scaling <nbins>
-
-<nbins> is accepted for the command line of nnp-scaling; the SF histogram
-files (Dataset::writeSymmetryFunctionHistograms) are not written.
 """
+import math
 import sys
 
 import numpy as np
@@ -82,11 +90,14 @@ def _batches(structures, max_atoms):
         yield cur
 
 
-def statistics(settings, structures, max_atoms=4_000_000, device=0):
+def statistics(settings, structures, max_atoms=4_000_000, device=0,
+               raw=False):
     """Per element [nsf x 4] of min, max, mean, sigma over all atoms of the
     structures [(r, elem, box or None), ...], in the model's internal units
     (positions converted with conv_length when the settings normalise, as
-    nnp-scaling's toNormalizedUnits)."""
+    nnp-scaling's toNormalizedUnits).  raw: also return the per-element
+    atom counts and [nsf x 4] accumulators (min, max, sum, sum of
+    squares)."""
     model = engine.Model.load(settings, None, "")
     used, _, _, conv_length = model.normalization()
     if not used:
@@ -127,15 +138,174 @@ def statistics(settings, structures, max_atoms=4_000_000, device=0):
         mean = a[:, 2] / n
         sigma = np.sqrt((a[:, 3] - a[:, 2] * a[:, 2] / n) / (n - 1.0))
         out.append(np.stack([a[:, 0], a[:, 1], mean, sigma], axis=1))
+    if raw:
+        return out, cnt, acc
     return out
+
+
+def _pad(name, width=12):
+    """utility.cpp:79-88 pad (right-filled with blanks)."""
+    return name + " " * max(0, width - len(name))
+
+
+def sf_info(model, settings_lines, e, symbols):
+    """#SFINFO lines of every SF of element e in SF order: the
+    parameterInfo of SymFnc.cpp:32-50, SymFncBaseCutoff.cpp:24-36,
+    SymFncExpRad.cpp:201-215, SymFncBaseExpAng.cpp:143-166, the weighted
+    SymFncExpRadWeighted / SymFncExpAngnWeighted, SymFncBaseComp.cpp:25-37,
+    SymFncCompRad.cpp:198-207 and SymFncBaseCompAng(Weighted).  Lengths are
+    printed as the reference does, converted back from internal units."""
+    _, _, _, conv = model.normalization()
+    if not model.normalization()[0]:
+        conv = 1.0
+    table = model.sf_table(e)
+    out = []
+    for idx, row in enumerate(table):
+        t, e1, e2, eta, rs, lam, zeta, rc, ct, alpha, _, ln = row
+        t, ln = int(t), int(ln)
+        w = settings_lines[ln].split("#", 1)[0].split()[1:]   # after keyword
+        v = [(_pad("lineNumber") + "%d") % (ln + 1),
+             (_pad("index") + "%d") % (idx + 1),
+             (_pad("type") + "%d") % t,
+             (_pad("ec") + "%s") % symbols[e],
+             (_pad("rc") + "%14.8E") % (rc / conv)]
+        if t in (2, 3, 9, 12, 13):
+            v += [(_pad("subtype") + "ct%d") % int(ct),
+                  (_pad("alpha") + "%14.8E") % alpha]
+            if t == 2:
+                v.append((_pad("e1") + "%s") % symbols[int(e1)])
+            if t in (3, 9):
+                v += [(_pad("e1") + "%s") % symbols[int(e1)],
+                      (_pad("e2") + "%s") % symbols[int(e2)]]
+            v.append((_pad("eta") + "%14.8E") % (eta * conv * conv))
+            if t in (3, 9, 13):
+                v += [(_pad("lambda") + "%14.8E") % lam,
+                      (_pad("zeta") + "%14.8E") % zeta]
+            v.append((_pad("rs") + "%14.8E") % (rs / conv))
+        else:
+            # compact: subtype and rl from the settings line
+            p = 2
+            if t in (20, 21, 22):
+                p += 1 if t == 20 else 2
+            rl = float(w[p]) * conv
+            ang = (w[p + 2], w[p + 3]) if t not in (20, 23) else None
+            sub = w[p + 4] if ang else w[p + 2]
+            v += [(_pad("subtype") + "%s") % sub,
+                  (_pad("rl") + "%14.8E") % (rl / conv)]
+            if t == 20:
+                v.append((_pad("e1") + "%s") % symbols[int(e1)])
+            if t in (21, 22):
+                v += [(_pad("e1") + "%s") % symbols[int(e1)],
+                      (_pad("e2") + "%s") % symbols[int(e2)]]
+            if ang:
+                v += [(_pad("angleLeft") + "%14.8E") % float(ang[0]),
+                      (_pad("angleRight") + "%14.8E") % float(ang[1])]
+        out.append(v)
+    return out
+
+
+HISTO_COLUMNS = (
+    [16, 16, 16],
+    ["symfunc_l", "symfunc_r", "hist"],
+    ["Symmetry function value, left bin limit.",
+     "Symmetry function value, right bin limit.", "Histogram count."])
+
+
+def histogram_edges(lo, hi, nbins):
+    """The reference's bins: nbins uniform bins over
+    [lo, hi + (hi - lo) / (nbins - 1)] (Dataset.cpp:1113-1127), edges as a
+    uniform GSL histogram sets them, lo + (i / n) (top - lo)."""
+    nb = nbins - 1
+    top = hi + (hi - lo) / nb
+    n = nb + 1
+    return np.array([lo + (i / n) * (top - lo) for i in range(n + 1)])
+
+
+def write_histograms(settings, structures, nbins, directory=".",
+                     max_atoms=4_000_000, device=0,
+                     fmt="sf.%03d.%04d.histo"):
+    """SF histogram files of nnp-scaling for the structures; returns the
+    number of files written (SFs whose minimum equals their maximum get
+    none, as in the reference)."""
+    import os
+    model = engine.Model.load(settings, None, "")
+    used, _, _, conv_length = model.normalization()
+    if not used:
+        conv_length = 1.0
+    stats, cnt, acc = statistics(settings, structures, max_atoms, device,
+                                 raw=True)
+    ne = model.num_elements
+    edges = [[None] * model.num_sf(k) for k in range(ne)]
+    counts = [np.zeros((model.num_sf(k), nbins)) for k in range(ne)]
+    for k in range(ne):
+        if acc[k] is None:
+            continue
+        for s_ in range(model.num_sf(k)):
+            lo, hi = acc[k][s_, 0], acc[k][s_, 1]
+            if lo < hi:
+                edges[k][s_] = histogram_edges(lo, hi, nbins)
+    ctx = engine.Context(model, device=device)
+    for batch in _batches(structures, max_atoms):
+        conv = []
+        for r, e, box in batch:
+            if conv_length != 1.0:
+                r, box = nio.to_normalized(r, box, conv_length)
+            conv.append((r, e, box))
+        ctx.set_structures(conv)
+        ctx.calculate_neighbor_list(model.max_cutoff)
+        ctx.calculate_symmetry_function_groups(derivatives=False)
+        G = ctx.G()
+        el = np.concatenate([np.asarray(e) for _, e, _ in conv])
+        for k in range(ne):
+            gk = G[el == k][:, :model.num_sf(k)]
+            for s_ in range(model.num_sf(k)):
+                ed = edges[k][s_]
+                if ed is None:
+                    continue
+                i = np.searchsorted(ed, gk[:, s_], side="right") - 1
+                i = i[(i >= 0) & (i < nbins)]
+                counts[k][s_] += np.bincount(i, minlength=nbins)
+    ctx.close()
+    lines = open(settings).read().split("\n")
+    symbols = [None] * ne
+    for l in lines:
+        w = l.split("#", 1)[0].split()
+        if w and w[0] == "elements":
+            symbols = w[1:1 + ne]
+    size, name, info = HISTO_COLUMNS
+    nfiles = 0
+    for k in range(ne):
+        infos = sf_info(model, lines, k, symbols)
+        z = model.atomic_number(k)
+        for s_ in range(model.num_sf(k)):
+            ed = edges[k][s_]
+            if ed is None:
+                continue
+            n = float(cnt[k])
+            a = acc[k][s_]
+            path = os.path.join(directory, fmt % (z, s_ + 1))
+            with open(path, "w") as f:
+                for l in infos[s_]:
+                    f.write("#SFINFO %s\n" % l)
+                f.write("#SFINFO min         %15.8E\n" % a[0])
+                f.write("#SFINFO max         %15.8E\n" % a[1])
+                f.write("#SFINFO mean        %15.8E\n" % (a[2] / n))
+                f.write("#SFINFO sigma       %15.8E\n"
+                        % math.sqrt(1.0 / (n - 1) * (a[3] - a[2] * a[2] / n)))
+                f.writelines(file_header(["Symmetry function histogram."],
+                                         size, name, info))
+                for i in range(nbins):
+                    f.write("%16.8E %16.8E %16.8E\n"
+                            % (ed[i], ed[i + 1], counts[k][s_, i]))
+            nfiles += 1
+    return nfiles
 
 
 def main(argv=None):
     argv = sys.argv if argv is None else argv
     if len(argv) != 2:
         print("USAGE: %s <nbins>\n"
-              "       <nbins> ... Number of symmetry function histogram bins "
-              "(histograms are not written).\n"
+              "       <nbins> ... Number of symmetry function histogram bins.\n"
               "       Execute in directory with these NNP files present:\n"
               "       - input.data (structure file)\n"
               "       - input.nn (NNP settings)" % argv[0])
@@ -145,8 +315,10 @@ def main(argv=None):
                   nio.read_structures("input.data", model.element_index)]
     stats = statistics("input.nn", structures)
     write_scaling_file("scaling.data", stats)
-    print("Wrote scaling.data: %d structures, %s SFs per element."
-          % (len(structures), [len(s) for s in stats]))
+    nfiles = write_histograms("input.nn", structures, int(argv[1]))
+    print("Wrote scaling.data: %d structures, %s SFs per element; %d "
+          "histogram files." % (len(structures), [len(s) for s in stats],
+                                nfiles))
     return 0
 
 

@@ -86,3 +86,66 @@ def test_command_line_tool(tmp_path):
     ref = np.concatenate(_golden())
     assert rows.shape == (len(ref), 6)
     assert _rel(rows[:, 2:5], ref[:, :3]) <= 1e-13
+
+
+def test_histograms(tmp_path):
+    """The SF histogram files of nnp-scaling
+    (Dataset::writeSymmetryFunctionHistograms) for the example data set.
+
+    - One file per element and SF with min < max.
+    - The bin counts sum to the element's atom count: the last bin reaches
+      one bin width beyond the maximum.
+    - The counts equal numpy's binning of the oracle's raw G on the same
+      edges, up to values within rounding of an edge (at most 2 per SF).
+    - The #SFINFO lines name each SF's settings line, type and centre
+      element.
+    The #SFINFO layout and the edge formula follow the reference's code.
+    No histogram file of the reference is at hand to compare the text
+    with."""
+    from tests.oracle_lib import OModel, OStruct
+    settings = os.path.join(GOLD, "input.nn")
+    sts = _structures()
+    nbins = 40
+    nfiles = SC.write_histograms(settings, sts, nbins, directory=str(tmp_path))
+    m = engine.Model.load(settings, None, "")
+    om = OModel.from_dir(GOLD, scaling=False)
+    used, _, _, conv = m.normalization()
+    G = [[] for _ in range(m.num_elements)]
+    for r, e, b in sts:
+        if used:
+            r, b = nio.to_normalized(r, b, conv)
+        s = OStruct(om, r, e, b)
+        s.neighbor_list(om.max_cutoff, cells=True)
+        s.symmetry_functions(False)
+        g = s.G()
+        for k in range(m.num_elements):
+            G[k].append(g[np.asarray(e) == k][:, :m.num_sf(k)])
+    lines = open(settings).read().split("\n")
+    seen = 0
+    for k in range(m.num_elements):
+        gk = np.concatenate(G[k])
+        z = m.atomic_number(k)
+        table = m.sf_table(k)
+        for s_ in range(m.num_sf(k)):
+            path = tmp_path / ("sf.%03d.%04d.histo" % (z, s_ + 1))
+            lo, hi = gk[:, s_].min(), gk[:, s_].max()
+            if not lo < hi:
+                assert not path.exists()
+                continue
+            seen += 1
+            txt = path.read_text().split("\n")
+            info = [l[8:] for l in txt if l.startswith("#SFINFO ")]
+            assert info[0].split()[0] == "lineNumber"
+            ln = int(info[0].split()[1]) - 1
+            assert lines[ln].split()[0] == "symfunction_short"
+            assert int(info[2].split()[1]) == int(table[s_, 0])
+            assert info[3].split()[1] == lines[ln].split()[1]
+            rows = np.array([[float(x) for x in l.split()] for l in txt
+                             if l.strip() and not l.startswith("#")])
+            assert rows.shape == (nbins, 3)
+            assert rows[:, 2].sum() == len(gk)
+            edges = np.append(rows[:, 0], rows[-1, 1])
+            ref, _ = np.histogram(gk[:, s_], bins=edges)
+            assert np.abs(rows[:, 2] - ref).sum() <= 2
+    say("histogram files", nfiles, "checked", seen)
+    assert nfiles == seen > 0
