@@ -808,7 +808,10 @@ void computeSF(hdnnp_ctx* c, bool raw)
     a.fp32 = fp32;
     hipEvent_t ev = nullptr;
     c->begin(K_SYMFUNC, ev);
-    PlanKernels const pk = raw ? PlanKernels() : c->planFor(fp32);
+    // Raw (unscaled) evaluations use the same kernels as scaled ones, so
+    // that statistics and later evaluations of a data set see the same G
+    // bit for bit.
+    PlanKernels const pk = c->planFor(fp32);
     if (pk.sf && c->n > 0)
     {
         void* params[] = {&a};

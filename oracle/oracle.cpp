@@ -1913,7 +1913,18 @@ orc_model* loadModel(string const& settingsFile, string const& scalingFile,
                     throw std::runtime_error("ERROR: NN topology unset.\n");
             nn.normalize = normalizeNeurons;
             // Mode.cpp:2270-2289 + NeuralNetwork.cpp:147-169 setConnections
-            vector<double> wts = readColumn0(formatZ(weightsFmt, m->el[i].Z));
+            // An empty weights format gives zero weights (the SF-only
+            // tools: statistics and histograms need no network).
+            vector<double> wts;
+            if (!weightsFmt.empty())
+                wts = readColumn0(formatZ(weightsFmt, m->el[i].Z));
+            else
+            {
+                size_t nw = 0;
+                for (int l = 1; l < nn.numLayers; ++l)
+                    nw += (size_t)nn.n[l] * (nn.n[l - 1] + 1);
+                wts.assign(nw, 0.0);
+            }
             nn.w.assign(nn.numLayers, {});
             nn.b.assign(nn.numLayers, {});
             size_t count = 0;

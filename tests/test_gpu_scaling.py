@@ -108,7 +108,7 @@ def test_histograms(tmp_path):
     nbins = 40
     nfiles = SC.write_histograms(settings, sts, nbins, directory=str(tmp_path))
     m = engine.Model.load(settings, None, "")
-    om = OModel.from_dir(GOLD, scaling=False)
+    om = OModel(settings, "", "")
     used, _, _, conv = m.normalization()
     G = [[] for _ in range(m.num_elements)]
     for r, e, b in sts:
@@ -144,7 +144,10 @@ def test_histograms(tmp_path):
                              if l.strip() and not l.startswith("#")])
             assert rows.shape == (nbins, 3)
             assert rows[:, 2].sum() == len(gk)
-            edges = np.append(rows[:, 0], rows[-1, 1])
+            # edges at full precision from the oracle's min and max (the
+            # file prints them to 9 digits)
+            edges = SC.histogram_edges(lo, hi, nbins)
+            assert np.allclose(edges[:-1], rows[:, 0], rtol=1e-8, atol=1e-12)
             ref, _ = np.histogram(gk[:, s_], bins=edges)
             assert np.abs(rows[:, 2] - ref).sum() <= 2
     say("histogram files", nfiles, "checked", seen)
