@@ -76,7 +76,7 @@ const char* hdnnp_last_error(void);
  * number, e.g. "weights.%03zu.data".  Every symmetry-function type of the
  * reference (2, 3, 9; weighted 12, 13; compact 20-25) and all cutoff,
  * scaling and activation types are supported; unknown SF types, more than
- * 32 compact SFs in one group and type-13 SFs with rs < 0 return
+ * 96 compact SFs in one group and type-13 SFs with rs < 0 return
  * HDNNP_E_UNSUPPORTED. */
 int  hdnnp_model_load(const char* settings_file, const char* scaling_file,
                       const char* weights_fmt, hdnnp_model** out);
@@ -305,9 +305,7 @@ int64_t hdnnp_md_partner_misses(hdnnp_ctx* c);
  * for all n_inner substeps instead.  Both contexts hold the same periodic
  * structure; masses, skin, ftoa, v and use_graph as for hdnnp_md_init.
  * The state lives in `sur` (hdnnp_md_get_state(sur, ...)).  No counterpart
- * in libnnp.  Currently switched off: hdnnp_mts_init returns
- * HDNNP_E_UNSUPPORTED (device runs of it ended in a memory fault whose
- * cause has not been found). */
+ * in libnnp. */
 int hdnnp_mts_init(hdnnp_ctx* full, hdnnp_ctx* sur, const double* masses,
                    double dt_outer, int n_inner, double skin, double ftoa,
                    const double* v, double rmse_gate, int use_graph);
@@ -370,6 +368,16 @@ int hdnnp_energy_weight_jacobian(hdnnp_ctx* c, double* J, int file_order);
  * [Gmin, Gmax] +- 1000 eps before scaling (Element.cpp:462-541).
  * per_atom (n, may be NULL) receives the count per atom. */
 int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom);
+/* The warnings themselves, one record per (atom, SF) in the order
+ * Element::updateSymmetryFunctionStatistics meets them (atoms in order, SFs
+ * in NN-input order; Element.cpp:479-539 called from Mode.cpp:1625-1633):
+ * atom index, SF index within the atom's element (SymFnc::getIndex), the
+ * unscaled value (SymFnc::unscale) and the SF's Gmin and Gmax.  Up to max
+ * records are written (any array may be NULL); returns the total number. */
+int64_t hdnnp_extrapolation_warning_list(hdnnp_ctx* c, int64_t max,
+                                         int32_t* atom, int32_t* sf,
+                                         double* value, double* gmin,
+                                         double* gmax);
 
 /* Symmetry-function statistics over the loaded structure as nnp-scaling
  * writes them to scaling.data (Dataset.cpp:974-1100): SFs evaluated without

@@ -163,7 +163,13 @@ struct hdnnp_ctx
     // caller's index of every centre and of every entry's neighbour, used
     // to scatter the forces.
     bool externalNL = false;
-    std::vector<int32_t> extCentre, extNbr;
+    // Per caller atom b < extTargets, the contributions it receives in the
+    // order of InterfaceLammps::getForces (k_scatter_ext): offsets and
+    // sources (entry q >= 0, or -(centre + 1) for a centre's self force).
+    int64_t extTargets = 0;
+    DBuf<int64_t> dScatOff;
+    DBuf<int> dScatSrc;
+    DBuf<double> dScatF;
     DBuf<int> dCellOfAtom, dRank, dCellCount, dCellStart, dCellAtoms, dCount,
         dOff, dStats, dBlockSums, dTotal;
     DBuf<uint64_t> dKeys;
@@ -356,6 +362,9 @@ struct hdnnp_ctx
         T.cgroup = dCompGroup.p;
         T.cmember = dCompMember.p;
         T.ncgroup = (int)flat.cgroup.size();
+        T.maxCompMembers = 1;
+        for (auto const& g : flat.cgroup)
+            T.maxCompMembers = std::max(T.maxCompMembers, g.nmem);
         ld = std::max(1, T.maxnsf);
         maxMembers = 0;
         maxCfg = 1;
@@ -788,7 +797,7 @@ void computeSF(hdnnp_ctx* c, bool raw)
     int const cap = std::max(c->maxNeigh, 1);
     int const fp32 = (!raw && c->precision != HDNNP_PRECISION_FP64) ? 1 : 0;
     if (sf_forward_lds_bytes(cap, fp32) > 65536
-        || (c->T.ncgroup > 0 && sf_compact_lds_bytes(cap) > 65536))
+        || (c->T.ncgroup > 0 && sf_compact_lds_bytes(c->T, cap) > 65536))
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the symmetry-"
                     "function kernel's LDS staging.\n");
@@ -907,7 +916,7 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
     int const fp32 = c->precision != HDNNP_PRECISION_FP64 ? 1 : 0;
     size_t const shm = force_backward_lds_bytes(cap, fp32);
     if (shm > 65536
-        || (c->T.ncgroup > 0 && force_compact_lds_bytes(cap) > 65536))
+        || (c->T.ncgroup > 0 && force_compact_lds_bytes(c->T, cap) > 65536))
         throw Error(HDNNP_E_CAPACITY,
                     "ERROR: Too many neighbours per atom for the force "
                     "kernel's LDS staging.\n");
@@ -1377,27 +1386,44 @@ int hdnnp_ctx_update_model(hdnnp_ctx* c)
     });
 }
 
-// Atoms, elements and per-element atom lists of a structure or a batch.
-static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* elem)
+// Forgets the loaded structure or caller-given list: every compute call
+// then fails with "No structure loaded" until a new load succeeds.  Loads
+// call this first, so a load rejected by its checks leaves no stale sizes
+// behind for the kernels.
+static void dropStructure(hdnnp_ctx* c)
 {
     mdEnd(c);
+    c->n = 0;
+    c->M = 0;
+    c->Mvalid = false;
+    c->externalNL = false;
+    c->batched = false;
+    c->nstruct = 0;
+    c->hostPosValid = false;
+    c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+}
+
+// Atoms, elements and per-element atom lists of a structure or a batch.
+// Every input is checked before any state of the context changes.
+static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* elem)
+{
+    dropStructure(c);
     if (n <= 0 || !r || !elem)
         throw Error(HDNNP_E_INVALID, "ERROR: Empty structure.\n");
     if (n > 200000000)
         throw Error(HDNNP_E_CAPACITY, "ERROR: Too many atoms.\n");
-    c->setDevice();
     int const ne = c->T.ne;
-    c->n = n;
-    c->hElem.assign(elem, elem + n);
-    c->hAtomsOfElem.assign(ne, {});
+    std::vector<std::vector<int>> atomsOfElem(ne);
     for (int64_t i = 0; i < n; ++i)
     {
         if (elem[i] < 0 || elem[i] >= ne)
             throw Error(HDNNP_E_INVALID,
                         "ERROR: Element index out of range.\n");
-        c->hAtomsOfElem[elem[i]].push_back((int)i);
+        atomsOfElem[elem[i]].push_back((int)i);
     }
-    c->externalNL = false;
+    c->setDevice();
+    c->hElem.assign(elem, elem + n);
+    c->hAtomsOfElem = std::move(atomsOfElem);
     c->dR.ensure(3 * n);
     c->dElemAtom.ensure(n);
     hipCheck(hipMemcpy(c->dR.p, r, 3 * n * sizeof(double),
@@ -1405,7 +1431,6 @@ static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* e
     hipCheck(hipMemcpy(c->dElemAtom.p, elem, n * sizeof(int32_t),
                        hipMemcpyHostToDevice), "elements");
     c->hR.assign(r, r + 3 * n);
-    c->hostPosValid = true;
     c->dAtomsOfElem.resize(ne);
     for (int e = 0; e < ne; ++e)
     {
@@ -1416,7 +1441,8 @@ static void loadAtoms(hdnnp_ctx* c, int64_t n, const double* r, const int32_t* e
                                c->hAtomsOfElem[e].size() * sizeof(int),
                                hipMemcpyHostToDevice), "element lists");
     }
-    c->haveNL = c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    c->hostPosValid = true;
+    c->n = n;
 }
 
 int hdnnp_set_structure(hdnnp_ctx* c, int64_t n, const double* r,
@@ -1442,6 +1468,7 @@ int hdnnp_set_structures(hdnnp_ctx* c, int64_t nstruct,
                          const int32_t* periodic)
 {
     return guarded([&]() {
+        dropStructure(c);
         if (nstruct <= 0 || !atom_offsets)
             throw Error(HDNNP_E_INVALID, "ERROR: Empty batch.\n");
         if (atom_offsets[0] != 0)
@@ -1794,6 +1821,10 @@ int hdnnp_md_init(hdnnp_ctx* c, const double* masses, double dt, double skin,
         m.hFlags[0] = m.hFlags[1] = m.hFlags[2] = m.hFlags[3] = 0;
         hipStream_t s = c->stream;
         hipCheck(hipMemsetAsync(m.flags.p, 0, 4 * sizeof(int), s), "flags");
+        // The step record's counter indexes epot / ekin: it must start at
+        // zero also for runs that never call hdnnp_md_run (the substeps of
+        // hdnnp_mts_run record through it too).
+        hipCheck(hipMemsetAsync(m.counter.p, 0, sizeof(int), s), "counter");
         if (v)
             hipCheck(hipMemcpyAsync(m.v.p, v, 3 * (size_t)n * sizeof(double),
                                     hipMemcpyHostToDevice, s), "velocities");
@@ -1868,8 +1899,6 @@ int64_t hdnnp_md_partner_misses(hdnnp_ctx* c)
 }
 
 // ------------------------------------------------------- MTS (r-RESPA) --
-
-constexpr int kMtsEnabled = 0;
 
 // Positions r_src (device, n x 3) into c, displacement check and filter of
 // its skin list, a rebuild if needed (or forced), then the forces.
@@ -1992,14 +2021,6 @@ int hdnnp_mts_init(hdnnp_ctx* full, hdnnp_ctx* sur, const double* masses,
                    const double* v, double rmse_gate, int use_graph)
 {
     return guarded([&]() {
-        // Switched off: runs of this integrator on the device have ended in
-        // a memory fault whose cause has not been found.  The code below is
-        // kept for the fix; the entry point refuses until then.
-        if (kMtsEnabled == 0)
-            throw Error(HDNNP_E_UNSUPPORTED,
-                        "ERROR: The multiple-time-step integrator is switched "
-                        "off in this build (an unresolved device memory "
-                        "fault).\n");
         if (!full || !sur || full == sur)
             throw Error(HDNNP_E_INVALID, "ERROR: MTS needs two contexts.\n");
         if (full->n != sur->n || full->device != sur->device)
@@ -2156,7 +2177,7 @@ int hdnnp_train_gradient(hdnnp_ctx* c, const double* f_ref, double e_ref,
         hipCheck(launch_train_u(A, s), "train u");
         {
             int const cap = std::max(c->maxNeigh, 1);
-            if (c->T.ncgroup > 0 && train_u_compact_lds_bytes(cap) > 65536)
+            if (c->T.ncgroup > 0 && train_u_compact_lds_bytes(c->T, cap) > 65536)
                 throw Error(HDNNP_E_CAPACITY,
                             "ERROR: Too many neighbours per atom for the "
                             "compact training kernel's LDS staging.\n");
@@ -2398,8 +2419,13 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
                             const int32_t* nbr_elem, const double* dr)
 {
     return guarded([&]() {
-        if (!c || ncentre <= 0 || !centre_elem || !offsets)
+        if (!c)
+            throw Error(HDNNP_E_INVALID, "ERROR: No context.\n");
+        dropStructure(c);
+        if (ncentre <= 0 || !centre_elem || !offsets)
             throw Error(HDNNP_E_INVALID, "ERROR: Empty neighbour list.\n");
+        if (offsets[0] != 0)
+            throw Error(HDNNP_E_INVALID, "ERROR: offsets[0] must be 0.\n");
         int64_t const M = offsets[ncentre];
         if (M < 0 || M > (int64_t)2000000000 || ncentre > 200000000)
             throw Error(HDNNP_E_CAPACITY, "ERROR: Neighbour list too large.\n");
@@ -2412,7 +2438,7 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
         std::vector<int32_t> elem((size_t)n + (size_t)M);
         std::vector<int> off(n + 1), cnt(n), nbrJ((size_t)M);
         std::vector<double4> geo((size_t)M);
-        c->hAtomsOfElem.assign(ne, {});
+        std::vector<std::vector<int>> atomsOfElem(ne);
         int maxc = 0;
         for (int i = 0; i < n; ++i)
         {
@@ -2420,7 +2446,7 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
                 throw Error(HDNNP_E_INVALID,
                             "ERROR: Element index out of range.\n");
             elem[i] = centre_elem[i];
-            c->hAtomsOfElem[centre_elem[i]].push_back(i);
+            atomsOfElem[centre_elem[i]].push_back(i);
             if (offsets[i + 1] < offsets[i])
                 throw Error(HDNNP_E_INVALID, "ERROR: Offsets not ascending.\n");
             off[i] = (int)offsets[i];
@@ -2428,6 +2454,24 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
             maxc = std::max(maxc, cnt[i]);
         }
         off[n] = (int)M;
+        // Caller indices of centres and neighbours (targets of the forces).
+        std::vector<int> ctr(n);
+        int64_t nt = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            ctr[i] = centre_index ? centre_index[i] : i;
+            if (ctr[i] < 0)
+                throw Error(HDNNP_E_INVALID, "ERROR: Centre index out of "
+                                             "range.\n");
+            nt = std::max<int64_t>(nt, (int64_t)ctr[i] + 1);
+        }
+        for (int64_t q = 0; q < M; ++q)
+        {
+            if (nbr_index[q] < 0)
+                throw Error(HDNNP_E_INVALID, "ERROR: Neighbour index out of "
+                                             "range.\n");
+            nt = std::max<int64_t>(nt, (int64_t)nbr_index[q] + 1);
+        }
         for (int64_t q = 0; q < M; ++q)
         {
             if (nbr_elem[q] < 0 || nbr_elem[q] >= ne)
@@ -2438,7 +2482,7 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
             double const x = dr[3 * q], y = dr[3 * q + 1], z = dr[3 * q + 2];
             geo[q] = make_double4(x, y, z, std::sqrt(x * x + y * y + z * z));
         }
-        c->n = n;
+        c->hAtomsOfElem = std::move(atomsOfElem);
         c->hElem.assign(centre_elem, centre_elem + n);
         c->periodic = false;
         c->pbc[0] = c->pbc[1] = c->pbc[2] = 0;
@@ -2460,15 +2504,34 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
         c->Mvalid = true;
         c->slabLayout = false;
         c->maxNeigh = maxc;
-        c->extCentre.resize(n);
-        for (int i = 0; i < n; ++i)
-            c->extCentre[i] = centre_index ? centre_index[i] : i;
-        c->extNbr.assign(nbr_index, nbr_index + M);
-        mdEnd(c);
+        {
+            // Contributions per caller atom, in the reference's loop order
+            // (a counting sort that keeps that order).
+            std::vector<int64_t> so((size_t)nt + 1, 0);
+            for (int i = 0; i < n; ++i)
+            {
+                ++so[(size_t)ctr[i] + 1];
+                for (int64_t q = offsets[i]; q < offsets[i + 1]; ++q)
+                    ++so[(size_t)nbr_index[q] + 1];
+            }
+            for (int64_t b = 0; b < nt; ++b) so[b + 1] += so[b];
+            std::vector<int64_t> pos(so.begin(), so.end() - 1);
+            std::vector<int> src((size_t)(n + M));
+            for (int i = 0; i < n; ++i)
+            {
+                src[(size_t)pos[ctr[i]]++] = -(i + 1);
+                for (int64_t q = offsets[i]; q < offsets[i + 1]; ++q)
+                    src[(size_t)pos[nbr_index[q]]++] = (int)q;
+            }
+            up(c->dScatOff, so);
+            up(c->dScatSrc, src);
+        }
+        c->extTargets = nt;
         c->externalNL = true;
         c->batched = false;
         c->nstruct = 1;
         c->hostPosValid = false;
+        c->n = n;
         c->haveNL = true;
         c->haveSF = c->haveNN = c->haveNNDeriv = false;
     });
@@ -2482,34 +2545,23 @@ int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f)
                                          "list.\n");
         if (!c->dCself.p || !c->dCslot.p)
             throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
+        if (nout < c->extTargets)
+            throw Error(HDNNP_E_INVALID, "ERROR: A centre or neighbour index "
+                                         "is beyond the force array.\n");
         c->setDevice();
-        size_t const n = (size_t)c->n, M = (size_t)c->M;
-        std::vector<double> self(3 * n), pair(3 * M);
-        copyOut(c, self.data(), c->dCself.p, 3 * n);
-        if (M) copyOut(c, pair.data(), c->dCslot.p, 3 * M);
+        // InterfaceLammps::getForces (InterfaceLammps.cpp:546-581) on the
+        // device: only the rows that receive contributions travel, in and
+        // out.  Stored sums carry +dE/dG dG/dr, hence the subtractions.
+        size_t const nt = (size_t)c->extTargets;
+        if (nt == 0) return;
+        c->dScatF.ensure(3 * nt);
+        hipCheck(hipMemcpyAsync(c->dScatF.p, f, 3 * nt * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream), "forces");
+        hipCheck(launch_scatter_ext((int)nt, c->dScatOff.p, c->dScatSrc.p,
+                                    c->dCslot.p, c->dCself.p, c->dScatF.p,
+                                    c->stream), "force scatter");
+        copyOut(c, f, c->dScatF.p, 3 * nt);
         c->sync();
-        // InterfaceLammps::getForces (InterfaceLammps.cpp:546-581): each
-        // centre's self force, then its pair forces onto the neighbours,
-        // in list order.  Stored sums carry +dE/dG dG/dr.
-        std::vector<int> off(n + 1);
-        copyOut(c, off.data(), c->dOff.p, n + 1);
-        c->sync();
-        for (size_t i = 0; i < n; ++i)
-        {
-            int64_t const a = c->extCentre[i];
-            if (a < 0 || a >= nout)
-                throw Error(HDNNP_E_INVALID, "ERROR: Centre index out of "
-                                             "range.\n");
-            for (int k = 0; k < 3; ++k) f[3 * a + k] -= self[3 * i + k];
-            for (int q = off[i]; q < off[i + 1]; ++q)
-            {
-                int64_t const b = c->extNbr[q];
-                if (b < 0 || b >= nout)
-                    throw Error(HDNNP_E_INVALID, "ERROR: Neighbour index out "
-                                                 "of range.\n");
-                for (int k = 0; k < 3; ++k) f[3 * b + k] -= pair[3 * q + k];
-            }
-        }
     });
 }
 
@@ -2557,6 +2609,64 @@ int64_t hdnnp_extrapolation_warnings(hdnnp_ctx* c, int32_t* per_atom)
         {
             total += h[i];
             if (per_atom) per_atom[i] = h[i];
+        }
+    });
+    return rc == HDNNP_OK ? total : rc;
+}
+
+int64_t hdnnp_extrapolation_warning_list(hdnnp_ctx* c, int64_t max,
+                                         int32_t* atom, int32_t* sf,
+                                         double* value, double* gmin,
+                                         double* gmax)
+{
+    int64_t total = -1;
+    int rc = guarded([&]() {
+        if (!c->haveSF)
+            throw Error(HDNNP_E_INVALID, "ERROR: No symmetry functions.\n");
+        c->setDevice();
+        int const n = (int)c->n;
+        c->dEw.ensure(n);
+        hipCheck(launch_ew(c->T, c->dG.p, c->ld, c->dElemAtom.p, n,
+                           c->dEw.p, c->stream), "extrapolation warnings");
+        std::vector<int32_t> cnt(n);
+        copyOut(c, cnt.data(), c->dEw.p, (size_t)n);
+        c->sync();
+        // Atoms with warnings (few, normally): their scaled G rows come back
+        // and are unscaled and tested here with the kernel's arithmetic
+        // (SymFnc::unscale, SymFnc.cpp:197-223), in the reference's order.
+        double const eps = 1000.0 * 2.220446049250313e-16;
+        std::vector<double> row((size_t)c->ld);
+        total = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            if (cnt[i] == 0) continue;
+            copyOut(c, row.data(), c->dG.p + (size_t)i * c->ld, (size_t)c->ld);
+            c->sync();
+            ElemDesc const& ed = c->flat.elem[c->hElem[i]];
+            for (int k = 0; k < ed.nsf; ++k)
+            {
+                ScaleDesc const& q = c->flat.scale[ed.sf0 + k];
+                double const g = row[k];
+                double v;
+                switch (q.type)
+                {
+                case ST_SCALE: v = (g - q.smin) / q.sfac + q.gmin; break;
+                case ST_CENTER: v = g + q.gmean; break;
+                case ST_SCALECENTER:
+                case ST_SCALESIGMA: v = (g - q.smin) / q.sfac + q.gmean; break;
+                default: v = g; break;
+                }
+                if (!(v + eps < q.gmin || v - eps > q.gmax)) continue;
+                if (total < max)
+                {
+                    if (atom) atom[total] = i;
+                    if (sf) sf[total] = k;
+                    if (value) value[total] = v;
+                    if (gmin) gmin[total] = q.gmin;
+                    if (gmax) gmax[total] = q.gmax;
+                }
+                ++total;
+            }
         }
     });
     return rc == HDNNP_OK ? total : rc;

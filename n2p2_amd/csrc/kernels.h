@@ -106,7 +106,7 @@ hipError_t launch_train_u(TrainArgs const& A, hipStream_t s);
 // The compact SF types' part of u (after launch_train_u); cap = the largest
 // neighbour count.
 hipError_t launch_train_u_compact(TrainArgs const& A, int cap, hipStream_t s);
-size_t train_u_compact_lds_bytes(int cap);
+size_t train_u_compact_lds_bytes(ModelTables const& T, int cap);
 hipError_t launch_train_grad(TrainArgs const& A, ElemDesc const& ed, int e,
                              const int* atoms, int na, double* part, int nw,
                              double* grad, hipStream_t s);
@@ -197,12 +197,17 @@ hipError_t launch_force_backward(ForceArgs const& a, hipStream_t s);
 // after launch_force_backward.
 hipError_t launch_sf_compact(SFArgs const& a, hipStream_t s);
 hipError_t launch_force_compact(ForceArgs const& a, hipStream_t s);
-size_t sf_compact_lds_bytes(int cap);
-size_t force_compact_lds_bytes(int cap);
+size_t sf_compact_lds_bytes(ModelTables const& T, int cap);
+size_t force_compact_lds_bytes(ModelTables const& T, int cap);
 size_t force_backward_lds_bytes(int cap, int fp32);
 
 // Entries of atom i: [off[i], off[i] + cnt[i]).  order: the atoms sorted
 // by cell (processing order only).
+// Forces of a caller-given list onto the caller's atoms, in the order of
+// InterfaceLammps::getForces (k_scatter_ext).
+hipError_t launch_scatter_ext(int nt, const int64_t* srcOff, const int* src,
+                              const double* cslot, const double* cself,
+                              double* f, hipStream_t s);
 hipError_t launch_force_gather(int n, const int* off, const int* cnt,
                                const int* nbrJ, const int* partner,
                                const double* cslot, const double* cself,

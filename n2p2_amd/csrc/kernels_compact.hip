@@ -25,8 +25,6 @@ namespace hdnnp {
 
 namespace {
 
-constexpr int kCM = kMaxCompMembers;
-
 // CompactFunction::fdf (CompactFunction.h:153-160): a = (x - center) scale,
 // f = core(|a|), f' = core'(|a|) copysign(scale, a).  The core polynomials
 // and the exponential are the cutoff cores on [0, 1) (a configuration with
@@ -216,7 +214,15 @@ __host__ __device__ constexpr int acc_doubles(int cap)
     return (3 * cap + 3) & ~3;
 }
 
-// LDS: per-lane member sums (64 x kCM), then the staged neighbours.
+// Per-lane member-sum rows (64 x ms doubles), rounded up to whole 32-byte
+// lines so that the staged double4 records that follow stay aligned.
+__host__ __device__ constexpr int acc_rows(int ms)
+{
+    return (64 * ms + 3) & ~3;
+}
+
+// LDS: per-lane member sums (64 x ms, ms = largest compact group), then the
+// staged neighbours.
 __global__ __launch_bounds__(64) void k_sf_compact(SFArgs a)
 {
     extern __shared__ __align__(16) double sm[];
@@ -225,12 +231,13 @@ __global__ __launch_bounds__(64) void k_sf_compact(SFArgs a)
     ElemDesc const& ed = T.elem[a.elem[i]];
     if (ed.ncg == 0) return;
     int const lane = threadIdx.x;
+    int const ms = T.maxCompMembers;
     double* ACC = sm;
-    double4* P = (double4*)(ACC + 64 * kCM);
+    double4* P = (double4*)(ACC + acc_rows(ms));
     int* E = (int*)(P + a.cap);
     int const s0 = a.off[i], n = a.cnt[i];
     stage_list(a.geo, a.nbrJ, a.elem, s0, n, P, E);
-    double* row = ACC + lane * kCM;
+    double* row = ACC + lane * ms;
     double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
     for (int cg = ed.cg0; cg < ed.cg0 + ed.ncg; ++cg)
     {
@@ -276,19 +283,19 @@ __global__ __launch_bounds__(64) void k_sf_compact(SFArgs a)
             }
         }
         __syncthreads();
-        // Member totals: lane l sums the 64 lane rows in lane order.
-        if (lane < g.nmem)
+        // Member totals: member l sums the 64 lane rows in lane order.
+        for (int l = lane; l < g.nmem; l += 64)
         {
             double v = 0.0;
-            for (int q = 0; q < 64; ++q) v += ACC[q * kCM + lane];
-            int const sf = mem[lane].sf;
+            for (int q = 0; q < 64; ++q) v += ACC[q * ms + l];
+            int const sf = mem[l].sf;
             Gi[sf] = a.raw ? v : scale_sf(T.scale[ed.sf0 + sf], v);
         }
         __syncthreads();
     }
 }
 
-// LDS: member weights (kCM), per-slot accumulators (3 x cap), staged
+// LDS: member weights (ms), per-slot accumulators (3 x cap), staged
 // neighbours and their elements.
 __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
 {
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
     int const lane = threadIdx.x;
     int const cap = a.cap;
     double* W = sm;
-    double* ACC = W + kCM;
+    double* ACC = W + ((T.maxCompMembers + 3) & ~3);
     double4* P = (double4*)(ACC + acc_doubles(cap));
     int* E = (int*)(P + cap);
     int const s0 = a.off[i], n = a.cnt[i];
@@ -312,7 +319,7 @@ __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
         CompGroup const g = T.cgroup[cg];
         CompMember const* __restrict__ mem = T.cmember + g.mem0;
         bool const weighted = sf_weighted(g.type);
-        if (lane < g.nmem) W[lane] = dEdGi[mem[lane].sf];
+        for (int l = lane; l < g.nmem; l += 64) W[l] = dEdGi[mem[l].sf];
         __syncthreads();
         if (sf_radial(g.type))
         {
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(64) void k_force_compact(ForceArgs a)
 
 // Training: u_as = cF sum_i d_i . dG_s/dr_i for the compact SFs s of atom
 // a (the part of the force-loss tangent k_train_u forms for the other
-// types), d = F - Fref.  LDS: per-lane member sums (64 x kCM), staged
+// types), d = F - Fref.  LDS: per-lane member sums (64 x ms), staged
 // neighbours, their residuals d_j and elements.
 __global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
 {
@@ -407,8 +414,9 @@ __global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
     ElemDesc const& ed = T.elem[A.elem[a]];
     if (ed.ncg == 0) return;
     int const lane = threadIdx.x;
+    int const ms = T.maxCompMembers;
     double* ACC = sm;
-    double4* P = (double4*)(ACC + 64 * kCM);
+    double4* P = (double4*)(ACC + acc_rows(ms));
     double* D = (double*)(P + cap);
     int* E = (int*)(D + 3 * cap);
     int const s0 = A.off[a], n = A.cnt[a];
@@ -422,7 +430,7 @@ __global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
     double da[3];
     for (int q = 0; q < 3; ++q)
         da[q] = A.F[3 * (size_t)a + q] - A.Fref[3 * (size_t)a + q];
-    double* row = ACC + lane * kCM;
+    double* row = ACC + lane * ms;
     double* __restrict__ U = A.U + (size_t)a * A.ldG;
     for (int cg = ed.cg0; cg < ed.cg0 + ed.ncg; ++cg)
     {
@@ -483,11 +491,11 @@ __global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
             }
         }
         __syncthreads();
-        if (lane < g.nmem)
+        for (int l = lane; l < g.nmem; l += 64)
         {
             double v = 0.0;
-            for (int q = 0; q < 64; ++q) v += ACC[q * kCM + lane];
-            U[mem[lane].sf] = A.cF * v;
+            for (int q = 0; q < 64; ++q) v += ACC[q * ms + l];
+            U[mem[l].sf] = A.cF * v;
         }
         __syncthreads();
     }
@@ -495,28 +503,30 @@ __global__ __launch_bounds__(64) void k_train_u_compact(TrainArgs A, int cap)
 
 }  // namespace
 
-size_t train_u_compact_lds_bytes(int cap)
+size_t train_u_compact_lds_bytes(ModelTables const& T, int cap)
 {
-    return (size_t)64 * kCM * sizeof(double)
+    return (size_t)acc_rows(T.maxCompMembers) * sizeof(double)
          + (size_t)cap * (sizeof(double4) + 3 * sizeof(double) + sizeof(int));
 }
 
 hipError_t launch_train_u_compact(TrainArgs const& A, int cap, hipStream_t s)
 {
     if (A.n <= 0 || A.T.ncgroup == 0) return hipSuccess;
-    k_train_u_compact<<<A.n, 64, train_u_compact_lds_bytes(cap), s>>>(A, cap);
+    k_train_u_compact<<<A.n, 64, train_u_compact_lds_bytes(A.T, cap), s>>>(
+        A, cap);
     return hipGetLastError();
 }
 
-size_t sf_compact_lds_bytes(int cap)
+size_t sf_compact_lds_bytes(ModelTables const& T, int cap)
 {
-    return (size_t)64 * kCM * sizeof(double) + (size_t)cap * sizeof(double4)
+    return (size_t)acc_rows(T.maxCompMembers) * sizeof(double)
+         + (size_t)cap * sizeof(double4)
          + (size_t)cap * sizeof(int);
 }
 
-size_t force_compact_lds_bytes(int cap)
+size_t force_compact_lds_bytes(ModelTables const& T, int cap)
 {
-    return (size_t)kCM * sizeof(double)
+    return (size_t)((T.maxCompMembers + 3) & ~3) * sizeof(double)
          + (size_t)acc_doubles(cap) * sizeof(double)
          + (size_t)cap * sizeof(double4) + (size_t)cap * sizeof(int);
 }
@@ -524,14 +534,14 @@ size_t force_compact_lds_bytes(int cap)
 hipError_t launch_sf_compact(SFArgs const& a, hipStream_t s)
 {
     if (a.n <= 0 || a.T.ncgroup == 0) return hipSuccess;
-    k_sf_compact<<<a.n, 64, sf_compact_lds_bytes(a.cap), s>>>(a);
+    k_sf_compact<<<a.n, 64, sf_compact_lds_bytes(a.T, a.cap), s>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launch_force_compact(ForceArgs const& a, hipStream_t s)
 {
     if (a.n <= 0 || a.T.ncgroup == 0) return hipSuccess;
-    k_force_compact<<<a.n, 64, force_compact_lds_bytes(a.cap), s>>>(a);
+    k_force_compact<<<a.n, 64, force_compact_lds_bytes(a.T, a.cap), s>>>(a);
     return hipGetLastError();
 }
 

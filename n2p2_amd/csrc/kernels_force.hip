@@ -63,6 +63,37 @@ __global__ __launch_bounds__(256) void k_gather(
     }
 }
 
+// InterfaceLammps::getForces (InterfaceLammps.cpp:533-621) for a
+// caller-given list: f[b] -= every contribution whose target is caller
+// atom b, in the reference's loop order (centres in order; per centre its
+// self force, then its entries in list order).  src[srcOff[b] ..
+// srcOff[b+1]) lists them in that order: src >= 0 is list entry q
+// (cslot[q]), src < 0 the self force of centre -src - 1 (cself).  One
+// thread per target: each f[b] is the same sequence of subtractions as the
+// reference's loop, so equal slot values give bitwise equal forces.
+__global__ __launch_bounds__(256) void k_scatter_ext(
+    int nt, const int64_t* __restrict__ srcOff, const int* __restrict__ src,
+    const double* __restrict__ cslot, const double* __restrict__ cself,
+    double* __restrict__ f)
+{
+    int const b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nt) return;
+    double fx = f[3 * (size_t)b], fy = f[3 * (size_t)b + 1],
+           fz = f[3 * (size_t)b + 2];
+    for (int64_t k = srcOff[b]; k < srcOff[b + 1]; ++k)
+    {
+        int const q = src[k];
+        double const* v = q >= 0 ? cslot + 3 * (size_t)q
+                                 : cself + 3 * (size_t)(-(int64_t)q - 1);
+        fx -= v[0];
+        fy -= v[1];
+        fz -= v[2];
+    }
+    f[3 * (size_t)b] = fx;
+    f[3 * (size_t)b + 1] = fy;
+    f[3 * (size_t)b + 2] = fz;
+}
+
 template <int MAXM, typename R>
 void launch_bwd(ForceArgs const& a, size_t shm, hipStream_t s)
 {
@@ -112,6 +143,16 @@ hipError_t launch_force_gather(int n, const int* off, const int* cnt,
     if (n <= 0) return hipSuccess;
     k_gather<<<(n + 3) / 4, 256, 0, s>>>(n, off, cnt, nbrJ, partner, cslot,
                                          cself, order, mode, nslots, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_ext(int nt, const int64_t* srcOff, const int* src,
+                              const double* cslot, const double* cself,
+                              double* f, hipStream_t s)
+{
+    if (nt <= 0) return hipSuccess;
+    k_scatter_ext<<<(nt + 255) / 256, 256, 0, s>>>(nt, srcOff, src, cslot,
+                                                  cself, f);
     return hipGetLastError();
 }
 

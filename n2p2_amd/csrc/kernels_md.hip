@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_md_mv2(
 }
 
 // Step record: ekin[k] = ekinScale * sum(part), epot[k] = *energy for
-// k = counter[0] (if below cap), then counter[0] += 1.  One block.
+// k = counter[0] (if 0 <= k < cap), then counter[0] += 1.  One block.
 __global__ __launch_bounds__(256) void k_md_record(
     int nb, const double* __restrict__ part, const double* __restrict__ energy,
     double ekinScale, int* __restrict__ counter, int capRec,
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_md_record(
     if (t == 0)
     {
         int const k = counter[0];
-        if (k < capRec)
+        if (k >= 0 && k < capRec)
         {
             ekin[k] = ekinScale * s[0];
             epot[k] = energy[0];

@@ -149,6 +149,9 @@ __global__ __launch_bounds__(256) void k_sf_stats(const double* __restrict__ G,
 
 // Element::updateSymmetryFunctionStatistics (Element.cpp:462-541): count
 // SFs whose unscaled value is outside [Gmin, Gmax] by more than 1000 eps.
+// Every SF is tested, unscaled ones (ST_NONE) too, as the reference does:
+// their Gmin / Gmax come from scaling.data, or are 0 without a scaling
+// file (Element::setScalingNone, Element.cpp:344-362).
 __global__ void k_ew(ModelTables T, const double* __restrict__ G, int ldG,
                      const int* __restrict__ elem, int n, int* __restrict__ ew)
 {
@@ -170,7 +173,7 @@ __global__ void k_ew(ModelTables T, const double* __restrict__ G, int ldG,
         case ST_SCALESIGMA: v = (g - s.smin) / s.sfac + s.gmean; break;
         default: v = g; break;
         }
-        if (s.type != ST_NONE && (v + eps < s.gmin || v - eps > s.gmax)) ++c;
+        if (v + eps < s.gmin || v - eps > s.gmax) ++c;
     }
     ew[i] = c;
 }

@@ -444,7 +444,7 @@ void Model::load(std::string const& settingsFile,
     for (auto const& e : el)
         for (auto const& g : e.groups)
             if (sf_compact(g.type) && (int)g.members.size() > kMaxCompMembers)
-                throw std::runtime_error("ERROR: More than 32 compact "
+                throw std::runtime_error("ERROR: More than 96 compact "
                                          "symmetry functions in one group "
                                          "are outside this engine.\n");
 

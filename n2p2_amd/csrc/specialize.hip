@@ -244,9 +244,26 @@ bool verbose()
     return v && *v && *v != '0';
 }
 
+// N2P2_AMD_RTC_DUMP=<directory>: the generated source of every compiled
+// specialisation is also written there (hdnnp_plan_f32.hip /
+// hdnnp_plan_f64.hip), for inspecting the code the kernels are built from.
+void dump_source(std::string const& src, int fp32)
+{
+    char const* d = std::getenv("N2P2_AMD_RTC_DUMP");
+    if (!d || !*d) return;
+    std::string const path = std::string(d) + "/hdnnp_plan_"
+                           + (fp32 ? "f32" : "f64") + ".hip";
+    if (FILE* f = std::fopen(path.c_str(), "w"))
+    {
+        std::fwrite(src.data(), 1, src.size(), f);
+        std::fclose(f);
+    }
+}
+
 std::shared_ptr<Module> compile(std::string const& src, int fp32, int device)
 {
     std::string const sfx = fp32 ? "_f32" : "_f64";
+    dump_source(src, fp32);
     auto t0 = std::chrono::steady_clock::now();
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "hdnnp_plan.hip", kRtcNumHeaders,

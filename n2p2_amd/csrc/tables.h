@@ -57,8 +57,10 @@ constexpr bool sf_weighted(int type)
 }
 
 // Members per compact group (SymGrpComp*: all compact SFs of one type and
-// centre / neighbour elements).
-constexpr int kMaxCompMembers = 32;
+// centre / neighbour elements).  The compact kernels keep one row of member
+// sums per lane in LDS, T.maxCompMembers doubles wide; 96 members fill
+// 48 KiB of the 64 KiB a workgroup may use.
+constexpr int kMaxCompMembers = 96;
 
 struct CutCfg
 {
@@ -193,6 +195,7 @@ struct ModelTables
     const CompGroup*  cgroup;       // compact groups (may be empty)
     const CompMember* cmember;
     int               ncgroup;      // compact groups over all elements
+    int               maxCompMembers;   // largest compact group (>= 1)
 };
 
 }  // namespace hdnnp

@@ -122,6 +122,9 @@ _SIGNATURES = [
     ("hdnnp_energy_weight_jacobian", ctypes.c_int,
      [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_extrapolation_warning_list", ctypes.c_int64,
+     [ctypes.c_void_p, ctypes.c_int64, _c_int32_p, _c_int32_p, _c_double_p,
+      _c_double_p, _c_double_p]),
     ("hdnnp_sf_statistics", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_sf_statistics_raw", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int, _c_double_p]),
     ("hdnnp_profile_enable", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -485,6 +488,10 @@ class Context:
         return e.value
 
     # --- results
+    def num_atoms(self):
+        """Atoms (or centres) of the loaded structure; 0 when none is."""
+        return int(_lib.hdnnp_num_atoms(self._h))
+
     def num_neighbor_entries(self):
         return _lib.hdnnp_num_neighbor_entries(self._h)
 
@@ -543,6 +550,22 @@ class Context:
         if tot < 0:
             _check(int(tot))
         return tot, per
+
+    def extrapolation_warning_list(self):
+        """Records (atom, sf, value, gmin, gmax) of every extrapolation
+        warning of the last SF evaluation, in the reference's order."""
+        tot = _lib.hdnnp_extrapolation_warning_list(self._h, 0, None, None,
+                                                    None, None, None)
+        if tot < 0:
+            _check(int(tot))
+        a = np.zeros(tot, dtype=np.int32)
+        k = np.zeros(tot, dtype=np.int32)
+        v, lo, hi = np.zeros(tot), np.zeros(tot), np.zeros(tot)
+        i32, f64 = ctypes.c_int32, ctypes.c_double
+        _lib.hdnnp_extrapolation_warning_list(self._h, tot, _ptr(a, i32),
+                                              _ptr(k, i32), _ptr(v, f64),
+                                              _ptr(lo, f64), _ptr(hi, f64))
+        return a, k, v, lo, hi
 
     def sf_statistics(self, e):
         out = np.zeros((self.model.num_sf(e), 4))

@@ -77,7 +77,6 @@ protected:
     std::map<std::size_t, int> mapElementToType;
     std::vector<bool> ignoreType;
     std::vector<std::size_t> indexMap;   // LAMMPS local index -> atom
-    std::size_t numExtrapolationWarningsLocal = 0;
     Structure structure;
 };
 

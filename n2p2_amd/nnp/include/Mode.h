@@ -129,9 +129,22 @@ public:
     Log log;
 
 protected:
+    // One SF's extrapolation warnings (SymFncStatistics::Container,
+    // SymFncStatistics.h:28-147, the warning part).
+    struct EWContainer
+    {
+        std::size_t countEW = 0;
+        std::size_t type = 0;
+        double Gmin = 0.0, Gmax = 0.0;
+        std::string element;
+        std::vector<std::size_t> indexStructureEW, indexAtomEW;
+        std::vector<double> valueEW;
+    };
+
     NNPType nnpType = NNPType::HDNNP_2G;
     bool normalize = false;
     bool checkExtrapolationWarnings = false;
+    bool collectExtrapolationWarnings = false;
     bool writeExtrapolationWarnings = false;
     bool stopOnExtrapolationWarnings = false;
     std::size_t numElements = 0;
@@ -142,6 +155,17 @@ protected:
     double convCharge = 1.0;
     std::vector<double> energyOffset;      // per element (atom_energy)
     std::vector<std::size_t> numSF;        // per element
+    std::vector<std::vector<std::size_t>> sfType;   // per element and SF
+    // Collected warnings per element, keyed by SF index (the reference's
+    // Element::statistics.data).
+    std::vector<std::map<std::size_t, EWContainer>> ewData;
+    // Element::updateSymmetryFunctionStatistics (Element.cpp:462-541) for
+    // every atom of the last SF evaluation on the engine, in the
+    // reference's order: counts, collects, writes to stderr and stops as
+    // the statistics flags say.  Returns the number of warnings.
+    std::size_t processExtrapolationWarnings(Structure const& structure);
+    // SymFncStatistics::getExtrapolationWarningLines of every element.
+    std::vector<std::string> extrapolationWarningLines() const;
 
 protected:
     // Creates the engine context on first use (after the weights are set).
@@ -160,7 +184,6 @@ private:
     hdnnp_ctx* ctx = nullptr;
     int device = -1;
     int precision = 0;
-    std::size_t numExtrapolationWarnings = 0;
     // Structure the context holds: atom count, elements and cell.
     mutable std::size_t upAtoms = 0;
     mutable std::vector<int32_t> upElem;

@@ -7,6 +7,7 @@
 #include "hdnnp.h"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <fstream>
 #include <limits>
@@ -229,11 +230,7 @@ void InterfaceLammps::process()
     std::vector<double> ea(n);
     check(hdnnp_get_atomic_energies(c, ea.data()));
     for (std::size_t i = 0; i < n; ++i) structure.atoms[i].energy = ea[i];
-    if (checkExtrapolationWarnings)
-    {
-        int64_t const w = hdnnp_extrapolation_warnings(c, nullptr);
-        if (w > 0) numExtrapolationWarningsLocal += (std::size_t)w;
-    }
+    if (checkExtrapolationWarnings) processExtrapolationWarnings(structure);
     structure.energy = e;
     if (normalize) structure.energy = physicalEnergy(structure, false);
     addEnergyOffset(structure, false);
@@ -285,37 +282,124 @@ void InterfaceLammps::getCharges(double* const& atomQ) const
     for (auto const& a : structure.atoms) atomQ[a.index] = 0.0;
 }
 
-long InterfaceLammps::getEWBufferSize() const { return (long)sizeof(long); }
+// The extrapolation-warning buffers pair_style hdnnp gathers on rank 0
+// (InterfaceLammps.cpp:638-749): per element the number of SF records, then
+// per record the SF index, countEW, type, Gmin, Gmax, the element symbol
+// (length with the terminating zero, then the characters) and the
+// structure indices, atom tags and values of its warnings.  The fields are
+// packed in this order in their native representation (what MPI_Pack
+// writes for MPI_SIZE_T, MPI_DOUBLE and MPI_CHAR on one host type).
+namespace
+{
+template <class T>
+void packBytes(char* buf, long bs, long& p, T const* v, std::size_t n)
+{
+    long const b = (long)(n * sizeof(T));
+    if (p + b > bs)
+        throw std::runtime_error("ERROR: Extrapolation-warning buffer too "
+                                 "small.\n");
+    if (b) std::memcpy(buf + p, v, (std::size_t)b);
+    p += b;
+}
 
-// The warning count of this rank travels as one long (the reference sends
-// the warning records; the engine keeps counts only).
+template <class T>
+void unpackBytes(char const* buf, long bs, long& p, T* v, std::size_t n)
+{
+    long const b = (long)(n * sizeof(T));
+    if (p + b > bs)
+        throw std::runtime_error("ERROR: Extrapolation-warning buffer too "
+                                 "short.\n");
+    if (b) std::memcpy(v, buf + p, (std::size_t)b);
+    p += b;
+}
+}  // namespace
+
+long InterfaceLammps::getEWBufferSize() const
+{
+    long const ss = sizeof(std::size_t), ds = sizeof(double), cs = 1;
+    long bs = 0;
+    for (auto const& m : ewData)
+    {
+        bs += ss;
+        for (auto const& kv : m)
+        {
+            EWContainer const& d = kv.second;
+            bs += 3 * ss + 2 * ds + ss;
+            bs += (long)(d.element.length() + 1) * cs;
+            bs += (long)d.countEW * (2 * ss + ds);
+        }
+    }
+    return bs;
+}
+
 void InterfaceLammps::fillEWBuffer(char* const& buf, int bs) const
 {
-    if (bs < (int)sizeof(long)) return;
-    long const v = (long)numExtrapolationWarningsLocal;
-    std::copy(reinterpret_cast<char const*>(&v),
-              reinterpret_cast<char const*>(&v) + sizeof(long), buf);
+    long p = 0;
+    for (auto const& m : ewData)
+    {
+        std::size_t const n = m.size();
+        packBytes(buf, bs, p, &n, 1);
+        for (auto const& kv : m)
+        {
+            EWContainer const& d = kv.second;
+            packBytes(buf, bs, p, &kv.first, 1);
+            packBytes(buf, bs, p, &d.countEW, 1);
+            packBytes(buf, bs, p, &d.type, 1);
+            packBytes(buf, bs, p, &d.Gmin, 1);
+            packBytes(buf, bs, p, &d.Gmax, 1);
+            std::size_t const ts = d.element.length() + 1;
+            packBytes(buf, bs, p, &ts, 1);
+            packBytes(buf, bs, p, d.element.c_str(), ts);
+            packBytes(buf, bs, p, d.indexStructureEW.data(), d.countEW);
+            packBytes(buf, bs, p, d.indexAtomEW.data(), d.countEW);
+            packBytes(buf, bs, p, d.valueEW.data(), d.countEW);
+        }
+    }
 }
 
 void InterfaceLammps::extractEWBuffer(char const* const& buf, int bs)
 {
-    if (bs < (int)sizeof(long)) return;
-    long v = 0;
-    std::copy(buf, buf + sizeof(long), reinterpret_cast<char*>(&v));
-    numExtrapolationWarningsLocal += (std::size_t)std::max(v, 0L);
+    long p = 0;
+    for (auto& m : ewData)
+    {
+        std::size_t n = 0;
+        unpackBytes(buf, bs, p, &n, 1);
+        for (std::size_t i = 0; i < n; ++i)
+        {
+            std::size_t index = 0, countEW = 0, ts = 0;
+            unpackBytes(buf, bs, p, &index, 1);
+            EWContainer& d = m[index];
+            unpackBytes(buf, bs, p, &countEW, 1);
+            unpackBytes(buf, bs, p, &d.type, 1);
+            unpackBytes(buf, bs, p, &d.Gmin, 1);
+            unpackBytes(buf, bs, p, &d.Gmax, 1);
+            unpackBytes(buf, bs, p, &ts, 1);
+            std::vector<char> el(ts + 1, '\0');
+            unpackBytes(buf, bs, p, el.data(), ts);
+            d.element = el.data();
+            d.indexStructureEW.resize(d.countEW + countEW);
+            unpackBytes(buf, bs, p, d.indexStructureEW.data() + d.countEW,
+                        countEW);
+            d.indexAtomEW.resize(d.countEW + countEW);
+            unpackBytes(buf, bs, p, d.indexAtomEW.data() + d.countEW,
+                        countEW);
+            d.valueEW.resize(d.countEW + countEW);
+            unpackBytes(buf, bs, p, d.valueEW.data() + d.countEW, countEW);
+            d.countEW += countEW;
+        }
+    }
 }
 
+// InterfaceLammps.cpp:751-765: every collected warning, one line each.
 void InterfaceLammps::writeExtrapolationWarnings()
 {
-    if (numExtrapolationWarningsLocal > 0)
-        log << strpr("### NNP EXTRAPOLATION WARNING ### %zu symmetry "
-                     "function values outside the scaling range.\n",
-                     numExtrapolationWarningsLocal);
+    for (std::string const& line : extrapolationWarningLines()) log << line;
 }
 
+// InterfaceLammps.cpp:767-776 (SymFncStatistics::clear).
 void InterfaceLammps::clearExtrapolationWarnings()
 {
-    numExtrapolationWarningsLocal = 0;
+    for (auto& m : ewData) m.clear();
 }
 
 void InterfaceLammps::writeToFile(std::string const fileName,

@@ -9,9 +9,11 @@
 #include "hdnnp.h"
 
 #include <algorithm>
+#include <cinttypes>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <stdexcept>
 
 namespace nnp
@@ -207,6 +209,7 @@ void Mode::setupSymmetryFunctionStatistics(bool collectStatistics,
     checkExtrapolationWarnings = collectExtrapolationWarnings
                               || writeExtrapolationWarnings
                               || stopOnExtrapolationWarnings;
+    this->collectExtrapolationWarnings = collectExtrapolationWarnings;
     this->writeExtrapolationWarnings = writeExtrapolationWarnings;
     this->stopOnExtrapolationWarnings = stopOnExtrapolationWarnings;
 }
@@ -247,10 +250,16 @@ void Mode::buildModel()
     convLength = cl;
     energyOffset.assign(numElements, 0.0);
     numSF.assign(numElements, 0);
+    sfType.assign(numElements, {});
+    ewData.assign(numElements, {});
     for (std::size_t e = 0; e < numElements; ++e)
     {
         energyOffset[e] = hdnnp_model_energy_offset(model, (int)e);
         numSF[e] = (std::size_t)hdnnp_model_num_sf(model, (int)e);
+        std::vector<double> tab(12 * numSF[e]);
+        if (numSF[e]) check(hdnnp_model_sf_table(model, (int)e, tab.data()));
+        for (std::size_t k = 0; k < numSF[e]; ++k)
+            sfType[e].push_back((std::size_t)tab[12 * k]);
     }
     log << strpr("Model set up on the engine: %zu elements, maximum cutoff "
                  "%.6f.\n", numElements, maxCutoffRadius);
@@ -351,18 +360,71 @@ void Mode::calculateSymmetryFunctionGroups(Structure& structure,
     }
     structure.hasSymmetryFunctions = true;
     structure.hasSymmetryFunctionDerivatives = derivatives;
-    if (checkExtrapolationWarnings)
+    if (checkExtrapolationWarnings) processExtrapolationWarnings(structure);
+}
+
+std::size_t Mode::processExtrapolationWarnings(Structure const& structure)
+{
+    int64_t const total = hdnnp_extrapolation_warning_list(
+        ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (total < 0) check((int)total);
+    if (total == 0) return 0;
+    std::size_t const m = (std::size_t)total;
+    std::vector<int32_t> atom(m), sf(m);
+    std::vector<double> value(m), gmin(m), gmax(m);
+    int64_t const again = hdnnp_extrapolation_warning_list(
+        ctx, total, atom.data(), sf.data(), value.data(), gmin.data(),
+        gmax.data());
+    if (again < 0) check((int)again);
+    // Element.cpp:489-538, record by record in the reference's order.
+    for (std::size_t w = 0; w < m; ++w)
     {
-        int64_t const w = hdnnp_extrapolation_warnings(ctx, nullptr);
-        if (w < 0) check((int)w);
-        numExtrapolationWarnings += (std::size_t)w;
-        if (w > 0 && writeExtrapolationWarnings)
-            log << strpr("### NNP EXTRAPOLATION WARNING ### %lld symmetry "
-                         "function values outside the scaling range in "
-                         "structure %zu.\n", (long long)w, structure.index);
-        if (w > 0 && stopOnExtrapolationWarnings)
-            throw std::out_of_range("ERROR: Extrapolation warning.\n");
+        Atom const& a = structure.atoms.at((std::size_t)atom[w]);
+        std::size_t const e = a.element;
+        std::size_t const k = (std::size_t)sf[w];
+        std::size_t const type = sfType.at(e).at(k);
+        std::string const symbol = elementMap.symbol(e);
+        if (collectExtrapolationWarnings)
+        {
+            EWContainer& d = ewData.at(e)[k];
+            d.countEW++;
+            d.Gmin = gmin[w];
+            d.Gmax = gmax[w];
+            d.type = type;
+            d.element = symbol;
+            d.valueEW.push_back(value[w]);
+            d.indexStructureEW.push_back(a.indexStructure);
+            d.indexAtomEW.push_back((std::size_t)a.tag);
+        }
+        std::string const line = strpr(
+            "### NNP EXTRAPOLATION WARNING ### STRUCTURE: %6zu ATOM: %9" PRId64
+            " ELEMENT: %2s SYMFUNC: %4zu TYPE: %2zu VALUE: %10.3E MIN: "
+            "%10.3E MAX: %10.3E\n", a.indexStructure, a.tag, symbol.c_str(),
+            k + 1, type, value[w], gmin[w], gmax[w]);
+        if (writeExtrapolationWarnings) std::cerr << line;
+        if (stopOnExtrapolationWarnings)
+            throw std::out_of_range(line + "ERROR: Symmetry function value "
+                                           "out of range.\n");
     }
+    return m;
+}
+
+std::vector<std::string> Mode::extrapolationWarningLines() const
+{
+    std::vector<std::string> vs;
+    for (auto const& el : ewData)
+        for (auto const& kv : el)
+        {
+            EWContainer const& d = kv.second;
+            for (std::size_t i = 0; i < d.valueEW.size(); ++i)
+                vs.push_back(strpr(
+                    "### NNP EXTRAPOLATION WARNING ### STRUCTURE: %6zu ATOM: "
+                    "%9zu ELEMENT: %2s SYMFUNC: %4zu TYPE: %2zu VALUE: %10.3E "
+                    "MIN: %10.3E MAX: %10.3E\n", d.indexStructureEW[i],
+                    d.indexAtomEW[i], d.element.c_str(), kv.first + 1, d.type,
+                    d.valueEW[i], d.Gmin, d.Gmax));
+        }
+    return vs;
 }
 
 // Mode.cpp:1642-1663: atomic energies, and dE/dG with derivatives.
@@ -609,12 +671,22 @@ void Mode::convertToPhysicalUnits(Structure& structure) const
 // 2G-HDNNPs have no Ewald sum; the reference logs cutoffs for 4G only.
 void Mode::logEwaldCutoffs() {}
 
+// Mode.cpp:2155-2178: the collected warnings of all elements; a reset
+// empties every SF's record and keeps the SF in the map
+// (SymFncStatistics::Container::resetExtrapolationWarnings).
 std::size_t Mode::getNumExtrapolationWarnings() const
 {
-    return numExtrapolationWarnings;
+    std::size_t n = 0;
+    for (auto const& el : ewData)
+        for (auto const& kv : el) n += kv.second.countEW;
+    return n;
 }
 
-void Mode::resetExtrapolationWarnings() { numExtrapolationWarnings = 0; }
+void Mode::resetExtrapolationWarnings()
+{
+    for (auto& el : ewData)
+        for (auto& kv : el) kv.second = EWContainer();
+}
 
 // ---------------------------------------------------------- Prediction --
 

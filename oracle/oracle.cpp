@@ -2346,6 +2346,44 @@ int orc_predict_file(orc_model* m, const char* path, double* energy_out,
     }, -1);
 }
 
+// Element::updateSymmetryFunctionStatistics (Element.cpp:462-541) over the
+// atoms in order (Mode.cpp:1625-1633): every SF whose unscaled value is
+// outside [Gmin - 1000 eps, Gmax + 1000 eps] is one warning record (atom,
+// SF index, value); up to cap records are written, the total is returned.
+int64_t orc_extrapolation_warnings(const orc_struct* s, const orc_model* m,
+                                   int32_t* atom, int32_t* sf, double* value,
+                                   int64_t cap)
+{
+    return guard([&]() -> int64_t {
+        int64_t count = 0;
+        double const epsilon = 1000.0 * std::numeric_limits<double>::epsilon();
+        for (size_t i = 0; i < s->atoms.size(); ++i)
+        {
+            Atom const& a = s->atoms[i];
+            ElementData const& el = m->el.at(a.element);
+            if (a.G.size() != el.sfs.size())
+                throw std::runtime_error("ERROR: Number of symmetry functions"
+                                    " does not match.\n");
+            for (size_t k = 0; k < a.G.size(); ++k)
+            {
+                SF const& f = el.sfs[k];
+                double const v = f.unscale(a.G[k]);
+                if (v + epsilon < f.Gmin || v - epsilon > f.Gmax)
+                {
+                    if (count < cap)
+                    {
+                        atom[count] = (int32_t)i;
+                        sf[count] = (int32_t)f.index;
+                        value[count] = v;
+                    }
+                    ++count;
+                }
+            }
+        }
+        return count;
+    }, (int64_t)-1);
+}
+
 int orc_sf_statistics(orc_struct* s, orc_model* m, int e, double* out)
 {
     return guard([&]() {

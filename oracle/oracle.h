@@ -117,6 +117,13 @@ double orc_energy_only(orc_struct* s, orc_model* m, int cells);
  * out holds 4*num_sf doubles for element e. */
 int orc_sf_statistics(orc_struct* s, orc_model* m, int e, double* out);
 
+/* Element::updateSymmetryFunctionStatistics over all atoms after
+ * orc_symmetry_functions: extrapolation-warning records (atom, SF index,
+ * unscaled value), up to cap written; returns the total. */
+int64_t orc_extrapolation_warnings(const orc_struct* s, const orc_model* m,
+                                   int32_t* atom, int32_t* sf, double* value,
+                                   int64_t cap);
+
 /* Central finite-difference forces for atoms in list[0..nlist): out 3*nlist.
  * Follows Dataset::prepareNumericForces + nnp-checkf (remap after shift). */
 int orc_numeric_forces(orc_struct* s, orc_model* m, double delta,

@@ -15,6 +15,10 @@ from tests.oracle_lib import OModel, OStruct
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 EXAMPLES = os.path.join(GOLDEN, "examples")
+# The 2G examples of examples/nnp-predict with ExamplePrediction.h goldens.
+# The last three use the compact SF types 20-22 in groups of 54 members.
+EXAMPLE_NAMES = ["H2O_RPBE-D3", "Cu2S_PBE", "Anisole_SCAN", "DMABN_SCAN",
+                 "Ethylbenzene_SCAN"]
 
 
 def load_json(name):

@@ -6,7 +6,9 @@ Writes (all data, no code):
   tests/golden/examples/<name>/{input.nn,scaling.data,weights.*.data,input.data}
       copies of examples/nnp-predict/<name> for the 2G examples whose SF and
       cutoff types this engine covers (H2O_RPBE-D3: G2/G3 + CT_TANHU;
-      Cu2S_PBE: G2/G3/G9 + CT_POLY2, softplus, sigma scaling);
+      Cu2S_PBE: G2/G3/G9 + CT_POLY2, softplus, sigma scaling;
+      Anisole_SCAN, DMABN_SCAN, Ethylbenzene_SCAN: the compact types 20 and
+      21/22 with groups of 54 members);
   tests/golden/example_prediction.json
       energies and all force components of those examples from
       test/cpp/ExamplePrediction.h (tested there at 1000 eps);
@@ -25,7 +27,8 @@ import shutil
 
 REF = "/root/reference"
 OUT = os.path.dirname(os.path.abspath(__file__))
-EXAMPLES = ["H2O_RPBE-D3", "Cu2S_PBE"]
+EXAMPLES = ["H2O_RPBE-D3", "Cu2S_PBE", "Anisole_SCAN", "DMABN_SCAN",
+            "Ethylbenzene_SCAN"]
 NUM = r"[-+]?[0-9]*\.?[0-9]+(?:[eE][-+]?[0-9]+)?"
 
 

@@ -57,6 +57,9 @@ _SIG = [
     ("orc_evaluate", ctypes.c_double, [_V, _V, ctypes.c_int, ctypes.c_int]),
     ("orc_energy_only", ctypes.c_double, [_V, _V, ctypes.c_int]),
     ("orc_sf_statistics", ctypes.c_int, [_V, _V, ctypes.c_int, _D]),
+    ("orc_extrapolation_warnings", ctypes.c_int64,
+     [_V, _V, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+      _D, ctypes.c_int64]),
     ("orc_numeric_forces", ctypes.c_int,
      [_V, _V, ctypes.c_double, _I64, ctypes.c_int64, ctypes.c_int, _D]),
     ("orc_num_threads", ctypes.c_int, []),
@@ -282,6 +285,23 @@ class OStruct:
         if n < 0:
             raise RuntimeError(_err())
         return n, out
+
+    def extrapolation_warnings(self):
+        """(atom, sf, value) records of Element::
+        updateSymmetryFunctionStatistics after the SFs were computed."""
+        i32 = ctypes.c_int32
+        z = np.zeros(1, dtype=np.int32)
+        zd = np.zeros(1)
+        n = lib().orc_extrapolation_warnings(self.h, self.model.h, _p(z, i32),
+                                             _p(z, i32), _p(zd), 0)
+        if n < 0:
+            raise RuntimeError(_err())
+        a = np.zeros(max(n, 1), dtype=np.int32)
+        k = np.zeros(max(n, 1), dtype=np.int32)
+        v = np.zeros(max(n, 1))
+        lib().orc_extrapolation_warnings(self.h, self.model.h, _p(a, i32),
+                                         _p(k, i32), _p(v), n)
+        return a[:n], k[:n], v[:n]
 
     def numeric_forces(self, atoms, delta=1e-4, cells=False):
         atoms = np.ascontiguousarray(atoms, dtype=np.int64)

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     assert sorted(engine.EXPORTED) == decl
 
 
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", cases.EXAMPLE_NAMES)
 def test_host_model_setup_matches_oracle(name):
     d = os.path.join(cases.EXAMPLES, name)
     m = engine.Model.load_dir(d)
@@ -78,7 +78,7 @@ def test_model_errors_are_reported(tmp_path):
         engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
     p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"
                  + "".join("symfunction_short H 20 H 0.0 %.1f p2\n"
-                           % (3.0 + 0.1 * k) for k in range(33)))
+                           % (3.0 + 0.05 * k) for k in range(97)))
     with pytest.raises(RuntimeError, match="outside this engine"):
         engine.Model.load(str(p), None, str(tmp_path / "w.%03zu.data"))
     p.write_text("number_of_elements 1\nelements H\ncutoff_type 1\n"

@@ -1,10 +1,14 @@
 """Caller-given neighbour lists (the InterfaceLammps path) against the
-periodic evaluation of the same box on the GPU: a LAMMPS-like domain with
+oracle's periodic evaluation of the same box: a LAMMPS-like domain with
 explicit ghost atoms (tests/lammps_like.py) evaluated through
-hdnnp_set_neighbor_list must give the periodic energy (1e-10 relative) and,
-after folding the ghost forces onto their owners, the periodic forces with
-self-images (HDNNP_FORCES_CONSISTENT) within 1e-9 eV/A.  Both precisions;
-the mixed path within its own bounds (1e-6 relative, 1e-5 eV/A)."""
+hdnnp_set_neighbor_list, its forces scattered onto local and ghost atoms on
+the device (InterfaceLammps::getForces, InterfaceLammps.cpp:533-621), must
+give the oracle's energy (1e-10 relative), atomic energies and, after
+folding the ghost forces onto their owners, the oracle's forces with
+self-images (force mode 1, -dE/dr) within 1e-9 eV/A.  Both precisions; the
+mixed path within its own bounds (1e-6 relative, 1e-5 eV/A).  The device
+scatter is also checked against the reference's loop, restated on the host
+over the per-slot forces: bit for bit."""
 import sys
 
 import numpy as np
@@ -13,6 +17,7 @@ import pytest
 from n2p2_amd import engine
 from tests import cases
 from tests.lammps_like import full_lists, ghost_domain
+from tests.oracle_lib import OModel, OStruct
 
 pytestmark = pytest.mark.gpu
 
@@ -22,13 +27,23 @@ def say(*a):
     sys.stdout.flush()
 
 
+def _loop_scatter(nout, off, idx, sf, pf):
+    """InterfaceLammps::getForces (InterfaceLammps.cpp:546-581) as the
+    reference loops: per centre its self force, then its pair forces onto
+    the neighbours in list order."""
+    f = np.zeros((nout, 3))
+    for i in range(len(off) - 1):
+        f[i] -= -sf[i]
+        for q in range(off[i], off[i + 1]):
+            f[idx[q]] -= -pf[q]
+    return f
+
+
 def _compare(d, r, e, box, prec, e_rel, f_abs):
     m = engine.Model.load_dir(d)
-    per = engine.Context(m)
-    per.set_precision(prec)
-    per.set_structure(r, e, box)
-    ep = per.evaluate(engine.FORCES_CONSISTENT)
-    fp = per.forces()
+    s = OStruct(OModel.from_dir(d), r, e, box)
+    ep = s.evaluate(cells=True, force_mode=1)
+    fp = s.forces()
     rc = m.max_cutoff
     x, el, owner, nl = ghost_domain(r, e, box, rc)
     off, idx, dr = full_lists(x, nl, rc)
@@ -39,15 +54,18 @@ def _compare(d, r, e, box, prec, e_rel, f_abs):
     ext.set_neighbor_list(el[:nl], off, idx, el[idx], dr)
     ee = ext.evaluate(engine.FORCES_CONSISTENT)
     fg = ext.scattered_forces(len(x))
+    # The device scatter is the reference's loop over the per-slot forces.
+    sf, pf = ext.pair_forces()
+    assert np.array_equal(fg, _loop_scatter(len(x), off, idx, sf, pf))
     folded = np.zeros((nl, 3))
     np.add.at(folded, owner, fg)
     de = abs(ee - ep) / abs(ep)
     df = float(np.abs(folded - fp).max())
-    say("  energy rel %.3g, folded force max %.3g" % (de, df))
+    say("  energy rel %.3g, folded force max %.3g (oracle)" % (de, df))
     assert de <= e_rel
     assert df <= f_abs
-    # The per-atom energies are those of the periodic evaluation.
-    assert np.allclose(ext.atomic_energies(), per.atomic_energies(),
+    # The per-atom energies are the oracle's periodic ones.
+    assert np.allclose(ext.atomic_energies(), s.atomic_energies(),
                        rtol=e_rel * 100, atol=1e-12)
 
 

@@ -17,12 +17,7 @@ from n2p2_amd import engine
 from n2p2_amd import workloads as W
 from tests import cases
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the multiple-time-step integrator is switched "
-                            "off: two device runs of it ended in a memory "
-                            "fault whose cause has not been found"),
-]
+pytestmark = pytest.mark.gpu
 
 DT_OUTER, N_INNER = 1.0, 4
 

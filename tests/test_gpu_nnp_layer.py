@@ -33,7 +33,7 @@ def _golden(name):
     return cases.load_json("example_prediction.json")["examples"][name]
 
 
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", cases.EXAMPLE_NAMES)
 def test_prediction_through_layer(name):
     exe = os.path.join(BIN, "predict_example")
     assert os.path.exists(exe), "predict_example not built"
@@ -54,7 +54,7 @@ def test_prediction_through_layer(name):
                     reason="the reference's nnp-predict was not built "
                            "against the layer (needs the reference checkout "
                            "at build time)")
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", cases.EXAMPLE_NAMES)
 def test_reference_nnp_predict_on_layer(tmp_path, name):
     say("reference nnp-predict on the layer:", name)
     src = os.path.join(cases.EXAMPLES, name)

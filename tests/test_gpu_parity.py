@@ -85,7 +85,7 @@ def test_neighbor_list_bit_exact(workdir):
             assert np.array_equal(ctx.pbc(), s.pbc())
 
 
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", cases.EXAMPLE_NAMES)
 def test_example_prediction_goldens(name):
     say("example prediction", name)
     g = cases.load_json("example_prediction.json")["examples"][name]

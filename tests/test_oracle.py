@@ -37,7 +37,7 @@ from tests.oracle_lib import OModel, OStruct
 EPS = np.finfo(float).eps
 
 
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", cases.EXAMPLE_NAMES)
 def test_example_prediction_goldens(name):
     g = cases.load_json("example_prediction.json")["examples"][name]
     d = os.path.join(cases.EXAMPLES, name)
