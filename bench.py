@@ -138,7 +138,9 @@ def run_batch(tmp, steps, warmup, log):
     out = {"structures": len(sts), "atoms": int(ctx.n)}
     res = {}
     for name, prec in (("fp64", engine.PRECISION_FP64),
-                       ("mixed", engine.PRECISION_MIXED)):
+                       ("mixed", engine.PRECISION_MIXED),
+                       ("mixed16", engine.PRECISION_MIXED16),
+                       ("mixed_bf16", engine.PRECISION_MIXED_BF16)):
         ctx.set_precision(prec)
         for _ in range(warmup):
             ctx.evaluate(engine.FORCES_REFERENCE)
@@ -154,12 +156,13 @@ def run_batch(tmp, steps, warmup, log):
         res[name] = (ctx.forces(), ctx.structure_energies())
         out[name] = {"atom_steps_per_s": ctx.n / dt, "ms_per_step": dt * 1e3,
                      "kernel_ms": kt}
-    df = res["mixed"][0] - res["fp64"][0]
-    de = np.abs(res["mixed"][1] - res["fp64"][1]) / np.abs(res["fp64"][1])
-    out["mixed_error_vs_fp64"] = {
-        "force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
-        "force_max_eV_A": float(np.abs(df).max()),
-        "structure_energy_rel_max": float(de.max())}
+    for name in ("mixed", "mixed16", "mixed_bf16"):
+        df = res[name][0] - res["fp64"][0]
+        de = np.abs(res[name][1] - res["fp64"][1]) / np.abs(res["fp64"][1])
+        out[name]["error_vs_fp64"] = {
+            "force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
+            "force_max_eV_A": float(np.abs(df).max()),
+            "structure_energy_rel_max": float(de.max())}
     out["mixed_vs_plain_fp64"] = (out["mixed"]["atom_steps_per_s"]
                                   / out["fp64"]["atom_steps_per_s"])
     ctx.close()
@@ -216,6 +219,114 @@ def train_surrogate(tmp, log):
     log("surrogate trained in %.2f s: loss %s -> %s" % (ttrain, hist[0],
                                                         hist[-1]))
     return engine.Model.load_dir(sd), hist, ttrain
+
+
+def _drift(etot, n):
+    """NVE energy drift of a run: slope of a least-squares line through the
+    total energy per atom (eV per atom per step) and the largest excursion
+    from the first value (eV per atom)."""
+    et = np.asarray(etot) / n
+    k = np.arange(len(et))
+    slope = float(np.polyfit(k, et, 1)[0]) if len(et) > 1 else 0.0
+    return {"slope_eV_per_atom_per_step": slope,
+            "max_excursion_eV_per_atom": float(np.abs(et - et[0]).max())}
+
+
+def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
+           epochs=20, nouter=2000, gate=5e-3):
+    """The learned surrogate for multiple time steps on the 196,608-atom
+    box: frames of the full model's own 300 K NVE trajectory (mixed path,
+    dt 0.25 fs, every `stride` steps), the distilled radial surrogate (4 eta
+    x 2 neighbour elements G2, 8-16-16-1 softplus) trained on them with the
+    engine's gradient and Adam kernels (lr 1e-3, `epochs` epochs, one frame
+    per step), then `nouter` outer steps of 1 fs (4 surrogate substeps of
+    0.25 fs) of r-RESPA: with the force-RMSE gate, without it, without it
+    for the untrained surrogate, and the full model's plain velocity Verlet
+    at 0.25 fs over the same time.  Throughput counts 0.25 fs steps."""
+    masses = np.array(W.WATER_MASSES)
+    n = len(r)
+    out = {"frames": nframes, "frame_stride_steps": stride, "epochs": epochs,
+           "outer_steps": nouter, "gate_eV_A": gate}
+    t0 = time.perf_counter()
+    R, E, F = S.generate_frames(model, r, elem, box, masses, nframes,
+                                stride=stride, dt=0.25, temp=300.0,
+                                seed=4242, precision=engine.PRECISION_MIXED)
+    out["frame_generation_s"] = time.perf_counter() - t0
+    log("C4: %d frames in %.1f s" % (nframes, out["frame_generation_s"]))
+    sd = os.path.join(tmp, "surrogate_c4")
+    S.write_model_gpu_scaled(sd, S.surrogate_settings(), W.WATER_ELEMENTS,
+                             (1, 8), S.SURROGATE_HIDDEN, R[0], elem, box,
+                             seed=21)
+    untrained = engine.Model.load_dir(sd)
+    ctx = engine.Context(untrained)
+    t0 = time.perf_counter()
+    hist = S.train(ctx, elem, box, R, E, F, epochs=epochs, lr=1e-3, log=log)
+    out["train_s"] = time.perf_counter() - t0
+    out["train_loss_first_epoch"] = list(hist[0])
+    out["train_loss_last_epoch"] = list(hist[-1])
+    td = os.path.join(tmp, "surrogate_c4_trained")
+    os.makedirs(td, exist_ok=True)
+    for f in ("input.nn", "scaling.data"):
+        with open(os.path.join(sd, f)) as a, open(os.path.join(td, f), "w") as b:
+            b.write(a.read())
+    S.write_weights(td, ctx.get_weights(),
+                    [untrained.num_sf(k) for k in range(untrained.num_elements)],
+                    (1, 8))
+    ctx.close()
+    del R, F
+    trained = engine.Model.load_dir(td)
+    v = W.maxwell_boltzmann(masses[elem], 300.0, seed=777)
+
+    def mts(sur_model, g, label):
+        full = engine.Context(model)
+        sur = engine.Context(sur_model)
+        full.set_precision(engine.PRECISION_MIXED)
+        for c in (full, sur):
+            c.set_structure(r, elem, box)
+        sur.mts_init(full, masses, dt_outer=1.0, n_inner=4, skin=1.0, v=v,
+                     rmse_gate=g)
+        t1 = time.perf_counter()
+        try:
+            ep, ek, rm, md = sur.mts_run(nouter)
+        except RuntimeError as err:
+            full.close()
+            sur.close()
+            log("C4 %s: %s" % (label, err))
+            return {"failed": str(err).strip()}
+        dt = time.perf_counter() - t1
+        res = {"atom_steps_per_s": n * 4 * nouter / dt,
+               "ms_per_outer_step": dt / nouter * 1e3,
+               "fallback_fraction": float(np.mean(md == 1)),
+               "surrogate_force_rmse_eV_A": {
+                   "mean": float(np.mean(rm)), "max": float(np.max(rm))},
+               "drift": _drift(ep + ek, n)}
+        full.close()
+        sur.close()
+        log("C4 %s: %s" % (label, res))
+        return res
+
+    out["mts_gated"] = mts(trained, gate, "MTS, gate %g eV/A" % gate)
+    out["mts_no_gate"] = mts(trained, 1e30, "MTS without gate")
+    out["mts_no_gate_untrained"] = mts(untrained, 1e30,
+                                       "MTS without gate, untrained")
+    ctx = engine.Context(model)
+    ctx.set_precision(engine.PRECISION_MIXED)
+    ctx.set_structure(r, elem, box)
+    ctx.md_init(masses, 0.25, skin=1.0, v=v)
+    t1 = time.perf_counter()
+    ep, ek = ctx.md_run(4 * nouter)
+    dt = time.perf_counter() - t1
+    out["full_verlet_0.25fs"] = {"atom_steps_per_s": n * 4 * nouter / dt,
+                                 "drift": _drift((ep + ek)[3::4], n)}
+    ctx.close()
+    log("C4 full model: %s" % out["full_verlet_0.25fs"])
+    for k in ("mts_gated", "mts_no_gate"):
+        if "failed" in out[k]:
+            continue
+        out[k]["speedup_vs_full"] = (out[k]["atom_steps_per_s"]
+                                     / out["full_verlet_0.25fs"][
+                                         "atom_steps_per_s"])
+    return out
 
 
 PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
@@ -297,8 +408,11 @@ def main():
     ap.add_argument("--nmol", type=int, default=65536)
     ap.add_argument("--box", type=float, default=124.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c4-frames", type=int, default=2000)
+    ap.add_argument("--c4-epochs", type=int, default=20)
+    ap.add_argument("--c4-outer", type=int, default=2000)
     ap.add_argument("--path", choices=("both", "mixed", "fp64", "md",
-                                       "batch"),
+                                       "batch", "mts"),
                     default="both",
                     help="time one path only (for profiling runs; the "
                          "reported line needs both)")
@@ -327,6 +441,13 @@ def main():
         if args.path == "batch":
             run_batch(tmp, args.steps, args.warmup, log)
             log("batch only: no JSON line")
+            return
+        if args.path == "mts":
+            c4 = run_c4(model, r, elem, box, tmp, log,
+                        nframes=args.c4_frames, epochs=args.c4_epochs,
+                        nouter=args.c4_outer)
+            print(json.dumps({"c4_multiple_time_steps": c4}), flush=True)
+            log("MTS only: no bench line")
             return
         if args.path == "md":
             res = run_md(model, r, elem, box, engine.PRECISION_MIXED,

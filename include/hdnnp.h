@@ -128,6 +128,18 @@ int  hdnnp_ctx_set_specialized(hdnnp_ctx* c, int enable);
  * enabled and compiled (compiling them now if they were not yet), 0 when
  * the generic kernels run.  No counterpart in the reference. */
 int  hdnnp_ctx_specialized_active(hdnnp_ctx* c, int* active);
+/* 1 (default unless the environment sets N2P2_AMD_PAIR_RECORD=0): on the
+ * mixed-precision paths the symmetry-function kernel records, per atom,
+ * the neighbour pairs of every angular group that pass the cutoff tests,
+ * and the force kernel of the same evaluation replays them instead of
+ * enumerating and testing all pairs again; 0: both kernels enumerate.
+ * Same results bit for bit.  No counterpart in the reference (an engine
+ * control like the precision). */
+int  hdnnp_ctx_set_pair_record(hdnnp_ctx* c, int enable);
+/* *active = 1 when symmetry-function evaluations of the current precision,
+ * model and neighbour list record the pairs.  No counterpart in the
+ * reference. */
+int  hdnnp_ctx_pair_record_active(hdnnp_ctx* c, int* active);
 /* Propagates a model change made after hdnnp_ctx_create (e.g.
  * hdnnp_model_set_scaling) to the context's device tables. */
 int  hdnnp_ctx_update_model(hdnnp_ctx* c);

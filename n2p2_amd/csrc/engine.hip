@@ -182,6 +182,13 @@ struct hdnnp_ctx
     bool haveSF = false, haveNN = false, haveNNDeriv = false;
     double energy = 0.0;
     int precision = HDNNP_PRECISION_FP64;
+    // Pair record of the mixed path: the SF kernel writes the passing
+    // neighbour pairs of every angular group, the force kernel replays them
+    // instead of enumerating and testing the pairs again.  pairsValid: the
+    // last SF evaluation recorded them for the current list.
+    bool pairRecord = pair_record_enabled_by_default();
+    bool pairsValid = false;
+    DBuf<unsigned short> dPairs;
     // Model-specialised SF / force kernels per arithmetic (0 FP64, 1 FP32),
     // compiled on first use (specialize.hip); generic kernels otherwise.
     bool specialized = plan_enabled_by_default();
@@ -790,6 +797,29 @@ int64_t totalEntries(hdnnp_ctx* c)
     return c->M;
 }
 
+// Entries per atom of the pair record for an LDS capacity of cap
+// neighbours: the group counts, then every unordered pair of the staged
+// list up to 4096 (groups beyond that are walked again); none when the
+// staged indices do not fit a byte.
+int pairRecordCap(int cap)
+{
+    if (cap > 256) return 0;
+    int64_t const all = (int64_t)cap * (cap - 1) / 2;
+    return kRecordGroups + (int)std::min<int64_t>(4096, (all + 63) / 64 * 64);
+}
+
+// Whether SF evaluations of this context record the passing pairs for the
+// force kernel: the mixed path, specialised or generic, of every model
+// whose element chunks fit the record's group counts.
+bool recordsPairs(hdnnp_ctx* c, bool raw)
+{
+    if (!c->pairRecord || raw || c->precision == HDNNP_PRECISION_FP64)
+        return false;
+    for (auto const& e : c->flat.elem)
+        if (e.ng > kRecordGroups) return false;
+    return pairRecordCap(std::max(c->maxNeigh, 1)) > 0;
+}
+
 void computeSF(hdnnp_ctx* c, bool raw)
 {
     requireNL(c);
@@ -815,6 +845,15 @@ void computeSF(hdnnp_ctx* c, bool raw)
     a.cap = cap;
     a.raw = raw ? 1 : 0;
     a.fp32 = fp32;
+    bool const rec = recordsPairs(c, raw);
+    if (rec)
+    {
+        int const pc = pairRecordCap(cap);
+        c->dPairs.ensure(std::max<size_t>((size_t)c->n * (size_t)pc, 1));
+        a.pairs = c->dPairs.p;
+        a.pcap = pc;
+    }
+    c->pairsValid = rec;
     hipEvent_t ev = nullptr;
     c->begin(K_SYMFUNC, ev);
     // Raw (unscaled) evaluations use the same kernels as scaled ones, so
@@ -903,6 +942,8 @@ double computeEnergy(hdnnp_ctx* c)
     return e;
 }
 
+void launchGather(hdnnp_ctx* c, int mode);
+
 void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
 {
     if (!c->haveNNDeriv)
@@ -937,7 +978,12 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
     a.cself = c->dCself.p;
     a.cap = cap;
     a.fp32 = fp32;
-    hipEvent_t ev = nullptr, ev2 = nullptr;
+    if (c->pairsValid && fp32)
+    {
+        a.pairs = c->dPairs.p;
+        a.pcap = pairRecordCap(cap);
+    }
+    hipEvent_t ev = nullptr;
     c->begin(K_FORCE, ev);
     PlanKernels const pk = c->planFor(fp32);
     if (pk.force && c->n > 0)
@@ -951,6 +997,16 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
     else hipCheck(launch_force_backward(a, c->stream), "force backward");
     hipCheck(launch_force_compact(a, c->stream), "compact force backward");
     c->end(K_FORCE, ev);
+    launchGather(c, mode);
+    if (wait) c->sync();
+}
+
+// F = -(cself + partner slots): the gather of the plain list, or of the MD
+// list through the skin partners.  Caller-given lists are scattered on
+// request (hdnnp_get_scattered_forces) instead.
+void launchGather(hdnnp_ctx* c, int mode)
+{
+    hipEvent_t ev2 = nullptr;
     if (!c->externalNL && c->md.on)
     {
         auto& m = c->md;
@@ -972,8 +1028,8 @@ void computeForces(hdnnp_ctx* c, int mode, bool wait = true)
                  "force gather");
         c->end(K_GATHER, ev2);
     }
-    if (wait) c->sync();
 }
+
 
 // ------------------------------------------------------------------ MD --
 
@@ -1079,6 +1135,11 @@ void mdRebuild(hdnnp_ctx* c)
     m.cap = std::min(c->maxNeigh, maxc + maxc / 8 + 8);
     c->maxNeigh = m.cap;
     c->dCslot.ensure(3 * E);
+    // The pair record, sized here so that graph captures of the steps
+    // allocate nothing.
+    if (recordsPairs(c, false))
+        c->dPairs.ensure(std::max<size_t>(
+            (size_t)n * (size_t)pairRecordCap(m.cap), 1));
     c->Mvalid = false;
     m.rebuilds += 1;
 }
@@ -1363,6 +1424,26 @@ int hdnnp_ctx_set_specialized(hdnnp_ctx* c, int enable)
         if ((enable != 0) != c->specialized)
             c->haveSF = c->haveNN = c->haveNNDeriv = false;
         c->specialized = enable != 0;
+    });
+}
+
+int hdnnp_ctx_set_pair_record(hdnnp_ctx* c, int enable)
+{
+    return guarded([&]() {
+        if (c->pairRecord != (enable != 0))
+        {
+            c->pairRecord = enable != 0;
+            c->haveSF = c->haveNN = c->haveNNDeriv = false;
+            c->pairsValid = false;
+        }
+    });
+}
+
+int hdnnp_ctx_pair_record_active(hdnnp_ctx* c, int* active)
+{
+    return guarded([&]() {
+        if (!active) throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        *active = recordsPairs(c, false) ? 1 : 0;
     });
 }
 
@@ -2546,8 +2627,9 @@ int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f)
         if (!c->dCself.p || !c->dCslot.p)
             throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
         if (nout < c->extTargets)
-            throw Error(HDNNP_E_INVALID, "ERROR: A centre or neighbour index "
-                                         "is beyond the force array.\n");
+            throw Error(HDNNP_E_INVALID, "ERROR: Centre or neighbour index "
+                                         "out of range (beyond the force "
+                                         "array).\n");
         c->setDevice();
         // InterfaceLammps::getForces (InterfaceLammps.cpp:546-581) on the
         // device: only the rows that receive contributions travel, in and

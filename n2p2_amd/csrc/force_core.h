@@ -357,6 +357,11 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
 
     stage(a, S, s0, n, sStart, sCnt);
 
+    // The SF kernel's pair record of this atom, if any (sf_core.h).
+    const unsigned short* __restrict__ rec =
+        a.pairs ? a.pairs + (size_t)i * (size_t)a.pcap : nullptr;
+    int pOff = kRecordGroups;
+
     int cfgCached = -1;
     for (int gi = ed.g0; gi < ed.g0 + ed.ng; ++gi)
     {
@@ -444,6 +449,24 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         }
         Members<R, MAXM> M;
         load_members(MP, nmem, M);
+        int const nrec = rec && (gi - ed.g0) < kRecordGroups
+                       ? (int)rec[gi - ed.g0] : (int)kRecordMissing;
+        if (nrec != (int)kRecordMissing)
+        {
+            // Replay the recorded pairs: the same triplets in the same lanes
+            // and order as the walk below would give.
+            for (int base = 0; base < nrec; base += 64)
+                if (base + lane < nrec)
+                {
+                    int const e = rec[pOff + base + lane];
+                    triplet_backward<CT, GENPOW, MAXM>(gi, S, e & 0xff,
+                                                       e >> 8, wide, cc, M,
+                                                       MP, nmem);
+                }
+            pOff += nrec;
+            __syncthreads();
+            continue;
+        }
         R const grc2 = (R)g.rc2;
         int qn = 0;
         int pc, pr;

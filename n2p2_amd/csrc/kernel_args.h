@@ -23,6 +23,12 @@ struct SFArgs
     int cap;                     // LDS neighbour capacity per wave
     int raw;
     int fp32;
+    // Pair record (null: none): per atom pcap entries, the first
+    // kRecordGroups the passing-pair count of each angular group chunk
+    // (kRecordMissing: not recorded), then the pairs in evaluation order as
+    // staged indices j | k << 8.
+    unsigned short* pairs;
+    int pcap;
 };
 
 struct ForceArgs
@@ -40,6 +46,9 @@ struct ForceArgs
     double* cself;               // 3 x n
     int cap;
     int fp32;                    // mixed-precision contraction
+    // The SF kernel's pair record of the same list (null: walk the pairs).
+    const unsigned short* pairs;
+    int pcap;
 };
 
 }  // namespace hdnnp

@@ -111,8 +111,9 @@ __global__ __launch_bounds__(256) void k_nn(NNArgs a)
 // v_mfma_f32_16x16x4f32 (mixed path).  MODE 2: v_mfma_f32_16x16x16f16,
 // FP16 operands with FP32 accumulation.  MODE 3:
 // v_mfma_f32_16x16x16bf16_1k, BF16 operands (round to nearest even) in the
-// FP16 operand layout, FP32 accumulation.  Biases, activations and
-// normalisation in FP64 in every mode.
+// FP16 operand layout, FP32 accumulation.  Modes 2 and 3 split every
+// operand into a high and a low 16-bit part (three MFMAs per step, Split4
+// below).  Biases, activations and normalisation in FP64 in every mode.
 // Operand layouts (16x16 tiles, lane l, v < 4): A[i = l % 16][k = K0 (l /
 // 16) + v'], B[k = K0 (l / 16) + v'][j = l % 16] with K0 = 1 (FP64, FP32;
 // one value per lane) or 4 (FP16, v' < 4); D[i][j = l % 16] in register v
@@ -186,26 +187,59 @@ template <> struct NNMma<1>
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// FP16 and BF16 operands split in two: x = hi + lo with hi = round(x) and
+// lo = round(x - hi) in the 16-bit type.  A tile step is three MFMAs,
+// hi.hi + hi.lo + lo.hi (the lo.lo term is below the FP32 accumulation),
+// so the products carry about twice the 16-bit significand: 22 bits for
+// FP16, 16 bits for BF16, against 11 and 8 for a single operand.
+template <typename H>
+struct Split4
+{
+    H hi, lo;
+};
+
+__device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo)
+{
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+__device__ __forceinline__ float bf16_to_f32(short b)
+{
+    return __uint_as_float(((unsigned)(unsigned short)b) << 16);
+}
+
+__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo)
+{
+    hi = to_bf16(x);
+    lo = to_bf16(x - bf16_to_f32(hi));
+}
+
 template <> struct NNMma<2>
 {
     typedef float Act;
     typedef nn_f4 Acc;
+    typedef Split4<nn_h4> Op;
     static constexpr int K = 16;
-    __device__ static nn_h4 a(Act const* X, int ld, int nk, int k0, int lane)
+    __device__ static Op a(Act const* X, int ld, int nk, int k0, int lane)
     {
         int const m = lane & 15, k = k0 + 4 * (lane >> 4);
-        nn_h4 v;
+        Op v;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            v[q] = (_Float16)(k + q < nk ? X[m * ld + k + q] : 0.0f);
+        {
+            _Float16 h, l;
+            split_f16(k + q < nk ? X[m * ld + k + q] : 0.0f, h, l);
+            v.hi[q] = h;
+            v.lo[q] = l;
+        }
         return v;
     }
-    __device__ static nn_h4 b(ModelTables const& T, int off, bool trans,
-                              int ldw, int nk, int nj, int k0, int j0,
-                              int lane)
+    __device__ static Op b(ModelTables const& T, int off, bool trans, int ldw,
+                           int nk, int nj, int k0, int j0, int lane)
     {
         int const j = j0 + (lane & 15), k = k0 + 4 * (lane >> 4);
-        nn_h4 v;
+        Op v;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
@@ -213,13 +247,18 @@ template <> struct NNMma<2>
             if (k + q < nk && j < nj)
                 w = T.weights32[off + (trans ? j * ldw + k + q
                                              : (k + q) * ldw + j)];
-            v[q] = (_Float16)w;
+            _Float16 h, l;
+            split_f16(w, h, l);
+            v.hi[q] = h;
+            v.lo[q] = l;
         }
         return v;
     }
-    __device__ static Acc mma(nn_h4 a, nn_h4 b, Acc c)
+    __device__ static Acc mma(Op const& a, Op const& b, Acc c)
     {
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, c, 0, 0, 0);
     }
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
@@ -228,22 +267,27 @@ template <> struct NNMma<3>
 {
     typedef float Act;
     typedef nn_f4 Acc;
+    typedef Split4<nn_s4> Op;
     static constexpr int K = 16;
-    __device__ static nn_s4 a(Act const* X, int ld, int nk, int k0, int lane)
+    __device__ static Op a(Act const* X, int ld, int nk, int k0, int lane)
     {
         int const m = lane & 15, k = k0 + 4 * (lane >> 4);
-        nn_s4 v;
+        Op v;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            v[q] = to_bf16(k + q < nk ? X[m * ld + k + q] : 0.0f);
+        {
+            short h, l;
+            split_bf16(k + q < nk ? X[m * ld + k + q] : 0.0f, h, l);
+            v.hi[q] = h;
+            v.lo[q] = l;
+        }
         return v;
     }
-    __device__ static nn_s4 b(ModelTables const& T, int off, bool trans,
-                              int ldw, int nk, int nj, int k0, int j0,
-                              int lane)
+    __device__ static Op b(ModelTables const& T, int off, bool trans, int ldw,
+                           int nk, int nj, int k0, int j0, int lane)
     {
         int const j = j0 + (lane & 15), k = k0 + 4 * (lane >> 4);
-        nn_s4 v;
+        Op v;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
@@ -251,13 +295,19 @@ template <> struct NNMma<3>
             if (k + q < nk && j < nj)
                 w = T.weights32[off + (trans ? j * ldw + k + q
                                              : (k + q) * ldw + j)];
-            v[q] = to_bf16(w);
+            short h, l;
+            split_bf16(w, h, l);
+            v.hi[q] = h;
+            v.lo[q] = l;
         }
         return v;
     }
-    __device__ static Acc mma(nn_s4 a, nn_s4 b, Acc c)
+    __device__ static Acc mma(Op const& a, Op const& b, Acc c)
     {
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.lo, b.hi, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.lo, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, c, 0, 0,
+                                                         0);
     }
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };

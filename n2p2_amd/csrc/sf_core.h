@@ -137,14 +137,16 @@ struct FwdMembers
 
 // One triplet, all members of the group: acc[m] += fg * plambda * pfc with
 // fg = exp(-eta sum r^2) plambda^(zeta - 1) (SymGrpExpAngn.cpp:276-297).
-template <int MAXM, int CT, bool GENPOW, typename R>
+// ST: a staging record set with P (x, y, z, r) and Q (f_c, 1/r, ...).
+template <int MAXM, int CT, bool GENPOW, typename R, typename ST>
 __device__ __forceinline__ void triplet_forward(
-    int gi, Stage<R> const& S, int ja, int kb, bool wide, CutCfg const& cc,
+    int gi, ST const& S, int ja, int kb, bool wide, CutCfg const& cc,
     FwdMembers<R, MAXM> const& M, MParam<R> const* MP, int nmem,
     R (&acc)[MAXM])
 {
     Vec4<R> const pj = S.P[ja], pk = S.P[kb];
-    Vec2<R> const qj = S.Q[ja], qk = S.Q[kb];
+    auto const qj = S.Q[ja];
+    auto const qk = S.Q[kb];
     R const rij = pj.w, rik = pk.w;
     R pfc, r2sum, rjk = R(0);
     if (!wide)
@@ -240,6 +242,15 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
 
     stage_by_element(a, S, s0, n, sStart, sCnt);
 
+    // Pair record (mixed path): group counts first, all marked missing
+    // until their group is done; the pairs follow from kRecordGroups on.
+    unsigned short* __restrict__ rec =
+        a.pairs ? a.pairs + (size_t)i * (size_t)a.pcap : nullptr;
+    bool const record = rec && cap <= 256;
+    if (rec)
+        for (int g = lane; g < kRecordGroups; g += 64) rec[g] = kRecordMissing;
+    int pOff = kRecordGroups;   // next free record entry
+
     double rcCached = -1.0;
     bool wCached = false;
     double* __restrict__ Gi = a.G + (size_t)i * a.ldG;
@@ -310,6 +321,11 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                 M.lam[m] = uniform_r(p.lambda);
                 M.code[m] = uniform(p.flags);
             }
+            // Passing pairs of this group go to the record while they fit,
+            // in the order and lane positions the force kernel would meet
+            // them (staged indices j | k << 8).
+            bool keep = record && (gi - ed.g0) < kRecordGroups;
+            int nrec = 0;
             int qn = 0;
             int pc = 0, pr = 0;
             dp.start(lane, pc, pr);
@@ -332,6 +348,13 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                     triplet_forward<MAXM, CT, GENPOW>(gi, S, e & 0xffff,
                                                       e >> 16, wide, cc, M, MP,
                                                       nmem, acc);
+                    if (keep && pOff + nrec + 64 <= a.pcap)
+                    {
+                        rec[pOff + nrec + lane] = (unsigned short)(
+                            (e & 0xff) | ((e >> 16) << 8));
+                        nrec += 64;
+                    }
+                    else keep = false;
                     qn -= 64;
                     __syncthreads();
                 }
@@ -341,6 +364,18 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                 int const e = Q[lane];
                 triplet_forward<MAXM, CT, GENPOW>(gi, S, e & 0xffff, e >> 16,
                                                   wide, cc, M, MP, nmem, acc);
+            }
+            if (keep && pOff + nrec + qn <= a.pcap)
+            {
+                if (lane < qn)
+                {
+                    int const e = Q[lane];
+                    rec[pOff + nrec + lane] = (unsigned short)(
+                        (e & 0xff) | ((e >> 16) << 8));
+                }
+                nrec += qn;
+                if (lane == 0) rec[gi - ed.g0] = (unsigned short)nrec;
+                pOff += nrec;
             }
         }
         // Member totals by the transposing butterfly: member m ends in the

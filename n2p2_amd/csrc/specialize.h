@@ -31,6 +31,8 @@ std::string plan_source(Model::Flat const& F);
 
 // False when the environment sets N2P2_AMD_RTC=0.
 bool plan_enabled_by_default();
+// False when the environment sets N2P2_AMD_PAIR_RECORD=0.
+bool pair_record_enabled_by_default();
 
 }  // namespace hdnnp
 

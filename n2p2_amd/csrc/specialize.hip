@@ -206,6 +206,7 @@ std::string full_source(std::string const& plan, ModelTables const& T,
       << "namespace hdnnp {\nnamespace {\n" << plan << "}\n}\n"
       << "#include \"force_core.h\"\n"
       << "#include \"sf_core.h\"\n"
+
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
       << "__attribute__((amdgpu_waves_per_eu("
       << (fp32 && maxm <= 8 ? 4 : 1) << ", 8)))\n"
@@ -299,6 +300,7 @@ std::shared_ptr<Module> compile(std::string const& src, int fp32, int device)
         || hipModuleGetFunction(&m->k.sf, m->mod,
                                 ("hdnnp_plan_sf" + sfx).c_str()) != hipSuccess)
         throw std::runtime_error("hipModuleGetFunction failed");
+
     if (verbose())
     {
         double const s = std::chrono::duration<double>(
@@ -338,6 +340,12 @@ std::string plan_source(Model::Flat const& F)
     b << "    default: break;\n    }\n}\n\n";
     f << "    default: break;\n    }\n}\n\n";
     return b.str() + f.str();
+}
+
+bool pair_record_enabled_by_default()
+{
+    char const* v = std::getenv("N2P2_AMD_PAIR_RECORD");
+    return !(v && std::strcmp(v, "0") == 0);
 }
 
 bool plan_enabled_by_default()

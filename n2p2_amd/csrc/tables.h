@@ -28,6 +28,11 @@ constexpr int kMaxMembers    = 16;   // SF members per device group chunk
 constexpr int kMaxLayers     = 8;
 constexpr int kMaxCutCfg     = 4;    // distinct cutoff radii per element
 constexpr int kMaxPairGroups = 8;    // angular group chunks per element pair
+// Pair record of the mixed path: group chunks per element whose passing
+// neighbour pairs are recorded, and the count that marks a chunk whose
+// pairs were not recorded (the force kernel walks them again).
+constexpr int kRecordGroups = 64;
+constexpr unsigned short kRecordMissing = 0xffff;
 
 // CutoffFunction.h:42-72 numbering.
 enum CutType { CT_HARD = 0, CT_COS, CT_TANHU, CT_TANH, CT_EXP, CT_POLY1,

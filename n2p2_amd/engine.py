@@ -121,6 +121,8 @@ _SIGNATURES = [
     ("hdnnp_kalman_update", ctypes.c_int, [ctypes.c_void_p, _c_double_p, ctypes.c_double]),
     ("hdnnp_energy_weight_jacobian", ctypes.c_int,
      [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
+    ("hdnnp_ctx_set_pair_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hdnnp_ctx_pair_record_active", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_extrapolation_warning_list", ctypes.c_int64,
      [ctypes.c_void_p, ctypes.c_int64, _c_int32_p, _c_int32_p, _c_double_p,
@@ -265,6 +267,17 @@ class Context:
         enabled and compiled (compiles them if needed)."""
         v = ctypes.c_int(0)
         _check(_lib.hdnnp_ctx_specialized_active(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def set_pair_record(self, on):
+        """Mixed paths: the SF kernel records the passing neighbour pairs
+        and the force kernel replays them (default), or both enumerate."""
+        _check(_lib.hdnnp_ctx_set_pair_record(self._h, 1 if on else 0))
+
+    def pair_record_active(self):
+        """True when SF evaluations now record the pairs."""
+        v = ctypes.c_int32(0)
+        _check(_lib.hdnnp_ctx_pair_record_active(self._h, ctypes.byref(v)))
         return bool(v.value)
 
     def update_model(self):
