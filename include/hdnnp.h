@@ -268,6 +268,15 @@ int hdnnp_set_neighbor_list(hdnnp_ctx* c, int64_t ncentre,
  * f[centre_index] and every entry's pair force to f[nbr_index] (f has nout
  * rows of 3, internal units), centres in order, entries in list order. */
 int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f);
+/* No counterpart in the reference.  hdnnp_get_scattered_forces for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation): the same kernel and the
+ * same result bit for bit, with the forces staged in a fixed 12 MiB device
+ * array of the library instead of a buffer the context allocates.  At most
+ * 524,288 caller atoms may receive forces (more is HDNNP_E_CAPACITY).  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU
+ * race. */
+int hdnnp_get_scattered_forces_noalloc(hdnnp_ctx* c, int64_t nout, double* f);
 
 /* Device pointer of the n x 3 force array (valid until the next
  * hdnnp_set_structure); for callers that keep MD state on the GPU. */

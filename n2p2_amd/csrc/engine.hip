@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -2643,6 +2644,69 @@ int hdnnp_get_scattered_forces(hdnnp_ctx* c, int64_t nout, double* f)
                                     c->dCslot.p, c->dCself.p, c->dScatF.p,
                                     c->stream), "force scatter");
         copyOut(c, f, c->dScatF.p, 3 * nt);
+        c->sync();
+    });
+}
+
+// ----------------------------------------------- allocation-free scatter --
+
+namespace {
+
+// Staging of hdnnp_get_scattered_forces_noalloc: the forces of up to
+// kNoAllocTargets caller atoms, 3 doubles each (12 MiB).  Every GPU the
+// library runs on holds its own copy until the process ends.
+constexpr int64_t kNoAllocTargets = (int64_t)1 << 19;
+static __device__ __align__(256) double scratch[3 * kNoAllocTargets];
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the host-side name is not a device address).
+std::mutex g_scratchMutex;
+std::map<int, double*> g_scratchPtr;
+
+double* scratchFor(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    auto it = g_scratchPtr.find(device);
+    if (it != g_scratchPtr.end()) return it->second;
+    double* p = nullptr;
+    hipCheck(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)),
+             "hipGetSymbolAddress");
+    g_scratchPtr[device] = p;
+    return p;
+}
+
+}  // namespace
+
+int hdnnp_get_scattered_forces_noalloc(hdnnp_ctx* c, int64_t nout, double* f)
+{
+    return guarded([&]() {
+        if (!c || !c->externalNL)
+            throw Error(HDNNP_E_INVALID, "ERROR: No caller-given neighbour "
+                                         "list.\n");
+        if (!c->dCself.p || !c->dCslot.p)
+            throw Error(HDNNP_E_INVALID, "ERROR: No forces.\n");
+        if (nout < c->extTargets)
+            throw Error(HDNNP_E_INVALID, "ERROR: Centre or neighbour index "
+                                         "out of range (beyond the force "
+                                         "array).\n");
+        if (c->extTargets > kNoAllocTargets)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: More than 524288 caller "
+                                          "atoms receive forces: beyond the "
+                                          "staging of the allocation-free "
+                                          "scatter.\n");
+        c->setDevice();
+        double* const buf = scratchFor(c->device);
+        size_t const nt = (size_t)c->extTargets;
+        if (nt == 0) return;
+        // The same kernel as hdnnp_get_scattered_forces, staged in `scratch`
+        // (every row it reads is copied in first, so nothing needs clearing).
+        hipCheck(hipMemcpyAsync(buf, f, 3 * nt * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream), "forces");
+        hipCheck(launch_scatter_ext((int)nt, c->dScatOff.p, c->dScatSrc.p,
+                                    c->dCslot.p, c->dCself.p, buf,
+                                    c->stream), "force scatter");
+        hipCheck(hipMemcpyAsync(f, buf, 3 * nt * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream), "forces");
         c->sync();
     });
 }

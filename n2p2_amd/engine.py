@@ -123,6 +123,8 @@ _SIGNATURES = [
      [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
     ("hdnnp_ctx_set_pair_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hdnnp_ctx_pair_record_active", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_get_scattered_forces_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, _c_double_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_extrapolation_warning_list", ctypes.c_int64,
      [ctypes.c_void_p, ctypes.c_int64, _c_int32_p, _c_int32_p, _c_double_p,
@@ -467,6 +469,14 @@ class Context:
         f = np.zeros((nout, 3))
         _check(_lib.hdnnp_get_scattered_forces(self._h, nout,
                                                _ptr(f, ctypes.c_double)))
+        return f
+
+    def scattered_forces_noalloc(self, nout):
+        """scattered_forces through the entry point that allocates nothing
+        (staging in the library's fixed device array)."""
+        f = np.zeros((nout, 3))
+        _check(_lib.hdnnp_get_scattered_forces_noalloc(
+            self._h, nout, _ptr(f, ctypes.c_double)))
         return f
 
     def set_positions(self, r):
