@@ -115,10 +115,32 @@ def run_path(ctx, precision, steps, warmup):
         energy = ctx.evaluate(engine.FORCES_REFERENCE)
     dt = (time.perf_counter() - t0) / steps
     kt = {k: ctx.profile_avg_ms(k)[0]
-          for k in ("neighbor", "symfunc", "network", "force", "gather")}
+          for k in ("neighbor", "partner", "symfunc", "network", "force",
+                    "gather")}
     ctx.profile(False)
     forces = ctx.forces()
     return dt, energy, forces, kt, spec
+
+
+def tune(ctx, precision, log, label):
+    """Sweeps the launch forms (member chunk, waves-per-SIMD bounds of the
+    specialised SF and force kernels, network tiles per wave) on the loaded
+    box and applies the fastest; untimed.  Returns the choice, the sweep
+    time and the rows (columns Context.TUNE_COLUMNS)."""
+    ctx.set_precision(precision)
+    t0 = time.perf_counter()
+    rows = ctx.autotune(engine.FORCES_REFERENCE, sweep=True, reps=10)
+    secs = time.perf_counter() - t0
+    log("tuning %s: %d settings in %.1f s, choice (chunk, sf waves, force "
+        "waves, nn tiles) = %s" % (label, len(rows), secs, ctx.tuning()))
+    log("  " + " ".join("%11s" % c for c in engine.Context.TUNE_COLUMNS))
+    for row in rows:
+        log("  " + " ".join("%11.4f" % x for x in row))
+    return {"choice": dict(zip(("member_chunk", "sf_waves_per_simd",
+                                "force_waves_per_simd", "nn_tiles_per_wave"),
+                               ctx.tuning())),
+            "sweep_seconds": secs, "settings_timed": len(rows),
+            "rows": [[round(float(x), 4) for x in row] for row in rows]}
 
 
 def run_batch(tmp, steps, warmup, log):
@@ -151,7 +173,8 @@ def run_batch(tmp, steps, warmup, log):
             ctx.evaluate(engine.FORCES_REFERENCE)
         dt = (time.perf_counter() - t0) / steps
         kt = {k: ctx.profile_avg_ms(k)[0]
-              for k in ("neighbor", "symfunc", "network", "force", "gather")}
+              for k in ("neighbor", "partner", "symfunc", "network",
+                        "force", "gather")}
         ctx.profile(False)
         res[name] = (ctx.forces(), ctx.structure_energies())
         out[name] = {"atom_steps_per_s": ctx.n / dt, "ms_per_step": dt * 1e3,
@@ -171,7 +194,7 @@ def run_batch(tmp, steps, warmup, log):
 
 
 def run_md(model, r, elem, box, precision, steps, warmup, skin=1.0,
-           dt_fs=0.25, temp=300.0):
+           dt_fs=0.25, temp=300.0, tuned=False):
     """NVE MD of the same box (velocity Verlet, Verlet-skin list, HIP-graph
     steps): `warmup` untimed steps, then `steps` timed ones.  md_run
     returns after its stream has drained, so the timed region is
@@ -179,6 +202,8 @@ def run_md(model, r, elem, box, precision, steps, warmup, skin=1.0,
     ctx = engine.Context(model)
     ctx.set_precision(precision)
     ctx.set_structure(r, elem, box)
+    if tuned:
+        ctx.autotune()   # the choice cached by the sweep of this process
     masses = np.array(W.WATER_MASSES)
     v = W.maxwell_boltzmann(masses[elem], temp, seed=20261202)
     ctx.md_init(masses, dt_fs, skin=skin, v=v)
@@ -338,11 +363,13 @@ def other_rooflines(ktm, kt64, counts, nnf, n):
     positions read, index, image key, dr, d and partner written per entry)
     and the network layers against the MFMA peak of their operand type
     (algorithmic flops: forward and backward GEMMs), both from the HIP-event
-    kernel times of the timed steps."""
+    kernel times of the timed steps.  The partner map runs on a second
+    stream beside the SF kernel; its time is added to the list kernels'
+    here (kernel time, not wall time)."""
     nb = costmodel.neighbor_bytes(n, int(counts[0]))
     out = {}
     for name, kt in (("mixed", ktm), ("plain_fp64", kt64)):
-        ms_nl = kt["neighbor"]
+        ms_nl = kt["neighbor"] + kt.get("partner", 0.0)
         ms_nn = kt["network"]
         peak = PEAK_MFMA_FP32_TFLOPS if name == "mixed" \
             else PEAK_MFMA_FP64_TFLOPS
@@ -408,6 +435,8 @@ def main():
     ap.add_argument("--nmol", type=int, default=65536)
     ap.add_argument("--box", type=float, default=124.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in launch forms (no sweep)")
     ap.add_argument("--c4-frames", type=int, default=2000)
     ap.add_argument("--c4-epochs", type=int, default=20)
     ap.add_argument("--c4-outer", type=int, default=2000)
@@ -454,9 +483,14 @@ def main():
                          args.steps, args.warmup)
             log("MD mixed only: %s (no JSON line)" % res)
             return
+        tuning = {}
+        if not args.no_tune and args.path in ("both", "fp64"):
+            tuning["fp64"] = tune(ctx, engine.PRECISION_FP64, log, "fp64")
         if args.path != "mixed":
             dt64, e64, f64, kt64, sp64 = run_path(
                 ctx, engine.PRECISION_FP64, args.steps, args.warmup)
+        if not args.no_tune and args.path in ("both", "mixed"):
+            tuning["mixed"] = tune(ctx, engine.PRECISION_MIXED, log, "mixed")
             log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
                 % (dt64 * 1e3, e64, kt64))
         if args.path != "fp64":
@@ -468,6 +502,8 @@ def main():
             log("single-path run: no JSON line")
             ctx.close()
             return
+        # The FP16 / BF16 legs run the mixed path's SF and force kernels:
+        # they keep its tuning (the context's current one).
         dt16, e16, f16, kt16, _ = run_path(ctx, engine.PRECISION_MIXED16,
                                            args.steps, args.warmup)
         dtb, eb, fb, ktb, _ = run_path(ctx, engine.PRECISION_MIXED_BF16,
@@ -501,7 +537,7 @@ def main():
         for name, prec in (("fp64", engine.PRECISION_FP64),
                            ("mixed", engine.PRECISION_MIXED)):
             md[name] = run_md(model, r, elem, box, prec, md_steps,
-                              args.warmup)
+                              args.warmup, tuned=name in tuning)
             log("MD %s: %s" % (name, md[name]))
         md["mixed_vs_plain_fp64"] = (md["mixed"]["atom_steps_per_s"]
                                      / md["fp64"]["atom_steps_per_s"])
@@ -516,8 +552,9 @@ def main():
             "train_seconds": ttrain,
             "train_loss_first_epoch": list(hist[0]),
             "train_loss_last_epoch": list(hist[-1]),
-            "multiple_time_step": "switched off (an unresolved device memory "
-                                  "fault in its runs); not measured"}
+            "multiple_time_step": "measured by bench.py --path mts (config "
+                                  "C4: trained surrogate, r-RESPA with a "
+                                  "force-RMSE gate)"}
         md["what"] = ("NVE velocity Verlet on the same box, dt 0.25 fs, "
                       "300 K Maxwell-Boltzmann start, 1.0 A Verlet skin "
                       "(rebuild when an atom moved > 0.5 A), list filtered "
@@ -562,16 +599,19 @@ def main():
                                    "kernel_ms": kt16,
                                    "error_vs_fp64": err16,
                                    "what": "as the mixed path with FP16 "
-                                           "MFMA network layers (FP32 "
-                                           "accumulation)"},
+                                           "MFMA network layers (operands "
+                                           "split hi + lo, three MFMAs per "
+                                           "step, FP32 accumulation)"},
                        "mixed_bf16": {"atom_steps_per_s": n / dtb,
                                       "ms_per_step": dtb * 1e3,
                                       "kernel_ms": ktb,
                                       "error_vs_fp64": errbf,
                                       "what": "as the mixed path with BF16 "
-                                              "MFMA network layers (FP32 "
-                                              "accumulation)"},
+                                              "MFMA network layers (operands "
+                                              "split hi + lo, three MFMAs per "
+                                              "step, FP32 accumulation)"},
                        "mixed_vs_plain_fp64": dt64 / dtm,
+                       "tuning": tuning,
                        "md": md,
                        "batch_512_structures": batch,
                        "mixed_error_vs_fp64": err,

@@ -144,6 +144,32 @@ int  hdnnp_ctx_pair_record_active(hdnnp_ctx* c, int* active);
  * hdnnp_model_set_scaling) to the context's device tables. */
 int  hdnnp_ctx_update_model(hdnnp_ctx* c);
 
+/* Tuning of the kernels' launch forms to the model and the hardware.  No
+ * counterpart in the reference (its CPU kernels have none).  The knobs, as
+ * four ints: SF members per group chunk of the device tables (1..16,
+ * default 16), least waves per SIMD of the specialised SF and force
+ * kernels (0 = the compiler's choice, -1 = built-in default, up to 8) and
+ * 16-atom tiles per wave of the MFMA networks (1, 2 or 4).  None changes
+ * what is computed; the member chunking fixes the order of the per-member
+ * force terms, so forces may differ in the last bits between chunkings. */
+#define HDNNP_TUNE_SWEEP 1 /* sweep even if a choice is cached */
+/* Sweeps the knobs on the loaded structure (and its neighbour list, built
+ * with the model's cutoff if there is none yet) at the current precision:
+ * every member chunking (that changes the groups) with
+ * six paired waves bounds, timing SF and force kernels of `reps`
+ * evaluations (force_mode HDNNP_FORCES_*), then network tiles 1, 2, 4 at
+ * the fastest kernels.  Applies the fastest and caches it for the process
+ * per (model shape, precision, device); without HDNNP_TUNE_SWEEP a cached
+ * choice is applied at once (*nrows = 0, no structure needed).  Rows of 8
+ * doubles into table (up to cap rows; *nrows = rows swept): chunk, SF
+ * waves, force waves, network tiles, SF ms, force ms, network ms, chosen
+ * (1: the SF choice, 2: the force choice, 3: both).  Ends an MD run. */
+int  hdnnp_ctx_autotune(hdnnp_ctx* c, int force_mode, int flags, int reps,
+                        double* table, int cap, int* nrows);
+/* The context's knobs (out4) / sets them (in4; -1 waves = default).  */
+int  hdnnp_ctx_tuning(hdnnp_ctx* c, int* out4);
+int  hdnnp_ctx_set_tuning(hdnnp_ctx* c, const int* in4);
+
 /* --------------------------------------------------------- structure -- */
 
 /* Loads a structure: n atoms, positions r (n x 3, row-major), element
@@ -416,8 +442,10 @@ int64_t hdnnp_sf_statistics_raw(hdnnp_ctx* c, int e, double* out);
 
 /* Average device time in milliseconds of each kernel class over the calls
  * since the last reset, measured with HIP events on the context's stream.
- * names: "neighbor", "symfunc", "network", "force", "gather".  Returns -1
- * for an unknown name.  enable != 0 turns event recording on. */
+ * names: "neighbor", "symfunc", "network", "force", "gather" and "partner"
+ * (the partner map of a list build, which runs on the context's second
+ * stream while the SF, network and force kernels run).  Returns -1 for an
+ * unknown name.  enable != 0 turns event recording on. */
 int    hdnnp_profile_enable(hdnnp_ctx* c, int enable);
 double hdnnp_profile_avg_ms(hdnnp_ctx* c, const char* name, int64_t* calls);
 int    hdnnp_profile_reset(hdnnp_ctx* c);

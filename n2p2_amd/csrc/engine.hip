@@ -89,9 +89,9 @@ template <class T> struct DBuf
 };
 
 enum KClass { K_NEIGHBOR = 0, K_SYMFUNC, K_NETWORK, K_FORCE, K_GATHER,
-              K_NUM };
+              K_PARTNER, K_NUM };
 const char* const kClassNames[K_NUM] = {"neighbor", "symfunc", "network",
-                                        "force", "gather"};
+                                        "force", "gather", "partner"};
 
 }  // namespace
 
@@ -195,12 +195,18 @@ struct hdnnp_ctx
     bool specialized = plan_enabled_by_default();
     PlanKernels plan[2];
     bool planTried[2] = {false, false};
+    // Tuning (hdnnp_ctx_autotune): SF members per group chunk of the device
+    // tables, waves-per-SIMD bounds of the specialised kernels, 16-atom
+    // tiles per wave of the MFMA networks.
+    int chunk = kMaxMembers;
+    PlanTuning tune;
+    int nnTiles = 1;
     PlanKernels planFor(int fp32)
     {
         if (!specialized) return PlanKernels();
         if (!planTried[fp32])
         {
-            plan[fp32] = plan_kernels(flat, T, fp32, device);
+            plan[fp32] = plan_kernels(flat, T, fp32, device, tune);
             planTried[fp32] = true;
         }
         return plan[fp32];
@@ -278,19 +284,34 @@ struct hdnnp_ctx
         hipCheck(hipEventCreate(&e), "hipEventCreate");
         return e;
     }
-    void begin(int cls, hipEvent_t& a)
+    void begin(int cls, hipEvent_t& a, hipStream_t on = nullptr)
     {
         (void)cls;
         if (!prof || capturing) return;
         a = getEvent();
-        hipCheck(hipEventRecord(a, stream), "hipEventRecord");
+        hipCheck(hipEventRecord(a, on ? on : stream), "hipEventRecord");
     }
-    void end(int cls, hipEvent_t a)
+    void end(int cls, hipEvent_t a, hipStream_t on = nullptr)
     {
         if (!prof || capturing) return;
         hipEvent_t b = getEvent();
-        hipCheck(hipEventRecord(b, stream), "hipEventRecord");
+        hipCheck(hipEventRecord(b, on ? on : stream), "hipEventRecord");
         pending.push_back(Pending{cls, a, b});
+    }
+
+    // The partner map of a list build runs on a second stream (default
+    // priority), concurrently with the SF, network and force kernels that
+    // do not read it; the gather, the next build and the MD rebuild wait
+    // for it (waitPartner).
+    hipStream_t side = nullptr;
+    hipEvent_t evListDone = nullptr, evPartnerDone = nullptr;
+    bool partnerPending = false;
+    void waitPartner()
+    {
+        if (!partnerPending) return;
+        hipCheck(hipStreamWaitEvent(stream, evPartnerDone, 0),
+                 "hipStreamWaitEvent");
+        partnerPending = false;
     }
     void collect()
     {
@@ -309,6 +330,8 @@ struct hdnnp_ctx
     void sync()
     {
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (side)
+            hipCheck(hipStreamSynchronize(side), "hipStreamSynchronize");
         collect();
     }
 
@@ -316,7 +339,7 @@ struct hdnnp_ctx
 
     void uploadModel()
     {
-        flat = model->m.flatten();
+        flat = model->m.flatten(chunk);
         planTried[0] = planTried[1] = false;
         auto up = [&](auto& buf, auto const& v) {
             buf.ensure(v.size());
@@ -456,8 +479,39 @@ void cellWidths(double const b[3][3], double rc, double proj[3], int pbc[3])
 // Structure.cpp:287-300); open structures use none.  The list uses fixed
 // per-atom slot ranges like the cell path: a rebuild of the same batch is
 // one fill pass; the first build (or an overflow) counts first.
+// The partner map of the list just built (k_partner), on the side stream
+// after the list kernels of the main stream; waitPartner() orders later
+// readers after it.
+void launchPartner(hdnnp_ctx* c, int n)
+{
+    if (!c->side)
+    {
+        hipCheck(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking),
+                 "hipStreamCreate");
+        hipCheck(hipEventCreateWithFlags(&c->evListDone,
+                                         hipEventDisableTiming),
+                 "hipEventCreate");
+        hipCheck(hipEventCreateWithFlags(&c->evPartnerDone,
+                                         hipEventDisableTiming),
+                 "hipEventCreate");
+    }
+    hipCheck(hipEventRecord(c->evListDone, c->stream), "hipEventRecord");
+    hipCheck(hipStreamWaitEvent(c->side, c->evListDone, 0),
+             "hipStreamWaitEvent");
+    hipEvent_t ev = nullptr;
+    c->begin(K_PARTNER, ev, c->side);
+    hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
+                               c->dCellAtoms.p, c->dPartner.p, c->entryCap,
+                               c->side),
+             "nl partner");
+    c->end(K_PARTNER, ev, c->side);
+    hipCheck(hipEventRecord(c->evPartnerDone, c->side), "hipEventRecord");
+    c->partnerPending = true;
+}
+
 void buildBatchNeighborList(hdnnp_ctx* c, double rc)
 {
+    c->waitPartner();
     int const n = (int)c->n;
     int64_t const ns = c->nstruct;
     hipStream_t s = c->stream;
@@ -550,11 +604,8 @@ void buildBatchNeighborList(hdnnp_ctx* c, double rc)
             throw Error(HDNNP_E_DEVICE,
                         "ERROR: Neighbour count changed between passes.\n");
     }
-    hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
-                               c->dCellAtoms.p, c->dPartner.p,
-                               c->entryCap, s),
-             "nl partner");
     c->end(K_NEIGHBOR, ev0);
+    launchPartner(c, n);
     c->slabLayout = true;
     c->Mvalid = false;
     c->maxNeigh = std::min(hstat[0], c->slab);
@@ -566,6 +617,7 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
 {
     requireStructure(c);
     c->setDevice();
+    c->waitPartner();
     if (c->batched)
     {
         buildBatchNeighborList(c, rc);
@@ -768,11 +820,8 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
     }
     c->slabLayout = cellKernels;
     c->Mvalid = !cellKernels;
-    hipCheck(launch_nl_partner(n, c->dOff.p, c->dCount.p, c->dKeys.p,
-                               c->dCellAtoms.p, c->dPartner.p,
-                               c->entryCap, s),
-             "nl partner");
     c->end(K_NEIGHBOR, ev0);
+    launchPartner(c, n);
     // Slot capacity of the SF and force kernels: the largest count (the
     // slab bounds it; a rebuild above raised the slab if it was exceeded).
     c->maxNeigh = cellKernels ? std::min(maxc, c->slab) : maxc;
@@ -905,7 +954,10 @@ void computeNN(hdnnp_ctx* c, bool deriv)
         a.mode = c->precision == HDNNP_PRECISION_MIXED_BF16 ? 3
                : c->precision == HDNNP_PRECISION_MIXED16 ? 2
                : c->precision == HDNNP_PRECISION_MIXED ? 1 : 0;
-        if (nn_mfma_lds_bytes(a.ed, a.mode) > 65536) a.mode = -1;
+        a.tiles = c->nnTiles;
+        while (a.tiles > 1 && nn_mfma_lds_bytes(a.ed, a.mode, a.tiles) > 65536)
+            a.tiles /= 2;
+        if (nn_mfma_lds_bytes(a.ed, a.mode, 1) > 65536) a.mode = -1;
         a.e = (int)e;
         a.atoms = c->dAtomsOfElem[e].p;
         a.na = (int)c->hAtomsOfElem[e].size();
@@ -1021,6 +1073,7 @@ void launchGather(hdnnp_ctx* c, int mode)
     }
     else if (!c->externalNL)
     {
+        c->waitPartner();
         c->begin(K_GATHER, ev2);
         hipCheck(launch_force_gather((int)c->n, c->dOff.p, c->dCount.p,
                                      c->dNbrJ.p, c->dPartner.p, c->dCslot.p,
@@ -1113,6 +1166,7 @@ void mdRebuild(hdnnp_ctx* c)
     hipCheck(launch_md_wrap(n, c->dR.p, m.cell, s), "md wrap");
     c->hostPosValid = false;
     buildNeighborList(c, m.rc + m.skin);
+    c->waitPartner();   // before any graph is captured on the main stream
     std::swap(c->dKeys, m.skinKeys);
     std::swap(c->dPartner, m.skinPartner);
     std::swap(c->dCount, m.skinCnt);
@@ -1361,7 +1415,10 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
     }
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (auto e : c->eventPool) (void)hipEventDestroy(e);
+    if (c->evListDone) (void)hipEventDestroy(c->evListDone);
+    if (c->evPartnerDone) (void)hipEventDestroy(c->evPartnerDone);
     c->dElem.release(); c->dCfg.release(); c->dGroup.release();
     c->dMember.release(); c->dRun.release(); c->dScale.release(); c->dPairGroups.release();
     c->dCompGroup.release(); c->dCompMember.release();
@@ -1401,6 +1458,7 @@ void hdnnp_ctx_destroy(hdnnp_ctx* c)
         m.hFlags = nullptr;
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -1468,12 +1526,247 @@ int hdnnp_ctx_update_model(hdnnp_ctx* c)
     });
 }
 
+namespace {
+
+// A tuning choice, kept per (model shape, precision, device) for the
+// process: contexts of the same model adopt it without sweeping again.
+struct TuneChoice
+{
+    int chunk, sfWaves, forceWaves, nnTiles;
+};
+std::mutex g_tuneMutex;
+std::map<std::string, TuneChoice> g_tuned;
+
+// The model's shape: the generated member code of its default tables (SF
+// types, groups, cutoffs) and the network widths, not its weights.
+std::string tuneKey(hdnnp_ctx* c)
+{
+    Model::Flat const F = c->model->m.flatten();
+    std::string k = std::to_string(c->device) + ":"
+                  + std::to_string(c->precision) + ":";
+    for (auto const& e : F.elem)
+    {
+        for (int l = 0; l < e.nlayers; ++l)
+            k += std::to_string(e.width[l]) + ",";
+        k += ";";
+    }
+    return k + plan_source(F);
+}
+
+void applyTuning(hdnnp_ctx* c, TuneChoice const& t)
+{
+    if (t.chunk != c->chunk)
+    {
+        hipCheck(hipStreamSynchronize(c->stream), "sync");
+        c->chunk = t.chunk;
+        c->uploadModel();
+    }
+    if (t.sfWaves != c->tune.sfWaves || t.forceWaves != c->tune.forceWaves)
+    {
+        c->tune.sfWaves = t.sfWaves;
+        c->tune.forceWaves = t.forceWaves;
+        c->planTried[0] = c->planTried[1] = false;
+    }
+    c->nnTiles = t.nnTiles;
+    c->haveSF = c->haveNN = c->haveNNDeriv = false;
+    c->pairsValid = false;
+}
+
+// Mean SF, force and network times (ms) of `reps` evaluations after one
+// untimed evaluation (which also compiles the specialised kernels).
+void timeEvaluations(hdnnp_ctx* c, int mode, int reps, double* ms)
+{
+    computeSF(c, false);
+    computeNN(c, true);
+    computeEnergy(c);
+    computeForces(c, mode);
+    c->sync();
+    for (int k = 0; k < K_NUM; ++k) c->profMs[k] = 0.0, c->profCalls[k] = 0;
+    bool const prof = c->prof;
+    c->prof = true;
+    for (int r = 0; r < reps; ++r)
+    {
+        computeSF(c, false);
+        computeNN(c, true);
+        computeEnergy(c);
+        computeForces(c, mode);
+    }
+    c->sync();
+    c->prof = prof;
+    int const cls[3] = {K_SYMFUNC, K_FORCE, K_NETWORK};
+    for (int q = 0; q < 3; ++q)
+        ms[q] = c->profCalls[cls[q]]
+              ? c->profMs[cls[q]] / c->profCalls[cls[q]] : 0.0;
+    for (int k = 0; k < K_NUM; ++k) c->profMs[k] = 0.0, c->profCalls[k] = 0;
+}
+
+}  // namespace
+
+int hdnnp_ctx_autotune(hdnnp_ctx* c, int force_mode, int flags, int reps,
+                       double* table, int cap, int* nrows)
+{
+    return guarded([&]() {
+        if (!c || !nrows || (cap > 0 && !table))
+            throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        if (reps < 1)
+            throw Error(HDNNP_E_INVALID, "ERROR: reps must be positive.\n");
+        if (force_mode < 0 || force_mode > 1)
+            throw Error(HDNNP_E_INVALID, "ERROR: Unknown force mode.\n");
+        *nrows = 0;
+        c->setDevice();
+        mdEnd(c);
+        std::string const key = tuneKey(c);
+        if (!(flags & HDNNP_TUNE_SWEEP))
+        {
+            std::lock_guard<std::mutex> lock(g_tuneMutex);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end())
+            {
+                applyTuning(c, it->second);
+                return;
+            }
+        }
+        requireStructure(c);
+        if (!c->haveNL && !c->externalNL)
+            buildNeighborList(c, c->T.maxCutoff);
+        requireNL(c);
+        int const fp32 = c->precision != HDNNP_PRECISION_FP64;
+        std::vector<double> rows;
+        auto addRow = [&](TuneChoice const& t, double const* ms) {
+            double const r[8] = {(double)t.chunk, (double)t.sfWaves,
+                                 (double)t.forceWaves, (double)t.nnTiles,
+                                 ms[0], ms[1], ms[2], 0.0};
+            rows.insert(rows.end(), r, r + 8);
+        };
+        // Stage 1: member chunk x waves bounds.  SF and force kernels are
+        // timed separately, so candidate q pairs the q-th SF bound with the
+        // q-th force bound (one compilation for both).
+        int const chunks[3] = {4, 8, 16};
+        int const sfW[6] = {0, 4, 5, 6, 7, 8};
+        int const fW[6] = {0, 2, 3, 4, 5, 6};
+        TuneChoice best{c->chunk, -1, -1, 1};
+        double bestTotal = 1e300;
+        size_t lastGroups = 0;
+        for (int ch : chunks)
+        {
+            size_t ng = 0;
+            try
+            {
+                ng = c->model->m.flatten(ch).group.size();
+            }
+            catch (std::exception const&)
+            {
+                continue;   // more chunks per element pair than supported
+            }
+            if (ng == lastGroups) continue;   // same partition as before
+            lastGroups = ng;
+            double bestSf = 1e300, bestF = 1e300;
+            int wSf = -1, wF = -1;
+            for (int q = 0; q < 6; ++q)
+            {
+                TuneChoice const t{ch, sfW[q], fW[q], 1};
+                applyTuning(c, t);
+                if (c->specialized)
+                {
+                    PlanKernels const pk = c->planFor(fp32);
+                    if (!pk.sf || !pk.force)
+                    {
+                        if (q > 0) break;   // generic kernels: no bounds
+                    }
+                }
+                double ms[3];
+                timeEvaluations(c, force_mode, reps, ms);
+                addRow(t, ms);
+                if (ms[0] < bestSf) bestSf = ms[0], wSf = sfW[q];
+                if (ms[1] < bestF) bestF = ms[1], wF = fW[q];
+                if (!c->specialized) break;
+            }
+            if (bestSf + bestF < bestTotal)
+            {
+                bestTotal = bestSf + bestF;
+                best = TuneChoice{ch, wSf, wF, 1};
+            }
+        }
+        // Stage 2: network tiles per wave at the chosen kernels.
+        double bestNN = 1e300;
+        int tiles = 1;
+        for (int nt : {1, 2, 4})
+        {
+            TuneChoice t = best;
+            t.nnTiles = nt;
+            applyTuning(c, t);
+            double ms[3];
+            timeEvaluations(c, force_mode, reps, ms);
+            addRow(t, ms);
+            if (ms[2] < bestNN) bestNN = ms[2], tiles = nt;
+        }
+        best.nnTiles = tiles;
+        applyTuning(c, best);
+        for (size_t r = 0; r < rows.size(); r += 8)
+            if ((int)rows[r] == best.chunk && (int)rows[r + 1] == best.sfWaves
+                && (int)rows[r + 3] == best.nnTiles)
+                rows[r + 7] += 1.0;
+        for (size_t r = 0; r < rows.size(); r += 8)
+            if ((int)rows[r] == best.chunk && (int)rows[r + 2] == best.forceWaves
+                && (int)rows[r + 3] == best.nnTiles)
+                rows[r + 7] += 2.0;
+        {
+            std::lock_guard<std::mutex> lock(g_tuneMutex);
+            g_tuned[key] = best;
+        }
+        int const nr = (int)(rows.size() / 8);
+        int const nc = std::min(nr, std::max(cap, 0));
+        if (nc > 0) std::memcpy(table, rows.data(), (size_t)nc * 8 * sizeof(double));
+        *nrows = nr;
+    });
+}
+
+int hdnnp_ctx_tuning(hdnnp_ctx* c, int* out4)
+{
+    return guarded([&]() {
+        if (!c || !out4) throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        out4[0] = c->chunk;
+        out4[1] = c->tune.sfWaves;
+        out4[2] = c->tune.forceWaves;
+        out4[3] = c->nnTiles;
+    });
+}
+
+int hdnnp_ctx_set_tuning(hdnnp_ctx* c, const int* in4)
+{
+    return guarded([&]() {
+        if (!c || !in4) throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        if (in4[0] < 1 || in4[0] > kMaxMembers)
+            throw Error(HDNNP_E_INVALID, "ERROR: Member chunk out of range.\n");
+        if (in4[1] < -1 || in4[1] > 8 || in4[2] < -1 || in4[2] > 8)
+            throw Error(HDNNP_E_INVALID,
+                        "ERROR: Waves per SIMD out of range.\n");
+        if (in4[3] != 1 && in4[3] != 2 && in4[3] != 4)
+            throw Error(HDNNP_E_INVALID,
+                        "ERROR: Network tiles per wave must be 1, 2 or 4.\n");
+        c->setDevice();
+        mdEnd(c);
+        // Validate the chunk before anything changes.
+        try
+        {
+            (void)c->model->m.flatten(in4[0]);
+        }
+        catch (std::exception const& e)
+        {
+            throw Error(HDNNP_E_INVALID, e.what());
+        }
+        applyTuning(c, TuneChoice{in4[0], in4[1], in4[2], in4[3]});
+    });
+}
+
 // Forgets the loaded structure or caller-given list: every compute call
 // then fails with "No structure loaded" until a new load succeeds.  Loads
 // call this first, so a load rejected by its checks leaves no stale sizes
 // behind for the kernels.
 static void dropStructure(hdnnp_ctx* c)
 {
+    c->waitPartner();
+    c->sync();   // the side stream no longer reads the list buffers
     mdEnd(c);
     c->n = 0;
     c->M = 0;

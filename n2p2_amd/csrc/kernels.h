@@ -215,9 +215,10 @@ hipError_t launch_force_gather(int n, const int* off, const int* cnt,
                                double* F, hipStream_t s);
 
 // Atomic networks for the atoms of one element.  mode -1: per-atom FP64
-// kernel (any width); 0: FP64 MFMA tiles; 1: FP32 MFMA tiles; 2: FP16 MFMA
-// tiles with FP32 accumulation.  The MFMA kernels need
-// nn_mfma_lds_bytes(ed, mode) of LDS.
+// kernel (any width); 0: FP64 MFMA tiles; 1: FP32 MFMA tiles; 2 / 3: FP16 /
+// BF16 MFMA tiles (hi + lo operand split) with FP32 accumulation.  The MFMA
+// kernels run `tiles` (1, 2 or 4) tiles of 16 atoms per wave and need
+// nn_mfma_lds_bytes(ed, mode, tiles) of LDS.
 struct NNArgs
 {
     ModelTables T;
@@ -232,9 +233,10 @@ struct NNArgs
     double* dEdG;                // n x ldG (indexed by atom)
     double* scratch;             // sum(width[l]) * 3 * na doubles
     int deriv;
+    int tiles;                   // 16-atom tiles per wave (MFMA modes)
 };
 hipError_t launch_nn(NNArgs const& a, hipStream_t s);
-size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode);
+size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode, int tiles);
 
 hipError_t launch_energy_sum(const double* E, int n, double* out, hipStream_t s);
 

@@ -312,7 +312,9 @@ template <> struct NNMma<3>
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <int MODE>
+// One wave computes R tiles of 16 atoms: every B operand (a weight tile)
+// is loaded once and used for the R tiles' MFMAs.
+template <int MODE, int R>
 __global__ __launch_bounds__(64) void k_nn_mfma(NNArgs a)
 {
     typedef NNMma<MODE> MM;
@@ -324,104 +326,127 @@ __global__ __launch_bounds__(64) void k_nn_mfma(NNArgs a)
     ElemDesc const& ed = T.elem[a.e];
     int const L = ed.nlayers;
     int const lane = threadIdx.x;
-    int const tile0 = blockIdx.x * 16;
-    // LDS: value planes of layers 0..L-1, then derivative planes of 1..L-1,
-    // each [16][width].
+    int const tile0 = blockIdx.x * 16 * R;
+    // LDS per tile: value planes of layers 0..L-1, then derivative planes
+    // of 1..L-1, each [16][width]; tile r at r * tstride.
     int voff[kMaxLayers], doff[kMaxLayers];
+    int tstride;
     {
         int o = 0;
         for (int l = 0; l < L; ++l) { voff[l] = o; o += 16 * ed.width[l]; }
         for (int l = 1; l < L; ++l) { doff[l] = o; o += 16 * ed.width[l]; }
+        tstride = o;
     }
-    // Input layer: the tile's scaled SFs (zero rows past the last atom).
+    // Input layer: the tiles' scaled SFs (zero rows past the last atom).
     {
         int const nin = ed.width[0];
-        for (int q = lane; q < 16 * nin; q += 64)
+        for (int q = lane; q < 16 * R * nin; q += 64)
         {
             int const m = q / nin, k = q - m * nin;
             int const t = tile0 + m;
             double g = 0.0;
             if (t < a.na) g = a.G[(size_t)a.atoms[t] * a.ldG + k];
-            act[voff[0] + q] = (Act)g;
+            act[(m >> 4) * tstride + voff[0] + (m & 15) * nin + k] = (Act)g;
         }
     }
     __syncthreads();
     for (int l = 1; l < L; ++l)
     {
         int const nin = ed.width[l - 1], nout = ed.width[l];
-        Act const* X = act + voff[l - 1];
         for (int j0 = 0; j0 < nout; j0 += 16)
         {
-            Acc acc;
+            Acc acc[R];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] = 0;
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[r][v] = 0;
             for (int k0 = 0; k0 < nin; k0 += MM::K)
-                acc = MM::mma(MM::a(X, nin, nin, k0, lane),
-                              MM::b(T, ed.woff[l], true, nin, nin, nout, k0,
-                                    j0, lane),
-                              acc);
+            {
+                auto const b = MM::b(T, ed.woff[l], true, nin, nin, nout, k0,
+                                     j0, lane);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    acc[r] = MM::mma(MM::a(act + r * tstride + voff[l - 1],
+                                           nin, nin, k0, lane),
+                                     b, acc[r]);
+            }
             int const j = j0 + (lane & 15);
             if (j < nout)
             {
                 double const bias = T.weights[ed.boff[l] + j];
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                {
-                    int const m = MM::row(lane, v);
-                    double z = (double)acc[v] + bias;
-                    if (ed.normalize) z /= nin;
-                    double val, d;
-                    activate(ed.act[l], z, val, d);
-                    act[voff[l] + m * nout + j] = (Act)val;
-                    act[doff[l] + m * nout + j] = (Act)d;
-                    if (l == L - 1 && tile0 + m < a.na)
-                        a.E[a.atoms[tile0 + m]] = val;   // one output neuron
-                }
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                    {
+                        int const m = MM::row(lane, v);
+                        double z = (double)acc[r][v] + bias;
+                        if (ed.normalize) z /= nin;
+                        double val, d;
+                        activate(ed.act[l], z, val, d);
+                        Act* const P = act + r * tstride;
+                        P[voff[l] + m * nout + j] = (Act)val;
+                        P[doff[l] + m * nout + j] = (Act)d;
+                        int const t = tile0 + 16 * r + m;
+                        if (l == L - 1 && t < a.na)
+                            a.E[a.atoms[t]] = val;   // one output neuron
+                    }
             }
         }
         __syncthreads();
     }
     if (!a.deriv) return;
     // Output delta (one neuron): f' / fan-in with normalize_nodes.
-    if (lane < 16)
+    for (int q = lane; q < 16 * R; q += 64)
     {
-        double g = (double)act[doff[L - 1] + lane];
+        Act* const P = act + (q >> 4) * tstride + doff[L - 1];
+        double g = (double)P[q & 15];
         if (ed.normalize) g /= ed.width[L - 2];
-        act[doff[L - 1] + lane] = (Act)g;
+        P[q & 15] = (Act)g;
     }
     __syncthreads();
     for (int l = L - 1; l >= 1; --l)
     {
         int const nin = ed.width[l - 1], nout = ed.width[l];
-        Act const* Dl = act + doff[l];
         for (int j0 = 0; j0 < nin; j0 += 16)
         {
-            Acc acc;
+            Acc acc[R];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] = 0;
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[r][v] = 0;
             for (int k0 = 0; k0 < nout; k0 += MM::K)
-                acc = MM::mma(MM::a(Dl, nout, nout, k0, lane),
-                              MM::b(T, ed.woff[l], false, nin, nout, nin, k0,
-                                    j0, lane),
-                              acc);
+            {
+                auto const b = MM::b(T, ed.woff[l], false, nin, nout, nin, k0,
+                                     j0, lane);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    acc[r] = MM::mma(MM::a(act + r * tstride + doff[l], nout,
+                                           nout, k0, lane),
+                                     b, acc[r]);
+            }
             int const j = j0 + (lane & 15);
             if (j < nin)
             {
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                {
-                    int const m = MM::row(lane, v);
-                    if (l >= 2)
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
                     {
-                        int const q = doff[l - 1] + m * nin + j;
-                        double g = (double)act[q];
-                        if (ed.normalize) g /= ed.width[l - 2];
-                        act[q] = (Act)((double)acc[v] * g);
+                        int const m = MM::row(lane, v);
+                        int const t = tile0 + 16 * r + m;
+                        if (l >= 2)
+                        {
+                            int const q = r * tstride + doff[l - 1] + m * nin
+                                        + j;
+                            double g = (double)act[q];
+                            if (ed.normalize) g /= ed.width[l - 2];
+                            act[q] = (Act)((double)acc[r][v] * g);
+                        }
+                        else if (t < a.na)
+                            a.dEdG[(size_t)a.atoms[t] * a.ldG + j]
+                                = (double)acc[r][v];
                     }
-                    else if (tile0 + m < a.na)
-                        a.dEdG[(size_t)a.atoms[tile0 + m] * a.ldG + j]
-                            = (double)acc[v];
-                }
             }
         }
         __syncthreads();
@@ -479,28 +504,40 @@ hipError_t launch_energy_segments(const double* E, const int* structOff,
     return hipGetLastError();
 }
 
-size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode)
+size_t nn_mfma_lds_bytes(ElemDesc const& ed, int mode, int tiles)
 {
     size_t w = 0;
     for (int l = 0; l < ed.nlayers; ++l) w += ed.width[l];
     for (int l = 1; l < ed.nlayers; ++l) w += ed.width[l];
-    return 16 * w * (mode >= 1 ? sizeof(float) : sizeof(double));
+    return (size_t)tiles * 16 * w
+         * (mode >= 1 ? sizeof(float) : sizeof(double));
 }
+
+namespace {
+
+template <int MODE>
+void launch_nn_tiles(NNArgs const& a, hipStream_t s)
+{
+    size_t const shm = nn_mfma_lds_bytes(a.ed, a.mode, a.tiles);
+    int const nb = (a.na + 16 * a.tiles - 1) / (16 * a.tiles);
+    if (a.tiles == 4) k_nn_mfma<MODE, 4><<<nb, 64, shm, s>>>(a);
+    else if (a.tiles == 2) k_nn_mfma<MODE, 2><<<nb, 64, shm, s>>>(a);
+    else k_nn_mfma<MODE, 1><<<nb, 64, shm, s>>>(a);
+}
+
+}  // namespace
 
 hipError_t launch_nn(NNArgs const& a, hipStream_t s)
 {
     if (a.na <= 0) return hipSuccess;
+    if (a.mode >= 0 && a.tiles != 1 && a.tiles != 2 && a.tiles != 4)
+        return hipErrorInvalidValue;
     if (a.mode < 0)
         k_nn<<<(a.na + 255) / 256, 256, 0, s>>>(a);
-    else
-    {
-        size_t const shm = nn_mfma_lds_bytes(a.ed, a.mode);
-        int const nb = (a.na + 15) / 16;
-        if (a.mode == 3) k_nn_mfma<3><<<nb, 64, shm, s>>>(a);
-        else if (a.mode == 2) k_nn_mfma<2><<<nb, 64, shm, s>>>(a);
-        else if (a.mode == 1) k_nn_mfma<1><<<nb, 64, shm, s>>>(a);
-        else k_nn_mfma<0><<<nb, 64, shm, s>>>(a);
-    }
+    else if (a.mode == 3) launch_nn_tiles<3>(a, s);
+    else if (a.mode == 2) launch_nn_tiles<2>(a, s);
+    else if (a.mode == 1) launch_nn_tiles<1>(a, s);
+    else launch_nn_tiles<0>(a, s);
     return hipGetLastError();
 }
 

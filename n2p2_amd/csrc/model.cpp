@@ -664,8 +664,10 @@ void Model::setScaling(int e, int type, double smin, double smax,
     }
 }
 
-Model::Flat Model::flatten() const
+Model::Flat Model::flatten(int chunk) const
 {
+    if (chunk < 1 || chunk > kMaxMembers)
+        throw std::runtime_error("ERROR: Group chunk size out of range.\n");
     Flat F;
     int const ne = (int)el.size();
     F.pairCount.assign((size_t)ne * ne * ne, 0);
@@ -736,7 +738,7 @@ Model::Flat Model::flatten() const
                         if (x.lambda != y.lambda) return x.lambda < y.lambda;
                         return x.zeta < y.zeta;
                     });
-                for (size_t c0 = 0; c0 < mem.size(); c0 += kMaxMembers)
+                for (size_t c0 = 0; c0 < mem.size(); c0 += (size_t)chunk)
                 {
                     GroupDesc gd{};
                     gd.type = g.type;
@@ -747,7 +749,8 @@ Model::Flat Model::flatten() const
                     gd.cfg = (int)(std::find(rcs.begin(), rcs.end(), g.rc)
                                    - rcs.begin());
                     gd.mem0 = (int)F.member.size();
-                    size_t const c1 = std::min(mem.size(), c0 + kMaxMembers);
+                    size_t const c1 = std::min(mem.size(),
+                                               c0 + (size_t)chunk);
                     gd.nmem = (int)(c1 - c0);
                     // zeta = 2^level: plambda^(zeta-1) is the level-th
                     // prefix product x x^2 x^4 ... of pow_int's squaring

@@ -99,7 +99,8 @@ public:
         std::vector<int>        pairGroups, pairCount;
         std::vector<double>     weights;
     };
-    Flat flatten() const;
+    // chunk: members per device group chunk (at most kMaxMembers).
+    Flat flatten(int chunk = kMaxMembers) const;
 
     std::vector<ElementHost> el;
     double maxCutoffRadius = 0.0;

@@ -222,7 +222,8 @@ __device__ __forceinline__ void triplet_forward(
 template <int MAXM, int CT, bool GENPOW, typename R>
 __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
 {
-    static_assert(MAXM == 8 || MAXM == 16, "member chunks of 8 or 16");
+    static_assert(MAXM == 4 || MAXM == 8 || MAXM == 16,
+                  "member chunks of 4, 8 or 16");
     extern __shared__ __align__(16) double sm[];
     __shared__ int sStart[kMaxElements], sCnt[kMaxElements];
     __shared__ MParam<R> MP[kMaxMembers];
@@ -384,7 +385,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
 #pragma unroll
         for (int m = 0; m < MAXM; ++m) dv[m] = (double)acc[m];
         double const v = wave_sum_transposed<MAXM>(dv);
-        constexpr int kSh = (MAXM == 8) ? 3 : 2;   // 6 - log2(MAXM)
+        constexpr int kSh = MAXM == 4 ? 4 : MAXM == 8 ? 3 : 2;   // 6 - log2
         int const m = lane >> kSh;
         if ((lane & ((1 << kSh) - 1)) == 0 && m < nmem)
         {

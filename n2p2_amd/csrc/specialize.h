@@ -20,10 +20,20 @@ struct PlanKernels
     hipFunction_t force = nullptr;
 };
 
+// Occupancy bounds of the specialised kernels: the least waves per SIMD
+// the compiler must leave room for (amdgpu_waves_per_eu), 0 = its own
+// choice, -1 = the built-in defaults.  The autotuner's knobs, with the
+// group chunk size of the model tables.
+struct PlanTuning
+{
+    int sfWaves = -1;
+    int forceWaves = -1;
+};
+
 // Specialised kernels for the model (fp32: mixed-precision arithmetic) on
 // the given device, compiled on first request and kept for the process.
 PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T, int fp32,
-                         int device);
+                         int device, PlanTuning const& tune = PlanTuning());
 
 // The generated member-loop code (the part of the kernel source that
 // depends on the model).

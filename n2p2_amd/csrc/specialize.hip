@@ -183,7 +183,10 @@ void emit_forward_case(std::ostringstream& o, int gi, GroupDesc const& g,
     o << "        break;\n    }\n";
 }
 
-int plan_maxm(ModelTables const& T) { return T.maxGroupMembers <= 8 ? 8 : 16; }
+int plan_maxm(ModelTables const& T)
+{
+    return T.maxGroupMembers <= 4 ? 4 : T.maxGroupMembers <= 8 ? 8 : 16;
+}
 
 int plan_cut(ModelTables const& T)
 {
@@ -191,8 +194,32 @@ int plan_cut(ModelTables const& T)
     return (c == CT_COS || c == CT_TANHU || c == CT_POLY2) ? c : -1;
 }
 
+// The waves-per-SIMD bounds of a tuning, with the built-in defaults for
+// -1: FP32 chunks of at most 8 members run the SF kernel at seven waves
+// (72 VGPRs; 3% faster than the compiler's choice on the 196,608-atom water
+// box) and the force kernel at four or more; other forms leave it to the
+// compiler.
+int sf_waves(PlanTuning const& t, int fp32, int maxm)
+{
+    if (t.sfWaves >= 0) return t.sfWaves;
+    return fp32 && maxm <= 8 ? 7 : 0;
+}
+
+int force_waves(PlanTuning const& t, int fp32, int maxm)
+{
+    if (t.forceWaves >= 0) return t.forceWaves;
+    return fp32 && maxm <= 8 ? 4 : 0;
+}
+
+std::string waves_attr(int w)
+{
+    if (w <= 0) return "";
+    return "__attribute__((amdgpu_waves_per_eu(" + std::to_string(w)
+         + ", 8)))\n";
+}
+
 std::string full_source(std::string const& plan, ModelTables const& T,
-                        int fp32)
+                        int fp32, PlanTuning const& tune)
 {
     std::ostringstream o;
     char const* R = fp32 ? "float" : "double";
@@ -208,20 +235,13 @@ std::string full_source(std::string const& plan, ModelTables const& T,
       << "#include \"sf_core.h\"\n"
 
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
-      << "__attribute__((amdgpu_waves_per_eu("
-      << (fp32 && maxm <= 8 ? 4 : 1) << ", 8)))\n"
+      << waves_attr(force_waves(tune, fp32, maxm))
       << "void hdnnp_plan_force_" << (fp32 ? "f32" : "f64")
       << "(hdnnp::ForceArgs a)\n"
       << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
       << R << ">(a); }\n"
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
-      // FP32 with up to 8 members per group chunk: at most 72 VGPRs, seven
-      // waves per SIMD (measured 3% faster than the compiler's choice on
-      // the 196,608-atom water box; eight waves 7% slower).  Chunks of 16
-      // members need more registers (the bound doubled the SF time of a
-      // G2 + G9 model).
-      << (fp32 && maxm <= 8 ? "__attribute__((amdgpu_waves_per_eu(7, 8)))\n"
-                            : "")
+      << waves_attr(sf_waves(tune, fp32, maxm))
       << "void hdnnp_plan_sf_" << (fp32 ? "f32" : "f64")
       << "(hdnnp::SFArgs a)\n"
       << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
@@ -355,9 +375,9 @@ bool plan_enabled_by_default()
 }
 
 PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T,
-                         int fp32, int device)
+                         int fp32, int device, PlanTuning const& tune)
 {
-    std::string const src = full_source(plan_source(F), T, fp32);
+    std::string const src = full_source(plan_source(F), T, fp32, tune);
     // Keyed by the full source (a few tens of KB): no collisions.
     std::string const key = std::to_string(device) + ":" + src;
     std::lock_guard<std::mutex> lock(g_mutex);

@@ -123,6 +123,11 @@ _SIGNATURES = [
      [ctypes.c_void_p, _c_double_p, ctypes.c_int]),
     ("hdnnp_ctx_set_pair_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hdnnp_ctx_pair_record_active", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_ctx_autotune", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p,
+      ctypes.c_int, _c_int32_p]),
+    ("hdnnp_ctx_tuning", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_ctx_set_tuning", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_get_scattered_forces_noalloc", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, _c_double_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
@@ -284,6 +289,34 @@ class Context:
 
     def update_model(self):
         _check(_lib.hdnnp_ctx_update_model(self._h))
+
+    TUNE_COLUMNS = ("chunk", "sf_waves", "force_waves", "nn_tiles", "sf_ms",
+                    "force_ms", "nn_ms", "chosen")
+
+    def autotune(self, force_mode=FORCES_CONSISTENT, sweep=False, reps=3):
+        """Times the kernels' launch forms on the loaded structure and list
+        and applies the fastest (or, unless sweep, a choice cached in this
+        process for the same model shape and precision).  Returns the sweep
+        rows as a float array with columns TUNE_COLUMNS (empty when a cached
+        choice was applied)."""
+        cap = 64
+        table = np.zeros((cap, 8), dtype=np.float64)
+        n = ctypes.c_int32(0)
+        _check(_lib.hdnnp_ctx_autotune(self._h, int(force_mode),
+                                       1 if sweep else 0, int(reps),
+                                       _ptr(table, ctypes.c_double), cap,
+                                       ctypes.byref(n)))
+        return table[:min(n.value, cap)].copy()
+
+    def tuning(self):
+        """(member chunk, SF waves, force waves, network tiles)."""
+        v = np.zeros(4, dtype=np.int32)
+        _check(_lib.hdnnp_ctx_tuning(self._h, _ptr(v, ctypes.c_int32)))
+        return tuple(int(x) for x in v)
+
+    def set_tuning(self, chunk=16, sf_waves=-1, force_waves=-1, nn_tiles=1):
+        v = np.array([chunk, sf_waves, force_waves, nn_tiles], dtype=np.int32)
+        _check(_lib.hdnnp_ctx_set_tuning(self._h, _ptr(v, ctypes.c_int32)))
 
     def set_structure(self, r, elem, box=None):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)

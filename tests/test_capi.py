@@ -101,3 +101,20 @@ def test_context_without_device_fails_cleanly():
     else:  # pragma: no cover - there is no device 999
         c.close()
         pytest.fail("device 999 accepted")
+
+
+def test_bindings_marshal_arguments_without_a_context():
+    """The ctypes wrappers of the tuning and allocation-free entry points
+    pass their arrays with the declared pointer types: called on a null
+    context, each reaches the library and comes back with its null-argument
+    error (no device is touched), not a ctypes argument error."""
+    engine.load_library()
+    c = object.__new__(engine.Context)
+    c._h = None
+    for call in (lambda: c.tuning(),
+                 lambda: c.set_tuning(8, -1, -1, 1),
+                 lambda: c.autotune(sweep=True),
+                 lambda: c.autotune(),
+                 lambda: c.scattered_forces_noalloc(4)):
+        with pytest.raises(RuntimeError, match="ERROR"):
+            call()
