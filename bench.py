@@ -258,24 +258,28 @@ def _drift(etot, n):
 
 
 def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
-           epochs=20, nouter=2000, gate=5e-3):
+           epochs=20, nouter=2000, gate=5e-3, chunk=100):
     """The learned surrogate for multiple time steps on the 196,608-atom
     box: frames of the full model's own 300 K NVE trajectory (mixed path,
     dt 0.25 fs, every `stride` steps), the distilled radial surrogate (4 eta
     x 2 neighbour elements G2, 8-16-16-1 softplus) trained on them with the
     engine's gradient and Adam kernels (lr 1e-3, `epochs` epochs, one frame
-    per step), then `nouter` outer steps of 1 fs (4 surrogate substeps of
-    0.25 fs) of r-RESPA: with the force-RMSE gate, without it, without it
-    for the untrained surrogate, and the full model's plain velocity Verlet
-    at 0.25 fs over the same time.  Throughput counts 0.25 fs steps."""
+    per step), then, from the trajectory's last state, `nouter` outer
+    steps of 1 fs (4 surrogate substeps of 0.25 fs) of r-RESPA: with the
+    force-RMSE gate, without it, without it for the untrained surrogate,
+    without it with eager steps instead of HIP graphs, and the full model's
+    plain velocity Verlet at 0.25 fs over the same time.  Throughput counts
+    0.25 fs steps."""
     masses = np.array(W.WATER_MASSES)
     n = len(r)
     out = {"frames": nframes, "frame_stride_steps": stride, "epochs": epochs,
            "outer_steps": nouter, "gate_eV_A": gate}
     t0 = time.perf_counter()
-    R, E, F = S.generate_frames(model, r, elem, box, masses, nframes,
-                                stride=stride, dt=0.25, temp=300.0,
-                                seed=4242, precision=engine.PRECISION_MIXED)
+    R, E, F, last = S.generate_frames(model, r, elem, box, masses, nframes,
+                                      stride=stride, dt=0.25, temp=300.0,
+                                      seed=4242,
+                                      precision=engine.PRECISION_MIXED,
+                                      final_state=True)
     out["frame_generation_s"] = time.perf_counter() - t0
     log("C4: %d frames in %.1f s" % (nframes, out["frame_generation_s"]))
     sd = os.path.join(tmp, "surrogate_c4")
@@ -300,25 +304,43 @@ def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
     ctx.close()
     del R, F
     trained = engine.Model.load_dir(td)
-    v = W.maxwell_boltzmann(masses[elem], 300.0, seed=777)
+    # The runs below continue the frames' trajectory from its last state:
+    # the generated box starts far from equilibrium (lattice oxygens,
+    # random orientations) and heats up over its first picosecond, which
+    # the frames cover.
+    r, v = last
 
-    def mts(sur_model, g, label):
+    def mts(sur_model, g, label, graph=True):
         full = engine.Context(model)
         sur = engine.Context(sur_model)
         full.set_precision(engine.PRECISION_MIXED)
         for c in (full, sur):
             c.set_structure(r, elem, box)
         sur.mts_init(full, masses, dt_outer=1.0, n_inner=4, skin=1.0, v=v,
-                     rmse_gate=g)
+                     rmse_gate=g, graph=graph)
+        # In chunks of outer steps (the run continues across calls), with a
+        # progress line per chunk.
+        parts = []
         t1 = time.perf_counter()
         try:
-            ep, ek, rm, md = sur.mts_run(nouter)
+            done = 0
+            while done < nouter:
+                k = min(chunk, nouter - done)
+                parts.append(sur.mts_run(k))
+                done += k
+                log("C4 %s: %d/%d outer steps, %.1f s, rebuilds full %d "
+                    "surrogate %d, fallbacks %d"
+                    % (label, done, nouter, time.perf_counter() - t1,
+                       full.md_rebuilds(), sur.md_rebuilds(),
+                       int(sum(int(np.sum(p[3] == 1)) for p in parts))))
         except RuntimeError as err:
             full.close()
             sur.close()
             log("C4 %s: %s" % (label, err))
             return {"failed": str(err).strip()}
         dt = time.perf_counter() - t1
+        ep, ek, rm, md = (np.concatenate([p[q] for p in parts])
+                          for q in range(4))
         res = {"atom_steps_per_s": n * 4 * nouter / dt,
                "ms_per_outer_step": dt / nouter * 1e3,
                "fallback_fraction": float(np.mean(md == 1)),
@@ -334,18 +356,31 @@ def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
     out["mts_no_gate"] = mts(trained, 1e30, "MTS without gate")
     out["mts_no_gate_untrained"] = mts(untrained, 1e30,
                                        "MTS without gate, untrained")
+    # The same trained run with eager steps instead of HIP graphs: the
+    # graphs are captured again after every list rebuild.
+    out["mts_no_gate_eager"] = mts(trained, 1e30, "MTS without gate, eager",
+                                   graph=False)
     ctx = engine.Context(model)
     ctx.set_precision(engine.PRECISION_MIXED)
     ctx.set_structure(r, elem, box)
     ctx.md_init(masses, 0.25, skin=1.0, v=v)
     t1 = time.perf_counter()
-    ep, ek = ctx.md_run(4 * nouter)
+    parts = []
+    done = 0
+    while done < 4 * nouter:
+        k = min(4 * chunk, 4 * nouter - done)
+        parts.append(ctx.md_run(k))
+        done += k
+        log("C4 full model: %d/%d steps, %.1f s"
+            % (done, 4 * nouter, time.perf_counter() - t1))
     dt = time.perf_counter() - t1
+    ep = np.concatenate([p[0] for p in parts])
+    ek = np.concatenate([p[1] for p in parts])
     out["full_verlet_0.25fs"] = {"atom_steps_per_s": n * 4 * nouter / dt,
                                  "drift": _drift((ep + ek)[3::4], n)}
     ctx.close()
     log("C4 full model: %s" % out["full_verlet_0.25fs"])
-    for k in ("mts_gated", "mts_no_gate"):
+    for k in ("mts_gated", "mts_no_gate", "mts_no_gate_eager"):
         if "failed" in out[k]:
             continue
         out[k]["speedup_vs_full"] = (out[k]["atom_steps_per_s"]

@@ -151,7 +151,7 @@ int  hdnnp_ctx_update_model(hdnnp_ctx* c);
  * kernels (0 = the compiler's choice, -1 = built-in default, up to 8) and
  * 16-atom tiles per wave of the MFMA networks (1, 2 or 4).  None changes
  * what is computed; the member chunking fixes the order of the per-member
- * force terms, so forces may differ in the last bits between chunkings. */
+ * terms, so results may differ in the last bits between chunkings. */
 #define HDNNP_TUNE_SWEEP 1 /* sweep even if a choice is cached */
 /* Sweeps the knobs on the loaded structure (and its neighbour list, built
  * with the model's cutoff if there is none yet) at the current precision:

@@ -646,10 +646,13 @@ __device__ __forceinline__ bool pair_in_range(V4 const& pj, V4 const& pk,
     return (pj.w < rc) & (pk.w < rc) & (wide | (d2 < rc2));
 }
 
-// Number of set bits of a 64-lane ballot below this lane.
+// Number of set bits of a 64-lane ballot below this lane (two mbcnt
+// instructions).
 __device__ __forceinline__ int lanes_below(uint64_t mask)
 {
-    return __popcll(mask & ((1ull << lane_id()) - 1ull));
+    return (int)__builtin_amdgcn_mbcnt_hi(
+        (unsigned)(mask >> 32),
+        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
 // Weight of staged neighbour t for the weighted SF types: the atomic number

@@ -216,6 +216,10 @@ struct hdnnp_ctx
     // lives in skinKeys / skinPartner / skinCnt; every step filters it into
     // the ordinary list buffers.  Graph A: kick, drift, filter, flags to
     // the host.  Graph B: the force evaluation, kick, energy record.
+    // MD flags: [0] moved more than half the skin, [1] filtered list over
+    // capacity, [2] unstable, [3] partner misses, [4] the last rebuild's
+    // wrap shifted atoms.
+    static constexpr int kMdFlags = 5;
     struct MDState
     {
         bool on = false;
@@ -224,6 +228,8 @@ struct hdnnp_ctx
         bool useGraph = true;
         int cap = 0;
         int64_t rebuilds = 0, steps = 0;
+        // Rebuilds whose wrap shifted atoms by lattice vectors (flags[4]).
+        int64_t wrapMoves = 0;
         MDCell cell{};
         DBuf<double> v, r0, accel, mass, part, epot, ekin;
         DBuf<int> flags, counter, skinPartner, skinCnt, newIdx;
@@ -1163,7 +1169,8 @@ void mdRebuild(hdnnp_ctx* c)
     int const n = (int)c->n;
     hipStream_t s = c->stream;
     mdDropGraphs(c);
-    hipCheck(launch_md_wrap(n, c->dR.p, m.cell, s), "md wrap");
+    hipCheck(hipMemsetAsync(m.flags.p + 4, 0, sizeof(int), s), "md flags");
+    hipCheck(launch_md_wrap(n, c->dR.p, m.cell, m.flags.p + 4, s), "md wrap");
     c->hostPosValid = false;
     buildNeighborList(c, m.rc + m.skin);
     c->waitPartner();   // before any graph is captured on the main stream
@@ -1186,7 +1193,10 @@ void mdRebuild(hdnnp_ctx* c)
     hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "md max");
     hipCheck(hipMemcpyAsync(&maxc, c->dStats.p, sizeof(int),
                             hipMemcpyDeviceToHost, s), "md max");
+    hipCheck(hipMemcpyAsync(m.hFlags + 4, m.flags.p + 4, sizeof(int),
+                            hipMemcpyDeviceToHost, s), "md wrap flag");
     hipCheck(hipStreamSynchronize(s), "sync");
+    if (m.hFlags[4]) m.wrapMoves += 1;
     m.cap = std::min(c->maxNeigh, maxc + maxc / 8 + 8);
     c->maxNeigh = m.cap;
     c->dCslot.ensure(3 * E);
@@ -1536,6 +1546,7 @@ struct TuneChoice
 };
 std::mutex g_tuneMutex;
 std::map<std::string, TuneChoice> g_tuned;
+constexpr int kTuneCols = 8;   // columns of a sweep row
 
 // The model's shape: the generated member code of its default tables (SF
 // types, groups, cutoffs) and the network widths, not its weights.
@@ -1633,10 +1644,10 @@ int hdnnp_ctx_autotune(hdnnp_ctx* c, int force_mode, int flags, int reps,
         int const fp32 = c->precision != HDNNP_PRECISION_FP64;
         std::vector<double> rows;
         auto addRow = [&](TuneChoice const& t, double const* ms) {
-            double const r[8] = {(double)t.chunk, (double)t.sfWaves,
-                                 (double)t.forceWaves, (double)t.nnTiles,
-                                 ms[0], ms[1], ms[2], 0.0};
-            rows.insert(rows.end(), r, r + 8);
+            double const r[kTuneCols] = {
+                (double)t.chunk, (double)t.sfWaves, (double)t.forceWaves,
+                (double)t.nnTiles, ms[0], ms[1], ms[2], 0.0};
+            rows.insert(rows.end(), r, r + kTuneCols);
         };
         // Stage 1: member chunk x waves bounds.  SF and force kernels are
         // timed separately, so candidate q pairs the q-th SF bound with the
@@ -1702,21 +1713,26 @@ int hdnnp_ctx_autotune(hdnnp_ctx* c, int force_mode, int flags, int reps,
         }
         best.nnTiles = tiles;
         applyTuning(c, best);
-        for (size_t r = 0; r < rows.size(); r += 8)
-            if ((int)rows[r] == best.chunk && (int)rows[r + 1] == best.sfWaves
-                && (int)rows[r + 3] == best.nnTiles)
-                rows[r + 7] += 1.0;
-        for (size_t r = 0; r < rows.size(); r += 8)
-            if ((int)rows[r] == best.chunk && (int)rows[r + 2] == best.forceWaves
-                && (int)rows[r + 3] == best.nnTiles)
-                rows[r + 7] += 2.0;
+        auto same = [&](size_t r) {
+            return (int)rows[r] == best.chunk
+                && (int)rows[r + 3] == best.nnTiles;
+        };
+        for (size_t r = 0; r < rows.size(); r += kTuneCols)
+        {
+            if (same(r) && (int)rows[r + 1] == best.sfWaves)
+                rows[r + kTuneCols - 1] += 1.0;
+            if (same(r) && (int)rows[r + 2] == best.forceWaves)
+                rows[r + kTuneCols - 1] += 2.0;
+        }
         {
             std::lock_guard<std::mutex> lock(g_tuneMutex);
             g_tuned[key] = best;
         }
-        int const nr = (int)(rows.size() / 8);
+        int const nr = (int)(rows.size() / kTuneCols);
         int const nc = std::min(nr, std::max(cap, 0));
-        if (nc > 0) std::memcpy(table, rows.data(), (size_t)nc * 8 * sizeof(double));
+        if (nc > 0)
+            std::memcpy(table, rows.data(),
+                        (size_t)nc * kTuneCols * sizeof(double));
         *nrows = nr;
     });
 }
@@ -2188,14 +2204,18 @@ int hdnnp_md_init(hdnnp_ctx* c, const double* masses, double dt, double skin,
         m.accel.ensure(ne);
         m.mass.ensure(ne);
         m.part.ensure(md_mv2_blocks(n));
-        m.flags.ensure(4);
+        m.flags.ensure(hdnnp_ctx::kMdFlags);
         m.counter.ensure(1);
         if (!m.hFlags)
-            hipCheck(hipHostMalloc((void**)&m.hFlags, 4 * sizeof(int),
+            hipCheck(hipHostMalloc((void**)&m.hFlags,
+                                   hdnnp_ctx::kMdFlags * sizeof(int),
                                    hipHostMallocDefault), "hipHostMalloc");
-        m.hFlags[0] = m.hFlags[1] = m.hFlags[2] = m.hFlags[3] = 0;
+        for (int k = 0; k < hdnnp_ctx::kMdFlags; ++k) m.hFlags[k] = 0;
+        m.wrapMoves = 0;
         hipStream_t s = c->stream;
-        hipCheck(hipMemsetAsync(m.flags.p, 0, 4 * sizeof(int), s), "flags");
+        hipCheck(hipMemsetAsync(m.flags.p, 0,
+                                hdnnp_ctx::kMdFlags * sizeof(int), s),
+                 "flags");
         // The step record's counter indexes epot / ekin: it must start at
         // zero also for runs that never call hdnnp_md_run (the substeps of
         // hdnnp_mts_run record through it too).
@@ -2324,6 +2344,57 @@ static double mtsRmse(hdnnp_ctx* sur)
     return std::sqrt(v / (3.0 * (double)n));
 }
 
+// Both contexts keep the same positions and lists built at the same
+// positions, so that two identical models give forces equal bit for bit.
+// When the leader rebuilds inside its substeps, the follower rebuilds its
+// list right away at the leader's (wrapped) positions; a wrap of wrapped
+// positions shifts nothing, and should rounding shift an atom after all,
+// the leader takes the follower's positions and rebuilds as well.
+static void mtsRebuildWith(hdnnp_ctx* follower, hdnnp_ctx* leader)
+{
+    int const n = (int)follower->n;
+    hipCheck(hipStreamSynchronize(leader->stream), "sync");
+    hipCheck(hipMemcpyAsync(follower->dR.p, leader->dR.p,
+                            3 * (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToDevice, follower->stream),
+             "positions");
+    int64_t const wm = follower->md.wrapMoves;
+    mdRebuild(follower);
+    if (follower->md.wrapMoves == wm) return;
+    hipCheck(hipStreamSynchronize(follower->stream), "sync");
+    hipCheck(hipMemcpyAsync(leader->dR.p, follower->dR.p,
+                            3 * (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToDevice, leader->stream),
+             "positions");
+    mdRebuild(leader);
+    mdForces(leader, true);
+}
+
+// The leader's substeps, each followed by the follower's rebuild when the
+// leader rebuilt.
+static void mtsSubsteps(hdnnp_ctx* leader, hdnnp_ctx* follower, int nsub)
+{
+    for (int k = 0; k < nsub; ++k)
+    {
+        int64_t const rb = leader->md.rebuilds;
+        mdStep(leader);
+        if (leader->md.rebuilds != rb) mtsRebuildWith(follower, leader);
+    }
+}
+
+// After the substeps the follower takes the leader's positions and
+// evaluates its forces there.  When its own check rebuilds it, the leader
+// rebuilds at the same positions and evaluates again.
+static void mtsFollow(hdnnp_ctx* follower, hdnnp_ctx* leader)
+{
+    int64_t const rb = follower->md.rebuilds;
+    mdEvalAt(follower, leader->dR.p, false);
+    hipCheck(hipStreamSynchronize(follower->stream), "sync");
+    if (follower->md.rebuilds == rb) return;
+    mdEvalAt(leader, follower->dR.p, true);
+    hipCheck(hipStreamSynchronize(leader->stream), "sync");
+}
+
 // One outer step.  r-RESPA (Tuckerman, Berne & Martyna 1992) with the
 // surrogate as the fast force and F_full - F_sur as the slow force:
 // half kick with the slow force, n_inner velocity-Verlet substeps on the
@@ -2343,15 +2414,9 @@ static void mtsOuter(hdnnp_ctx* sur, double* rmseOut, int* modeOut)
         hipCheck(launch_md_kick_diff(n, sur->md.v.p, full->dF.p, sur->dF.p,
                                      sur->dElemAtom.p, sur->md.accel.p, h,
                                      sur->stream), "slow kick");
-        int64_t const rb = sur->md.rebuilds;
-        for (int k = 0; k < S.nInner; ++k) mdStep(sur);
-        bool const surRebuilt = sur->md.rebuilds != rb;
+        mtsSubsteps(sur, full, S.nInner);
         hipCheck(hipStreamSynchronize(sur->stream), "sync");
-        if (mdEvalAt(full, sur->dR.p, surRebuilt))
-        {
-            hipCheck(hipStreamSynchronize(full->stream), "sync");
-            mdEvalAt(sur, full->dR.p, true);
-        }
+        mtsFollow(full, sur);
         double const rmse = mtsRmse(sur);
         hipCheck(launch_md_kick_diff(n, sur->md.v.p, full->dF.p, sur->dF.p,
                                      sur->dElemAtom.p, sur->md.accel.p, h,
@@ -2365,18 +2430,12 @@ static void mtsOuter(hdnnp_ctx* sur, double* rmseOut, int* modeOut)
         hipCheck(hipMemcpyAsync(full->md.v.p, sur->md.v.p,
                                 3 * (size_t)n * sizeof(double),
                                 hipMemcpyDeviceToDevice, full->stream), "v");
-        int64_t const rb = full->md.rebuilds;
-        for (int k = 0; k < S.nInner; ++k) mdStep(full);
-        bool const fullRebuilt = full->md.rebuilds != rb;
+        mtsSubsteps(full, sur, S.nInner);
         hipCheck(hipMemcpyAsync(sur->md.v.p, full->md.v.p,
                                 3 * (size_t)n * sizeof(double),
                                 hipMemcpyDeviceToDevice, full->stream), "v");
         hipCheck(hipStreamSynchronize(full->stream), "sync");
-        if (mdEvalAt(sur, full->dR.p, fullRebuilt))
-        {
-            hipCheck(hipStreamSynchronize(sur->stream), "sync");
-            mdEvalAt(full, sur->dR.p, true);
-        }
+        mtsFollow(sur, full);
         *rmseOut = mtsRmse(sur);
         *modeOut = 1;
         S.fallbacks += 1;

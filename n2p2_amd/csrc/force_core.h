@@ -75,7 +75,10 @@ struct BLds
 {
     Vec4<R>* P;      // (x, y, z, r) per staged neighbour
     Vec4<R>* Q;      // (f_c(r), 1/r, f_c'(r)/r, 0) for the current cutoff
-    double* ACC;     // 3 per staged neighbour (FP64 on both paths)
+    double* ACC;     // x, y, z planes of cap doubles per staged neighbour
+                     // (FP64 on both paths): lanes on neighbouring slots
+                     // hit neighbouring banks
+    int cap;
     int* SLOT;       // original slot of each staged neighbour
     int* QUEUE;      // kQueue passing pairs, j | k << 16
 };
@@ -148,9 +151,9 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
                     qq.w = R(0);
                     S.Q[pos] = qq;
                     S.SLOT[pos] = s;
-                    S.ACC[3 * pos] = 0.0;
-                    S.ACC[3 * pos + 1] = 0.0;
-                    S.ACC[3 * pos + 2] = 0.0;
+                    S.ACC[pos] = 0.0;
+                    S.ACC[S.cap + pos] = 0.0;
+                    S.ACC[2 * S.cap + pos] = 0.0;
                 }
                 start[q] += __popcll(mask);
             }
@@ -321,14 +324,15 @@ __device__ __forceinline__ void triplet_backward(
     // FP64 LDS adds on both paths (the FP32 LDS add is much slower on this
     // hardware, and the FP64 sums keep the mixed path's slot totals exact
     // to FP64 rounding).
-    double* const aj = S.ACC + 3 * j;
-    double* const ak = S.ACC + 3 * k;
+    double* const aj = S.ACC + j;
+    double* const ak = S.ACC + k;
+    int const c = S.cap;
     lds_add(aj, (double)(-(P1j * pj.x + P3 * dx)));
-    lds_add(aj + 1, (double)(-(P1j * pj.y + P3 * dy)));
-    lds_add(aj + 2, (double)(-(P1j * pj.z + P3 * dz)));
+    lds_add(aj + c, (double)(-(P1j * pj.y + P3 * dy)));
+    lds_add(aj + 2 * c, (double)(-(P1j * pj.z + P3 * dz)));
     lds_add(ak, (double)(P3 * dx - P1k * pk.x));
-    lds_add(ak + 1, (double)(P3 * dy - P1k * pk.y));
-    lds_add(ak + 2, (double)(P3 * dz - P1k * pk.z));
+    lds_add(ak + c, (double)(P3 * dy - P1k * pk.y));
+    lds_add(ak + 2 * c, (double)(P3 * dz - P1k * pk.z));
 }
 
 // Body of the backward kernel (one 64-lane workgroup per atom).
@@ -343,6 +347,7 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     S.P = (Vec4<R>*)sm;
     S.Q = S.P + cap;
     S.ACC = (double*)(S.Q + cap);
+    S.cap = cap;
     S.SLOT = (int*)(S.ACC + 3 * cap);
     S.QUEUE = S.SLOT + cap;
 
@@ -428,9 +433,9 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
                     R const pexp = exp_r(-eta * d * d);
                     P += MP[m].w * ((dfcinv - R(2) * eta * d * fcinv) * pexp);
                 }
-                S.ACC[3 * j] -= (double)(P * pj.x);
-                S.ACC[3 * j + 1] -= (double)(P * pj.y);
-                S.ACC[3 * j + 2] -= (double)(P * pj.z);
+                S.ACC[j] -= (double)(P * pj.x);
+                S.ACC[cap + j] -= (double)(P * pj.y);
+                S.ACC[2 * cap + j] -= (double)(P * pj.z);
             }
             __syncthreads();
             continue;
@@ -479,7 +484,7 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
             if (!valid) j = k = dp.s0;
             dp.advance(pc, pr);
             bool const pass = valid
-                && pair_in_range<R>(S.P[j], S.P[k], wide, grc, grc2);
+                & pair_in_range<R>(S.P[j], S.P[k], wide, grc, grc2);
             uint64_t const mask = __ballot(pass);
             if (pass) S.QUEUE[qn + lanes_below(mask)] = j | (k << 16);
             qn += __popcll(mask);
@@ -507,9 +512,9 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     for (int t = lane; t < n; t += 64)
     {
         size_t const q = (size_t)(s0 + S.SLOT[t]);
-        double const cx = S.ACC[3 * t];
-        double const cy = S.ACC[3 * t + 1];
-        double const cz = S.ACC[3 * t + 2];
+        double const cx = S.ACC[t];
+        double const cy = S.ACC[cap + t];
+        double const cz = S.ACC[2 * cap + t];
         a.cslot[3 * q] = cx;
         a.cslot[3 * q + 1] = cy;
         a.cslot[3 * q + 2] = cz;

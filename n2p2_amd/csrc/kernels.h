@@ -148,7 +148,9 @@ hipError_t launch_md_kick_diff(int n, double* v, const double* Ff,
                                const double* accel, double h, hipStream_t s);
 hipError_t launch_md_fdiff2(int n, const double* F1, const double* F2,
                             double* d2, hipStream_t s);
-hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s);
+// moved[0] is set to 1 if an atom was shifted (the caller clears it).
+hipError_t launch_md_wrap(int n, double* r, MDCell const& C, int* moved,
+                          hipStream_t s);
 hipError_t launch_md_filter(int n, MDCell const& C, double rc2, const double* r,
                             const int* off, const int* skinCnt,
                             const uint64_t* skinKeys, int* cnt, int* nbrJ,

@@ -91,7 +91,9 @@ __global__ void k_md_clear_flags(int* __restrict__ flags)
 
 // Atoms that left the cell are shifted back by whole lattice vectors,
 // r -= floor(f) box with f = r invbox; atoms inside keep their bits.
-__global__ void k_md_wrap(int n, double* __restrict__ r, MDCell C)
+// moved[0] = 1 if any atom was shifted.
+__global__ void k_md_wrap(int n, double* __restrict__ r, MDCell C,
+                          int* __restrict__ moved)
 {
     int const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -109,6 +111,7 @@ __global__ void k_md_wrap(int n, double* __restrict__ r, MDCell C)
     for (int k = 0; k < 3; ++k)
         r[3 * i + k] -= sh[0] * C.box[0][k] + sh[1] * C.box[1][k]
                       + sh[2] * C.box[2][k];
+    moved[0] = 1;
 }
 
 // Entries of the skin list within rc at the current positions, in order,
@@ -366,10 +369,11 @@ hipError_t launch_md_clear_flags(int* flags, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_md_wrap(int n, double* r, MDCell const& C, hipStream_t s)
+hipError_t launch_md_wrap(int n, double* r, MDCell const& C, int* moved,
+                          hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    k_md_wrap<<<(n + 255) / 256, 256, 0, s>>>(n, r, C);
+    k_md_wrap<<<(n + 255) / 256, 256, 0, s>>>(n, r, C, moved);
     return hipGetLastError();
 }
 

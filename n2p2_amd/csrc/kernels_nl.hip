@@ -411,7 +411,39 @@ __global__ __launch_bounds__(256) void k_nl_cell(
             }
         }
     }
-    if (MODE == 1 && tid == 0) sMax = 0;
+    if (MODE == 1)
+    {
+        // Candidate coordinates into sorted order (slot u holds the
+        // candidate of the u-th key), so that the scan below reads them
+        // with consecutive, conflict-free LDS accesses.
+        constexpr int kPer = kCellCand / 256;
+        double px[kPer], py[kPer], pz[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+        {
+            int const u = tid + 256 * q;
+            if (u < total)
+            {
+                int const t = (int)(cKey[u] & 1023);
+                px[q] = cX[t];
+                py[q] = cY[t];
+                pz[q] = cZ[t];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+        {
+            int const u = tid + 256 * q;
+            if (u < total)
+            {
+                cX[u] = px[q];
+                cY[u] = py[q];
+                cZ[u] = pz[q];
+            }
+        }
+        if (tid == 0) sMax = 0;
+    }
     __syncthreads();
     int const a0 = cellStart[cell];
     for (int a = w; a < nAt; a += kCellWaves)
@@ -434,7 +466,7 @@ __global__ __launch_bounds__(256) void k_nl_cell(
                 {
                     j = (int)(ck >> 31);
                     img = (int)((ck >> 10) & 0x1fffff);
-                    t = (int)(ck & 1023);
+                    t = u;   // coordinates are in sorted order
                 }
                 else
                 {

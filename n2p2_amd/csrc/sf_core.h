@@ -337,8 +337,10 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                 dp.pair(pc, pr, ja, kb);
                 if (!valid) ja = kb = dp.s0;
                 dp.advance(pc, pr);
+                // Invalid lanes test (s0, s0), in bounds: no divergent
+                // branch around the LDS reads.
                 bool const pass = valid
-                    && pair_in_range<R>(S.P[ja], S.P[kb], wide, grc, grc2);
+                    & pair_in_range<R>(S.P[ja], S.P[kb], wide, grc, grc2);
                 uint64_t const mask = __ballot(pass);
                 if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
                 qn += __popcll(mask);

@@ -65,26 +65,42 @@ void emit_exp_sum(std::ostringstream& o, LamRun const& r, bool wide)
     else o << "        R const s_ = r2sum;\n";
 }
 
+// The member sums of one group chunk.  Every sum starts with an assignment
+// of its first term, not "0 +" it: the compiler may not drop an addition of
+// +0 (it would change -0 into +0), so a zero start costs an instruction per
+// sum and triplet.
 void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
                         Model::Flat const& F)
 {
     bool const wide = (g.type == 9);
     o << "    case " << gi << ":\n    {\n"
-      << "        R S1 = R(0), U = R(0), vexp = R(0);\n"
+      << "        R S1, U, vexp;\n"
       << "        R E0 = R(0), Ejx = R(0), Ekx = R(0), E3x = R(0);\n";
     bool anyZero = false, anyShift = false;
+    bool firstT = true, firstE0 = true, firstEx = true, firstS = true;
     double eta = 0.0, rs = 0.0;
+    auto acc = [&](bool& first) { char const* op = first ? " = " : " += ";
+                                  first = false; return op; };
     auto close_run = [&]() {
-        o << "        T1 += vexp * S1;\n"
-          << "        { R const vu = vexp * U; V += vu;";
+        bool ft = firstT;
+        o << "        T1" << acc(ft) << "vexp * S1;\n"
+          << "        { R const vu = vexp * U; V" << acc(firstT) << "vu;";
         if (rs > 0.0)
         {
+            bool fx = firstEx, fy = firstEx;
             o << " R const te = " << lit(2.0 * eta) << " * vu;"
-              << " Ejx += te * a_ * invj; Ekx += te * b_ * invk;";
-            if (!wide) o << " E3x += te * c_ * invjk;";
+              << " Ejx" << acc(fx) << "te * a_ * invj; Ekx" << acc(fy)
+              << "te * b_ * invk;";
+            if (!wide)
+            {
+                bool fz = firstEx;
+                o << " E3x" << acc(fz) << "te * c_ * invjk;";
+            }
+            firstEx = false;
         }
-        else o << " E0 += " << lit(2.0 * eta) << " * vu;";
-        o << " }\n        S1 = U = R(0);\n        }\n";
+        else o << " E0" << acc(firstE0) << lit(2.0 * eta) << " * vu;";
+        o << " }\n        }\n";
+        firstS = true;
     };
     bool open = false;
     for (int ri = 0; ri < g.nrun; ++ri)
@@ -108,7 +124,8 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
         // FMAs per member; U += plambda W_ and S1 += Z_ at the run's end.
         o << "        {\n        R const pl = " << plambda(r.lambda) << ";\n"
           << "        R P = pl > R(0) ? R(1) : R(0), X = pl;\n"
-          << "        R W_ = R(0), Z_ = R(0);\n";
+          << "        R W_, Z_;\n";
+        bool firstW = true;
         int level = 0;
         int m = r.m0;
         for (int l = 0; l <= r.lmax; ++l)
@@ -117,8 +134,9 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
                 MemberDesc const& md = F.member[g.mem0 + m];
                 for (; level < md.level; ++level)
                     o << "        P *= X; X *= X;\n";
-                o << "        W_ += w[" << m << "] * P; Z_ += wz[" << m
-                  << "] * P;\n";
+                bool fz = firstW;
+                o << "        W_" << acc(firstW) << "w[" << m << "] * P; Z_"
+                  << acc(fz) << "wz[" << m << "] * P;\n";
             }
         for (; m < r.mend; ++m)
         {
@@ -126,12 +144,18 @@ void emit_backward_case(std::ostringstream& o, int gi, GroupDesc const& g,
             o << "        { R const pw_ = ";
             if (md.useint) o << "pow_int(pl, " << md.zint - 1 << ")";
             else o << "pow(pl, " << lit(md.zeta - 1.0) << ")";
-            o << "; R const f_ = pl > R(0) ? pw_ : R(0); W_ += w[" << m
-              << "] * f_; Z_ += wz[" << m << "] * f_; }\n";
+            bool fz = firstW;
+            o << "; R const f_ = pl > R(0) ? pw_ : R(0); W_" << acc(firstW)
+              << "w[" << m << "] * f_; Z_" << acc(fz) << "wz[" << m
+              << "] * f_; }\n";
         }
-        o << "        S1 += Z_; U += W_ * pl;\n        }\n";
+        if (firstW) o << "        W_ = Z_ = R(0);\n";   // no members
+        bool fu = firstS;
+        o << "        S1" << acc(firstS) << "Z_; U" << acc(fu)
+          << "W_ * pl;\n        }\n";
     }
     if (open) close_run();
+    else o << "        T1 = V = R(0);\n";
     if (anyZero && !anyShift)
         o << "        Ej = E0; Ek = E0; E3 = E0;\n";
     else if (!anyZero)

@@ -300,7 +300,7 @@ class Context:
         rows as a float array with columns TUNE_COLUMNS (empty when a cached
         choice was applied)."""
         cap = 64
-        table = np.zeros((cap, 8), dtype=np.float64)
+        table = np.zeros((cap, len(self.TUNE_COLUMNS)), dtype=np.float64)
         n = ctypes.c_int32(0)
         _check(_lib.hdnnp_ctx_autotune(self._h, int(force_mode),
                                        1 if sweep else 0, int(reps),

@@ -91,10 +91,13 @@ def write_weights(d, w, nsf_per_element, z_numbers, hidden=SURROGATE_HIDDEN):
 
 
 def generate_frames(model, r, elem, box, masses, nframes, stride=4,
-                    dt=0.25, temp=300.0, seed=1, precision=None):
+                    dt=0.25, temp=300.0, seed=1, precision=None,
+                    final_state=False):
     """Frames of the full model's own NVE trajectory (velocity Verlet,
     Verlet-skin list): every `stride` steps the positions (wrapped into the
-    cell), the energy and the forces.  Returns (R, E, F) arrays."""
+    cell), the energy and the forces.  Returns (R, E, F) arrays, and with
+    final_state the trajectory's last positions (wrapped) and velocities
+    as a fourth item."""
     ctx = engine.Context(model)
     if precision is not None:
         ctx.set_precision(precision)
@@ -110,8 +113,12 @@ def generate_frames(model, r, elem, box, masses, nframes, stride=4,
         R[k] = W.remap(x, box)
         E[k] = ep[-1]
         F[k] = ctx.forces()
+    last = None
+    if final_state:
+        x, v = ctx.md_state()
+        last = (W.remap(x, box), v.copy())
     ctx.close()
-    return R, E, F
+    return (R, E, F, last) if final_state else (R, E, F)
 
 
 def train(ctx, elem, box, R, E, F, epochs=20, lr=1e-3, w_e=1.0, w_f=1.0,

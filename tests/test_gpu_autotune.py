@@ -133,3 +133,4 @@ def test_tuning_rejects_bad_values(workdir):
     with pytest.raises(RuntimeError, match="No structure"):
         ctx.autotune(sweep=True)
     ctx.close()
+
