@@ -258,18 +258,21 @@ def _drift(etot, n):
 
 
 def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
-           epochs=20, nouter=2000, gate=5e-3, chunk=100):
+           epochs=20, nouter=2000, gate=5e-3, chunk=100, equil=2000,
+           nouter_extra=500):
     """The learned surrogate for multiple time steps on the 196,608-atom
-    box: frames of the full model's own 300 K NVE trajectory (mixed path,
-    dt 0.25 fs, every `stride` steps), the distilled radial surrogate (4 eta
+    box: frames of the full model's own 300 K trajectory (mixed path, dt
+    0.25 fs, `equil` steps of equilibration, velocities rescaled to 300 K
+    every 20 steps, a frame every `stride` steps), the distilled radial
+    surrogate (4 eta
     x 2 neighbour elements G2, 8-16-16-1 softplus) trained on them with the
     engine's gradient and Adam kernels (lr 1e-3, `epochs` epochs, one frame
     per step), then, from the trajectory's last state, `nouter` outer
     steps of 1 fs (4 surrogate substeps of 0.25 fs) of r-RESPA: with the
-    force-RMSE gate, without it, without it for the untrained surrogate,
-    without it with eager steps instead of HIP graphs, and the full model's
-    plain velocity Verlet at 0.25 fs over the same time.  Throughput counts
-    0.25 fs steps."""
+    force-RMSE gate, without it, and the full model's plain velocity Verlet
+    at 0.25 fs over the same time; then `nouter_extra` outer steps without
+    the gate for the untrained surrogate and with eager steps instead of
+    HIP graphs.  Throughput counts 0.25 fs steps."""
     masses = np.array(W.WATER_MASSES)
     n = len(r)
     out = {"frames": nframes, "frame_stride_steps": stride, "epochs": epochs,
@@ -279,7 +282,8 @@ def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
                                       stride=stride, dt=0.25, temp=300.0,
                                       seed=4242,
                                       precision=engine.PRECISION_MIXED,
-                                      final_state=True)
+                                      final_state=True, equil=equil,
+                                      rescale_every=20, log=log)
     out["frame_generation_s"] = time.perf_counter() - t0
     log("C4: %d frames in %.1f s" % (nframes, out["frame_generation_s"]))
     sd = os.path.join(tmp, "surrogate_c4")
@@ -304,13 +308,14 @@ def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
     ctx.close()
     del R, F
     trained = engine.Model.load_dir(td)
-    # The runs below continue the frames' trajectory from its last state:
-    # the generated box starts far from equilibrium (lattice oxygens,
-    # random orientations) and heats up over its first picosecond, which
-    # the frames cover.
+    # The runs below continue the frames' trajectory from its last state
+    # (NVE from there).  The generated box starts far from equilibrium
+    # (lattice oxygens, random orientations): without the thermostat of
+    # the equilibration and the frames it heats up to thousands of kelvin.
     r, v = last
+    out["start_temperature_K"] = S.temperature(v, masses[elem])
 
-    def mts(sur_model, g, label, graph=True):
+    def mts(sur_model, g, label, graph=True, nouter=nouter):
         full = engine.Context(model)
         sur = engine.Context(sur_model)
         full.set_precision(engine.PRECISION_MIXED)
@@ -355,11 +360,12 @@ def run_c4(model, r, elem, box, tmp, log, nframes=2000, stride=2,
     out["mts_gated"] = mts(trained, gate, "MTS, gate %g eV/A" % gate)
     out["mts_no_gate"] = mts(trained, 1e30, "MTS without gate")
     out["mts_no_gate_untrained"] = mts(untrained, 1e30,
-                                       "MTS without gate, untrained")
-    # The same trained run with eager steps instead of HIP graphs: the
-    # graphs are captured again after every list rebuild.
+                                       "MTS without gate, untrained",
+                                       nouter=nouter_extra)
+    # The trained run with eager steps instead of HIP graphs: the graphs
+    # are captured again after every list rebuild.
     out["mts_no_gate_eager"] = mts(trained, 1e30, "MTS without gate, eager",
-                                   graph=False)
+                                   graph=False, nouter=nouter_extra)
     ctx = engine.Context(model)
     ctx.set_precision(engine.PRECISION_MIXED)
     ctx.set_structure(r, elem, box)

@@ -459,15 +459,19 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         if (nrec != (int)kRecordMissing)
         {
             // Replay the recorded pairs: the same triplets in the same lanes
-            // and order as the walk below would give.
+            // and order as the walk below would give.  The next batch's
+            // entries are loaded while this batch is evaluated.
+            int next = lane < nrec ? (int)rec[pOff + lane] : 0;
             for (int base = 0; base < nrec; base += 64)
+            {
+                int const e = next;
+                if (base + 64 + lane < nrec)
+                    next = rec[pOff + base + 64 + lane];
                 if (base + lane < nrec)
-                {
-                    int const e = rec[pOff + base + lane];
                     triplet_backward<CT, GENPOW, MAXM>(gi, S, e & 0xff,
                                                        e >> 8, wide, cc, M,
                                                        MP, nmem);
-                }
+            }
             pOff += nrec;
             __syncthreads();
             continue;

@@ -90,20 +90,44 @@ def write_weights(d, w, nsf_per_element, z_numbers, hidden=SURROGATE_HIDDEN):
     assert p == len(w)
 
 
+def temperature(v, atom_masses):
+    """Kinetic temperature (K) of velocities v (A/fs), masses in amu."""
+    m = np.asarray(atom_masses, dtype=np.float64)
+    return float((m[:, None] * v * v).sum()) / (3 * len(v) * W.KB_EV * W.FTOA)
+
+
 def generate_frames(model, r, elem, box, masses, nframes, stride=4,
                     dt=0.25, temp=300.0, seed=1, precision=None,
-                    final_state=False):
-    """Frames of the full model's own NVE trajectory (velocity Verlet,
+                    final_state=False, equil=0, rescale_every=0, log=None):
+    """Frames of the full model's own trajectory (velocity Verlet,
     Verlet-skin list): every `stride` steps the positions (wrapped into the
-    cell), the energy and the forces.  Returns (R, E, F) arrays, and with
+    cell), the energy and the forces.  With rescale_every > 0 the
+    velocities are scaled to `temp` every rescale_every steps (a simple
+    velocity-rescaling thermostat), through `equil` steps before the first
+    frame and on between the frames.  Returns (R, E, F) arrays, and with
     final_state the trajectory's last positions (wrapped) and velocities
     as a fourth item."""
     ctx = engine.Context(model)
     if precision is not None:
         ctx.set_precision(precision)
     ctx.set_structure(r, elem, box)
-    v = W.maxwell_boltzmann(np.asarray(masses)[elem], temp, seed=seed)
+    m_atom = np.asarray(masses)[elem]
+    v = W.maxwell_boltzmann(m_atom, temp, seed=seed)
     ctx.md_init(masses, dt, skin=1.0, v=v)
+
+    def rescale():
+        _, u = ctx.md_state()
+        t = temperature(u, m_atom)
+        ctx.md_init(masses, dt, skin=1.0, v=u * np.sqrt(temp / t))
+        return t
+
+    if rescale_every > 0:
+        for k in range(equil // rescale_every):
+            ctx.md_run(rescale_every)
+            t = rescale()
+            if log and (k + 1) % 20 == 0:
+                log("equilibration step %d: %.0f K before rescaling"
+                    % ((k + 1) * rescale_every, t))
     R = np.zeros((nframes, len(r), 3))
     E = np.zeros(nframes)
     F = np.zeros((nframes, len(r), 3))
@@ -113,6 +137,8 @@ def generate_frames(model, r, elem, box, masses, nframes, stride=4,
         R[k] = W.remap(x, box)
         E[k] = ep[-1]
         F[k] = ctx.forces()
+        if rescale_every > 0 and ((k + 1) * stride) % rescale_every == 0:
+            rescale()
     last = None
     if final_state:
         x, v = ctx.md_state()
