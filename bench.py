@@ -119,7 +119,9 @@ def run_path(ctx, precision, steps, warmup):
                     "gather")}
     ctx.profile(False)
     forces = ctx.forces()
-    return dt, energy, forces, kt, spec
+    names = {"force": ctx.kernel_name("force"),
+             "symfunc": ctx.kernel_name("sf"), "network": "k_nn_mfma"}
+    return dt, energy, forces, kt, names if spec else None
 
 
 def tune(ctx, precision, log, label):
@@ -428,8 +430,9 @@ def other_rooflines(ktm, kt64, counts, nnf, n):
 
 
 def roofline(kt, counts, nnf, n, fp32, spec):
-    """Roofline object for the dominant kernel of one path; spec: the
-    model-specialised SF and force kernels ran (their names differ)."""
+    """Roofline object for the dominant kernel of one path; spec: the names
+    of the model-specialised SF and force kernels that ran (None: the
+    generic kernels ran)."""
     fwd = int(costmodel.forward_flops(counts))
     bwd = int(costmodel.backward_flops(counts))
     peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
@@ -444,16 +447,18 @@ def roofline(kt, counts, nnf, n, fp32, spec):
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                 "avg_launch_ms": ms}
     if spec:
-        sfx = "_f32" if fp32 else "_f64"
-        kname = {"force": "hdnnp_plan_force" + sfx,
-                 "symfunc": "hdnnp_plan_sf" + sfx, "network": "k_nn_mfma"}[dom]
+        kname = spec[dom]
+        # traffic.json keys the specialised kernels without the launch form
+        # (hdnnp_plan_force_f32_m8_w4 -> hdnnp_plan_force_f32)
+        key = kname.split("_m")[0] if kname.startswith("hdnnp_plan") \
+            else kname
     else:
         kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
                  "network": "k_nn_mfma"}[dom]
+        key = kname + ("_fp32" if fp32 else "")
     achieved = flops / (ms * 1e-3) / 1e12
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
-    key = kname if spec else kname + ("_fp32" if fp32 else "")
     if os.path.exists(tp) and n == 196608:
         with open(tp) as f:
             tk = json.load(f).get("kernels", {}).get(key)

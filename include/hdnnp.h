@@ -169,6 +169,12 @@ int  hdnnp_ctx_autotune(hdnnp_ctx* c, int force_mode, int flags, int reps,
 /* The context's knobs (out4) / sets them (in4; -1 waves = default).  */
 int  hdnnp_ctx_tuning(hdnnp_ctx* c, int* out4);
 int  hdnnp_ctx_set_tuning(hdnnp_ctx* c, const int* in4);
+/* Name of the SF (which = 0) or force (1) kernel the context launches at
+ * its precision and tuning, as profilers list it: the specialised
+ * hdnnp_plan_{sf,force}_{f32,f64}_m<chunk>_w<waves> (which compiles it if
+ * needed) or the generic k_sf_forward / k_force_backward.  No counterpart
+ * in the reference. */
+int  hdnnp_ctx_kernel_name(hdnnp_ctx* c, int which, char* buf, int cap);
 
 /* --------------------------------------------------------- structure -- */
 

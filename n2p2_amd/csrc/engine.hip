@@ -1748,6 +1748,26 @@ int hdnnp_ctx_tuning(hdnnp_ctx* c, int* out4)
     });
 }
 
+int hdnnp_ctx_kernel_name(hdnnp_ctx* c, int which, char* buf, int cap)
+{
+    return guarded([&]() {
+        if (!c || !buf || cap <= 0)
+            throw Error(HDNNP_E_INVALID, "ERROR: Null argument.\n");
+        if (which != 0 && which != 1)
+            throw Error(HDNNP_E_INVALID, "ERROR: Kernel 0 (SF) or 1 (force)."
+                                         "\n");
+        c->setDevice();
+        int const fp32 = c->precision != HDNNP_PRECISION_FP64;
+        std::string name = which ? "k_force_backward" : "k_sf_forward";
+        PlanKernels const pk = c->planFor(fp32);
+        if (pk.sf && pk.force)
+            name = plan_kernel_name(c->T, fp32, c->tune, which == 1);
+        if ((int)name.size() >= cap)
+            throw Error(HDNNP_E_CAPACITY, "ERROR: Name buffer too small.\n");
+        std::memcpy(buf, name.c_str(), name.size() + 1);
+    });
+}
+
 int hdnnp_ctx_set_tuning(hdnnp_ctx* c, const int* in4)
 {
     return guarded([&]() {

@@ -35,6 +35,12 @@ struct PlanTuning
 PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T, int fp32,
                          int device, PlanTuning const& tune = PlanTuning());
 
+// Name of the specialised SF (force = false) or force kernel for the
+// tables, arithmetic and tuning: hdnnp_plan_{sf,force}_{f32,f64}_m<members
+// per chunk>_w<waves-per-SIMD bound, 0 = compiler's choice>.
+std::string plan_kernel_name(ModelTables const& T, int fp32,
+                             PlanTuning const& tune, bool force);
+
 // The generated member-loop code (the part of the kernel source that
 // depends on the model).
 std::string plan_source(Model::Flat const& F);

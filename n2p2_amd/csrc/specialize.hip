@@ -242,6 +242,24 @@ std::string waves_attr(int w)
          + ", 8)))\n";
 }
 
+}  // namespace
+
+// Kernel names carry the arithmetic, the member chunk and the waves bound,
+// so that traces of different launch forms (a tuning sweep) keep them
+// apart: hdnnp_plan_{sf,force}_{f32,f64}_m<chunk>_w<waves>.
+std::string plan_kernel_name(ModelTables const& T, int fp32,
+                             PlanTuning const& tune, bool force)
+{
+    int const maxm = plan_maxm(T);
+    int const w = force ? force_waves(tune, fp32, maxm)
+                        : sf_waves(tune, fp32, maxm);
+    return std::string(force ? "hdnnp_plan_force" : "hdnnp_plan_sf")
+         + (fp32 ? "_f32" : "_f64") + "_m" + std::to_string(maxm) + "_w"
+         + std::to_string(w);
+}
+
+namespace {
+
 std::string full_source(std::string const& plan, ModelTables const& T,
                         int fp32, PlanTuning const& tune)
 {
@@ -260,13 +278,13 @@ std::string full_source(std::string const& plan, ModelTables const& T,
 
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
       << waves_attr(force_waves(tune, fp32, maxm))
-      << "void hdnnp_plan_force_" << (fp32 ? "f32" : "f64")
+      << "void " << plan_kernel_name(T, fp32, tune, true)
       << "(hdnnp::ForceArgs a)\n"
       << "{ hdnnp::force_backward_body<" << ct << ", false, " << maxm << ", "
       << R << ">(a); }\n"
       << "extern \"C\" __global__ __launch_bounds__(64)\n"
       << waves_attr(sf_waves(tune, fp32, maxm))
-      << "void hdnnp_plan_sf_" << (fp32 ? "f32" : "f64")
+      << "void " << plan_kernel_name(T, fp32, tune, false)
       << "(hdnnp::SFArgs a)\n"
       << "{ hdnnp::sf_forward_body<" << maxm << ", " << ct << ", false, "
       << R << ">(a); }\n";
@@ -305,9 +323,10 @@ void dump_source(std::string const& src, int fp32)
     }
 }
 
-std::shared_ptr<Module> compile(std::string const& src, int fp32, int device)
+std::shared_ptr<Module> compile(std::string const& src, int fp32, int device,
+                                std::string const& sfName,
+                                std::string const& forceName)
 {
-    std::string const sfx = fp32 ? "_f32" : "_f64";
     dump_source(src, fp32);
     auto t0 = std::chrono::steady_clock::now();
     hiprtcProgram prog;
@@ -339,10 +358,10 @@ std::shared_ptr<Module> compile(std::string const& src, int fp32, int device)
     auto m = std::make_shared<Module>();
     if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess)
         throw std::runtime_error("hipModuleLoadData failed");
-    if (hipModuleGetFunction(&m->k.force, m->mod,
-                             ("hdnnp_plan_force" + sfx).c_str()) != hipSuccess
-        || hipModuleGetFunction(&m->k.sf, m->mod,
-                                ("hdnnp_plan_sf" + sfx).c_str()) != hipSuccess)
+    if (hipModuleGetFunction(&m->k.force, m->mod, forceName.c_str())
+            != hipSuccess
+        || hipModuleGetFunction(&m->k.sf, m->mod, sfName.c_str())
+            != hipSuccess)
         throw std::runtime_error("hipModuleGetFunction failed");
 
     if (verbose())
@@ -410,7 +429,8 @@ PlanKernels plan_kernels(Model::Flat const& F, ModelTables const& T,
     std::shared_ptr<Module> m;
     try
     {
-        m = compile(src, fp32, device);
+        m = compile(src, fp32, device, plan_kernel_name(T, fp32, tune, false),
+                    plan_kernel_name(T, fp32, tune, true));
     }
     catch (std::exception const& e)
     {

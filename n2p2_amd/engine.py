@@ -128,6 +128,8 @@ _SIGNATURES = [
       ctypes.c_int, _c_int32_p]),
     ("hdnnp_ctx_tuning", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
     ("hdnnp_ctx_set_tuning", ctypes.c_int, [ctypes.c_void_p, _c_int32_p]),
+    ("hdnnp_ctx_kernel_name", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     ("hdnnp_get_scattered_forces_noalloc", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, _c_double_p]),
     ("hdnnp_extrapolation_warnings", ctypes.c_int64, [ctypes.c_void_p, _c_int32_p]),
@@ -313,6 +315,15 @@ class Context:
         v = np.zeros(4, dtype=np.int32)
         _check(_lib.hdnnp_ctx_tuning(self._h, _ptr(v, ctypes.c_int32)))
         return tuple(int(x) for x in v)
+
+    def kernel_name(self, which):
+        """Name of the SF ("sf") or force ("force") kernel as profilers
+        list it, at the context's precision and tuning."""
+        buf = ctypes.create_string_buffer(128)
+        _check(_lib.hdnnp_ctx_kernel_name(self._h,
+                                          {"sf": 0, "force": 1}[which], buf,
+                                          128))
+        return buf.value.decode()
 
     def set_tuning(self, chunk=16, sf_waves=-1, force_waves=-1, nn_tiles=1):
         v = np.array([chunk, sf_waves, force_waves, nn_tiles], dtype=np.int32)

@@ -115,6 +115,7 @@ def test_bindings_marshal_arguments_without_a_context():
                  lambda: c.set_tuning(8, -1, -1, 1),
                  lambda: c.autotune(sweep=True),
                  lambda: c.autotune(),
-                 lambda: c.scattered_forces_noalloc(4)):
+                 lambda: c.scattered_forces_noalloc(4),
+                 lambda: c.kernel_name("force")):
         with pytest.raises(RuntimeError, match="ERROR"):
             call()

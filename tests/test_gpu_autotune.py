@@ -103,6 +103,12 @@ def test_sweep_and_cache(workdir):
     assert np.any(chosen & 1) and np.any(chosen & 2)
     t = ctx.tuning()
     assert t[0] in (4, 8, 16) and t[3] in (1, 2, 4)
+    # Profilers list the chosen launch form under its own kernel names.
+    for which, w in (("sf", t[1]), ("force", t[2])):
+        name = ctx.kernel_name(which)
+        say("%s kernel: %s" % (which, name))
+        assert name.startswith("hdnnp_plan_%s_f32_m" % which)
+        assert name.endswith("_w%d" % w)
     en = ctx.evaluate(engine.FORCES_CONSISTENT)
     _check(engine.PRECISION_MIXED, en, ctx.forces(), oe, of)
     # A second context of the same model shape adopts the cached choice
