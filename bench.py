@@ -198,13 +198,17 @@ def run_batch(tmp, steps, warmup, log):
 def run_md(model, r, elem, box, precision, steps, warmup, skin=1.0,
            dt_fs=0.25, temp=300.0, tuned=False):
     """NVE MD of the same box (velocity Verlet, Verlet-skin list, HIP-graph
-    steps): `warmup` untimed steps, then `steps` timed ones.  md_run
+    steps): `warmup` untimed steps, then `steps` timed ones.  `tuned`: a
+    launch form (chunk, SF waves, force waves, tiles), True for the choice
+    a sweep of this process cached, False for the built-in one.  md_run
     returns after its stream has drained, so the timed region is
     synchronous on both sides."""
     ctx = engine.Context(model)
     ctx.set_precision(precision)
     ctx.set_structure(r, elem, box)
-    if tuned:
+    if isinstance(tuned, tuple):
+        ctx.set_tuning(*tuned)
+    elif tuned:
         ctx.autotune()   # the choice cached by the sweep of this process
     masses = np.array(W.WATER_MASSES)
     v = W.maxwell_boltzmann(masses[elem], temp, seed=20261202)
@@ -448,22 +452,24 @@ def roofline(kt, counts, nnf, n, fp32, spec):
                 "avg_launch_ms": ms}
     if spec:
         kname = spec[dom]
-        # traffic.json keys the specialised kernels without the launch form
-        # (hdnnp_plan_force_f32_m8_w4 -> hdnnp_plan_force_f32)
-        key = kname.split("_m")[0] if kname.startswith("hdnnp_plan") \
-            else kname
     else:
         kname = {"force": "k_force_backward", "symfunc": "k_sf_forward",
                  "network": "k_nn_mfma"}[dom]
-        key = kname + ("_fp32" if fp32 else "")
     achieved = flops / (ms * 1e-3) / 1e12
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp) and n == 196608:
         with open(tp) as f:
-            tk = json.load(f).get("kernels", {}).get(key)
-        if tk:
-            traffic = tk["read_bytes"] + tk["write_bytes"]
+            tk = json.load(f).get("kernels", {})
+        # The waves-per-SIMD bound (the _w suffix) does not change what a
+        # kernel reads and writes: a profile of another bound of the same
+        # arithmetic and member chunk stands in when the sweep chose a
+        # bound that was not profiled.
+        base = kname.rsplit("_w", 1)[0] + "_w"
+        hit = tk.get(kname) or next(
+            (v for k, v in sorted(tk.items()) if k.startswith(base)), None)
+        if hit:
+            traffic = hit["read_bytes"] + hit["write_bytes"]
     return {"kernel": kname, "bound": "valu-fp32" if fp32 else "valu-fp64",
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,
@@ -483,6 +489,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the built-in launch forms (no sweep)")
+    ap.add_argument("--set-tuning", action="append", default=[],
+                    metavar="PATH=C,S,F,T",
+                    help="apply a launch form without a sweep (PATH fp64 "
+                         "or mixed; member chunk, SF and force waves per "
+                         "SIMD, network tiles per wave), e.g. for a "
+                         "profiled run of the form a sweep chose")
     ap.add_argument("--c4-frames", type=int, default=2000)
     ap.add_argument("--c4-epochs", type=int, default=20)
     ap.add_argument("--c4-outer", type=int, default=2000)
@@ -497,6 +509,13 @@ def main():
               "(spatial decomposition with halo exchange) is not built, so "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    fixed = {}
+    for item in args.set_tuning:
+        key, _, vals = item.partition("=")
+        if key not in ("fp64", "mixed") or len(vals.split(",")) != 4:
+            ap.error("--set-tuning takes fp64=C,S,F,T or mixed=C,S,F,T")
+        fixed[key] = tuple(int(v) for v in vals.split(","))
 
     r, elem, box = W.water(args.nmol, args.box, 20261202)
     n = len(r)
@@ -526,19 +545,33 @@ def main():
             return
         if args.path == "md":
             res = run_md(model, r, elem, box, engine.PRECISION_MIXED,
-                         args.steps, args.warmup)
+                         args.steps, args.warmup,
+                         tuned=fixed.get("mixed", False))
             log("MD mixed only: %s (no JSON line)" % res)
             return
         tuning = {}
-        if not args.no_tune and args.path in ("both", "fp64"):
-            tuning["fp64"] = tune(ctx, engine.PRECISION_FP64, log, "fp64")
-        if args.path != "mixed":
+
+        def choose(precision, label):
+            if label in fixed:
+                ctx.set_precision(precision)
+                ctx.set_tuning(*fixed[label])
+                log("tuning %s: set to %s" % (label, fixed[label]))
+                tuning[label] = {"choice": dict(zip(
+                    ("member_chunk", "sf_waves_per_simd",
+                     "force_waves_per_simd", "nn_tiles_per_wave"),
+                    fixed[label])), "sweep_seconds": 0.0,
+                    "settings_timed": 0, "rows": []}
+            elif not args.no_tune:
+                tuning[label] = tune(ctx, precision, log, label)
+
+        if args.path in ("both", "fp64"):
+            choose(engine.PRECISION_FP64, "fp64")
             dt64, e64, f64, kt64, sp64 = run_path(
                 ctx, engine.PRECISION_FP64, args.steps, args.warmup)
-        if not args.no_tune and args.path in ("both", "mixed"):
-            tuning["mixed"] = tune(ctx, engine.PRECISION_MIXED, log, "mixed")
             log("plain FP64: %.3f ms/step, E=%.10f, kernel ms %s"
                 % (dt64 * 1e3, e64, kt64))
+        if args.path in ("both", "mixed"):
+            choose(engine.PRECISION_MIXED, "mixed")
         if args.path != "fp64":
             dtm, em, fm, ktm, spm = run_path(
                 ctx, engine.PRECISION_MIXED, args.steps, args.warmup)
@@ -583,7 +616,8 @@ def main():
         for name, prec in (("fp64", engine.PRECISION_FP64),
                            ("mixed", engine.PRECISION_MIXED)):
             md[name] = run_md(model, r, elem, box, prec, md_steps,
-                              args.warmup, tuned=name in tuning)
+                              args.warmup,
+                              tuned=fixed.get(name, name in tuning))
             log("MD %s: %s" % (name, md[name]))
         md["mixed_vs_plain_fp64"] = (md["mixed"]["atom_steps_per_s"]
                                      / md["fp64"]["atom_steps_per_s"])
