@@ -145,12 +145,14 @@ def tune(ctx, precision, log, label):
             "rows": [[round(float(x), 4) for x in row] for row in rows]}
 
 
-def run_batch(tmp, steps, warmup, log):
+def run_batch(tmp, steps, warmup, log, sweep=True):
     """The 512-structure batch (three elements B/C/N at 0.4/0.4/0.2, 16-512
     atoms each, 0.08 atoms/A^3, 10 G2 per element and 12 G9 per element
     pair, polynomial cutoff 6 A, 102-30-30-1 softplus): one E+F evaluation
     of all structures per step, neighbour lists rebuilt, FP64 and mixed;
-    the mixed path's force error against FP64 over all atoms."""
+    the mixed path's force error against FP64 over all atoms.  With
+    `sweep`, the FP64 and mixed legs first sweep the launch forms on the
+    batch (untimed); the FP16 / BF16 legs keep the mixed choice."""
     sts = W.batch_structures(512, 16, 512, seed=2024)
     d = os.path.join(tmp, "batch_model")
     r0, e0, b0 = sts[0]
@@ -166,6 +168,10 @@ def run_batch(tmp, steps, warmup, log):
                        ("mixed16", engine.PRECISION_MIXED16),
                        ("mixed_bf16", engine.PRECISION_MIXED_BF16)):
         ctx.set_precision(prec)
+        if sweep and name in ("fp64", "mixed"):
+            rows = ctx.autotune(engine.FORCES_REFERENCE, sweep=True, reps=5)
+            log("batch tuning %s: %d settings, choice %s"
+                % (name, len(rows), ctx.tuning()))
         for _ in range(warmup):
             ctx.evaluate(engine.FORCES_REFERENCE)
         ctx.profile(True)
@@ -180,7 +186,11 @@ def run_batch(tmp, steps, warmup, log):
         ctx.profile(False)
         res[name] = (ctx.forces(), ctx.structure_energies())
         out[name] = {"atom_steps_per_s": ctx.n / dt, "ms_per_step": dt * 1e3,
-                     "kernel_ms": kt}
+                     "kernel_ms": kt,
+                     "tuning": dict(zip(("member_chunk", "sf_waves_per_simd",
+                                         "force_waves_per_simd",
+                                         "nn_tiles_per_wave"),
+                                        ctx.tuning()))}
     for name in ("mixed", "mixed16", "mixed_bf16"):
         df = res[name][0] - res["fp64"][0]
         de = np.abs(res[name][1] - res["fp64"][1]) / np.abs(res["fp64"][1])
@@ -533,7 +543,8 @@ def main():
                       for e in range(model.num_elements)))
 
         if args.path == "batch":
-            run_batch(tmp, args.steps, args.warmup, log)
+            run_batch(tmp, args.steps, args.warmup, log,
+                      sweep=not args.no_tune)
             log("batch only: no JSON line")
             return
         if args.path == "mts":
@@ -622,7 +633,8 @@ def main():
         md["mixed_vs_plain_fp64"] = (md["mixed"]["atom_steps_per_s"]
                                      / md["fp64"]["atom_steps_per_s"])
         sur_model, hist, ttrain = train_surrogate(tmp, log)
-        batch = run_batch(tmp, args.steps, args.warmup, log)
+        batch = run_batch(tmp, args.steps, args.warmup, log,
+                          sweep=not args.no_tune)
         md["surrogate"] = {
             "what": "radial-only HDNNP, 8 G2 per element (4 eta x 2 "
                     "neighbour elements), 8-16-16-1 softplus, trained on the "

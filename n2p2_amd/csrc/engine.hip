@@ -586,6 +586,7 @@ void buildBatchNeighborList(hdnnp_ctx* c, double rc)
                                  c->dStructOff.p, c->dCells.p, c->dCount.p,
                                  c->dStats.p, c->dOff.p, c->slab, c->dKeys.p,
                                  c->dNbrJ.p, c->dGeo.p, s), "batch nl fill");
+        hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "count max");
         readStats();
     };
     bool built = false;

@@ -73,8 +73,8 @@ hipError_t launch_batch_extent(int nstruct, const double* r,
                                hipStream_t s);
 // Batched structures: per-atom lists within each atom's structure
 // (structOf, structOff) over its cell's images.  mode 0 counts into count;
-// mode 1 writes at most `limit` entries per atom from off, stores count,
-// raises stats[0] to the largest count and sets stats[3] on overflow.
+// mode 1 writes at most `limit` entries per atom from off, stores count
+// and sets stats[3] on overflow (the largest count: launch_max_int).
 hipError_t launch_nl_batch(int mode, int n, double rc, const double* r,
                            const int* structOf, const int* structOff,
                            const BatchCell* cells, int* count, int* stats,

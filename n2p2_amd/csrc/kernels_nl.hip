@@ -581,13 +581,21 @@ __device__ __forceinline__ void image_range(double f, double fmin,
 
 // Batched structures: atom i of structure s = structOf[i] searches the
 // atoms [structOff[s], structOff[s+1]) of its own structure over the
-// structure's periodic images |b_k| <= pbc_k (Structure.cpp:287-399),
-// candidates in the reference's loop order (j, then b0, b1, b2), 64 per
-// step; the ballot compaction therefore yields the list in order.  Only
-// the images image_range keeps are enumerated, which drops no entry and
-// keeps the order.  MODE 0 counts into count[i].  MODE 1 writes at most
-// `limit` entries from off[i], stores the count, raises stats[0] to it and
-// sets stats[3] when it exceeds the limit.  One wave per atom.
+// structure's periodic images |b_k| <= pbc_k (Structure.cpp:287-399); its
+// list holds the hits in the reference's loop order (j, then b0, b1, b2).
+// Only the images image_range keeps are enumerated, which drops no entry
+// and keeps the order.  MODE 0 counts into count[i].  MODE 1 writes at most
+// `limit` entries from off[i], stores the count and sets stats[3] when it
+// exceeds the limit.  One wave per atom.
+//
+// At most 64 images (every structure at least about 2 rc wide, and most
+// others): lanes take 64 atoms j at a time and walk the images in a
+// uniform loop, so each lane computes r_i - r_j once and adds the images'
+// shifts to it in image_dr's order and rounding; a lane's hits are kept as
+// a bit mask over the images, and an exclusive scan of the lanes' counts
+// places each lane's hits, in image order, behind those of the lanes
+// (atoms j) before it.  More images: candidates (j, image) in list order,
+// 64 per step, and a ballot compaction.
 template <int MODE>
 __global__ __launch_bounds__(64) void k_nl_batch(
     int n, double rc2, double rcm, const double* __restrict__ r,
@@ -627,53 +635,127 @@ __global__ __launch_bounds__(64) void k_nl_batch(
     int const total = na * nimg;
     int const o = MODE == 1 ? off[i] : 0;
     int hits = 0;
-    for (int base = 0; base < total; base += 64)
+    if (nimg <= 64)
     {
-        int const t = base + lane;
-        bool hit = false;
-        int j = 0, b0 = 0, b1 = 0, b2 = 0;
-        double dr[3] = {0.0, 0.0, 0.0};
-        if (t < total)
+        for (int jb = 0; jb < na; jb += 64)
         {
-            int jj, bi, q0, r0, q1, r2;
-            divmod_small(t, nimg, invImg, jj, bi);
-            divmod_small(bi, w12, invW12, q0, r0);
-            divmod_small(r0, w2, invW2, q1, r2);
-            j = a0 + jj;
-            b0 = lo[0] + q0;
-            b1 = lo[1] + q1;
-            b2 = lo[2] + r2;
-            if (!(j == i && b0 == 0 && b1 == 0 && b2 == 0))
+            int const jj = jb + lane;
+            bool const live = jj < na;
+            int const j = a0 + (live ? jj : 0);
+            double const rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                                  r[3 * (size_t)j + 2]};
+            uint64_t m = 0;
             {
-                double const rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
-                                      r[3 * (size_t)j + 2]};
-                image_dr(p, ri, rj, b0, b1, b2, dr);
-                hit = norm2_pinned(dr) <= rc2;
+#pragma clang fp contract(off)
+                double const d0 = ri[0] - rj[0], d1 = ri[1] - rj[1],
+                             d2 = ri[2] - rj[2];
+                int bi = 0;
+                for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
+                    for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
+                        for (int b2 = lo[2]; b2 <= hi[2]; ++b2, ++bi)
+                        {
+                            double const f0 = (double)b0, f1 = (double)b1,
+                                         f2 = (double)b2;
+                            double dr[3];
+                            dr[0] = d0 + f0 * p.box[0][0];
+                            dr[0] = dr[0] + f1 * p.box[1][0];
+                            dr[0] = dr[0] + f2 * p.box[2][0];
+                            dr[1] = d1 + f0 * p.box[0][1];
+                            dr[1] = dr[1] + f1 * p.box[1][1];
+                            dr[1] = dr[1] + f2 * p.box[2][1];
+                            dr[2] = d2 + f0 * p.box[0][2];
+                            dr[2] = dr[2] + f1 * p.box[1][2];
+                            dr[2] = dr[2] + f2 * p.box[2][2];
+                            bool const self = j == i && b0 == 0 && b1 == 0
+                                           && b2 == 0;
+                            bool const hit = live && !self
+                                          && norm2_pinned(dr) <= rc2;
+                            m |= (uint64_t)hit << bi;
+                        }
             }
-        }
-        uint64_t const mask = __ballot(hit);
-        if (MODE == 1 && hit)
-        {
-            int const h = hits + lanes_below(mask);
-            if (h < limit)
+            int const c = __popcll(m);
+            int inc = c;
+            for (int s = 1; s < 64; s <<= 1)
             {
-                int const q = o + h;
-                keysOut[q] = make_key(j, b0, b1, b2);
-                nbrJ[q] = j;
-                geo[q] = make_double4(dr[0], dr[1], dr[2],
-                                      sqrt(norm2_pinned(dr)));
+                int const v = __shfl_up(inc, s, 64);
+                if (lane >= s) inc += v;
             }
+            if (MODE == 1)
+            {
+                int h = hits + inc - c;
+                while (m)
+                {
+                    int const bi = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    if (h < limit)
+                    {
+                        int q0, rem, q1, q2;
+                        divmod_small(bi, w12, invW12, q0, rem);
+                        divmod_small(rem, w2, invW2, q1, q2);
+                        int const b0 = lo[0] + q0, b1 = lo[1] + q1,
+                                  b2 = lo[2] + q2;
+                        double dr[3];
+                        image_dr(p, ri, rj, b0, b1, b2, dr);
+                        int const q = o + h;
+                        keysOut[q] = make_key(j, b0, b1, b2);
+                        nbrJ[q] = j;
+                        geo[q] = make_double4(dr[0], dr[1], dr[2],
+                                              sqrt(norm2_pinned(dr)));
+                    }
+                    ++h;
+                }
+            }
+            hits += __shfl(inc, 63, 64);
         }
-        hits += __popcll(mask);
     }
+    else
+    {
+        for (int base = 0; base < total; base += 64)
+        {
+            int const t = base + lane;
+            bool hit = false;
+            int j = 0, b0 = 0, b1 = 0, b2 = 0;
+            double dr[3] = {0.0, 0.0, 0.0};
+            if (t < total)
+            {
+                int jj, bi, q0, r0, q1, r2;
+                divmod_small(t, nimg, invImg, jj, bi);
+                divmod_small(bi, w12, invW12, q0, r0);
+                divmod_small(r0, w2, invW2, q1, r2);
+                j = a0 + jj;
+                b0 = lo[0] + q0;
+                b1 = lo[1] + q1;
+                b2 = lo[2] + r2;
+                if (!(j == i && b0 == 0 && b1 == 0 && b2 == 0))
+                {
+                    double const rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1],
+                                          r[3 * (size_t)j + 2]};
+                    image_dr(p, ri, rj, b0, b1, b2, dr);
+                    hit = norm2_pinned(dr) <= rc2;
+                }
+            }
+            uint64_t const mask = __ballot(hit);
+            if (MODE == 1 && hit)
+            {
+                int const h = hits + lanes_below(mask);
+                if (h < limit)
+                {
+                    int const q = o + h;
+                    keysOut[q] = make_key(j, b0, b1, b2);
+                    nbrJ[q] = j;
+                    geo[q] = make_double4(dr[0], dr[1], dr[2],
+                                          sqrt(norm2_pinned(dr)));
+                }
+            }
+            hits += __popcll(mask);
+        }
+    }
+    // The largest count comes from a reduction over count[] afterwards: one
+    // atomic per atom on a single word would serialise in L2.
     if (lane == 0)
     {
         count[i] = hits;
-        if (MODE == 1)
-        {
-            atomicMax(&stats[0], hits);
-            if (hits > limit) stats[3] = 1;
-        }
+        if (MODE == 1 && hits > limit) stats[3] = 1;
     }
 }
 
