@@ -813,15 +813,25 @@ __global__ __launch_bounds__(64) void k_partner(
     }
 }
 
-// out[0] = max(in[0..n)); one block, fixed-order tree (the per-atom
-// neighbour counts, whose maximum sizes the fill's LDS).
+// out[0] = max(in[0..n)) of non-negative counts; one block, fixed-order
+// tree (the per-atom neighbour counts, whose maximum sizes the fill's
+// LDS).  `in` is 16-byte aligned (a device allocation): four counts per
+// load, four loads in flight per thread.
 __global__ __launch_bounds__(1024) void k_max(const int* __restrict__ in,
                                               int n, int* __restrict__ out)
 {
     __shared__ int s[1024];
     int const t = threadIdx.x;
     int m = 0;
-    for (int q = t; q < n; q += 1024) m = max(m, in[q]);
+    int const n4 = n >> 2;
+    const int4* __restrict__ in4 = reinterpret_cast<const int4*>(in);
+#pragma unroll 4
+    for (int q = t; q < n4; q += 1024)
+    {
+        int4 const v = in4[q];
+        m = max(m, max(max(v.x, v.y), max(v.z, v.w)));
+    }
+    for (int q = 4 * n4 + t; q < n; q += 1024) m = max(m, in[q]);
     s[t] = m;
     __syncthreads();
     for (int o = 512; o > 0; o >>= 1)
