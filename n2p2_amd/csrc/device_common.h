@@ -646,6 +646,16 @@ __device__ __forceinline__ bool pair_in_range(V4 const& pj, V4 const& pk,
     return (pj.w < rc) & (pk.w < rc) & (wide | (d2 < rc2));
 }
 
+// Orders this wave's LDS accesses around a queue hand-off between lanes,
+// where the workgroup is one wave: a wave's LDS operations complete in
+// issue order, so a wavefront-scope fence (no wait, unlike the workgroup
+// barrier's fence) is enough and later LDS reads no longer wait for the
+// queue writes before them to complete.
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // Number of set bits of a 64-lane ballot below this lane (two mbcnt
 // instructions).
 __device__ __forceinline__ int lanes_below(uint64_t mask)

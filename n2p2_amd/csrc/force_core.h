@@ -492,14 +492,14 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
             uint64_t const mask = __ballot(pass);
             if (pass) S.QUEUE[qn + lanes_below(mask)] = j | (k << 16);
             qn += __popcll(mask);
-            __syncthreads();
+            wave_lds_fence();
             if (qn >= 64)
             {
                 int const e = S.QUEUE[qn - 64 + lane];
                 triplet_backward<CT, GENPOW, MAXM>(gi, S, e & 0xffff, e >> 16,
                                                    wide, cc, M, MP, nmem);
                 qn -= 64;
-                __syncthreads();
+                wave_lds_fence();
             }
         }
         if (lane < qn)

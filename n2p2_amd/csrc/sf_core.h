@@ -344,7 +344,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                 uint64_t const mask = __ballot(pass);
                 if (pass) Q[qn + lanes_below(mask)] = ja | (kb << 16);
                 qn += __popcll(mask);
-                __syncthreads();
+                wave_lds_fence();
                 if (qn >= 64)
                 {
                     int const e = Q[qn - 64 + lane];
@@ -359,7 +359,7 @@ __device__ __forceinline__ void sf_forward_body(SFArgs const& a)
                     }
                     else keep = false;
                     qn -= 64;
-                    __syncthreads();
+                    wave_lds_fence();
                 }
             }
             if (lane < qn)
