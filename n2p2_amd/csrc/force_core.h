@@ -79,7 +79,6 @@ struct BLds
                      // (FP64 on both paths): lanes on neighbouring slots
                      // hit neighbouring banks
     int cap;
-    int* SLOT;       // original slot of each staged neighbour
     int* QUEUE;      // kQueue passing pairs, j | k << 16
 };
 
@@ -150,7 +149,6 @@ __device__ __forceinline__ void stage(ForceArgs const& a, BLds<R> const& S,
                     qq.z = R(0);
                     qq.w = R(0);
                     S.Q[pos] = qq;
-                    S.SLOT[pos] = s;
                     S.ACC[pos] = 0.0;
                     S.ACC[S.cap + pos] = 0.0;
                     S.ACC[2 * S.cap + pos] = 0.0;
@@ -348,8 +346,7 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
     S.Q = S.P + cap;
     S.ACC = (double*)(S.Q + cap);
     S.cap = cap;
-    S.SLOT = (int*)(S.ACC + 3 * cap);
-    S.QUEUE = S.SLOT + cap;
+    S.QUEUE = (int*)(S.ACC + 3 * cap);
 
     int const i = blockIdx.x;
     int const lane = threadIdx.x;
@@ -510,21 +507,14 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         }
         __syncthreads();
     }
-    // cslot in FP64; cself = -(sum of all slots), summed per lane in slot
-    // order and then by the fixed butterfly.
+    // cself = -(sum of all slots), summed per lane in staged order and
+    // then by the fixed butterfly.
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int t = lane; t < n; t += 64)
     {
-        size_t const q = (size_t)(s0 + S.SLOT[t]);
-        double const cx = S.ACC[t];
-        double const cy = S.ACC[cap + t];
-        double const cz = S.ACC[2 * cap + t];
-        a.cslot[3 * q] = cx;
-        a.cslot[3 * q + 1] = cy;
-        a.cslot[3 * q + 2] = cz;
-        sx += cx;
-        sy += cy;
-        sz += cz;
+        sx += S.ACC[t];
+        sy += S.ACC[cap + t];
+        sz += S.ACC[2 * cap + t];
     }
     sx = wave_sum(sx);
     sy = wave_sum(sy);
@@ -534,6 +524,36 @@ __device__ __forceinline__ void force_backward_body(ForceArgs const& a)
         a.cself[3 * (size_t)i] = -sx;
         a.cself[3 * (size_t)i + 1] = -sy;
         a.cself[3 * (size_t)i + 2] = -sz;
+    }
+    // cslot in FP64, in slot order: each slot's staged position follows
+    // from its element as in stage() (no slot map kept in LDS, which leaves
+    // room for more workgroups per CU).
+    int start[kMaxElements];
+#pragma unroll
+    for (int q = 0; q < kMaxElements; ++q) start[q] = sStart[q];
+    int const ne = T.ne;
+    for (int base = 0; base < n; base += 64)
+    {
+        int const s = base + lane;
+        int const e = s < n ? a.elem[a.nbrJ[s0 + s]] : -1;
+        int pos = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxElements; ++q)
+        {
+            if (q < ne)
+            {
+                uint64_t const mask = __ballot(e == q);
+                if (e == q) pos = start[q] + lanes_below(mask);
+                start[q] += __popcll(mask);
+            }
+        }
+        if (s < n)
+        {
+            size_t const q = (size_t)(s0 + s);
+            a.cslot[3 * q] = S.ACC[pos];
+            a.cslot[3 * q + 1] = S.ACC[cap + pos];
+            a.cslot[3 * q + 2] = S.ACC[2 * cap + pos];
+        }
     }
 }
 

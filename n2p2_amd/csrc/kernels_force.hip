@@ -120,7 +120,6 @@ size_t force_backward_lds_bytes(int cap, int fp32)
     size_t b = 0;
     b += (size_t)cap * 8 * r;                     // P (4) and Q (4)
     b += (size_t)cap * 3 * sizeof(double);        // ACC
-    b += (size_t)cap * sizeof(int);               // SLOT
     b += (size_t)kQueue * sizeof(int);            // QUEUE
     return b;
 }
