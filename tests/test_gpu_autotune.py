@@ -13,7 +13,10 @@ oracle's, as the member sums are reduced in an order that depends on the
 chunking.  The sweep returns its rows and marks the choice; a second
 context of the same model adopts the cached choice without sweeping.
 Cases: water (G2 + G3, groups of 8 members) and three elements with G2 +
-G9 (groups of 10 and 12: three chunks of 4 per element pair).
+G9 (groups of 10 and 12: three chunks of 4 per element pair); a batch of
+structures of the second model (hdnnp_set_structures), whose settings and
+sweep must give the default setting's FP64 results within 1e-12 (energies,
+relative) and 1e-10 eV/A.
 """
 import sys
 
@@ -21,6 +24,7 @@ import numpy as np
 import pytest
 
 from n2p2_amd import engine
+from n2p2_amd import workloads as W
 from tests import cases
 from tests.oracle_lib import OModel, OStruct
 
@@ -123,6 +127,41 @@ def test_sweep_and_cache(workdir):
     assert en2 == en
     assert np.array_equal(ctx2.forces(), ctx.forces())
     ctx2.close()
+    ctx.close()
+
+
+def test_batch_settings_and_sweep(workdir):
+    d, _, _, _ = cases.mixed_case(workdir, natoms=96, seed=3)
+    sts = [W.random_structure(nat, seed=300 + k)
+           for k, nat in enumerate((16, 40, 96, 150, 23))]
+    m = engine.Model.load_dir(d)
+    ctx = engine.Context(m)
+    ctx.set_structures(sts)
+    ctx.evaluate(engine.FORCES_REFERENCE)
+    e0, f0 = ctx.structure_energies().copy(), ctx.forces().copy()
+    g0 = ctx.G().copy()
+    for t in ((4, 0, 2, 1), (8, 7, 4, 2), (16, 4, 0, 4)):
+        say("batch fp64 setting %s" % (t,))
+        ctx.set_tuning(*t)
+        ctx.evaluate(engine.FORCES_REFERENCE)
+        assert np.all(np.abs(ctx.structure_energies() - e0)
+                      <= 1e-12 * np.abs(e0))
+        assert np.abs(ctx.forces() - f0).max() <= 1e-10
+        assert np.abs(ctx.G() - g0).max() <= 1e-12 * np.abs(g0).max()
+    rows = ctx.autotune(engine.FORCES_REFERENCE, sweep=True, reps=2)
+    say("batch fp64 sweep: %d rows, choice %s" % (len(rows), ctx.tuning()))
+    assert len(rows) >= 4
+    ctx.evaluate(engine.FORCES_REFERENCE)
+    assert np.abs(ctx.forces() - f0).max() <= 1e-10
+    ctx.set_precision(engine.PRECISION_MIXED)
+    rows = ctx.autotune(engine.FORCES_REFERENCE, sweep=True, reps=2)
+    say("batch mixed sweep: %d rows, choice %s" % (len(rows), ctx.tuning()))
+    ctx.evaluate(engine.FORCES_REFERENCE)
+    err = ctx.forces() - f0
+    assert float(np.sqrt(np.mean(err ** 2))) <= 1e-4
+    assert np.abs(err).max() <= 1e-3
+    assert np.all(np.abs(ctx.structure_energies() - e0)
+                  <= 1e-6 * np.abs(e0))
     ctx.close()
 
 
