@@ -124,6 +124,25 @@ def run_path(ctx, precision, steps, warmup):
     return dt, energy, forces, kt, names if spec else None
 
 
+def run_host_io(ctx, r, precision, steps, warmup):
+    """The same evaluation with the caller's host arrays on both sides of
+    every step: positions copied in (hdnnp_set_positions) and forces copied
+    out (hdnnp_get_forces), as a caller without device buffers pays."""
+    ctx.set_precision(precision)
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    for _ in range(warmup):
+        ctx.set_positions(rr)
+        ctx.evaluate(engine.FORCES_REFERENCE)
+        ctx.forces()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.set_positions(rr)
+        ctx.evaluate(engine.FORCES_REFERENCE)
+        f = ctx.forces()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, f
+
+
 def tune(ctx, precision, log, label):
     """Sweeps the launch forms (member chunk, waves-per-SIMD bounds of the
     specialised SF and force kernels, network tiles per wave) on the loaded
@@ -606,6 +625,10 @@ def main():
             % (dtb * 1e3, errbf))
         log("mixed with FP16 networks: %.3f ms/step, E=%.10f, kernel ms %s"
             % (dt16 * 1e3, e16, kt16))
+        dth, fh = run_host_io(ctx, r, engine.PRECISION_MIXED, args.steps,
+                              args.warmup)
+        log("mixed with host positions in and forces out: %.3f ms/step"
+            % (dth * 1e3))
         df = fm - f64
         err = {"force_rmse_eV_A": float(np.sqrt(np.mean(df ** 2))),
                "force_max_eV_A": float(np.abs(df).max()),
@@ -702,6 +725,15 @@ def main():
                                               "MFMA network layers (operands "
                                               "split hi + lo, three MFMAs per "
                                               "step, FP32 accumulation)"},
+                       "mixed_host_io": {
+                           "atom_steps_per_s": n / dth,
+                           "ms_per_step": dth * 1e3,
+                           "forces_equal_device_path": bool(
+                               np.array_equal(fh, fm)),
+                           "what": "as the mixed path with the positions "
+                                   "copied in from host memory and the "
+                                   "forces copied out to it every step "
+                                   "(PCIe included; not the value)"},
                        "mixed_vs_plain_fp64": dt64 / dtm,
                        "tuning": tuning,
                        "md": md,
