@@ -755,6 +755,7 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
                                      c->dCount.p, c->dStats.p, c->dOff.p,
                                      c->slab, c->dKeys.p, c->dNbrJ.p,
                                      c->dGeo.p, s), "nl fill");
+        hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s), "count max");
         readStats();
         if (hstat[2] > 0) cellKernels = false;    // staging overflow
         else if (hstat[3] == 0) built = true;
@@ -800,6 +801,8 @@ void buildNeighborList(hdnnp_ctx* c, double rc)
                                          c->dCount.p, c->dStats.p, c->dOff.p,
                                          c->slab, c->dKeys.p, c->dNbrJ.p,
                                          c->dGeo.p, s), "nl fill");
+            hipCheck(launch_max_int(c->dCount.p, n, c->dStats.p, s),
+                     "count max");
         }
         else
         {

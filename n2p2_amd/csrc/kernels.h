@@ -50,8 +50,8 @@ hipError_t launch_nl_count_cell(NLParams const& p, int ncell, const double* r,
                                 const int* cellAtoms, int* count, int* stats,
                                 hipStream_t s);
 // Writes atom i's entries from off[i] on, at most limit of them; count[i]
-// receives the true count, stats[0] the largest, stats[3] = 1 if a count
-// exceeds limit.
+// receives the true count, stats[3] = 1 if a count exceeds limit (the
+// largest count: launch_max_int over count).
 hipError_t launch_nl_fill_cell(NLParams const& p, int ncell, const double* r,
                                const int* cellStart, const int* cellCount,
                                const int* cellAtoms, int* count, int* stats,

@@ -279,8 +279,9 @@ __global__ __launch_bounds__(64) void k_nl(
 // and falls back to k_nl when a cell has more than kCellCand candidates
 // (stats[2]).  MODE 1 writes atom i's entries from off[i] on, at most
 // limit of them (a fixed slot range per atom), stores the true count in
-// count[i], raises stats[0] to the largest count and sets stats[3] when a
-// count exceeds limit (the host then rebuilds with a larger range).
+// count[i] and sets stats[3] when a count exceeds limit (the host then
+// rebuilds with a larger range); the largest count comes from a reduction
+// over count[] afterwards (launch_max_int).
 constexpr int kCellCand = 1024;
 constexpr int kCellWaves = 4;
 
@@ -294,7 +295,6 @@ __global__ __launch_bounds__(256) void k_nl_cell(
     double4* __restrict__ geo)
 {
     __shared__ double cX[kCellCand], cY[kCellCand], cZ[kCellCand];
-    __shared__ int sMax;
     // MODE 0: (j << 21 | image).  MODE 1: (j << 31 | image << 10 | t), t the
     // staging slot, sorted.
     __shared__ uint64_t cKey[kCellCand];
@@ -442,7 +442,6 @@ __global__ __launch_bounds__(256) void k_nl_cell(
                 cZ[u] = pz[q];
             }
         }
-        if (tid == 0) sMax = 0;
     }
     __syncthreads();
     int const a0 = cellStart[cell];
@@ -499,17 +498,8 @@ __global__ __launch_bounds__(256) void k_nl_cell(
         if (lane == 0)
         {
             count[i] = hits;
-            if (MODE == 1)
-            {
-                atomicMax(&sMax, hits);
-                if (hits > limit) stats[3] = 1;
-            }
+            if (MODE == 1 && hits > limit) stats[3] = 1;
         }
-    }
-    if (MODE == 1)
-    {
-        __syncthreads();
-        if (tid == 0) atomicMax(&stats[0], sMax);
     }
 }
 
